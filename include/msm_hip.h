@@ -1,0 +1,209 @@
+/*
+ * msm_hip.h -- C-ABI of the MI355X-native tabulated FS/EAM molecular-dynamics force path
+ * (libmsm_hip.so: hand-written gfx950 HIP kernels, FP64).
+ *
+ * The entry points replace the device-side call surface of the reference MDLIB:
+ *   force class            MDLIB/sor/CommonGPU/MD_ForceClass_Register_GPU.F90:248-259, 363-442
+ *   FS force               MDLIB/sor/CommonGPU/MD_FS_ForceTable_GPU.F90:178-257, 934-994, 1451-1700
+ *   EAM force              MDLIB/sor/CommonGPU/MD_EAM_ForceTable_GPU.F90:192-278, 891-1010, 1470-1700
+ *   neighbour list         MDLIB/sor/CommonGPU/MD_NeighborsList_GPU.F90:121-181, 604-693, 1347-1732
+ *   integrator             MDLIB/sor/CommonGPU/MD_DiffScheme_GPU.F90:604-682, 821-847
+ *   step loop              MDLIB/sor/Appshell/MD_Method_GenericMD_GPU.F90:496-659
+ * and the host-side table / box-setup helpers that feed them:
+ *   table generation       MDLIB/sor/Common/MD_TypeDef_ForceTable.F90:530-622, 890-1056, 1151-1230
+ *   potentials             MDLIB/sor/Potentials/EM_TB_WangJun_W-HE_2010/FS_Ackland_WW.F90:23-91,
+ *                          MDLIB/sor/Potentials/EM_TB_Cleri_Rosato_PRB48_22/TB_CR_AuAu.F90
+ *   RNG / Maxwell          LIB/sor/f/RandGenerators/DRAND32.F90:60-85,
+ *                          MDLIB/sor/Common/MD_TypeDef_SimBox.F90:4865-4942
+ *
+ * Array conventions are those of the reference's Fortran arrays, so Fortran callers pass their
+ * arrays unchanged through ISO_C_BINDING:
+ *   - XP(N,3), XP1(N,3), FP(N,3), DIS(N,3): column-major, x at [i], y at [i+N], z at [i+2N];
+ *   - ITYP(N): 1-based atom group; STATU(N): CP_STATU_* bit set;
+ *   - force tables POTR(NKIND,NTAB) etc. column-major: T(k,i) at [(k-1) + (i-1)*NKIND];
+ *     FEMBD(NKIND1,NEMBD) likewise; KPAIR(NGROUP,NGROUP) at [(i-1) + (j-1)*NGROUP] holding a
+ *     1-based kind; KEMBD(NGROUP) 1-based;
+ *   - RU(NGROUP,NGROUP), NB_RM(NGROUP,NGROUP) column-major;
+ *   - neighbour lists KVOIS(N), INDI(LD, MXKVOIS) slot-major with 1-based atom ids, in the
+ *     caller's atom order (as Copyout_NeighboreList_B0, MD_NeighborsList_GPU.F90:604-693).
+ * The library is unit-agnostic: positions, table abscissae, masses and time step must be in one
+ * consistent unit system (the reference uses CGS; the tests use eV, Angstrom, fs with masses in
+ * eV fs^2 / Angstrom^2).  Box shapes are orthorhombic (BOXSHAPE = identity).
+ *
+ * Every entry point returns MSM_OK (0) or an MSM_ERR_* code; nothing stops the process or reads
+ * stdin.  msm_last_error() gives a message for the last failing call on a context.
+ */
+#ifndef MSM_HIP_H
+#define MSM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSM_FS_TYPE  0   /* F = -sqrt(rho) built in ("FS_TYPE") */
+#define MSM_EAM_TYPE 1   /* F(rho) tabulated ("EAM_TYPE") */
+
+/* status bits, MDLIB/sor/Common/MD_Const.F90:79-99 */
+#define MSM_STATU_ACTIVE    1
+#define MSM_STATU_FIXPOSX   2
+#define MSM_STATU_FIXPOSY   4
+#define MSM_STATU_FIXPOSZ   8
+#define MSM_STATU_FIXVELX   16
+#define MSM_STATU_FIXVELY   32
+#define MSM_STATU_FIXVELZ   64
+#define MSM_STATU_OUTOFBOX  65536
+#define MSM_STATU_REFLECT   131072
+#define MSM_STATU_TRANSMIT  262144
+#define MSM_STATU_PASSBOUND 524288
+
+#define MSM_OK                  0
+#define MSM_ERR_ARG             1  /* invalid argument or size */
+#define MSM_ERR_HIP             2  /* HIP runtime error */
+#define MSM_ERR_NOMEM           3  /* device allocation failed */
+#define MSM_ERR_NLIST_OVERFLOW  4  /* an atom has more than MXKVOIS neighbours */
+#define MSM_ERR_OUTOFBOX        5  /* an active atom lies outside its box's cell grid */
+#define MSM_ERR_EMBD_RANGE      6  /* a density fell beyond the embedding table */
+#define MSM_ERR_STATE           7  /* call out of order (no table / box / atoms / list yet) */
+
+typedef struct msm_sim msm_sim;
+
+/* Pair/embedding callback: (r, out value, out derivative term) by reference, the shape of the
+ * reference's NNFORCE/NEFORCE/EMBDF interfaces (MD_TypeDef_ForceTable.F90:46-78), so a Fortran
+ * bind(C) subroutine can be passed with c_funloc. */
+typedef void (*msm_pairfn)(const double* r, double* v, double* dv);
+
+const char* msm_strerror(int code);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Host-side table generation (no GPU needed)                                                  */
+/* ------------------------------------------------------------------------------------------ */
+
+/* One kind of Create_Pairwise_ForceTable (MD_TypeDef_ForceTable.F90:949-976): entries
+ * i=1..ntab at r_i=(i/CSI)^2, CSI=ntab/sqrt(rmax); POTR=nn.v*r, FPOTR=nn.dv*r, POTB=ne.v,
+ * FPOTB=ne.dv (ne may be NULL: zeros).  Output arrays hold ntab values with element stride
+ * `stride` (use NKIND to write straight into a POTR(NKIND,NTAB) array at offset k-1).
+ * rhomx_inout accumulates max(POTB*rhomult) as FTable%RHOMX does (start it at -1). */
+int msm_table_pairwise(int ntab, double rmax, msm_pairfn nn, msm_pairfn ne, double rhomult,
+                       double* rhomx_inout, double* potr, double* fpotr, double* potb,
+                       double* fpotb, int stride);
+/* One kind of Create_EMBDFUNTable (:1043-1048): F(i),F'(i) at rho=(i-1)*rhod, i=1..nembd. */
+int msm_table_embd(int nembd, double rhod, msm_pairfn embd, double* fembd, double* dfembd,
+                   int stride);
+
+/* Built-in potential forms in eV / Angstrom (NN: 0.5*V and -dV/dr; NE: rho and -drho/dr;
+ * EMB: F and dF/drho). */
+void msm_pot_wfs_nn(const double* r, double* potr, double* fpotr);       /* benchmark W FS pair */
+void msm_pot_wfs_ne(const double* r, double* potb, double* fpotb);       /* A^2 (r-4.4002)^2    */
+void msm_pot_ackland_ww_nn(const double* r, double* potr, double* fpotr);/* FS_Ackland_WW.F90   */
+void msm_pot_ackland_ww_ne(const double* r, double* potb, double* fpotb);
+void msm_emb_fs(const double* rho, double* f, double* df);               /* -sqrt(rho)          */
+/* Cleri-Rosato TB-SMA (PRB 48 (1993) 22) in the reference's form (TB_CR_AuAu.F90:17-53):
+ * NN = A exp(-p(r/r0-1)), 2 NN p/r0; NE = xi^2 exp(-2q(r/r0-1)), 2 q/r0 NE; truncated only by
+ * the force cutoff, as in the reference.  pair = 0:Cu-Cu 1:Ni-Ni 2:Cu-Ni (Cu, Ni from the
+ * paper's table; Cu-Ni by arithmetic means of A, p, q, r0 and the geometric mean of xi). */
+void msm_pot_tbsma_nn(int pair, const double* r, double* potr, double* fpotr);
+void msm_pot_tbsma_ne(int pair, const double* r, double* potb, double* fpotb);
+void msm_pot_tbsma_cucu_nn(const double* r, double* potr, double* fpotr);
+void msm_pot_tbsma_cucu_ne(const double* r, double* potb, double* fpotb);
+void msm_pot_tbsma_nini_nn(const double* r, double* potr, double* fpotr);
+void msm_pot_tbsma_nini_ne(const double* r, double* potb, double* fpotb);
+void msm_pot_tbsma_cuni_nn(const double* r, double* potr, double* fpotr);
+void msm_pot_tbsma_cuni_ne(const double* r, double* potb, double* fpotb);
+
+/* DRAND32 (DRAND32.F90:60-85): returns the next number and advances seed[2]. */
+double msm_drand32(int* seed);
+/* Thermalize_MC_SimMDBox (MD_TypeDef_SimBox.F90:4865-4942) over atoms [0,n) in index order:
+ * Maxwell components v0*sqrt(-ln z1)*cos(2 pi z2), v0=sqrt(2 T kb / m), minus the mass-weighted
+ * mean.  cm is per group (1-based ITYP); writes XP1(N,3). */
+int msm_thermalize_mc(int n, const int* ityp, const double* cm, double temp, double kb,
+                      int* seed, double* xp1);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Device context                                                                              */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Create a context on HIP device `device` (Initialize_DEVICES / Initialize_Globle_Variables_DEV). */
+int msm_create(msm_sim** out, int device);
+int msm_destroy(msm_sim* sim);
+const char* msm_last_error(const msm_sim* sim);
+
+/* Copy a force table to the device (INITIALIZE_FS_Force_Table_DEV / INITIALIZE_EAM_Force_Table_DEV).
+ * pot_type MSM_FS_TYPE ignores the embedding arguments (may be 0/NULL). */
+int msm_set_force_table(msm_sim* sim, int pot_type, int ngroup, int nkind, int ntab, double csi,
+                        const int* kpair, const double* potr, const double* fpotr,
+                        const double* potb, const double* fpotb, int nkind1, int nembd,
+                        double rhod, const int* kembd, const double* fembd,
+                        const double* dfembd);
+
+/* Cutoffs: force cutoff RU(NGROUP,NGROUP) (the device kernels use max(RU^2), as the reference
+ * kernels do), neighbour cutoff NB_RM(NGROUP,NGROUP), list capacity MXKVOIS, and per-group
+ * masses CM(NGROUP). */
+int msm_set_ctrl(msm_sim* sim, int ngroup, const double* ru, const double* nb_rm, int mxkvois,
+                 const double* cm);
+
+/* Boxes (MULTIBOX generalised to uneven sizes): nbox boxes, box b holding natoms[b] consecutive
+ * atoms of the arrays given to msm_set_atoms; lo[3*b..], len[3*b..] in length units; ifpd[3*b..]
+ * periodic flags. */
+int msm_set_boxes(msm_sim* sim, int nbox, const int* natoms, const double* lo, const double* len,
+                  const int* ifpd);
+
+/* Upload atoms (copy-in of SimBox XP/XP1/ITYP/STATU; DIS and FP reset to zero). */
+int msm_set_atoms(msm_sim* sim, int n, const double* xp, const double* xp1, const int* ityp,
+                  const int* statu);
+
+/* Copy-out in the caller's atom order; any pointer may be NULL. */
+int msm_get_atoms(msm_sim* sim, double* xp, double* xp1, double* fp, int* statu, double* dis);
+/* F'(rho) per atom (the reference's DEN array) and per-atom potential energy EPOT. */
+int msm_get_den(msm_sim* sim, double* den);
+int msm_get_epot(msm_sim* sim, double* epot);
+/* Total potential energy per box (pairwise tree sum on the device); valid after msm_calc_epot. */
+int msm_get_box_epot(msm_sim* sim, double* ebox);
+
+/* Cal_NeighBoreList_DEV: device cell binning + deterministic sort + 27-cell full list, FP64
+ * distances with the reference host predicate (strict <, shifted images xj+L). */
+int msm_build_neighbors(msm_sim* sim);
+/* Copyout_NeighboreList: KVOIS(N) and INDI(ld, MXKVOIS) (1-based ids) in caller order. */
+int msm_get_neighbors(msm_sim* sim, int* kvois, int* indi, int ld);
+
+/* CALFORCE_*_Force_Table2A_DEV: density pass then force pass. */
+int msm_calc_force(msm_sim* sim);
+/* CALEPOT_*_Force_Table2A_DEV: per-atom potential energy (and box totals). */
+int msm_calc_epot(msm_sim* sim);
+/* Predictor_DEV / Correction_DEV (Predictor_KERNEL0 / Correction_KERNEL semantics). */
+int msm_predictor(msm_sim* sim, double h);
+int msm_correction(msm_sim* sim, double h);
+
+/* nsteps of For_One_Step on the device: predictor, list rebuild when (step0+s) % nb_uptab == 0,
+ * density, force, corrector.  The corrector of step s and the predictor of step s+1 run fused in
+ * the force kernel; the sequence between rebuilds is replayed from a captured HIP graph.
+ * Requires forces of the current positions (msm_calc_force) on entry. */
+int msm_run(msm_sim* sim, int nsteps, int nb_uptab, int step0, double h);
+
+/* Wait for all device work of the context. */
+int msm_sync(msm_sim* sim);
+
+/* Neighbour-list statistics of the last build: total entries (sum of KVOIS) and max KVOIS. */
+int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
+
+/* Options.  MSM_OPT_TRACK_DIS (default 1): accumulate DIS in the fused predictor of msm_run.
+ * MSM_OPT_KERNEL_TIMING (default 0): msm_run enqueues its kernels directly (no graph replay) with
+ * HIP events around each launch and accumulates per-class device time. */
+#define MSM_OPT_TRACK_DIS     1
+#define MSM_OPT_KERNEL_TIMING 2
+int msm_set_option(msm_sim* sim, int option, int value);
+
+/* Kernel classes for msm_kernel_times. */
+#define MSM_KCLASS_DENSITY   0
+#define MSM_KCLASS_FORCE     1  /* force pass with the fused corrector/predictor */
+#define MSM_KCLASS_REBUILD   2  /* whole neighbour-list rebuild (binning, sort, gather, list) */
+#define MSM_KCLASS_PREDICTOR 3
+#define MSM_KCLASS_NLIST     4  /* the list kernel alone (inside the rebuild) */
+#define MSM_KCLASS_COUNT     5
+/* Accumulated device milliseconds and launch counts per kernel class since the last reset
+ * (collected while MSM_OPT_KERNEL_TIMING is on); reset != 0 clears the counters after reading. */
+int msm_kernel_times(msm_sim* sim, double* ms, long long* count, int nclass, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+"""In-tree build of libmsm_hip.so (hipcc, gfx950) and of the Fortran drop-in layer (amdflang).
+
+    python -m msmpscu_amd.build          # library + Fortran module and driver
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+FSRC = os.path.join(HERE, "fortran")
+LIB = os.path.join(HERE, "libmsm_hip.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # no packed FP32 instructions in device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "is not a recognized feature" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+    return out
+
+
+def build_library() -> str:
+    srcs = [os.path.join(CSRC, f) for f in ("msm_api.hip", "msm_kernels.hip", "msm_host.cpp")]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+          "-o", tmp, *srcs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran() -> list:
+    """Fortran drop-in module + test driver, linked against libmsm_hip.so."""
+    outs = []
+    mod_src = os.path.join(FSRC, "MD_ForceLib_HIP.F90")
+    if not os.path.exists(mod_src) or shutil.which(FC) is None and not os.path.exists(FC):
+        return outs
+    bdir = os.path.join(FSRC, "_fbuild")
+    os.makedirs(bdir, exist_ok=True)
+    _run([FC, "-O2", "-fPIC", "-c", mod_src, "-module-dir", bdir, "-o",
+          os.path.join(bdir, "MD_ForceLib_HIP.o")])
+    drv = os.path.join(FSRC, "fs_w_driver.F90")
+    exe = os.path.join(FSRC, "fs_w_driver")
+    if os.path.exists(drv):
+        _run([FC, "-O2", drv, os.path.join(bdir, "MD_ForceLib_HIP.o"), "-I" + bdir,
+              "-module-dir", bdir, "-L" + HERE, "-lmsm_hip", "-Wl,-rpath," + HERE, "-o", exe])
+        outs.append(exe)
+    return outs
+
+
+def build_all() -> None:
+    build_library()
+    build_fortran()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", LIB)

@@ -1,0 +1,963 @@
+// msm_api.hip -- C-ABI entry points of libmsm_hip.so (declared in include/msm_hip.h): device
+// context, table / box / atom upload, neighbour-list build, force, energy, integrator and the
+// graph-replayed step loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "msm_hip.h"
+#include "msm_internal.h"
+
+using msm::AtomsDev;
+using msm::BoxDev;
+using msm::ListDev;
+using msm::SortBufs;
+using msm::StepParams;
+using msm::TabDev;
+
+namespace {
+
+struct AtomSet {  // one set of per-atom arrays (cell-sorted order)
+    double* v = nullptr;    // 3 * n_alloc
+    double* f = nullptr;    // 3 * n_alloc
+    double* dis = nullptr;  // 3 * n_alloc
+    int* ityp = nullptr;
+    int* statu = nullptr;
+    int* boxid = nullptr;
+    int* gid = nullptr;
+    int* cell = nullptr;
+};
+
+struct GraphKey {
+    int set, pos, rebuild, nsteps, last, track;
+    bool operator<(const GraphKey& o) const {
+        return std::tie(set, pos, rebuild, nsteps, last, track) <
+               std::tie(o.set, o.pos, o.rebuild, o.nsteps, o.last, o.track);
+    }
+};
+
+}  // namespace
+
+struct msm_sim {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // force table
+    bool has_table = false;
+    TabDev tab{};
+    double2* d_b2 = nullptr;
+    double4 *d_f4 = nullptr, *d_e4 = nullptr, *d_m4 = nullptr;
+    int tab_ngroup = 0;
+
+    // control
+    bool has_ctrl = false;
+    int ngroup = 0, mx = 0;
+    std::vector<double> ru, nb_rm, cm;
+    double* d_rnl2 = nullptr;
+
+    // boxes
+    bool has_boxes = false;
+    std::vector<BoxDev> boxes;
+    BoxDev* d_boxes = nullptr;
+    int ncell_total = 0;
+
+    // atoms
+    int n = 0, n_alloc = 0;
+    double4* pos[2] = {nullptr, nullptr};
+    int pcur = 0;  // index of the current position buffer
+    AtomSet set[2];
+    int scur = 0;  // index of the current atom set
+    double* epot = nullptr;
+    bool has_atoms = false;
+
+    // sort + list
+    SortBufs sb{};
+    int sort_cap = 0;
+    int* kvois = nullptr;
+    int* indi = nullptr;
+    size_t indi_cap = 0;
+    bool has_list = false;
+
+    // flags and scratch
+    int* d_err = nullptr;
+    int* h_err = nullptr;
+    double* d_ebox = nullptr;
+    unsigned long long* d_stats = nullptr;
+
+    // options
+    int track_dis = 1;
+    int timing = 0;
+
+    // per-class kernel timing (MSM_OPT_KERNEL_TIMING)
+    std::vector<hipEvent_t> evpool;
+    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> pending;
+    double kms[MSM_KCLASS_COUNT] = {0, 0, 0, 0, 0};
+    long long kcnt[MSM_KCLASS_COUNT] = {0, 0, 0, 0, 0};
+
+    std::map<GraphKey, hipGraphExec_t> graphs;
+};
+
+namespace {
+
+int set_err(msm_sim* s, int code, const std::string& msg) {
+    if (s) s->err = msg;
+    return code;
+}
+
+#define HIPCHK(sim, call)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return set_err(sim, MSM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(msm_sim* s, T** p, size_t count) {
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(s, MSM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return MSM_OK;
+}
+
+template <typename T>
+void dfree(T** p) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+}
+
+void clear_graphs(msm_sim* s) {
+    for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
+    s->graphs.clear();
+}
+
+void free_atoms(msm_sim* s) {
+    for (int k = 0; k < 2; ++k) {
+        dfree(&s->pos[k]);
+        dfree(&s->set[k].v);
+        dfree(&s->set[k].f);
+        dfree(&s->set[k].dis);
+        dfree(&s->set[k].ityp);
+        dfree(&s->set[k].statu);
+        dfree(&s->set[k].boxid);
+        dfree(&s->set[k].gid);
+        dfree(&s->set[k].cell);
+    }
+    dfree(&s->epot);
+    s->n = s->n_alloc = 0;
+    s->has_atoms = false;
+    s->has_list = false;
+}
+
+AtomsDev atoms_view(msm_sim* s, int set, int pcur) {
+    AtomsDev a{};
+    const size_t na = static_cast<size_t>(s->n_alloc);
+    AtomSet& z = s->set[set];
+    a.pos = s->pos[pcur];
+    a.pos_out = s->pos[1 - pcur];
+    a.vx = z.v; a.vy = z.v + na; a.vz = z.v + 2 * na;
+    a.fx = z.f; a.fy = z.f + na; a.fz = z.f + 2 * na;
+    a.dx = z.dis; a.dy = z.dis + na; a.dz = z.dis + 2 * na;
+    a.ityp = z.ityp;
+    a.statu = z.statu;
+    a.boxid = z.boxid;
+    a.epot = s->epot;
+    a.n = s->n;
+    return a;
+}
+
+ListDev list_view(msm_sim* s) {
+    ListDev l{};
+    l.kvois = s->kvois;
+    l.indi = s->indi;
+    l.stride = s->n_alloc;
+    return l;
+}
+
+int check_ready(msm_sim* s, bool need_list) {
+    if (!s) return MSM_ERR_ARG;
+    if (!s->has_table) return set_err(s, MSM_ERR_STATE, "no force table (msm_set_force_table)");
+    if (!s->has_ctrl) return set_err(s, MSM_ERR_STATE, "no control parameters (msm_set_ctrl)");
+    if (!s->has_boxes) return set_err(s, MSM_ERR_STATE, "no boxes (msm_set_boxes)");
+    if (!s->has_atoms) return set_err(s, MSM_ERR_STATE, "no atoms (msm_set_atoms)");
+    if (s->tab_ngroup != s->ngroup)
+        return set_err(s, MSM_ERR_ARG, "force table and control parameters disagree on NGROUP");
+    if (need_list && !s->has_list)
+        return set_err(s, MSM_ERR_STATE, "no neighbour list (msm_build_neighbors)");
+    return MSM_OK;
+}
+
+// ---- optional per-launch timing with HIP events on the context's stream ----
+hipEvent_t take_event(msm_sim* s) {
+    if (!s->evpool.empty()) {
+        hipEvent_t e = s->evpool.back();
+        s->evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct TimedScope {
+    msm_sim* s;
+    int cls;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TimedScope(msm_sim* sim, int c) : s(sim), cls(c) {
+        if (s->timing) {
+            e0 = take_event(s);
+            e1 = take_event(s);
+            (void)hipEventRecord(e0, s->stream);
+        }
+    }
+    ~TimedScope() {
+        if (s->timing) {
+            (void)hipEventRecord(e1, s->stream);
+            s->pending.emplace_back(cls, e0, e1);
+        }
+    }
+};
+
+void resolve_timing(msm_sim* s) {
+    for (auto& p : s->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, std::get<1>(p), std::get<2>(p)) == hipSuccess) {
+            s->kms[std::get<0>(p)] += ms;
+            s->kcnt[std::get<0>(p)] += 1;
+        }
+        s->evpool.push_back(std::get<1>(p));
+        s->evpool.push_back(std::get<2>(p));
+    }
+    s->pending.clear();
+}
+
+// Device error flags: cleared with a fill kernel so the clear can sit inside captured graphs.
+void clear_flags(msm_sim* s) { msm::launch_fill_int(s->d_err, 0, msm::kErrSlots, s->stream); }
+
+int read_flags(msm_sim* s) {
+    HIPCHK(s, hipMemcpyAsync(s->h_err, s->d_err, sizeof(int) * msm::kErrSlots,
+                             hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    resolve_timing(s);
+    if (s->h_err[msm::kErrOutOfBox])
+        return set_err(s, MSM_ERR_OUTOFBOX, "an active atom lies outside its box's cell grid");
+    if (s->h_err[msm::kErrOverflow])
+        return set_err(s, MSM_ERR_NLIST_OVERFLOW,
+                       "an atom has more neighbours than MXKVOIS (list truncated)");
+    if (s->h_err[msm::kErrEmbd])
+        return set_err(s, MSM_ERR_EMBD_RANGE, "a density fell beyond the embedding table");
+    return MSM_OK;
+}
+
+// Neighbour-list rebuild on the stream (capturable: kernels only).  Reads the current set and
+// position buffer, writes the other set and buffer, then flips both.
+void enqueue_rebuild(msm_sim* s) {
+    TimedScope ts(s, MSM_KCLASS_REBUILD);
+    const int src = s->scur, dst = 1 - s->scur;
+    const int p0 = s->pcur, p1 = 1 - s->pcur;
+    AtomsDev a_src = atoms_view(s, src, p0);
+    AtomsDev a_dst = atoms_view(s, dst, p1);
+    a_dst.pos = s->pos[p1];
+    const int nc = s->ncell_total;
+    msm::launch_fill_int(s->sb.count, 0, nc + 2, s->stream);
+    msm::launch_cell_assign(a_src.pos, a_src.statu, a_src.boxid, s->d_boxes, s->n, nc, s->sb,
+                            s->d_err, s->stream);
+    msm::launch_scan_exclusive(s->sb.count, s->sb.start, nc + 2, s->sb.scan_tmp, s->stream);
+    msm::launch_scatter_sort(s->n, nc, s->sb, s->set[src].gid, s->stream);
+    msm::launch_gather(s->n, s->n_alloc, s->sb.perm, a_src, s->set[src].gid, a_dst,
+                       s->set[dst].gid, s->sb.key, s->set[dst].cell, s->stream);
+    TimedScope tl(s, MSM_KCLASS_NLIST);
+    msm::launch_nlist(s->n, s->pos[p1], s->set[dst].ityp, s->set[dst].statu, s->set[dst].boxid,
+                      s->set[dst].cell, s->sb.start, s->d_boxes, s->ngroup, s->d_rnl2, s->mx,
+                      s->kvois, s->indi, s->n_alloc, s->d_err, s->stream);
+    s->scur = dst;
+    s->pcur = p1;
+}
+
+bool multibox(const msm_sim* s) { return s->boxes.size() > 1; }
+
+void enqueue_density(msm_sim* s) {
+    TimedScope ts(s, MSM_KCLASS_DENSITY);
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_density(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), s->d_err,
+                        s->stream);
+}
+
+// mode 0: force only; 1: force + corrector; 2: force + corrector + next predictor (flips pos).
+void enqueue_force(msm_sim* s, int mode, int write_f, double h) {
+    TimedScope ts(s, MSM_KCLASS_FORCE);
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    StepParams sp{};
+    sp.h = h;
+    sp.hs2 = h * 0.5;
+    sp.h2s2 = h * sp.hs2;
+    sp.mode = mode;
+    sp.write_f = write_f;
+    sp.track_dis = s->track_dis;
+    msm::launch_force(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), sp,
+                      s->stream);
+    if (mode == 2) s->pcur = 1 - s->pcur;
+}
+
+// One segment of the step loop: [rebuild] then nsteps x (density, fused force); the final step
+// of the run applies only the corrector and stores forces.
+void enqueue_segment(msm_sim* s, int rebuild, int nsteps, int last, double h) {
+    if (rebuild) enqueue_rebuild(s);
+    for (int k = 0; k < nsteps; ++k) {
+        enqueue_density(s);
+        const bool final_step = last && (k == nsteps - 1);
+        if (final_step)
+            enqueue_force(s, 1, 1, h);
+        else
+            enqueue_force(s, 2, 0, h);
+    }
+}
+
+// Host-side state change of a segment without enqueuing work (used when a cached graph replays
+// the segment).
+void advance_segment_state(msm_sim* s, int rebuild, int nsteps, int last) {
+    if (rebuild) {
+        s->scur = 1 - s->scur;
+        s->pcur = 1 - s->pcur;
+    }
+    const int flips = last ? nsteps - 1 : nsteps;
+    if (flips & 1) s->pcur = 1 - s->pcur;
+}
+
+int run_segment(msm_sim* s, int rebuild, int nsteps, int last, double h) {
+    if (s->timing) {  // direct launches, each bracketed by events
+        enqueue_segment(s, rebuild, nsteps, last, h);
+        return MSM_OK;
+    }
+    GraphKey key{s->scur, s->pcur, rebuild, nsteps, last, s->track_dis};
+    auto it = s->graphs.find(key);
+    if (it == s->graphs.end()) {
+        const int scur0 = s->scur, pcur0 = s->pcur;
+        HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_segment(s, rebuild, nsteps, last, h);
+        hipGraph_t g = nullptr;
+        HIPCHK(s, hipStreamEndCapture(s->stream, &g));
+        hipGraphExec_t ge = nullptr;
+        hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess)
+            return set_err(s, MSM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        it = s->graphs.emplace(key, ge).first;
+        // the capture advanced the host state; rewind so the launch below replays from it
+        s->scur = scur0;
+        s->pcur = pcur0;
+    }
+    HIPCHK(s, hipGraphLaunch(it->second, s->stream));
+    advance_segment_state(s, rebuild, nsteps, last);
+    return MSM_OK;
+}
+
+}  // namespace
+
+// ============================================================================================
+// C-ABI
+// ============================================================================================
+
+extern "C" {
+
+const char* msm_strerror(int code) {
+    switch (code) {
+        case MSM_OK: return "ok";
+        case MSM_ERR_ARG: return "invalid argument";
+        case MSM_ERR_HIP: return "HIP runtime error";
+        case MSM_ERR_NOMEM: return "device allocation failed";
+        case MSM_ERR_NLIST_OVERFLOW: return "neighbour list overflow";
+        case MSM_ERR_OUTOFBOX: return "active atom out of box";
+        case MSM_ERR_EMBD_RANGE: return "density beyond embedding table";
+        case MSM_ERR_STATE: return "call out of order";
+        default: return "unknown error";
+    }
+}
+
+int msm_create(msm_sim** out, int device) {
+    if (!out) return MSM_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MSM_ERR_HIP;
+    if (device < 0 || device >= ndev) return MSM_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return MSM_ERR_HIP;
+    msm_sim* s = new msm_sim();
+    s->device = device;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return MSM_ERR_HIP;
+    }
+    int rc = dalloc(s, &s->d_err, msm::kErrSlots);
+    if (rc == MSM_OK) rc = dalloc(s, &s->d_stats, 2);
+    if (rc == MSM_OK && hipHostMalloc(reinterpret_cast<void**>(&s->h_err),
+                                      sizeof(int) * msm::kErrSlots) != hipSuccess)
+        rc = MSM_ERR_NOMEM;
+    if (rc != MSM_OK) {
+        msm_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return MSM_OK;
+}
+
+int msm_destroy(msm_sim* s) {
+    if (!s) return MSM_OK;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    clear_graphs(s);
+    free_atoms(s);
+    for (auto& p : s->pending) {
+        (void)hipEventDestroy(std::get<1>(p));
+        (void)hipEventDestroy(std::get<2>(p));
+    }
+    for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
+    dfree(&s->d_b2);
+    dfree(&s->d_f4);
+    dfree(&s->d_e4);
+    dfree(&s->d_m4);
+    dfree(&s->d_rnl2);
+    dfree(&s->d_boxes);
+    dfree(&s->sb.key);
+    dfree(&s->sb.slot);
+    dfree(&s->sb.count);
+    dfree(&s->sb.start);
+    dfree(&s->sb.perm);
+    dfree(&s->sb.scan_tmp);
+    dfree(&s->kvois);
+    dfree(&s->indi);
+    dfree(&s->d_err);
+    dfree(&s->d_ebox);
+    dfree(&s->d_stats);
+    if (s->h_err) (void)hipHostFree(s->h_err);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return MSM_OK;
+}
+
+const char* msm_last_error(const msm_sim* s) { return s ? s->err.c_str() : "null context"; }
+
+int msm_set_force_table(msm_sim* s, int pot_type, int ngroup, int nkind, int ntab, double csi,
+                        const int* kpair, const double* potr, const double* fpotr,
+                        const double* potb, const double* fpotb, int nkind1, int nembd,
+                        double rhod, const int* kembd, const double* fembd,
+                        const double* dfembd) {
+    if (!s) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (pot_type != MSM_FS_TYPE && pot_type != MSM_EAM_TYPE)
+        return set_err(s, MSM_ERR_ARG, "pot_type must be MSM_FS_TYPE or MSM_EAM_TYPE");
+    if (ngroup < 1 || ngroup > msm::kMaxGroup || nkind < 1 || ntab < 2 || !(csi > 0.0) ||
+        !kpair || !potr || !fpotr || !potb || !fpotb)
+        return set_err(s, MSM_ERR_ARG, "bad pair-table arguments");
+    if (pot_type == MSM_EAM_TYPE &&
+        (nkind1 < 1 || nembd < 2 || !(rhod > 0.0) || !kembd || !fembd || !dfembd))
+        return set_err(s, MSM_ERR_ARG, "bad embedding-table arguments");
+    const int ntab2 = ntab + 2;
+    // kind-major padded knots with forward differences (v(i+1) - v(i) is the same IEEE
+    // subtraction the reference evaluates per lookup, so precomputing it changes no bit)
+    auto knot = [&](const double* T, int k, int i) -> double {
+        if (i < 1 || i > ntab) return 0.0;
+        return T[static_cast<size_t>(k) + static_cast<size_t>(i - 1) * nkind];
+    };
+    std::vector<double2> hb2(static_cast<size_t>(nkind) * ntab2);
+    std::vector<double4> hf4(static_cast<size_t>(nkind) * ntab2), he4(hf4.size());
+    for (int k = 0; k < nkind; ++k)
+        for (int i = 0; i < ntab2; ++i) {
+            const size_t a = static_cast<size_t>(k) * ntab2 + i;
+            const double pb = knot(potb, k, i), pb1 = knot(potb, k, i + 1);
+            hb2[a] = make_double2(pb, pb1 - pb);
+            const double fr = knot(fpotr, k, i), fr1 = knot(fpotr, k, i + 1);
+            const double fb = knot(fpotb, k, i), fb1 = knot(fpotb, k, i + 1);
+            hf4[a] = make_double4(fr, fr1 - fr, fb, fb1 - fb);
+            const double pr = knot(potr, k, i), pr1 = knot(potr, k, i + 1);
+            he4[a] = make_double4(pr, pr1 - pr, pb, pb1 - pb);
+        }
+    TabDev t{};
+    t.pot_type = pot_type;
+    t.ngroup = ngroup;
+    t.ntab2 = ntab2;
+    t.csi = csi;
+    for (int i = 0; i < ngroup; ++i)
+        for (int j = 0; j < ngroup; ++j) {
+            const int kd = kpair[i + j * ngroup];
+            if (kd < 1 || kd > nkind) return set_err(s, MSM_ERR_ARG, "KPAIR entry out of range");
+            t.kpair[i * ngroup + j] = kd - 1;
+        }
+    int rc = dalloc(s, &s->d_b2, hb2.size());
+    if (rc == MSM_OK) rc = dalloc(s, &s->d_f4, hf4.size());
+    if (rc == MSM_OK) rc = dalloc(s, &s->d_e4, he4.size());
+    if (rc != MSM_OK) return rc;
+    HIPCHK(s, hipMemcpy(s->d_b2, hb2.data(), hb2.size() * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->d_f4, hf4.data(), hf4.size() * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->d_e4, he4.data(), he4.size() * sizeof(double4), hipMemcpyHostToDevice));
+    if (pot_type == MSM_EAM_TYPE) {
+        const int nembd2 = nembd + 2;
+        auto eknot = [&](const double* T, int k, int i) -> double {
+            if (i < 1 || i > nembd) return 0.0;
+            return T[static_cast<size_t>(k) + static_cast<size_t>(i - 1) * nkind1];
+        };
+        std::vector<double4> hm4(static_cast<size_t>(nkind1) * nembd2);
+        for (int k = 0; k < nkind1; ++k)
+            for (int i = 0; i < nembd2; ++i) {
+                const double f0 = eknot(fembd, k, i), f1 = eknot(fembd, k, i + 1);
+                const double d0 = eknot(dfembd, k, i), d1 = eknot(dfembd, k, i + 1);
+                hm4[static_cast<size_t>(k) * nembd2 + i] = make_double4(f0, f1 - f0, d0, d1 - d0);
+            }
+        for (int i = 0; i < ngroup; ++i) {
+            if (kembd[i] < 1 || kembd[i] > nkind1)
+                return set_err(s, MSM_ERR_ARG, "KEMBD entry out of range");
+            t.kembd[i] = kembd[i] - 1;
+        }
+        rc = dalloc(s, &s->d_m4, hm4.size());
+        if (rc != MSM_OK) return rc;
+        HIPCHK(s, hipMemcpy(s->d_m4, hm4.data(), hm4.size() * sizeof(double4), hipMemcpyHostToDevice));
+        t.nembd = nembd;
+        t.nembd2 = nembd2;
+        t.rhod = rhod;
+    } else {
+        dfree(&s->d_m4);
+        t.nembd = 0;
+        t.nembd2 = 0;
+        t.rhod = 1.0;
+    }
+    t.b2 = s->d_b2;
+    t.f4 = s->d_f4;
+    t.e4 = s->d_e4;
+    t.m4 = s->d_m4;
+    // keep control-dependent fields if the control block came first
+    t.ru2max = s->tab.ru2max;
+    for (int g = 0; g < msm::kMaxGroup; ++g) t.cm[g] = s->tab.cm[g];
+    s->tab = t;
+    s->tab_ngroup = ngroup;
+    s->has_table = true;
+    clear_graphs(s);
+    return MSM_OK;
+}
+
+int msm_set_ctrl(msm_sim* s, int ngroup, const double* ru, const double* nb_rm, int mxkvois,
+                 const double* cm) {
+    if (!s) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (ngroup < 1 || ngroup > msm::kMaxGroup || !ru || !nb_rm || !cm || mxkvois < 1)
+        return set_err(s, MSM_ERR_ARG, "bad control arguments");
+    s->ngroup = ngroup;
+    s->mx = mxkvois;
+    s->ru.assign(ru, ru + ngroup * ngroup);
+    s->nb_rm.assign(nb_rm, nb_rm + ngroup * ngroup);
+    s->cm.assign(cm, cm + ngroup);
+    double ru2max = 0.0;
+    std::vector<double> rnl2(static_cast<size_t>(ngroup) * ngroup);
+    for (int i = 0; i < ngroup; ++i)
+        for (int j = 0; j < ngroup; ++j) {
+            const double r = ru[i + j * ngroup];
+            ru2max = std::max(ru2max, r * r);
+            const double q = nb_rm[i + j * ngroup];
+            rnl2[static_cast<size_t>(i) * ngroup + j] = q * q;
+        }
+    for (int g = 0; g < ngroup; ++g) {
+        if (!(cm[g] > 0.0)) return set_err(s, MSM_ERR_ARG, "masses must be positive");
+        s->tab.cm[g] = cm[g];
+    }
+    s->tab.ru2max = ru2max;
+    int rc = dalloc(s, &s->d_rnl2, rnl2.size());
+    if (rc != MSM_OK) return rc;
+    HIPCHK(s, hipMemcpy(s->d_rnl2, rnl2.data(), rnl2.size() * sizeof(double), hipMemcpyHostToDevice));
+    s->has_ctrl = true;
+    s->has_list = false;
+    clear_graphs(s);
+    return MSM_OK;
+}
+
+int msm_set_boxes(msm_sim* s, int nbox, const int* natoms, const double* lo, const double* len,
+                  const int* ifpd) {
+    if (!s) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (!s->has_ctrl) return set_err(s, MSM_ERR_STATE, "set control parameters before boxes");
+    if (nbox < 1 || !natoms || !lo || !len || !ifpd)
+        return set_err(s, MSM_ERR_ARG, "bad box arguments");
+    double rmax = 0.0;
+    for (double r : s->nb_rm) rmax = std::max(rmax, r);
+    std::vector<BoxDev> bx(nbox);
+    long long cells = 0;
+    for (int b = 0; b < nbox; ++b) {
+        BoxDev& d = bx[b];
+        std::memset(&d, 0, sizeof(d));
+        if (natoms[b] < 0) return set_err(s, MSM_ERR_ARG, "negative atom count");
+        d.natoms = natoms[b];
+        int nc = 1;
+        for (int k = 0; k < 3; ++k) {
+            d.lo[k] = lo[3 * b + k];
+            d.len[k] = len[3 * b + k];
+            if (!(d.len[k] > 0.0)) return set_err(s, MSM_ERR_ARG, "box length must be positive");
+            d.hlen[k] = d.len[k] * 0.5;
+            d.ifpd[k] = ifpd[3 * b + k] ? 1 : 0;
+            // NCELL = max(3, int(L / max(NB_RM) - eps)), MD_NeighborsList.F90:474-483
+            int c = static_cast<int>(d.len[k] / (1.0 * rmax) - 0.0001);
+            if (c < 3) c = 3;
+            d.ncell[k] = c;
+            nc *= c;
+        }
+        d.cell_off = static_cast<int>(cells);
+        cells += nc;
+        if (cells > (1LL << 30)) return set_err(s, MSM_ERR_ARG, "too many cells");
+    }
+    int rc = dalloc(s, &s->d_boxes, bx.size());
+    if (rc != MSM_OK) return rc;
+    HIPCHK(s, hipMemcpy(s->d_boxes, bx.data(), bx.size() * sizeof(BoxDev), hipMemcpyHostToDevice));
+    s->boxes = bx;
+    s->ncell_total = static_cast<int>(cells);
+    const int need = s->ncell_total + 2;
+    if (need > s->sort_cap) {
+        rc = dalloc(s, &s->sb.count, need);
+        if (rc == MSM_OK) rc = dalloc(s, &s->sb.start, need);
+        if (rc == MSM_OK) rc = dalloc(s, &s->sb.scan_tmp, need / 1024 + 2);
+        if (rc != MSM_OK) return rc;
+        s->sort_cap = need;
+    }
+    rc = dalloc(s, &s->d_ebox, nbox);
+    if (rc != MSM_OK) return rc;
+    s->has_boxes = true;
+    s->has_atoms = false;
+    s->has_list = false;
+    clear_graphs(s);
+    return MSM_OK;
+}
+
+int msm_set_atoms(msm_sim* s, int n, const double* xp, const double* xp1, const int* ityp,
+                  const int* statu) {
+    if (!s) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (!s->has_boxes) return set_err(s, MSM_ERR_STATE, "set boxes before atoms");
+    if (n < 0 || (n > 0 && (!xp || !xp1 || !ityp || !statu)))
+        return set_err(s, MSM_ERR_ARG, "bad atom arguments");
+    long long tot = 0;
+    for (const BoxDev& b : s->boxes) tot += b.natoms;
+    if (tot != n) return set_err(s, MSM_ERR_ARG, "sum of box atom counts differs from n");
+    for (int i = 0; i < n; ++i)
+        if (ityp[i] < 1 || ityp[i] > s->ngroup) return set_err(s, MSM_ERR_ARG, "ITYP out of range");
+    clear_graphs(s);
+    free_atoms(s);
+    const int na = std::max(64, (n + 63) / 64 * 64);
+    int rc = MSM_OK;
+    for (int k = 0; k < 2 && rc == MSM_OK; ++k) {
+        rc = dalloc(s, &s->pos[k], na);
+        if (rc == MSM_OK) rc = dalloc(s, &s->set[k].v, 3 * static_cast<size_t>(na));
+        if (rc == MSM_OK) rc = dalloc(s, &s->set[k].f, 3 * static_cast<size_t>(na));
+        if (rc == MSM_OK) rc = dalloc(s, &s->set[k].dis, 3 * static_cast<size_t>(na));
+        if (rc == MSM_OK) rc = dalloc(s, &s->set[k].ityp, na);
+        if (rc == MSM_OK) rc = dalloc(s, &s->set[k].statu, na);
+        if (rc == MSM_OK) rc = dalloc(s, &s->set[k].boxid, na);
+        if (rc == MSM_OK) rc = dalloc(s, &s->set[k].gid, na);
+        if (rc == MSM_OK) rc = dalloc(s, &s->set[k].cell, na);
+    }
+    if (rc == MSM_OK) rc = dalloc(s, &s->epot, na);
+    if (rc == MSM_OK) rc = dalloc(s, &s->sb.key, na);
+    if (rc == MSM_OK) rc = dalloc(s, &s->sb.slot, na);
+    if (rc == MSM_OK) rc = dalloc(s, &s->sb.perm, na);
+    if (rc == MSM_OK) rc = dalloc(s, &s->kvois, na);
+    const size_t icap = static_cast<size_t>(na) * s->mx;
+    if (rc == MSM_OK) rc = dalloc(s, &s->indi, icap);
+    if (rc != MSM_OK) {
+        free_atoms(s);
+        return rc;
+    }
+    s->indi_cap = icap;
+    s->n = n;
+    s->n_alloc = na;
+    s->scur = 0;
+    s->pcur = 0;
+    std::vector<double4> hp(na, make_double4(0, 0, 0, 0));
+    std::vector<double> hv(3 * static_cast<size_t>(na), 0.0);
+    std::vector<int> ht(na, 0), hs(na, 0), hb(na, 0), hg(na, 0);
+    for (int i = 0; i < n; ++i) {
+        hp[i] = make_double4(xp[i], xp[i + n], xp[i + 2 * static_cast<size_t>(n)], 0.0);
+        for (int k = 0; k < 3; ++k)
+            hv[i + static_cast<size_t>(k) * na] = xp1[i + static_cast<size_t>(k) * n];
+        ht[i] = ityp[i] - 1;
+        hs[i] = statu[i];
+        hg[i] = i;
+    }
+    int off = 0;
+    for (size_t b = 0; b < s->boxes.size(); ++b) {
+        for (int i = 0; i < s->boxes[b].natoms; ++i) hb[off + i] = static_cast<int>(b);
+        off += s->boxes[b].natoms;
+    }
+    AtomSet& z = s->set[0];
+    HIPCHK(s, hipMemcpy(s->pos[0], hp.data(), na * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(z.v, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::fill(hv.begin(), hv.end(), 0.0);
+    HIPCHK(s, hipMemcpy(z.f, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(z.dis, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(z.ityp, ht.data(), na * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(z.statu, hs.data(), na * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(z.boxid, hb.data(), na * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(z.gid, hg.data(), na * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<double> hz(na, 0.0);
+    HIPCHK(s, hipMemcpy(s->epot, hz.data(), na * sizeof(double), hipMemcpyHostToDevice));
+    s->has_atoms = true;
+    s->has_list = false;
+    return MSM_OK;
+}
+
+int msm_get_atoms(msm_sim* s, double* xp, double* xp1, double* fp, int* statu, double* dis) {
+    if (!s) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (!s->has_atoms) return set_err(s, MSM_ERR_STATE, "no atoms");
+    const int n = s->n;
+    const size_t na = s->n_alloc;
+    AtomSet& z = s->set[s->scur];
+    double* tmp = nullptr;
+    int* itmp = nullptr;
+    int rc = dalloc(s, &tmp, 3 * na);
+    if (rc == MSM_OK) rc = dalloc(s, &itmp, na);
+    if (rc != MSM_OK) {
+        dfree(&tmp);
+        return rc;
+    }
+    auto pull3 = [&](const double* src3, double* host) -> int {
+        for (int k = 0; k < 3; ++k)
+            msm::launch_unsort_d(src3 + k * na, tmp + k * static_cast<size_t>(n), z.gid, n, s->stream);
+        HIPCHK(s, hipMemcpyAsync(host, tmp, 3 * static_cast<size_t>(n) * sizeof(double),
+                                 hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        return MSM_OK;
+    };
+    if (xp && n > 0) {
+        for (int k = 0; k < 3; ++k)
+            msm::launch_unsort_pos(s->pos[s->pcur], tmp + k * static_cast<size_t>(n), z.gid, n, k, s->stream);
+        HIPCHK(s, hipMemcpyAsync(xp, tmp, 3 * static_cast<size_t>(n) * sizeof(double),
+                                 hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+    }
+    if (rc == MSM_OK && xp1 && n > 0) rc = pull3(z.v, xp1);
+    if (rc == MSM_OK && fp && n > 0) rc = pull3(z.f, fp);
+    if (rc == MSM_OK && dis && n > 0) rc = pull3(z.dis, dis);
+    if (rc == MSM_OK && statu && n > 0) {
+        msm::launch_unsort_i(z.statu, itmp, z.gid, n, s->stream);
+        hipError_t e = hipMemcpyAsync(statu, itmp, n * sizeof(int), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) rc = set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    }
+    dfree(&tmp);
+    dfree(&itmp);
+    return rc;
+}
+
+static int pull_sorted_d(msm_sim* s, const double* src, int comp_pos, double* host) {
+    const int n = s->n;
+    if (n == 0) return MSM_OK;
+    double* tmp = nullptr;
+    int rc = dalloc(s, &tmp, n);
+    if (rc != MSM_OK) return rc;
+    AtomSet& z = s->set[s->scur];
+    if (comp_pos >= 0)
+        msm::launch_unsort_pos(s->pos[s->pcur], tmp, z.gid, n, comp_pos, s->stream);
+    else
+        msm::launch_unsort_d(src, tmp, z.gid, n, s->stream);
+    hipError_t e = hipMemcpyAsync(host, tmp, n * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    dfree(&tmp);
+    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    return MSM_OK;
+}
+
+int msm_get_den(msm_sim* s, double* den) {
+    if (!s || !den) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (!s->has_atoms) return set_err(s, MSM_ERR_STATE, "no atoms");
+    return pull_sorted_d(s, nullptr, 3, den);
+}
+
+int msm_get_epot(msm_sim* s, double* epot) {
+    if (!s || !epot) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (!s->has_atoms) return set_err(s, MSM_ERR_STATE, "no atoms");
+    return pull_sorted_d(s, s->epot, -1, epot);
+}
+
+int msm_get_box_epot(msm_sim* s, double* ebox) {
+    if (!s || !ebox) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (!s->has_list) return set_err(s, MSM_ERR_STATE, "no neighbour list");
+    msm::launch_box_sum(s->epot, s->sb.start, s->d_boxes, static_cast<int>(s->boxes.size()),
+                        s->d_ebox, s->stream);
+    HIPCHK(s, hipMemcpyAsync(ebox, s->d_ebox, s->boxes.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MSM_OK;
+}
+
+int msm_build_neighbors(msm_sim* s) {
+    int rc = check_ready(s, false);
+    if (rc != MSM_OK) return rc;
+    (void)hipSetDevice(s->device);
+    clear_flags(s);
+    enqueue_rebuild(s);
+    s->has_list = true;
+    return read_flags(s);
+}
+
+int msm_get_neighbors(msm_sim* s, int* kvois, int* indi, int ld) {
+    if (!s || !kvois || !indi) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (!s->has_list) return set_err(s, MSM_ERR_STATE, "no neighbour list");
+    const int n = s->n;
+    if (ld < n) return set_err(s, MSM_ERR_ARG, "ld smaller than the atom count");
+    if (n == 0) return MSM_OK;
+    int *kv = nullptr, *ix = nullptr;
+    int rc = dalloc(s, &kv, n);
+    if (rc == MSM_OK) rc = dalloc(s, &ix, static_cast<size_t>(ld) * s->mx);
+    if (rc != MSM_OK) {
+        dfree(&kv);
+        return rc;
+    }
+    msm::launch_unsort_list(s->kvois, s->indi, s->n_alloc, s->set[s->scur].gid, n, s->mx, kv, ix,
+                            ld, s->stream);
+    hipError_t e = hipMemcpyAsync(kvois, kv, n * sizeof(int), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(indi, ix, static_cast<size_t>(ld) * s->mx * sizeof(int),
+                           hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    dfree(&kv);
+    dfree(&ix);
+    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    return MSM_OK;
+}
+
+int msm_calc_force(msm_sim* s) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    (void)hipSetDevice(s->device);
+    clear_flags(s);
+    enqueue_density(s);
+    enqueue_force(s, 0, 1, 0.0);
+    return read_flags(s);
+}
+
+int msm_calc_epot(msm_sim* s) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    (void)hipSetDevice(s->device);
+    clear_flags(s);
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_epot(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), s->d_err,
+                     s->stream);
+    return read_flags(s);
+}
+
+int msm_predictor(msm_sim* s, double h) {
+    int rc = check_ready(s, false);
+    if (rc != MSM_OK) return rc;
+    (void)hipSetDevice(s->device);
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_predictor(a, s->tab, s->boxes[0], s->d_boxes, multibox(s), h, s->stream);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MSM_OK;
+}
+
+int msm_correction(msm_sim* s, double h) {
+    int rc = check_ready(s, false);
+    if (rc != MSM_OK) return rc;
+    (void)hipSetDevice(s->device);
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_correction(a, s->tab, h, s->stream);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MSM_OK;
+}
+
+int msm_run(msm_sim* s, int nsteps, int nb_uptab, int step0, double h) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (nsteps < 0) return set_err(s, MSM_ERR_ARG, "nsteps must be >= 0");
+    if (nsteps == 0) return MSM_OK;
+    (void)hipSetDevice(s->device);
+    clear_flags(s);
+    // predictor of the first step (stand-alone kernel; later predictors run fused)
+    {
+        TimedScope ts(s, MSM_KCLASS_PREDICTOR);
+        AtomsDev a = atoms_view(s, s->scur, s->pcur);
+        msm::launch_predictor(a, s->tab, s->boxes[0], s->d_boxes, multibox(s), h, s->stream);
+    }
+    const int first = step0 + 1, last = step0 + nsteps;
+    int sstep = first;
+    while (sstep <= last) {
+        const int rebuild = (nb_uptab > 0 && sstep % nb_uptab == 0) ? 1 : 0;
+        // the segment runs until the step before the next rebuild or the end of the run
+        int end = last;
+        if (nb_uptab > 0) {
+            int next = (sstep / nb_uptab + 1) * nb_uptab;
+            end = std::min(last, next - 1);
+        }
+        const int cnt = end - sstep + 1;
+        rc = run_segment(s, rebuild, cnt, end == last ? 1 : 0, h);
+        if (rc != MSM_OK) return rc;
+        sstep = end + 1;
+    }
+    return read_flags(s);
+}
+
+int msm_sync(msm_sim* s) {
+    if (!s) return MSM_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MSM_OK;
+}
+
+int msm_set_option(msm_sim* s, int option, int value) {
+    if (!s) return MSM_ERR_ARG;
+    switch (option) {
+        case MSM_OPT_TRACK_DIS:
+            s->track_dis = value ? 1 : 0;
+            return MSM_OK;
+        case MSM_OPT_KERNEL_TIMING:
+            s->timing = value ? 1 : 0;
+            return MSM_OK;
+        default:
+            return set_err(s, MSM_ERR_ARG, "unknown option");
+    }
+}
+
+int msm_kernel_times(msm_sim* s, double* ms, long long* count, int nclass, int reset) {
+    if (!s || nclass < 0) return MSM_ERR_ARG;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    resolve_timing(s);
+    for (int c = 0; c < nclass && c < MSM_KCLASS_COUNT; ++c) {
+        if (ms) ms[c] = s->kms[c];
+        if (count) count[c] = s->kcnt[c];
+    }
+    if (reset)
+        for (int c = 0; c < MSM_KCLASS_COUNT; ++c) {
+            s->kms[c] = 0.0;
+            s->kcnt[c] = 0;
+        }
+    return MSM_OK;
+}
+
+int msm_list_stats(msm_sim* s, long long* total, int* maxk) {
+    if (!s || !total || !maxk) return MSM_ERR_ARG;
+    if (!s->has_list) return set_err(s, MSM_ERR_STATE, "no neighbour list");
+    (void)hipSetDevice(s->device);
+    msm::launch_fill_int(reinterpret_cast<int*>(s->d_stats), 0, 4, s->stream);
+    msm::launch_list_stats(s->kvois, s->n, s->d_stats,
+                           reinterpret_cast<int*>(s->d_stats + 1), s->stream);
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(s, hipMemcpyAsync(h, s->d_stats, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    *total = static_cast<long long>(h[0]);
+    *maxk = static_cast<int>(h[1] & 0xffffffffu);
+    return MSM_OK;
+}
+
+}  // extern "C"

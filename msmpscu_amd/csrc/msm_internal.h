@@ -1,0 +1,116 @@
+// msm_internal.h -- device data layout and kernel launchers shared by the HIP translation units
+// of libmsm_hip.so.
+//
+// HBM layout (all per-atom arrays in cell-sorted order, rebuilt every neighbour update):
+//   pos[2][n]   double4 {x, y, z, w}; w holds F'(rho_i) after the density pass, so the force pass
+//               gathers a neighbour's position and F' with one 32-byte load.  Two buffers: the
+//               fused force+integrator kernel reads one and writes the next positions to the other.
+//   v, f, dis   three SoA double arrays each (x|y|z, stride n_alloc), owned by one thread per atom.
+//   ityp, statu, boxid, gid, cell   int32 per atom (gid = caller's index of the sorted atom).
+//   kvois[n], indi[mx][n]           full neighbour list, slot-major (coalesced per slot).
+//   tables      kind-major knots 0..ntab+1 holding {value, next-minus-value} pairs, so one
+//               lerp v(KK) + DK*(v(KK+1)-v(KK)) is one 16-byte load and one multiply-add.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace msm {
+
+constexpr int kMaxGroup = 16;
+constexpr int kBlock = 256;
+
+enum ErrSlot { kErrOverflow = 0, kErrOutOfBox = 1, kErrEmbd = 2, kErrSlots = 4 };
+
+struct BoxDev {
+    double lo[3], len[3], hlen[3];
+    int ifpd[3];
+    int ncell[3];
+    int cell_off;  // first global cell id of this box
+    int natoms;    // atoms assigned to this box
+    int pad;
+};
+
+struct TabDev {
+    const double2* b2;  // density table {POTB, dPOTB}, nkind x (ntab+2)
+    const double4* f4;  // force table {FPOTR, dFPOTR, FPOTB, dFPOTB}
+    const double4* e4;  // energy table {POTR, dPOTR, POTB, dPOTB}
+    const double4* m4;  // embedding {FEMBD, dFEMBD, DFEMBD, dDFEMBD}, nkind1 x (nembd+2)
+    int ntab2, nembd2, nembd, ngroup;
+    double csi, rhod, ru2max;
+    int pot_type;
+    int kpair[kMaxGroup * kMaxGroup];  // 0-based kinds, [ti*ngroup + tj]
+    int kembd[kMaxGroup];
+    double cm[kMaxGroup];
+};
+
+struct AtomsDev {
+    double4* pos;      // positions read by this launch (w = F')
+    double4* pos_out;  // next positions (fused integrator)
+    double *vx, *vy, *vz;
+    double *fx, *fy, *fz;
+    double *dx, *dy, *dz;
+    int* ityp;
+    int* statu;
+    int* boxid;
+    double* epot;
+    int n;
+};
+
+struct ListDev {
+    const int* kvois;
+    const int* indi;
+    int stride;
+};
+
+struct StepParams {
+    double h, hs2, h2s2;
+    int mode;      // 0: force only, 1: force + corrector, 2: force + corrector + next predictor
+    int write_f;   // store forces
+    int track_dis; // accumulate DIS in the fused predictor
+};
+
+struct SortBufs {
+    int* key;    // cell id per atom (current order)
+    int* slot;   // arrival slot within the cell
+    int* count;  // ncell_total + 2
+    int* start;  // ncell_total + 2 (exclusive scan of count)
+    int* perm;   // sorted position -> current index
+    int* scan_tmp;
+};
+
+// ---- launchers (msm_kernels.hip) ----
+void launch_fill_int(int* p, int v, int n, hipStream_t s);
+void launch_cell_assign(const double4* pos, const int* statu, const int* boxid,
+                        const BoxDev* boxes, int n, int ncell_total, SortBufs sb, int* err,
+                        hipStream_t s);
+void launch_scan_exclusive(const int* in, int* out, int n, int* tmp, hipStream_t s);
+void launch_scatter_sort(int n, int ncell_total, SortBufs sb, const int* gid, hipStream_t s);
+void launch_gather(int n, int n_alloc, const int* perm, const AtomsDev& src, const int* gid_src,
+                   AtomsDev& dst, int* gid_dst, const int* key, int* cell_dst, hipStream_t s);
+void launch_nlist(int n, const double4* pos, const int* ityp, const int* statu, const int* boxid,
+                  const int* cell, const int* start, const BoxDev* boxes, int ngroup,
+                  const double* rnl2, int mx, int* kvois, int* indi, int stride, int* err,
+                  hipStream_t s);
+void launch_density(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                    const BoxDev* boxes, int multibox, int* err, hipStream_t s);
+void launch_force(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                  const BoxDev* boxes, int multibox, const StepParams& sp, hipStream_t s);
+void launch_epot(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                 const BoxDev* boxes, int multibox, int* err, hipStream_t s);
+void launch_predictor(const AtomsDev& a, const TabDev& t, const BoxDev& box0,
+                      const BoxDev* boxes, int multibox, double h, hipStream_t s);
+void launch_correction(const AtomsDev& a, const TabDev& t, double h, hipStream_t s);
+void launch_unsort_d(const double* src, double* dst, const int* gid, int n, hipStream_t s);
+void launch_unsort_i(const int* src, int* dst, const int* gid, int n, hipStream_t s);
+void launch_unsort_pos(const double4* src, double* dst, const int* gid, int n, int comp,
+                       hipStream_t s);
+void launch_unsort_list(const int* kvois, const int* indi, int stride, const int* gid, int n,
+                        int mx, int* kv_out, int* indi_out, int ld, hipStream_t s);
+void launch_box_sum(const double* epot, const int* start, const BoxDev* boxes, int nbox,
+                    double* out, hipStream_t s);
+void launch_list_stats(const int* kvois, int n, unsigned long long* total, int* maxk,
+                       hipStream_t s);
+
+}  // namespace msm

@@ -1,0 +1,790 @@
+// msm_kernels.hip -- gfx950 kernels of the tabulated FS/EAM force path (FP64).
+//
+// One thread per atom over cell-sorted atoms; workgroups of 256 (four wave64s).  Workgroup ids
+// are remapped so that each of the eight XCDs (which receive workgroups round-robin) walks one
+// contiguous eighth of the sorted atoms: the neighbour gathers of an XCD then stay within one
+// spatial slab and hit that XCD's own L2.
+//
+// Reference kernels restated (semantics, not code):
+//   cell ids         MDLIB/sor/CommonGPU/MD_NeighborsList_GPU.F90:759-819 with the host formula
+//                    of MDLIB/sor/Common/MD_NeighborsList.F90:255-276 (FP64, eps = 1e-4)
+//   list             MD_NeighborsList.F90:396-633 (27-cell stencil :427-429, strict <, shifted
+//                    images x_j + L) -- the host predicate, exactly, rather than the FP32 device one
+//   density          MD_FS_ForceTable_GPU.F90:341-516, MD_EAM_ForceTable_GPU.F90:369-548
+//   force            MD_FS_ForceTable_GPU.F90:623-806, MD_EAM_ForceTable_GPU.F90:646-829
+//   energy           MD_FS_ForceTable_GPU.F90:1451-1609, MD_EAM_ForceTable_GPU.F90:1470-1636
+//   integrator       MD_DiffScheme_GPU.F90:242-384 (Predictor_KERNEL0), :686-758 (Correction_KERNEL)
+#include "msm_internal.h"
+
+namespace msm {
+
+namespace {
+
+constexpr int STATU_ACTIVE = 1;
+constexpr int STATU_FIXPOSX = 2, STATU_FIXPOSY = 4, STATU_FIXPOSZ = 8;
+constexpr int STATU_FIXVELX = 16, STATU_FIXVELY = 32, STATU_FIXVELZ = 64;
+constexpr int STATU_OUTOFBOX = 65536, STATU_REFLECT = 131072, STATU_TRANSMIT = 262144,
+              STATU_PASSBOUND = 524288;
+
+// Workgroup id -> logical block so that XCD x (receiving ids x, x+8, ...) gets the contiguous
+// logical range [x*q + min(x,r), ...) of a bijection over [0, nblk).
+__device__ __forceinline__ int xcd_block(int nblk) {
+    const int b = blockIdx.x;
+    const int q = nblk >> 3, r = nblk & 7;
+    const int x = b & 7, k = b >> 3;
+    const int base = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+    return base + k;
+}
+
+__device__ __forceinline__ int atom_index() {
+    return xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
+}
+
+inline unsigned grid_for(int n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+// Minimum image as the device kernels write it: if periodic and |s| > L/2 then s - sign(L, s).
+__device__ __forceinline__ double min_image(double s, double len, double hlen, int pd) {
+    return (pd && fabs(s) > hlen) ? s - copysign(len, s) : s;
+}
+
+template <int MB>
+__device__ __forceinline__ const BoxDev& box_of(const BoxDev& box0, const BoxDev* boxes,
+                                                const int* boxid, int i) {
+    if constexpr (MB) return boxes[boxid[i]];
+    return box0;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------------------------
+// utilities
+// --------------------------------------------------------------------------------------------
+
+__global__ void k_fill_int(int* p, int v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_int(int* p, int v, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(n)), dim3(kBlock), 0, s, p, v, n);
+}
+
+// --------------------------------------------------------------------------------------------
+// neighbour-list build: cell ids, counting sort, deterministic in-cell order, gather, list
+// --------------------------------------------------------------------------------------------
+
+__global__ void k_cell_assign(const double4* __restrict__ pos, const int* __restrict__ statu,
+                              const int* __restrict__ boxid, const BoxDev* __restrict__ boxes,
+                              int n, int ncell_total, int* __restrict__ key,
+                              int* __restrict__ slot, int* __restrict__ count, int* err) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = ncell_total;  // sentinel bucket: atoms outside every cell grid
+    if ((statu[i] & STATU_OUTOFBOX) != STATU_OUTOFBOX) {
+        const BoxDev& b = boxes[boxid[i]];
+        const double4 p = pos[i];
+        const double xyz[3] = {p.x, p.y, p.z};
+        int ic[3];
+        bool in = true;
+        for (int k = 0; k < 3; ++k) {
+            // (x - LB)/L * NC - eps, truncated toward zero, as MD_NeighborsList.F90:258
+            const double s = (xyz[k] - b.lo[k]) / b.len[k] * static_cast<double>(b.ncell[k]) - 1.0e-4;
+            ic[k] = static_cast<int>(s);
+            in = in && ic[k] >= 0 && ic[k] < b.ncell[k];
+        }
+        if (in)
+            c = b.cell_off + ic[0] + b.ncell[0] * (ic[1] + b.ncell[1] * ic[2]);
+        else
+            atomicOr(&err[kErrOutOfBox], 1);
+    }
+    key[i] = c;
+    slot[i] = atomicAdd(&count[c], 1);
+}
+
+void launch_cell_assign(const double4* pos, const int* statu, const int* boxid,
+                        const BoxDev* boxes, int n, int ncell_total, SortBufs sb, int* err,
+                        hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cell_assign, dim3(grid_for(n)), dim3(kBlock), 0, s, pos, statu, boxid,
+                       boxes, n, ncell_total, sb.key, sb.slot, sb.count, err);
+}
+
+// Exclusive scan of int: block-level inclusive scans (wave64 shuffles + LDS), one-block scan of
+// the block totals, then a fix-up pass.
+constexpr int kScanBlock = 1024;
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int block_incl_scan(int v, int* lds) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    v = wave_incl_scan(v);
+    if (lane == 63) lds[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+        int w = (lane < (int)(blockDim.x >> 6)) ? lds[lane] : 0;
+        w = wave_incl_scan(w);
+        if (lane < (int)(blockDim.x >> 6)) lds[lane] = w;
+    }
+    __syncthreads();
+    if (wid > 0) v += lds[wid - 1];
+    __syncthreads();
+    return v;
+}
+
+__global__ void k_scan_blocks(const int* __restrict__ in, int* __restrict__ out,
+                              int* __restrict__ sums, int n) {
+    __shared__ int lds[kScanBlock / 64];
+    const int i = blockIdx.x * kScanBlock + threadIdx.x;
+    const int v = (i < n) ? in[i] : 0;
+    const int inc = block_incl_scan(v, lds);
+    if (i < n) out[i] = inc - v;  // exclusive within the block
+    if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = inc;
+}
+
+__global__ void k_scan_sums(int* sums, int nb) {
+    __shared__ int lds[kScanBlock / 64];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += kScanBlock) {
+        const int i = base + threadIdx.x;
+        const int v = (i < nb) ? sums[i] : 0;
+        const int inc = block_incl_scan(v, lds);
+        const int c = carry;
+        if (i < nb) sums[i] = c + inc - v;
+        __syncthreads();
+        if (threadIdx.x == kScanBlock - 1) carry = c + inc;
+        __syncthreads();
+    }
+}
+
+__global__ void k_scan_add(int* out, const int* __restrict__ sums, int n) {
+    const int i = blockIdx.x * kScanBlock + threadIdx.x;
+    if (i < n) out[i] += sums[blockIdx.x];
+}
+
+void launch_scan_exclusive(const int* in, int* out, int n, int* tmp, hipStream_t s) {
+    if (n <= 0) return;
+    const int nb = (n + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(kScanBlock), 0, s, in, out, tmp, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, nb);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanBlock), 0, s, out, tmp, n);
+}
+
+__global__ void k_scatter(int n, const int* __restrict__ key, const int* __restrict__ slot,
+                          const int* __restrict__ start, int* __restrict__ perm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[start[key[i]] + slot[i]] = i;
+}
+
+// One thread per cell: order the cell's atoms by caller index (insertion sort), which makes the
+// sorted order, the list order and every summation order independent of atomic arrival order.
+__global__ void k_cell_order(int ncells, const int* __restrict__ start, int* __restrict__ perm,
+                             const int* __restrict__ gid) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    const int b = start[c], e = start[c + 1];
+    for (int a = b + 1; a < e; ++a) {
+        const int v = perm[a];
+        const int g = gid[v];
+        int q = a - 1;
+        while (q >= b && gid[perm[q]] > g) {
+            perm[q + 1] = perm[q];
+            --q;
+        }
+        perm[q + 1] = v;
+    }
+}
+
+void launch_scatter_sort(int n, int ncell_total, SortBufs sb, const int* gid, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, n, sb.key, sb.slot,
+                       sb.start, sb.perm);
+    const int nc = ncell_total + 1;
+    hipLaunchKernelGGL(k_cell_order, dim3(grid_for(nc)), dim3(kBlock), 0, s, nc, sb.start,
+                       sb.perm, gid);
+}
+
+__global__ void k_gather(int n, const int* __restrict__ perm, AtomsDev src,
+                         const int* __restrict__ gid_src, AtomsDev dst, int* __restrict__ gid_dst,
+                         const int* __restrict__ key, int* __restrict__ cell_dst) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = perm[p];
+    double4 x = src.pos[i];
+    x.w = 0.0;
+    dst.pos[p] = x;
+    dst.vx[p] = src.vx[i];
+    dst.vy[p] = src.vy[i];
+    dst.vz[p] = src.vz[i];
+    dst.fx[p] = src.fx[i];
+    dst.fy[p] = src.fy[i];
+    dst.fz[p] = src.fz[i];
+    dst.dx[p] = src.dx[i];
+    dst.dy[p] = src.dy[i];
+    dst.dz[p] = src.dz[i];
+    dst.ityp[p] = src.ityp[i];
+    dst.statu[p] = src.statu[i];
+    dst.boxid[p] = src.boxid[i];
+    gid_dst[p] = gid_src[i];
+    cell_dst[p] = key[i];
+}
+
+void launch_gather(int n, int n_alloc, const int* perm, const AtomsDev& src, const int* gid_src,
+                   AtomsDev& dst, int* gid_dst, const int* key, int* cell_dst, hipStream_t s) {
+    (void)n_alloc;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, n, perm, src, gid_src, dst,
+                       gid_dst, key, cell_dst);
+}
+
+// Full-stencil order of MD_NeighborsList.F90:427-429 (entry 0 = own cell).
+__constant__ int c_nix[27] = {0, -1, -1, -1, 0, 0, -1, 1, -1, 0, 1, -1, 0, 1,
+                              1, 1, 1, 0, 0, -1, 1, -1, 0, 1, -1, 0, 1};
+__constant__ int c_niy[27] = {0, 0, -1, 1, 1, 0, 0, 0, -1, -1, -1, 1, 1, 1,
+                              0, 1, -1, -1, 0, 0, 0, -1, -1, -1, 1, 1, 1};
+__constant__ int c_niz[27] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                              0, 0, 0, 0, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+
+__global__ __launch_bounds__(kBlock) void k_nlist(
+    int n, const double4* __restrict__ pos, const int* __restrict__ ityp,
+    const int* __restrict__ statu, const int* __restrict__ boxid, const int* __restrict__ cell,
+    const int* __restrict__ start, const BoxDev* __restrict__ boxes, int ngroup,
+    const double* __restrict__ rnl2, int mx, int* __restrict__ kvois, int* __restrict__ indi,
+    int stride, int* err) {
+#pragma clang fp contract(off)
+    const int p = atom_index();
+    if (p >= n) return;
+    if ((statu[p] & STATU_OUTOFBOX) == STATU_OUTOFBOX) {
+        kvois[p] = 0;
+        return;
+    }
+    const BoxDev& b = boxes[boxid[p]];
+    const int lc = cell[p] - b.cell_off;
+    const int ix = lc % b.ncell[0];
+    const int iy = (lc / b.ncell[0]) % b.ncell[1];
+    const int iz = lc / (b.ncell[0] * b.ncell[1]);
+    const double4 xi = pos[p];
+    const int ti = ityp[p];
+    int cnt = 0;
+    for (int s = 0; s < 27; ++s) {
+        int j3[3] = {ix + c_nix[s], iy + c_niy[s], iz + c_niz[s]};
+        double cx[3] = {0.0, 0.0, 0.0};
+        bool skip = false;
+        for (int k = 0; k < 3; ++k) {
+            if (b.ifpd[k]) {
+                if (j3[k] >= b.ncell[k]) {
+                    j3[k] = 0;
+                    cx[k] = b.len[k];
+                } else if (j3[k] < 0) {
+                    j3[k] = b.ncell[k] - 1;
+                    cx[k] = -b.len[k];
+                }
+            }
+            skip = skip || j3[k] < 0 || j3[k] >= b.ncell[k];
+        }
+        if (skip) continue;
+        const int g = b.cell_off + j3[0] + b.ncell[0] * (j3[1] + b.ncell[1] * j3[2]);
+        const int qe = start[g + 1];
+        for (int q = start[g]; q < qe; ++q) {
+            if (q == p) continue;
+            const double4 xj = pos[q];
+            // XPT(I) - (XP(J) + CXYZ), squared and summed left to right, no contraction
+            const double dx = xi.x - (xj.x + cx[0]);
+            const double dy = xi.y - (xj.y + cx[1]);
+            const double dz = xi.z - (xj.z + cx[2]);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < rnl2[ti * ngroup + ityp[q]]) {
+                if (cnt < mx) indi[p + static_cast<size_t>(cnt) * stride] = q;
+                ++cnt;
+            }
+        }
+    }
+    if (cnt > mx) {
+        atomicOr(&err[kErrOverflow], 1);
+        cnt = mx;
+    }
+    kvois[p] = cnt;
+}
+
+void launch_nlist(int n, const double4* pos, const int* ityp, const int* statu, const int* boxid,
+                  const int* cell, const int* start, const BoxDev* boxes, int ngroup,
+                  const double* rnl2, int mx, int* kvois, int* indi, int stride, int* err,
+                  hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_nlist, dim3(grid_for(n)), dim3(kBlock), 0, s, n, pos, ityp, statu, boxid,
+                       cell, start, boxes, ngroup, rnl2, mx, kvois, indi, stride, err);
+}
+
+// --------------------------------------------------------------------------------------------
+// density pass: rho_i = sum_j POTB(r_ij); F'(rho_i) stored in pos[i].w
+// --------------------------------------------------------------------------------------------
+
+template <int MB, int MT, int EAM>
+__global__ __launch_bounds__(kBlock) void k_density(AtomsDev a, ListDev l, TabDev t, BoxDev box0,
+                                                    const BoxDev* __restrict__ boxes, int* err) {
+    const int i = atom_index();
+    if (i >= a.n) return;
+    double den0 = 0.0;
+    if ((a.statu[i] & STATU_ACTIVE) == STATU_ACTIVE) {
+        const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
+        const double2 pxy = *reinterpret_cast<const double2*>(&a.pos[i]);
+        const double pz = a.pos[i].z;
+        const int ti = MT ? a.ityp[i] : 0;
+        const int kv = l.kvois[i];
+        const int* __restrict__ li = l.indi + i;
+        for (int iw = 0; iw < kv; ++iw) {
+            const int j = li[static_cast<size_t>(iw) * l.stride];
+            const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
+            const double qz = a.pos[j].z;
+            const double sx = min_image(pxy.x - qxy.x, b.len[0], b.hlen[0], b.ifpd[0]);
+            const double sy = min_image(pxy.y - qxy.y, b.len[1], b.hlen[1], b.ifpd[1]);
+            const double sz = min_image(pz - qz, b.len[2], b.hlen[2], b.ifpd[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            if (r2 <= t.ru2max) {
+                const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
+                const double r = sqrt(r2);
+                const double sk = sqrt(r) * t.csi;
+                const int kk = static_cast<int>(sk);
+                const double2 e = t.b2[kind * t.ntab2 + kk];
+                den0 = den0 + (e.x + (sk - kk) * e.y);
+            }
+        }
+        if constexpr (!EAM) {
+            if (den0 > 0.0) den0 = -0.5 / sqrt(den0);
+        } else {
+            if (den0 > 0.0) {
+                const double sk = den0 / t.rhod + 1.0;
+                int kk = static_cast<int>(sk + 0.000001);
+                if (kk > t.nembd) {
+                    atomicOr(&err[kErrEmbd], 1);
+                    kk = t.nembd;
+                }
+                const double4 m = t.m4[t.kembd[ti] * t.nembd2 + kk];
+                den0 = m.z + (sk - kk) * m.w;
+            }
+        }
+    }
+    a.pos[i].w = den0;
+}
+
+void launch_density(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                    const BoxDev* boxes, int multibox, int* err, hipStream_t s) {
+    if (a.n <= 0) return;
+    const dim3 g(grid_for(a.n)), b(kBlock);
+    const bool mt = t.ngroup > 1, eam = t.pot_type != 0;
+#define MSM_DEN(MB, MT, E) \
+    hipLaunchKernelGGL((k_density<MB, MT, E>), g, b, 0, s, a, l, t, box0, boxes, err)
+    if (multibox) {
+        if (mt) { if (eam) MSM_DEN(1, 1, 1); else MSM_DEN(1, 1, 0); }
+        else    { if (eam) MSM_DEN(1, 0, 1); else MSM_DEN(1, 0, 0); }
+    } else {
+        if (mt) { if (eam) MSM_DEN(0, 1, 1); else MSM_DEN(0, 1, 0); }
+        else    { if (eam) MSM_DEN(0, 0, 1); else MSM_DEN(0, 0, 0); }
+    }
+#undef MSM_DEN
+}
+
+// --------------------------------------------------------------------------------------------
+// force pass (+ optional fused corrector of this step and predictor of the next step)
+// --------------------------------------------------------------------------------------------
+
+template <int MB, int MT, int MODE>
+__global__ __launch_bounds__(kBlock) void k_force(AtomsDev a, ListDev l, TabDev t, BoxDev box0,
+                                                  const BoxDev* __restrict__ boxes, StepParams sp) {
+    const int i = atom_index();
+    if (i >= a.n) return;
+    const int st = a.statu[i];
+    const bool active = (st & STATU_ACTIVE) == STATU_ACTIVE;
+    const double4 pi = a.pos[i];
+    const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
+    double fpx = 0.0, fpy = 0.0, fpz = 0.0;
+    const int ti = MT ? a.ityp[i] : 0;
+    if (active) {
+        const double denki = pi.w;
+        const int kv = l.kvois[i];
+        const int* __restrict__ li = l.indi + i;
+        for (int iw = 0; iw < kv; ++iw) {
+            const int j = li[static_cast<size_t>(iw) * l.stride];
+            const double4 pj = a.pos[j];
+            const double sx = min_image(pi.x - pj.x, b.len[0], b.hlen[0], b.ifpd[0]);
+            const double sy = min_image(pi.y - pj.y, b.len[1], b.hlen[1], b.ifpd[1]);
+            const double sz = min_image(pi.z - pj.z, b.len[2], b.hlen[2], b.ifpd[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            if (r2 <= t.ru2max) {
+                int kind, kind1;
+                if constexpr (MT) {
+                    const int tj = a.ityp[j];
+                    kind = t.kpair[ti * t.ngroup + tj];
+                    kind1 = t.kpair[tj * t.ngroup + ti];
+                } else {
+                    kind = kind1 = t.kpair[0];
+                }
+                const double r = sqrt(r2);
+                const double sk = sqrt(r) * t.csi;
+                const int kk = static_cast<int>(sk);
+                const double dk = sk - kk;
+                const double4 f = t.f4[kind * t.ntab2 + kk];
+                double fb1;
+                if constexpr (MT) {
+                    const double4 f1 = t.f4[kind1 * t.ntab2 + kk];
+                    fb1 = f1.z + dk * f1.w;
+                } else {
+                    fb1 = f.z + dk * f.w;
+                }
+                const double fortot =
+                    (f.x + dk * f.y) / r2 + ((f.z + dk * f.w) * denki + fb1 * pj.w) / r;
+                fpx = fpx + fortot * sx;
+                fpy = fpy + fortot * sy;
+                fpz = fpz + fortot * sz;
+            }
+        }
+    }
+    if (sp.write_f) {
+        a.fx[i] = fpx;
+        a.fy[i] = fpy;
+        a.fz[i] = fpz;
+    }
+    if constexpr (MODE >= 1) {
+        if (active) {
+#pragma clang fp contract(off)
+            const double cm0 = t.cm[MT ? ti : 0];
+            const double ax = fpx / cm0, ay = fpy / cm0, az = fpz / cm0;
+            double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+            // Correction_KERNEL
+            if (!(st & (STATU_FIXVELX | STATU_FIXPOSX))) vx = vx + sp.hs2 * ax;
+            if (!(st & (STATU_FIXVELY | STATU_FIXPOSY))) vy = vy + sp.hs2 * ay;
+            if (!(st & (STATU_FIXVELZ | STATU_FIXPOSZ))) vz = vz + sp.hs2 * az;
+            if constexpr (MODE == 2) {
+                // Predictor_KERNEL0 of the next step with the same forces
+                int stat = st;
+                double x = pi.x, y = pi.y, z = pi.z;
+                double dx = sp.h * vx + sp.h2s2 * ax;
+                if (stat & STATU_FIXPOSX) dx = 0.0;
+                x = x + dx;
+                if (b.ifpd[0]) {
+                    if (x > b.lo[0] + b.len[0]) { x = x - b.len[0]; stat |= STATU_PASSBOUND; }
+                    else if (x < b.lo[0])       { x = x + b.len[0]; stat |= STATU_PASSBOUND; }
+                }
+                double dy = sp.h * vy + sp.h2s2 * ay;
+                if (stat & STATU_FIXPOSY) dy = 0.0;
+                y = y + dy;
+                if (b.ifpd[1]) {
+                    if (y > b.lo[1] + b.len[1]) { y = y - b.len[1]; stat |= STATU_PASSBOUND; }
+                    else if (y < b.lo[1])       { y = y + b.len[1]; stat |= STATU_PASSBOUND; }
+                }
+                double dz = sp.h * vz + sp.h2s2 * az;
+                if (stat & STATU_FIXPOSZ) dz = 0.0;
+                z = z + dz;
+                if (b.ifpd[2]) {
+                    if (z > b.lo[2] + b.len[2]) { z = z - b.len[2]; stat |= STATU_PASSBOUND; }
+                    else if (z < b.lo[2])       { z = z + b.len[2]; stat |= STATU_PASSBOUND; }
+                }
+                if (!(stat & (STATU_FIXVELX | STATU_FIXPOSX))) vx = vx + sp.hs2 * ax;
+                if (!(stat & (STATU_FIXVELY | STATU_FIXPOSY))) vy = vy + sp.hs2 * ay;
+                if (!(stat & (STATU_FIXVELZ | STATU_FIXPOSZ))) vz = vz + sp.hs2 * az;
+                if (sp.track_dis) {
+                    a.dx[i] = a.dx[i] + dx;
+                    a.dy[i] = a.dy[i] + dy;
+                    a.dz[i] = a.dz[i] + dz;
+                }
+                if (x > b.lo[0] + b.len[0] || y > b.lo[1] + b.len[1] || z > b.lo[2] + b.len[2])
+                    stat = STATU_OUTOFBOX | STATU_REFLECT;
+                else if (x < b.lo[0] || y < b.lo[1] || z < b.lo[2])
+                    stat = STATU_OUTOFBOX | STATU_TRANSMIT;
+                if (stat != st) a.statu[i] = stat;
+                a.pos_out[i] = make_double4(x, y, z, 0.0);
+            }
+            a.vx[i] = vx;
+            a.vy[i] = vy;
+            a.vz[i] = vz;
+        } else if constexpr (MODE == 2) {
+            a.pos_out[i] = make_double4(pi.x, pi.y, pi.z, 0.0);
+        }
+    }
+}
+
+void launch_force(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                  const BoxDev* boxes, int multibox, const StepParams& sp, hipStream_t s) {
+    if (a.n <= 0) return;
+    const dim3 g(grid_for(a.n)), b(kBlock);
+    const bool mt = t.ngroup > 1;
+#define MSM_F(MB, MT, M) \
+    hipLaunchKernelGGL((k_force<MB, MT, M>), g, b, 0, s, a, l, t, box0, boxes, sp)
+#define MSM_FM(MB, MT)                      \
+    do {                                    \
+        if (sp.mode == 0) MSM_F(MB, MT, 0); \
+        else if (sp.mode == 1) MSM_F(MB, MT, 1); \
+        else MSM_F(MB, MT, 2);              \
+    } while (0)
+    if (multibox) {
+        if (mt) MSM_FM(1, 1); else MSM_FM(1, 0);
+    } else {
+        if (mt) MSM_FM(0, 1); else MSM_FM(0, 0);
+    }
+#undef MSM_FM
+#undef MSM_F
+}
+
+// --------------------------------------------------------------------------------------------
+// energy pass
+// --------------------------------------------------------------------------------------------
+
+template <int MB, int EAM>
+__global__ __launch_bounds__(kBlock) void k_epot(AtomsDev a, ListDev l, TabDev t, BoxDev box0,
+                                                 const BoxDev* __restrict__ boxes, int* err) {
+    const int i = atom_index();
+    if (i >= a.n) return;
+    double er0 = 0.0, den0 = 0.0;
+    if ((a.statu[i] & STATU_ACTIVE) == STATU_ACTIVE) {
+        const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
+        const double4 pi = a.pos[i];
+        const int ti = a.ityp[i];
+        const int kv = l.kvois[i];
+        for (int iw = 0; iw < kv; ++iw) {
+            const int j = l.indi[i + static_cast<size_t>(iw) * l.stride];
+            const double4 pj = a.pos[j];
+            const double sx = min_image(pi.x - pj.x, b.len[0], b.hlen[0], b.ifpd[0]);
+            const double sy = min_image(pi.y - pj.y, b.len[1], b.hlen[1], b.ifpd[1]);
+            const double sz = min_image(pi.z - pj.z, b.len[2], b.hlen[2], b.ifpd[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            if (r2 <= t.ru2max) {
+                const int kind = t.kpair[ti * t.ngroup + a.ityp[j]];
+                const double r = sqrt(r2);
+                const double sk = sqrt(r) * t.csi;
+                const int kk = static_cast<int>(sk);
+                const double dk = sk - kk;
+                const double4 e = t.e4[kind * t.ntab2 + kk];
+                er0 = er0 + (e.x + dk * e.y) / r;
+                den0 = den0 + (e.z + dk * e.w);
+            }
+        }
+        if constexpr (!EAM) {
+            den0 = -sqrt(den0);
+        } else {
+            const double sk = den0 / t.rhod + 1.0;
+            int kk = static_cast<int>(sk + 0.000001);
+            if (kk < 0) kk = 0;
+            if (kk > t.nembd) {
+                atomicOr(&err[kErrEmbd], 1);
+                kk = t.nembd;
+            }
+            const double4 m = t.m4[t.kembd[ti] * t.nembd2 + kk];
+            den0 = m.x + (sk - kk) * m.y;
+        }
+    }
+    a.epot[i] = er0 + den0;
+}
+
+void launch_epot(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                 const BoxDev* boxes, int multibox, int* err, hipStream_t s) {
+    if (a.n <= 0) return;
+    const dim3 g(grid_for(a.n)), b(kBlock);
+    const bool eam = t.pot_type != 0;
+    if (multibox) {
+        if (eam) hipLaunchKernelGGL((k_epot<1, 1>), g, b, 0, s, a, l, t, box0, boxes, err);
+        else     hipLaunchKernelGGL((k_epot<1, 0>), g, b, 0, s, a, l, t, box0, boxes, err);
+    } else {
+        if (eam) hipLaunchKernelGGL((k_epot<0, 1>), g, b, 0, s, a, l, t, box0, boxes, err);
+        else     hipLaunchKernelGGL((k_epot<0, 0>), g, b, 0, s, a, l, t, box0, boxes, err);
+    }
+}
+
+// --------------------------------------------------------------------------------------------
+// stand-alone integrator kernels (Predictor_DEV / Correction_DEV)
+// --------------------------------------------------------------------------------------------
+
+template <int MB>
+__global__ __launch_bounds__(kBlock) void k_predictor(AtomsDev a, TabDev t, BoxDev box0,
+                                                      const BoxDev* __restrict__ boxes, double h) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    int stat = a.statu[i];
+    if ((stat & STATU_ACTIVE) != STATU_ACTIVE) return;
+    const int st0 = stat;
+    const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
+    const double hs2 = h * 0.5, h2s2 = h * hs2;
+    const double cm0 = t.cm[a.ityp[i]];
+    const double ax = a.fx[i] / cm0, ay = a.fy[i] / cm0, az = a.fz[i] / cm0;
+    double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+    const double4 p = a.pos[i];
+    double x = p.x, y = p.y, z = p.z;
+    double dx = h * vx + h2s2 * ax;
+    if (stat & STATU_FIXPOSX) dx = 0.0;
+    x = x + dx;
+    if (b.ifpd[0]) {
+        if (x > b.lo[0] + b.len[0]) { x = x - b.len[0]; stat |= STATU_PASSBOUND; }
+        else if (x < b.lo[0])       { x = x + b.len[0]; stat |= STATU_PASSBOUND; }
+    }
+    double dy = h * vy + h2s2 * ay;
+    if (stat & STATU_FIXPOSY) dy = 0.0;
+    y = y + dy;
+    if (b.ifpd[1]) {
+        if (y > b.lo[1] + b.len[1]) { y = y - b.len[1]; stat |= STATU_PASSBOUND; }
+        else if (y < b.lo[1])       { y = y + b.len[1]; stat |= STATU_PASSBOUND; }
+    }
+    double dz = h * vz + h2s2 * az;
+    if (stat & STATU_FIXPOSZ) dz = 0.0;
+    z = z + dz;
+    if (b.ifpd[2]) {
+        if (z > b.lo[2] + b.len[2]) { z = z - b.len[2]; stat |= STATU_PASSBOUND; }
+        else if (z < b.lo[2])       { z = z + b.len[2]; stat |= STATU_PASSBOUND; }
+    }
+    if (!(stat & (STATU_FIXVELX | STATU_FIXPOSX))) vx = vx + hs2 * ax;
+    if (!(stat & (STATU_FIXVELY | STATU_FIXPOSY))) vy = vy + hs2 * ay;
+    if (!(stat & (STATU_FIXVELZ | STATU_FIXPOSZ))) vz = vz + hs2 * az;
+    a.pos[i] = make_double4(x, y, z, p.w);
+    a.vx[i] = vx;
+    a.vy[i] = vy;
+    a.vz[i] = vz;
+    a.dx[i] = a.dx[i] + dx;
+    a.dy[i] = a.dy[i] + dy;
+    a.dz[i] = a.dz[i] + dz;
+    if (x > b.lo[0] + b.len[0] || y > b.lo[1] + b.len[1] || z > b.lo[2] + b.len[2])
+        stat = STATU_OUTOFBOX | STATU_REFLECT;
+    else if (x < b.lo[0] || y < b.lo[1] || z < b.lo[2])
+        stat = STATU_OUTOFBOX | STATU_TRANSMIT;
+    if (stat != st0) a.statu[i] = stat;
+}
+
+void launch_predictor(const AtomsDev& a, const TabDev& t, const BoxDev& box0,
+                      const BoxDev* boxes, int multibox, double h, hipStream_t s) {
+    if (a.n <= 0) return;
+    if (multibox)
+        hipLaunchKernelGGL(k_predictor<1>, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a, t, box0,
+                           boxes, h);
+    else
+        hipLaunchKernelGGL(k_predictor<0>, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a, t, box0,
+                           boxes, h);
+}
+
+__global__ __launch_bounds__(kBlock) void k_correction(AtomsDev a, TabDev t, double h) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int st = a.statu[i];
+    if ((st & STATU_ACTIVE) != STATU_ACTIVE) return;
+    const double hs2 = h * 0.5;
+    const double cm0 = t.cm[a.ityp[i]];
+    if (!(st & (STATU_FIXVELX | STATU_FIXPOSX))) a.vx[i] = a.vx[i] + hs2 * (a.fx[i] / cm0);
+    if (!(st & (STATU_FIXVELY | STATU_FIXPOSY))) a.vy[i] = a.vy[i] + hs2 * (a.fy[i] / cm0);
+    if (!(st & (STATU_FIXVELZ | STATU_FIXPOSZ))) a.vz[i] = a.vz[i] + hs2 * (a.fz[i] / cm0);
+}
+
+void launch_correction(const AtomsDev& a, const TabDev& t, double h, hipStream_t s) {
+    if (a.n <= 0) return;
+    hipLaunchKernelGGL(k_correction, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a, t, h);
+}
+
+// --------------------------------------------------------------------------------------------
+// copy-out helpers (sorted -> caller order), box totals, list statistics
+// --------------------------------------------------------------------------------------------
+
+__global__ void k_unsort_d(const double* __restrict__ src, double* __restrict__ dst,
+                           const int* __restrict__ gid, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[gid[p]] = src[p];
+}
+__global__ void k_unsort_i(const int* __restrict__ src, int* __restrict__ dst,
+                           const int* __restrict__ gid, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[gid[p]] = src[p];
+}
+__global__ void k_unsort_pos(const double4* __restrict__ src, double* __restrict__ dst,
+                             const int* __restrict__ gid, int n, int comp) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double4 v = src[p];
+    const double c = comp == 0 ? v.x : comp == 1 ? v.y : comp == 2 ? v.z : v.w;
+    dst[gid[p]] = c;
+}
+__global__ void k_unsort_list(const int* __restrict__ kvois, const int* __restrict__ indi,
+                              int stride, const int* __restrict__ gid, int n, int mx,
+                              int* __restrict__ kv_out, int* __restrict__ indi_out, int ld) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int g = gid[p];
+    const int kv = kvois[p];
+    kv_out[g] = kv;
+    for (int iw = 0; iw < kv && iw < mx; ++iw)
+        indi_out[g + static_cast<size_t>(iw) * ld] = gid[indi[p + static_cast<size_t>(iw) * stride]] + 1;
+}
+
+void launch_unsort_d(const double* src, double* dst, const int* gid, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_unsort_d, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, gid, n);
+}
+void launch_unsort_i(const int* src, int* dst, const int* gid, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_unsort_i, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, gid, n);
+}
+void launch_unsort_pos(const double4* src, double* dst, const int* gid, int n, int comp,
+                       hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_unsort_pos, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, gid, n, comp);
+}
+void launch_unsort_list(const int* kvois, const int* indi, int stride, const int* gid, int n,
+                        int mx, int* kv_out, int* indi_out, int ld, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_unsort_list, dim3(grid_for(n)), dim3(kBlock), 0, s, kvois, indi,
+                           stride, gid, n, mx, kv_out, indi_out, ld);
+}
+
+// One workgroup per box: fixed-order strided partial sums then a fixed-shape tree, so the total
+// is the same on every run.
+__global__ __launch_bounds__(kBlock) void k_box_sum(const double* __restrict__ epot,
+                                                    const int* __restrict__ start,
+                                                    const BoxDev* __restrict__ boxes,
+                                                    double* __restrict__ out) {
+    __shared__ double lds[kBlock];
+    const BoxDev b = boxes[blockIdx.x];
+    const int c0 = b.cell_off, c1 = b.cell_off + b.ncell[0] * b.ncell[1] * b.ncell[2];
+    const int a0 = start[c0], a1 = start[c1];
+    double s = 0.0;
+    for (int i = a0 + threadIdx.x; i < a1; i += kBlock) s += epot[i];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = lds[0];
+}
+
+void launch_box_sum(const double* epot, const int* start, const BoxDev* boxes, int nbox,
+                    double* out, hipStream_t s) {
+    if (nbox > 0)
+        hipLaunchKernelGGL(k_box_sum, dim3(nbox), dim3(kBlock), 0, s, epot, start, boxes, out);
+}
+
+__global__ void k_list_stats(const int* __restrict__ kvois, int n, unsigned long long* total,
+                             int* maxk) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int v = (p < n) ? kvois[p] : 0;
+    unsigned long long s = static_cast<unsigned long long>(v);
+    int m = v;
+    for (int d = 32; d > 0; d >>= 1) {
+        s += __shfl_down(s, d, 64);
+        m = max(m, __shfl_down(m, d, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(total, s);
+        atomicMax(maxk, m);
+    }
+}
+
+void launch_list_stats(const int* kvois, int n, unsigned long long* total, int* maxk,
+                       hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_list_stats, dim3(grid_for(n)), dim3(kBlock), 0, s, kvois, n, total, maxk);
+}
+
+}  // namespace msm

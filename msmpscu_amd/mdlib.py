@@ -1,0 +1,610 @@
+"""Host-side mirror of the reference MDLIB surface for the force hot path.
+
+Names, argument meaning and error behaviour follow the reference modules so that callers (and the
+tests) read like MDLIB code:
+
+* ``MDForceTable``, ``Register_ForceTableProc``, ``New_ForceTable``,
+  ``Create_Pairwise_ForceTable``, ``Create_EMBDFUNTable``, ``Create_Interaction_ForceTable``,
+  ``Export_ForceTable``, ``Import_ForceTable``
+  -- MDLIB/sor/Common/MD_TypeDef_ForceTable.F90:117-155, 530-1056, 1151-1230, 1315-1459
+* ``SimMDBox`` / ``SimMDCtrl`` (the fields this path reads)
+  -- MDLIB/sor/Common/MD_TypeDef_SimBox.F90:43-132, MD_TypeDef_SimCtrlParam.F90:56-142
+* ``MDForceClassGPU``, ``Register_ForceClass``, ``Init_Forcetable_Dev``, ``CalForce_ForceClass``,
+  ``CalEpot_ForceClass`` -- MDLIB/sor/CommonGPU/MD_ForceClass_Register_GPU.F90:248-442, 483-822
+* ``Initialize_Globle_Variables_DEV``, ``CopyOut_SimBox_DEV``
+  -- MDLIB/sor/CommonGPU/MD_Globle_Variables_GPU.F90:515-654
+* ``Cal_NeighBoreList_DEV``, ``Copyout_NeighboreList_DEV``
+  -- MDLIB/sor/CommonGPU/MD_NeighborsList_GPU.F90:121-181, 604-693
+* ``Predictor_DEV``, ``Correction_DEV`` -- MDLIB/sor/CommonGPU/MD_DiffScheme_GPU.F90:604-847
+* ``For_One_Step`` loop -- MDLIB/sor/Appshell/MD_Method_GenericMD_GPU.F90:496-659
+
+All compute goes through libmsm_hip.so; where the reference stops the process, these functions
+raise ``MSMError`` (device) or ``ValueError`` (host setup).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Sequence, Union
+
+import numpy as np
+
+from . import _lib
+from ._lib import MSM_EAM_TYPE, MSM_FS_TYPE, check, dptr, iptr, lib
+
+PotFn = Union[str, Callable[[float], tuple], None]
+
+
+# ----------------------------------------------------------------------------------------------
+# force tables
+# ----------------------------------------------------------------------------------------------
+
+@dataclass
+class _ExtProc:
+    POTTypeID: int
+    POTSymbID: str
+    NNFORCE: PotFn
+    NEFORCE: PotFn
+    EMBDF: PotFn
+    NOTE: str
+
+
+@dataclass
+class MDForceTable:
+    """MDForceTable (MD_TypeDef_ForceTable.F90:117-155).
+
+    Table arrays keep the reference's Fortran memory order: ``POTR`` has shape (NKIND, NTAB) in
+    Fortran order, i.e. element (k, i) at flat offset k + i*NKIND, which is what the C-ABI takes.
+    """
+    PotType: str = "FS_TYPE"
+    NKIND: int = 0
+    NTAB: int = 10000
+    Rmax: float = -1.0
+    RmaxSQRT: float = 0.0
+    CSI: float = 0.0
+    CSIV: float = 0.0
+    FPAIR: List[int] = field(default_factory=list)
+    KPAIR: Optional[np.ndarray] = None
+    POTR: Optional[np.ndarray] = None
+    FPOTR: Optional[np.ndarray] = None
+    POTB: Optional[np.ndarray] = None
+    FPOTB: Optional[np.ndarray] = None
+    NEMBD: int = 10000
+    NKIND1: int = 0
+    FPAIR1: List[int] = field(default_factory=list)
+    KEMBD: Optional[np.ndarray] = None
+    FEMBD: Optional[np.ndarray] = None
+    DFEMBD: Optional[np.ndarray] = None
+    RHOMX: float = -1.0
+    RHOD: float = 1.0
+    RHOMULT: float = 20.0
+    REGPROCS: Dict[int, _ExtProc] = field(default_factory=dict)
+    HASREG: Dict[int, bool] = field(default_factory=dict)
+    HASREG1: Dict[int, bool] = field(default_factory=dict)
+
+
+def _fn_address(fn: PotFn, keep: list):
+    """C callable for a potential: a built-in name ("msm_pot_wfs_nn") or a Python callable
+    r -> (value, derivative-term)."""
+    if fn is None:
+        return None
+    if isinstance(fn, str):
+        return _lib.builtin_fn(fn)
+
+    def tramp(rp, vp, dvp, _f=fn):
+        v, dv = _f(rp[0])
+        vp[0] = v
+        dvp[0] = dv
+
+    cb = _lib.PAIRFN(tramp)
+    keep.append(cb)
+    return C.cast(cb, C.c_void_p).value
+
+
+def Register_ForceTableProc(IFORCE, FTable: MDForceTable, NNFORCE: PotFn, NEFORCE: PotFn = None,
+                            EMBDF: PotFn = None, NOTE: str = "", Tgt: Optional[str] = None):
+    """RegisterByTypeID_ForceTableProc / RegisterBySymbID_ForceTableProc (:626-852).
+
+    ``IFORCE`` is the integer type id; with ``Tgt`` given, ``IFORCE`` is read as the source symbol
+    and the id is assigned as max(existing)+1 with symbol "TGT<-SOR"."""
+    if Tgt is not None:
+        symb = (Tgt.strip() + "<-" + str(IFORCE).strip()).upper()
+        if any(p.POTSymbID == symb for p in FTable.REGPROCS.values()):
+            raise ValueError(f"MDPSCU Error: the potential type by symb {symb} has been registed")
+        tid = max([0] + list(FTable.REGPROCS)) + 1
+        FTable.REGPROCS[tid] = _ExtProc(tid, symb, NNFORCE, NEFORCE, EMBDF, NOTE)
+        return tid
+    if IFORCE in FTable.REGPROCS:
+        raise ValueError(f"MDPSCU Error: the potential type #{IFORCE} has been registed")
+    FTable.REGPROCS[IFORCE] = _ExtProc(IFORCE, "", NNFORCE, NEFORCE, EMBDF, NOTE)
+    return IFORCE
+
+
+def SymbID2TypeID_ForceTableProc(Sor: str, Tgt: str, FTable: MDForceTable) -> int:
+    symb = (Tgt.strip() + "<-" + Sor.strip()).upper()
+    for tid, p in FTable.REGPROCS.items():
+        if p.POTSymbID == symb:
+            return tid
+    return 0
+
+
+def New_ForceTable(SimBox: "SimMDBox", CtrlParam: "SimMDCtrl", FTable: MDForceTable):
+    """New_ForceTable (:530-622): distinct pair types -> kinds, grid constants, zeroed tables."""
+    n = SimBox.NGROUP
+    fpair: List[int] = []
+    for i in range(n):
+        for j in range(n):
+            t = int(SimBox.PTYPE[i, j])
+            if t not in fpair:
+                fpair.append(t)
+    FTable.FPAIR = fpair
+    FTable.NKIND = len(fpair)
+    nr = int(CtrlParam.NUMFTABR)
+    FTable.NTAB = nr
+    FTable.KPAIR = np.zeros((n, n), dtype=np.int32, order="F")
+    for name in ("POTR", "FPOTR", "POTB", "FPOTB"):
+        setattr(FTable, name, np.zeros((FTable.NKIND, nr), dtype=np.float64, order="F"))
+    FTable.Rmax = float(np.max(CtrlParam.RU))
+    FTable.RmaxSQRT = float(np.sqrt(FTable.Rmax))
+    FTable.CSI = float(nr) / FTable.RmaxSQRT
+    FTable.CSIV = 1.0 / FTable.CSI
+    FTable.RHOMX = -1.0
+    FTable.HASREG = {}
+    FTable.HASREG1 = {}
+    if FTable.PotType.strip() == "EAM_TYPE":
+        fpair1: List[int] = []
+        for i in range(n):
+            t = int(SimBox.PTYPE[i, i])
+            if t not in fpair1:
+                fpair1.append(t)
+        FTable.FPAIR1 = fpair1
+        FTable.NKIND1 = len(fpair1)
+        FTable.NEMBD = int(CtrlParam.NUMFTABE)
+        FTable.KEMBD = np.zeros(n, dtype=np.int32)
+        FTable.FEMBD = np.zeros((FTable.NKIND1, FTable.NEMBD), dtype=np.float64, order="F")
+        FTable.DFEMBD = np.zeros((FTable.NKIND1, FTable.NEMBD), dtype=np.float64, order="F")
+        FTable.RHOMULT = float(CtrlParam.RHOSCAL)
+
+
+def _col(a: np.ndarray, k: int):
+    """Pointer to element (k, 0) of a Fortran-ordered (NKIND, NTAB) array (stride NKIND)."""
+    base = a.ctypes.data + k * a.itemsize
+    return C.cast(C.c_void_p(base), C.POINTER(C.c_double))
+
+
+def Create_Pairwise_ForceTable(IFORCE: int, FTable: MDForceTable):
+    """Create_Pairwise_ForceTable (:890-982), evaluated by msm_table_pairwise."""
+    if IFORCE not in FTable.FPAIR:
+        return
+    k = FTable.FPAIR.index(IFORCE)
+    if FTable.HASREG.get(k):
+        return
+    proc = FTable.REGPROCS.get(IFORCE)
+    if proc is None:
+        raise ValueError(f"MDPSCU Error: the potential type #{IFORCE} is not available")
+    keep: list = []
+    nn = _fn_address(proc.NNFORCE, keep)
+    ne = _fn_address(proc.NEFORCE, keep)
+    rhomx = C.c_double(FTable.RHOMX)
+    rc = lib().msm_table_pairwise(FTable.NTAB, FTable.Rmax, nn, ne, FTable.RHOMULT,
+                                  C.byref(rhomx), _col(FTable.POTR, k), _col(FTable.FPOTR, k),
+                                  _col(FTable.POTB, k), _col(FTable.FPOTB, k), FTable.NKIND)
+    check(rc, "msm_table_pairwise")
+    FTable.RHOMX = rhomx.value
+    FTable.RHOD = FTable.RHOMX / float(FTable.NEMBD)
+    FTable.HASREG[k] = True
+
+
+def Create_EMBDFUNTable(IFORCE: int, FTable: MDForceTable):
+    """Create_EMBDFUNTable (:986-1056), evaluated by msm_table_embd."""
+    if IFORCE not in FTable.FPAIR1:
+        return
+    k = FTable.FPAIR1.index(IFORCE)
+    if FTable.HASREG1.get(k):
+        return
+    proc = FTable.REGPROCS.get(IFORCE)
+    if proc is None:
+        raise ValueError(f"MDPSCU Error: the embedment function of potential type #{IFORCE} "
+                         "is not available")
+    keep: list = []
+    em = _fn_address(proc.EMBDF, keep)
+    rc = lib().msm_table_embd(FTable.NEMBD, FTable.RHOD, em, _col(FTable.FEMBD, k),
+                              _col(FTable.DFEMBD, k), FTable.NKIND1)
+    check(rc, "msm_table_embd")
+    FTable.HASREG1[k] = True
+
+
+def Create_Interaction_ForceTable(SimBox: "SimMDBox", CtrlParam: "SimMDCtrl",
+                                  FTable: MDForceTable):
+    """Create_Interaction_ForceTable (:1151-1230)."""
+    n = SimBox.NGROUP
+    for i in range(n):
+        for j in range(n):
+            if SimBox.PTYPE[i, j] == 0 and SimBox.SYMB:
+                SimBox.PTYPE[i, j] = SymbID2TypeID_ForceTableProc(SimBox.SYMB[j], SimBox.SYMB[i],
+                                                                  FTable)
+            if SimBox.PTYPE[i, j] == 0:
+                raise ValueError("MDPSCU Error: cannot find the registed force table for between "
+                                 f"atom groups {i + 1} and {j + 1}")
+    New_ForceTable(SimBox, CtrlParam, FTable)
+    FTable.KPAIR[:, :] = -1
+    for i in range(n):
+        for j in range(n):
+            iforce = int(SimBox.PTYPE[i, j])
+            Create_Pairwise_ForceTable(iforce, FTable)
+            FTable.KPAIR[i, j] = FTable.FPAIR.index(iforce) + 1
+    if FTable.PotType.strip() == "EAM_TYPE":
+        FTable.KEMBD[:] = -1
+        for i in range(n):
+            iforce = int(SimBox.PTYPE[i, i])
+            Create_EMBDFUNTable(iforce, FTable)
+            FTable.KEMBD[i] = FTable.FPAIR1.index(iforce) + 1
+        if FTable.RHOD <= 1.0e-64:
+            FTable.RHOD = 1.0
+
+
+# ----------------------------------------------------------------------------------------------
+# box and control parameters
+# ----------------------------------------------------------------------------------------------
+
+@dataclass
+class SimMDBox:
+    """The SimMDBox fields this path reads or writes (MD_TypeDef_SimBox.F90:43-132).
+
+    XP, XP1, FP, DIS have shape (NPRT, 3) in Fortran order (x column, then y, then z)."""
+    NPRT: int
+    NGROUP: int
+    ZL: np.ndarray
+    BOXLOW: np.ndarray
+    XP: np.ndarray
+    XP1: np.ndarray
+    ITYP: np.ndarray
+    STATU: np.ndarray
+    CM: np.ndarray
+    PTYPE: np.ndarray
+    SYMB: List[str] = field(default_factory=list)
+    FP: Optional[np.ndarray] = None
+    DIS: Optional[np.ndarray] = None
+    EPOT: Optional[np.ndarray] = None
+    VTENSOR: Optional[np.ndarray] = None
+
+    @property
+    def BOXUP(self) -> np.ndarray:
+        return self.BOXLOW + self.ZL
+
+    def __post_init__(self):
+        self.ZL = np.asarray(self.ZL, dtype=np.float64)
+        self.BOXLOW = np.asarray(self.BOXLOW, dtype=np.float64)
+        self.XP = np.asfortranarray(self.XP, dtype=np.float64)
+        self.XP1 = np.asfortranarray(self.XP1, dtype=np.float64)
+        self.ITYP = np.ascontiguousarray(self.ITYP, dtype=np.int32)
+        self.STATU = np.ascontiguousarray(self.STATU, dtype=np.int32)
+        self.CM = np.ascontiguousarray(self.CM, dtype=np.float64)
+        self.PTYPE = np.asarray(self.PTYPE, dtype=np.int32)
+        if self.FP is None:
+            self.FP = np.zeros((self.NPRT, 3), dtype=np.float64, order="F")
+        if self.DIS is None:
+            self.DIS = np.zeros((self.NPRT, 3), dtype=np.float64, order="F")
+        if self.EPOT is None:
+            self.EPOT = np.zeros(self.NPRT, dtype=np.float64)
+
+
+@dataclass
+class SimMDCtrl:
+    """The SimMDCtrl fields this path reads (MD_TypeDef_SimCtrlParam.F90:56-142).  RU and NB_RM
+    are (NGROUP, NGROUP) in length units (the reference converts from lattice units at input)."""
+    RU: np.ndarray
+    NB_RM: np.ndarray
+    NB_UPTAB: int = 10
+    NB_MXNBS: int = 64
+    NUMFTABR: int = 5000
+    NUMFTABE: int = 5000
+    RHOSCAL: float = 20.0
+    IFPD: Sequence[int] = (1, 1, 1)
+    H: float = 1.0
+    MULTIBOX: int = 1
+
+    def __post_init__(self):
+        self.RU = np.asfortranarray(np.atleast_2d(self.RU), dtype=np.float64)
+        self.NB_RM = np.asfortranarray(np.atleast_2d(self.NB_RM), dtype=np.float64)
+
+
+# ----------------------------------------------------------------------------------------------
+# device side
+# ----------------------------------------------------------------------------------------------
+
+class DeviceMD:
+    """One libmsm_hip context: the device copy of the boxes, the neighbour list and the force
+    table (dm_WorkSpace, dm_Neighbors, dm_FTableWS of the reference)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().msm_create(C.byref(h), device), "msm_create")
+        self.h = h
+        self.n = 0
+        self.nbox = 0
+        self.mx = 0
+        self.natoms: List[int] = []
+
+    # ---- setup ----
+    def set_force_table(self, FTable: MDForceTable):
+        pt = MSM_EAM_TYPE if FTable.PotType.strip() == "EAM_TYPE" else MSM_FS_TYPE
+        ng = FTable.KPAIR.shape[0]
+        kpair = np.ascontiguousarray(np.asarray(FTable.KPAIR, dtype=np.int32).ravel(order="F"))
+        potr = np.ascontiguousarray(FTable.POTR.ravel(order="F"))
+        fpotr = np.ascontiguousarray(FTable.FPOTR.ravel(order="F"))
+        potb = np.ascontiguousarray(FTable.POTB.ravel(order="F"))
+        fpotb = np.ascontiguousarray(FTable.FPOTB.ravel(order="F"))
+        if pt == MSM_EAM_TYPE:
+            kembd = np.ascontiguousarray(FTable.KEMBD, dtype=np.int32)
+            fembd = np.ascontiguousarray(FTable.FEMBD.ravel(order="F"))
+            dfembd = np.ascontiguousarray(FTable.DFEMBD.ravel(order="F"))
+            rc = lib().msm_set_force_table(self.h, pt, ng, FTable.NKIND, FTable.NTAB, FTable.CSI,
+                                           iptr(kpair), dptr(potr), dptr(fpotr), dptr(potb),
+                                           dptr(fpotb), FTable.NKIND1, FTable.NEMBD, FTable.RHOD,
+                                           iptr(kembd), dptr(fembd), dptr(dfembd))
+        else:
+            rc = lib().msm_set_force_table(self.h, pt, ng, FTable.NKIND, FTable.NTAB, FTable.CSI,
+                                           iptr(kpair), dptr(potr), dptr(fpotr), dptr(potb),
+                                           dptr(fpotb), 0, 0, 1.0, None, None, None)
+        check(rc, "msm_set_force_table", self.h)
+
+    def set_ctrl(self, CtrlParam: SimMDCtrl, CM: np.ndarray):
+        ng = CtrlParam.RU.shape[0]
+        ru = np.ascontiguousarray(CtrlParam.RU.ravel(order="F"))
+        nb = np.ascontiguousarray(CtrlParam.NB_RM.ravel(order="F"))
+        cm = np.ascontiguousarray(CM, dtype=np.float64)
+        check(lib().msm_set_ctrl(self.h, ng, dptr(ru), dptr(nb), int(CtrlParam.NB_MXNBS),
+                                 dptr(cm)), "msm_set_ctrl", self.h)
+        self.mx = int(CtrlParam.NB_MXNBS)
+
+    def set_boxes(self, boxes: Sequence[SimMDBox], ifpd: Sequence[int]):
+        nb = len(boxes)
+        nat = np.array([b.NPRT for b in boxes], dtype=np.int32)
+        lo = np.ascontiguousarray(np.concatenate([b.BOXLOW for b in boxes]), dtype=np.float64)
+        ln = np.ascontiguousarray(np.concatenate([b.ZL for b in boxes]), dtype=np.float64)
+        pd = np.ascontiguousarray(np.tile(np.asarray(ifpd, dtype=np.int32), nb))
+        check(lib().msm_set_boxes(self.h, nb, iptr(nat), dptr(lo), dptr(ln), iptr(pd)),
+              "msm_set_boxes", self.h)
+        self.nbox = nb
+        self.natoms = [int(x) for x in nat]
+
+    def set_atoms(self, xp, xp1, ityp, statu):
+        n = int(np.asarray(ityp).shape[0])
+        xp = np.ascontiguousarray(np.asarray(xp, dtype=np.float64).ravel(order="F"))
+        xp1 = np.ascontiguousarray(np.asarray(xp1, dtype=np.float64).ravel(order="F"))
+        ityp = np.ascontiguousarray(ityp, dtype=np.int32)
+        statu = np.ascontiguousarray(statu, dtype=np.int32)
+        check(lib().msm_set_atoms(self.h, n, dptr(xp), dptr(xp1), iptr(ityp), iptr(statu)),
+              "msm_set_atoms", self.h)
+        self.n = n
+
+    # ---- compute ----
+    def build_neighbors(self):
+        check(lib().msm_build_neighbors(self.h), "msm_build_neighbors", self.h)
+
+    def calc_force(self):
+        check(lib().msm_calc_force(self.h), "msm_calc_force", self.h)
+
+    def calc_epot(self):
+        check(lib().msm_calc_epot(self.h), "msm_calc_epot", self.h)
+
+    def predictor(self, h: float):
+        check(lib().msm_predictor(self.h, h), "msm_predictor", self.h)
+
+    def correction(self, h: float):
+        check(lib().msm_correction(self.h, h), "msm_correction", self.h)
+
+    def run(self, nsteps: int, nb_uptab: int, step0: int, h: float):
+        check(lib().msm_run(self.h, nsteps, nb_uptab, step0, h), "msm_run", self.h)
+
+    def sync(self):
+        check(lib().msm_sync(self.h), "msm_sync", self.h)
+
+    # ---- copy-out ----
+    def get_atoms(self):
+        n = self.n
+        xp = np.zeros(3 * n)
+        xp1 = np.zeros(3 * n)
+        fp = np.zeros(3 * n)
+        dis = np.zeros(3 * n)
+        st = np.zeros(n, dtype=np.int32)
+        check(lib().msm_get_atoms(self.h, dptr(xp), dptr(xp1), dptr(fp), iptr(st), dptr(dis)),
+              "msm_get_atoms", self.h)
+        f = lambda a: a.reshape(3, n).T.copy(order="F")  # noqa: E731
+        return f(xp), f(xp1), f(fp), st, f(dis)
+
+    def get_den(self) -> np.ndarray:
+        d = np.zeros(self.n)
+        check(lib().msm_get_den(self.h, dptr(d)), "msm_get_den", self.h)
+        return d
+
+    def get_epot(self) -> np.ndarray:
+        d = np.zeros(self.n)
+        check(lib().msm_get_epot(self.h, dptr(d)), "msm_get_epot", self.h)
+        return d
+
+    def get_box_epot(self) -> np.ndarray:
+        d = np.zeros(self.nbox)
+        check(lib().msm_get_box_epot(self.h, dptr(d)), "msm_get_box_epot", self.h)
+        return d
+
+    def get_neighbors(self):
+        n = self.n
+        kv = np.zeros(n, dtype=np.int32)
+        ind = np.zeros(n * self.mx, dtype=np.int32)
+        check(lib().msm_get_neighbors(self.h, iptr(kv), iptr(ind), n), "msm_get_neighbors",
+              self.h)
+        return kv, ind.reshape(self.mx, n).T  # INDI(N, MXKVOIS), 1-based ids
+
+    def list_stats(self):
+        tot = C.c_longlong()
+        mk = C.c_int()
+        check(lib().msm_list_stats(self.h, C.byref(tot), C.byref(mk)), "msm_list_stats", self.h)
+        return tot.value, mk.value
+
+    def close(self):
+        if self.h:
+            lib().msm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class MDForceClassGPU:
+    """MDForceClassGPU (MD_ForceClass_Register_GPU.F90:248-259): the force table and the device
+    context it was initialised on."""
+    ForceTable: MDForceTable = field(default_factory=MDForceTable)
+    Device: Optional[DeviceMD] = None
+
+
+def Register_ForceClass(classname: str, ForceClass: MDForceClassGPU):
+    """Register_ForceClass (:363-442): FS_TYPE or EAM_TYPE."""
+    name = classname.strip()
+    if name not in ("FS_TYPE", "EAM_TYPE"):
+        raise ValueError(f"MDPSCU Error: unsupported type of potential {classname}")
+    ForceClass.ForceTable.PotType = name
+
+
+def _boxes(SimBox) -> List[SimMDBox]:
+    return list(SimBox) if isinstance(SimBox, (list, tuple)) else [SimBox]
+
+
+def Initialize_Globle_Variables_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU,
+                                    device: int = 0):
+    """Initialize_Globle_Variables_DEV (MD_Globle_Variables_GPU.F90:515-654) generalised to
+    boxes of uneven size: creates the device context, uploads control data, boxes and atoms."""
+    boxes = _boxes(SimBox)
+    if ForceClass.Device is None:
+        ForceClass.Device = DeviceMD(device)
+    dev = ForceClass.Device
+    dev.set_ctrl(CtrlParam, boxes[0].CM)
+    dev.set_boxes(boxes, CtrlParam.IFPD)
+    xp = np.concatenate([b.XP for b in boxes], axis=0)
+    xp1 = np.concatenate([b.XP1 for b in boxes], axis=0)
+    ityp = np.concatenate([b.ITYP for b in boxes])
+    st = np.concatenate([b.STATU for b in boxes])
+    dev.set_atoms(xp, xp1, ityp, st)
+    return dev
+
+
+def Init_Forcetable_Dev(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
+    """Init_Forcetable_Dev (:483-559): copy the registered table to the device."""
+    if ForceClass.Device is None:
+        raise ValueError("MDPSCU Error: device workspace not initialised")
+    ForceClass.Device.set_force_table(ForceClass.ForceTable)
+
+
+def Cal_NeighBoreList_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
+    ForceClass.Device.build_neighbors()
+
+
+def Copyout_NeighboreList_DEV(ForceClass: MDForceClassGPU):
+    return ForceClass.Device.get_neighbors()
+
+
+def CalForce_ForceClass(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
+    ForceClass.Device.calc_force()
+
+
+def CalEpot_ForceClass(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
+    ForceClass.Device.calc_epot()
+
+
+def Predictor_DEV(ITIME: int, SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
+    ForceClass.Device.predictor(CtrlParam.H)
+
+
+def Correction_DEV(ITIME: int, SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
+    ForceClass.Device.correction(CtrlParam.H)
+
+
+def For_One_Steps(ITIME0: int, NSTEPS: int, SimBox, CtrlParam: SimMDCtrl,
+                  ForceClass: MDForceClassGPU):
+    """NSTEPS consecutive For_One_Step calls (predictor, rebuild when mod(ITIME, NB_UPTAB)=0,
+    force, corrector), replayed on the device."""
+    ForceClass.Device.run(NSTEPS, CtrlParam.NB_UPTAB, ITIME0, CtrlParam.H)
+
+
+def CopyOut_SimBox_DEV(SimBox, ForceClass: MDForceClassGPU, with_epot: bool = False):
+    """CopyOut_SimBox_DEV: device state back into the SimMDBox objects."""
+    boxes = _boxes(SimBox)
+    dev = ForceClass.Device
+    xp, xp1, fp, st, dis = dev.get_atoms()
+    ep = dev.get_epot() if with_epot else None
+    off = 0
+    for b in boxes:
+        s = slice(off, off + b.NPRT)
+        b.XP = np.asfortranarray(xp[s])
+        b.XP1 = np.asfortranarray(xp1[s])
+        b.FP = np.asfortranarray(fp[s])
+        b.DIS = np.asfortranarray(dis[s])
+        b.STATU = st[s].copy()
+        if ep is not None:
+            b.EPOT = ep[s].copy()
+        off += b.NPRT
+
+
+# ----------------------------------------------------------------------------------------------
+# table files (.pair / .embd), Export_ForceTable (:1315-1459) layout in eV and Angstrom
+# ----------------------------------------------------------------------------------------------
+
+def Export_ForceTable(fname: str, FTable: MDForceTable, rho_unit: float = 1.0):
+    """Write ``fname.pair`` and ``fname.embd`` in the reference's column layout: r, r*V, -r*dV/dr,
+    RHO, -dRHO/dr per table id (POTR is stored as 0.5*V*r, so the r*V column is 2*POTR)."""
+    ids = sorted(FTable.FPAIR)
+    with open(fname + ".pair", "w") as fh:
+        fh.write("&MDPSCU_POTTAB.Pair\n")
+        fh.write(f'&POTTYPE "{FTable.PotType.strip()}"\n')
+        fh.write(f"&NUMTABLE {len(ids):7d} table IDs: " + "".join(f"{i:5d}" for i in ids) + "\n")
+        fh.write(f"&NUMPOINT {FTable.NTAB:7d}\n")
+        for it in range(1, FTable.NTAB + 1):
+            t = it * FTable.CSIV
+            r = t * t
+            cols = []
+            for tid in ids:
+                k = FTable.FPAIR.index(tid)
+                cols += [FTable.POTR[k, it - 1] * 2.0, FTable.FPOTR[k, it - 1],
+                         FTable.POTB[k, it - 1] * rho_unit, FTable.FPOTB[k, it - 1] * rho_unit]
+            fh.write(f" {it:6d} " + "".join(f"{v:21.9E}" for v in [r] + cols) + "\n")
+    with open(fname + ".embd", "w") as fh:
+        fh.write("&MDPSCU_POTTAB.Embd\n")
+        fh.write(f'&POTTYPE "{FTable.PotType.strip()}"\n')
+        ids1 = sorted(FTable.FPAIR1) if FTable.FPAIR1 else [1]
+        fh.write(f"&NUMTABLE {len(ids1):7d} table IDs: " + "".join(f"{i:5d}" for i in ids1) + "\n")
+        fh.write(f"&NUMPOINT {FTable.NEMBD:7d}\n")
+        for it in range(1, FTable.NEMBD + 1):
+            rho = (it - 1) * FTable.RHOD
+            cols = []
+            for tid in ids1:
+                if FTable.PotType.strip() == "FS_TYPE" or FTable.FEMBD is None:
+                    f = -np.sqrt(rho) if rho > 0 else 0.0
+                    df = -0.5 / np.sqrt(rho) if rho > 0 else 0.0
+                else:
+                    k = FTable.FPAIR1.index(tid)
+                    f, df = FTable.FEMBD[k, it - 1], FTable.DFEMBD[k, it - 1]
+                cols += [f, df]
+            fh.write(f" {it:6d} " + "".join(f"{v:16.8E}" for v in [rho] + cols) + "\n")
+
+
+def read_table_file(path: str):
+    """Read a .pair or .embd table (reference layout): returns (header dict, ndarray rows x cols)
+    with the leading row index dropped."""
+    head = {}
+    rows = []
+    with open(path) as fh:
+        for line in fh:
+            s = line.strip()
+            if not s or s.startswith("!"):
+                continue
+            if s.startswith("&"):
+                parts = s.split()
+                head[parts[0].upper()] = parts[1:]
+                continue
+            rows.append([float(x) for x in s.split()[1:]])
+    return head, np.array(rows, dtype=np.float64)

@@ -1,0 +1,291 @@
+"""ctypes front end of the CPU oracle (oracle/liborcmd.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg, always as the checker.  The product package (msmpscu_amd) never imports it.
+
+Every function takes and returns numpy arrays in the oracle's conventions (0-based groups and
+neighbour ids, XP(N,3) in Fortran order, padded kind-major tables); see orc_md.h.
+
+Parity status of each restated function (how it is pinned):
+  orc_table_pairwise   pinned: reference-exported table fixture (tests/golden), closed forms
+  orc_table_embd       pinned: reference-exported .embd fixture (grid and F column)
+  orc_nlist_half/full  pinned: brute-force O(N^2) list, half <-> full symmetry, lattice shells
+  orc_force_fs_half    pinned: perfect-lattice F = 0, sum F = 0, F = -dE/dx (central FD),
+                       analytic lattice energy, full-list (device-kernel) restatement agreement
+  orc_force_full       pinned: as above, plus FS == EAM with F = -sqrt(rho) tabulated
+  orc_predictor/corr.  pinned: closed-form free flight and constant-force motion, NVE drift
+  orc_drand32          pinned: exact modular arithmetic of L'Ecuyer's generator
+  orc_thermalize_mc    pinned: zero net momentum, temperature statistics
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liborcmd.so")
+
+STATU_ACTIVE = 1
+STATU_OUTOFBOX = 65536
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+PAIRFN = C.CFUNCTYPE(None, _dp, _dp, _dp)
+
+
+class _FTable(C.Structure):
+    _fields_ = [("pot_type", C.c_int), ("ngroup", C.c_int), ("nkind", C.c_int), ("ntab", C.c_int),
+                ("csi", C.c_double), ("kpair", _ip),
+                ("potr", _dp), ("fpotr", _dp), ("potb", _dp), ("fpotb", _dp),
+                ("nkind1", C.c_int), ("nembd", C.c_int), ("rhod", C.c_double), ("kembd", _ip),
+                ("fembd", _dp), ("dfembd", _dp)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        L.orc_drand32.restype = C.c_double
+        L.orc_sum.restype = C.c_double
+        L.orc_sum.argtypes = [_dp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def fn(name: str):
+    """C function pointer of an oracle potential (e.g. "orc_pot_wfs_nn")."""
+    return C.cast(getattr(lib(), name), C.c_void_p)
+
+
+class Table:
+    """Padded kind-major oracle table: arrays of shape (nkind, ntab+2), pads zero."""
+
+    def __init__(self, pot_type: int, ngroup: int, nkind: int, ntab: int, rmax: float,
+                 kpair: np.ndarray, nkind1: int = 0, nembd: int = 0):
+        self.pot_type = pot_type
+        self.ngroup = ngroup
+        self.nkind = nkind
+        self.ntab = ntab
+        self.rmax = rmax
+        self.csi = float(ntab) / np.sqrt(rmax)
+        self.kpair = np.ascontiguousarray(kpair, dtype=np.int32)  # [ti, tj] 0-based kinds
+        self.potr = np.zeros((nkind, ntab + 2))
+        self.fpotr = np.zeros_like(self.potr)
+        self.potb = np.zeros_like(self.potr)
+        self.fpotb = np.zeros_like(self.potr)
+        self.nkind1 = nkind1
+        self.nembd = nembd
+        self.rhod = 1.0
+        self.rhomx = -1.0
+        self.kembd = np.zeros(max(ngroup, 1), dtype=np.int32)
+        self.fembd = np.zeros((max(nkind1, 1), nembd + 2))
+        self.dfembd = np.zeros_like(self.fembd)
+
+    def fill_pair(self, k: int, nn: str, ne: Optional[str], rhomult: float = 20.0):
+        rh = C.c_double(self.rhomx)
+        lib().orc_table_pairwise(C.c_int(self.ntab), C.c_double(self.rmax), fn(nn),
+                                 fn(ne) if ne else None, C.c_double(rhomult), C.byref(rh),
+                                 _d(self.potr[k]), _d(self.fpotr[k]), _d(self.potb[k]),
+                                 _d(self.fpotb[k]))
+        self.rhomx = rh.value
+
+    def fill_embd(self, k: int, emb: str, rhod: float):
+        self.rhod = rhod
+        lib().orc_table_embd(C.c_int(self.nembd), C.c_double(rhod), fn(emb), _d(self.fembd[k]),
+                             _d(self.dfembd[k]))
+
+    def struct(self) -> _FTable:
+        s = _FTable()
+        s.pot_type = self.pot_type
+        s.ngroup = self.ngroup
+        s.nkind = self.nkind
+        s.ntab = self.ntab
+        s.csi = self.csi
+        s.kpair = _i(self.kpair)
+        s.potr, s.fpotr, s.potb, s.fpotb = (_d(self.potr), _d(self.fpotr), _d(self.potb),
+                                            _d(self.fpotb))
+        s.nkind1 = self.nkind1
+        s.nembd = self.nembd
+        s.rhod = self.rhod
+        s.kembd = _i(self.kembd)
+        s.fembd, s.dfembd = _d(self.fembd), _d(self.dfembd)
+        self._keep = s
+        return s
+
+
+def from_mdlib(ft) -> Table:
+    """Oracle table holding the same numbers as a product MDForceTable (1-based KPAIR/KEMBD)."""
+    ng = ft.KPAIR.shape[0]
+    pt = 1 if ft.PotType.strip() == "EAM_TYPE" else 0
+    t = Table(pt, ng, ft.NKIND, ft.NTAB, ft.Rmax, np.asarray(ft.KPAIR) - 1,
+              ft.NKIND1 if pt else 0, ft.NEMBD if pt else 0)
+    t.csi = ft.CSI
+    for k in range(ft.NKIND):
+        t.potr[k, 1:-1] = ft.POTR[k]
+        t.fpotr[k, 1:-1] = ft.FPOTR[k]
+        t.potb[k, 1:-1] = ft.POTB[k]
+        t.fpotb[k, 1:-1] = ft.FPOTB[k]
+    if pt:
+        t.rhod = ft.RHOD
+        t.kembd = np.ascontiguousarray(np.asarray(ft.KEMBD) - 1, dtype=np.int32)
+        for k in range(ft.NKIND1):
+            t.fembd[k, 1:-1] = ft.FEMBD[k]
+            t.dfembd[k, 1:-1] = ft.DFEMBD[k]
+    return t
+
+
+def _xp(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel(order="F"))
+
+
+def _v3(a, dtype=np.float64):
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype).reshape(3))
+
+
+def nlist(kind: str, xp, ityp0, statu, lo, ln, ifpd, nb_rm, mx):
+    """kind in {"half", "full", "brute"}; returns (kvois, indi[N, mx]) with 0-based ids."""
+    n = int(np.asarray(ityp0).shape[0])
+    x = _xp(xp)
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    st = np.ascontiguousarray(statu, dtype=np.int32)
+    nb = np.ascontiguousarray(np.asarray(nb_rm, dtype=np.float64))
+    ng = int(round(np.sqrt(nb.size)))
+    kv = np.zeros(n, dtype=np.int32)
+    ind = np.zeros(n * mx, dtype=np.int32)
+    if kind == "brute":
+        rc = lib().orc_nlist_brute(n, _d(x), _i(it), _i(st), _d(_v3(ln)), _i(_v3(ifpd, np.int32)),
+                                   ng, _d(nb), mx, _i(kv), _i(ind))
+    else:
+        f = lib().orc_nlist_half if kind == "half" else lib().orc_nlist_full
+        rc = f(n, _d(x), _i(it), _i(st), _d(_v3(lo)), _d(_v3(ln)), _i(_v3(ifpd, np.int32)), ng,
+               _d(nb), mx, _i(kv), _i(ind))
+    if rc != 0:
+        raise RuntimeError(f"oracle nlist({kind}) error {rc}")
+    return kv, ind.reshape(mx, n).T
+
+
+def force_fs_half(xp, ityp0, kvois, indi, ln, ifpd, ru, table: Table):
+    n = int(np.asarray(ityp0).shape[0])
+    mx = indi.shape[1]
+    x = _xp(xp)
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    kv = np.ascontiguousarray(kvois, dtype=np.int32)
+    ind = np.ascontiguousarray(np.asarray(indi, dtype=np.int32).T.ravel())
+    r = np.ascontiguousarray(np.asarray(ru, dtype=np.float64))
+    fp = np.zeros(3 * n)
+    ep = np.zeros(n)
+    den = np.zeros(n)
+    dden = np.zeros(n)
+    vt = np.zeros(9)
+    rc = lib().orc_force_fs_half(n, _d(x), _i(it), _i(kv), _i(ind), mx, _d(_v3(ln)),
+                                 _i(_v3(ifpd, np.int32)), _d(r), C.byref(table.struct()), _d(fp),
+                                 _d(ep), _d(den), _d(dden), _d(vt))
+    if rc != 0:
+        raise RuntimeError(f"oracle force error {rc}")
+    return fp.reshape(3, n).T.copy(), ep, den, dden, vt.reshape(3, 3).T.copy()
+
+
+def force_full(xp, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table: Table):
+    n = int(np.asarray(ityp0).shape[0])
+    mx = indi.shape[1]
+    x = _xp(xp)
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    st = np.ascontiguousarray(statu, dtype=np.int32)
+    kv = np.ascontiguousarray(kvois, dtype=np.int32)
+    ind = np.ascontiguousarray(np.asarray(indi, dtype=np.int32).T.ravel())
+    fp = np.zeros(3 * n)
+    dden = np.zeros(n)
+    ep = np.zeros(n)
+    rc = lib().orc_force_full(n, _d(x), _i(it), _i(st), _i(kv), _i(ind), mx, _d(_v3(ln)),
+                              _i(_v3(ifpd, np.int32)), C.c_double(ru2max),
+                              C.byref(table.struct()), _d(fp), _d(dden), _d(ep))
+    if rc != 0:
+        raise RuntimeError(f"oracle force_full error {rc}")
+    return fp.reshape(3, n).T.copy(), dden, ep
+
+
+def predictor(xp, xp1, fp, dis, statu, ityp0, cm, h, lo, ln, ifpd):
+    n = int(np.asarray(ityp0).shape[0])
+    x, v, f, d = _xp(xp), _xp(xp1), _xp(fp), _xp(dis)
+    st = np.ascontiguousarray(statu, dtype=np.int32).copy()
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    c = np.ascontiguousarray(cm, dtype=np.float64)
+    lib().orc_predictor(n, _d(x), _d(v), _d(f), _d(d), _i(st), _i(it), _d(c), C.c_double(h),
+                        _d(_v3(lo)), _d(_v3(ln)), _i(_v3(ifpd, np.int32)))
+    r = lambda a: a.reshape(3, n).T.copy()  # noqa: E731
+    return r(x), r(v), r(d), st
+
+
+def correction(xp1, fp, statu, ityp0, cm, h):
+    n = int(np.asarray(ityp0).shape[0])
+    v, f = _xp(xp1), _xp(fp)
+    st = np.ascontiguousarray(statu, dtype=np.int32)
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    c = np.ascontiguousarray(cm, dtype=np.float64)
+    ek = np.zeros(n)
+    lib().orc_correction(n, _d(v), _d(f), _i(st), _i(it), _d(c), C.c_double(h), _d(ek))
+    return v.reshape(3, n).T.copy(), ek
+
+
+def md_run(xp, xp1, statu, ityp0, cm, h, nsteps, nb_uptab, step0, lo, ln, ifpd, nb_rm, ru, mx,
+           table: Table, fp=None, init=True):
+    """Half-list FS MD with the reference's step order; returns (xp, xp1, fp, dis, statu, epot,
+    kvois, indi)."""
+    n = int(np.asarray(ityp0).shape[0])
+    x, v = _xp(xp), _xp(xp1)
+    f = _xp(fp) if fp is not None else np.zeros(3 * n)
+    d = np.zeros(3 * n)
+    st = np.ascontiguousarray(statu, dtype=np.int32).copy()
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    c = np.ascontiguousarray(cm, dtype=np.float64)
+    nb = np.ascontiguousarray(np.asarray(nb_rm, dtype=np.float64))
+    r = np.ascontiguousarray(np.asarray(ru, dtype=np.float64))
+    kv = np.zeros(n, dtype=np.int32)
+    ind = np.zeros(n * mx, dtype=np.int32)
+    ep = np.zeros(n)
+    rc = lib().orc_md_run(n, _d(x), _d(v), _d(f), _d(d), _i(st), _i(it), _d(c), C.c_double(h),
+                          nsteps, nb_uptab, step0, 1 if init else 0, _d(_v3(lo)), _d(_v3(ln)),
+                          _i(_v3(ifpd, np.int32)), _d(nb), _d(r), mx, C.byref(table.struct()),
+                          _i(kv), _i(ind), _d(ep))
+    if rc != 0:
+        raise RuntimeError(f"oracle md_run error {rc}")
+    rs = lambda a: a.reshape(3, n).T.copy()  # noqa: E731
+    return rs(x), rs(v), rs(f), rs(d), st, ep, kv, ind.reshape(mx, n).T
+
+
+def drand32(seed):
+    s = (C.c_int * 2)(*seed)
+    v = lib().orc_drand32(s)
+    return v, (s[0], s[1])
+
+
+def thermalize(ityp0, cm, temp, kb, seed):
+    n = int(np.asarray(ityp0).shape[0])
+    s = (C.c_int * 2)(*seed)
+    v = np.zeros(3 * n)
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    c = np.ascontiguousarray(cm, dtype=np.float64)
+    lib().orc_thermalize_mc(n, _i(it), _d(c), len(c), C.c_double(temp), C.c_double(kb),
+                            C.cast(s, _ip), _d(v))
+    return v.reshape(3, n).T.copy()
+
+
+def tree_sum(a) -> float:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return lib().orc_sum(_d(a), int(a.size))

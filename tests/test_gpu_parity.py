@@ -1,0 +1,332 @@
+"""GPU parity of the HIP path (through the C-ABI) against the CPU oracle on the same seeded
+inputs.  Bars: neighbour lists bit-exact as sorted sets; per-atom forces within 1e-10 eV/A
+(max-abs); total energy within 1e-12 relative; integrator within rounding of the reference
+formulas."""
+import sys
+
+import numpy as np
+import pytest
+
+import msmpscu_amd.configs as cfg
+import msmpscu_amd.mdlib as md
+from msmpscu_amd._lib import MSMError
+
+pytestmark = pytest.mark.gpu
+
+F_TOL = 1e-10      # eV/A, max-abs per atom and component
+E_REL = 1e-12      # relative, total energy
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def device_for(boxes, ctrl, ft):
+    boxes = boxes if isinstance(boxes, list) else [boxes]
+    fc = md.MDForceClassGPU(ForceTable=ft)
+    md.Register_ForceClass(ft.PotType, fc)
+    md.Initialize_Globle_Variables_DEV(boxes, ctrl, fc)
+    md.Init_Forcetable_Dev(boxes, ctrl, fc)
+    return fc.Device
+
+
+def oracle_half(orc, box, ctrl, ft, xp=None):
+    xp = box.XP if xp is None else xp
+    t = orc.from_mdlib(ft)
+    it = box.ITYP - 1
+    kv, ind = orc.nlist("half", xp, it, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD, ctrl.NB_RM,
+                        ctrl.NB_MXNBS)
+    return orc.force_fs_half(xp, it, kv, ind, box.ZL, ctrl.IFPD, ctrl.RU, t)
+
+
+def oracle_full(orc, box, ctrl, ft, xp=None, statu=None):
+    xp = box.XP if xp is None else xp
+    st = box.STATU if statu is None else statu
+    t = orc.from_mdlib(ft)
+    it = box.ITYP - 1
+    kv, ind = orc.nlist("full", xp, it, st, box.BOXLOW, box.ZL, ctrl.IFPD, ctrl.NB_RM,
+                        ctrl.NB_MXNBS)
+    return orc.force_full(xp, it, st, kv, ind, box.ZL, ctrl.IFPD, float(np.max(ctrl.RU)) ** 2, t)
+
+
+def pairset(kv, ind, one_based):
+    s = set()
+    off = 1 if one_based else 0
+    for i in range(len(kv)):
+        for j in ind[i, : kv[i]]:
+            s.add((i, int(j) - off))
+    return s
+
+
+@pytest.fixture(scope="module")
+def p0():
+    box = cfg.make_bcc_box((16, 16, 16), cfg.W_A, 0.03, 300.0, 2024)
+    ctrl = cfg.w_ctrl(mx=64)
+    ft = cfg.w_forcetable(box, ctrl)
+    return box, ctrl, ft
+
+
+@pytest.mark.parametrize("case", ["p0", "disordered"])
+def test_neighbour_list_bit_exact(orc, p0, case):
+    if case == "p0":
+        box, ctrl, ft = p0
+    else:
+        box = cfg.make_bcc_box((10, 10, 10), cfg.W_A, 0.15, 900.0, 77, n_vac=5, n_dumbbell=3)
+        ctrl = cfg.w_ctrl(mx=64)
+        ft = cfg.w_forcetable(box, ctrl)
+    say(f"nlist {case} n={box.NPRT}")
+    dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    kv, ind = dev.get_neighbors()
+    okv, oind = orc.nlist("full", box.XP, box.ITYP - 1, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
+                          ctrl.NB_RM, ctrl.NB_MXNBS)
+    assert np.array_equal(kv, okv)
+    assert pairset(kv, ind, True) == pairset(okv, oind, False)
+    tot, mk = dev.list_stats()
+    assert tot == int(okv.sum()) and mk == int(okv.max())
+    dev.close()
+
+
+@pytest.mark.parametrize("pot_type", ["FS_TYPE", "EAM_TYPE"])
+def test_force_energy_vs_oracle(orc, p0, pot_type):
+    box, ctrl, _ = p0
+    ft = cfg.w_forcetable(box, ctrl, pot_type=pot_type)
+    say(f"force {pot_type} n={box.NPRT}")
+    dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    dev.calc_force()
+    dev.calc_epot()
+    _, _, fp, _, _ = dev.get_atoms()
+    den = dev.get_den()
+    ep = dev.get_epot()
+    ebox = dev.get_box_epot()
+    fo, do, eo = oracle_full(orc, box, ctrl, ft)
+    assert np.max(np.abs(fp - fo)) <= F_TOL
+    assert np.max(np.abs(den - do)) <= 1e-13
+    assert abs(ep.sum() - eo.sum()) <= E_REL * abs(eo.sum())
+    assert abs(ebox[0] - orc.tree_sum(eo)) <= E_REL * abs(eo.sum())
+    if pot_type == "FS_TYPE":
+        fh, eh, _, ddh, _ = oracle_half(orc, box, ctrl, ft)
+        assert np.max(np.abs(fp - fh)) <= F_TOL
+        assert np.max(np.abs(den - ddh)) <= 1e-13
+        assert abs(ebox[0] - orc.tree_sum(eh)) <= E_REL * abs(eh.sum())
+    dev.close()
+
+
+def test_two_species_slab(orc):
+    box = cfg.make_cuni_slab((6, 6, 6), 3.57, 600.0, 5)
+    rng = np.random.default_rng(1)
+    box.XP = np.asfortranarray(box.XP + rng.normal(0, 0.05, box.XP.shape))
+    box.XP[:, :2] = cfg.wrap_into(box.XP, box.BOXLOW, box.ZL, (1, 1, 0))[:, :2]
+    ctrl = cfg.cuni_ctrl()
+    ft = cfg.cuni_forcetable(box, ctrl)
+    say(f"cu-ni slab n={box.NPRT}")
+    dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    kv, ind = dev.get_neighbors()
+    okv, oind = orc.nlist("full", box.XP, box.ITYP - 1, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
+                          ctrl.NB_RM, ctrl.NB_MXNBS)
+    assert pairset(kv, ind, True) == pairset(okv, oind, False)
+    dev.calc_force()
+    dev.calc_epot()
+    _, _, fp, _, _ = dev.get_atoms()
+    fh, eh, _, ddh, _ = oracle_half(orc, box, ctrl, ft)
+    assert np.max(np.abs(fp - fh)) <= F_TOL
+    assert abs(dev.get_epot().sum() - eh.sum()) <= E_REL * abs(eh.sum())
+    dev.close()
+
+
+def test_asymmetric_density_channels(orc):
+    """KPAIR(i,j) != KPAIR(j,i) with different density tables: checks the KTAB/KTAB1 pairing."""
+    box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.08, 300.0, 8)
+    box.NGROUP = 2
+    box.ITYP = (np.arange(box.NPRT) % 2 + 1).astype(np.int32)
+    box.CM = np.array([box.CM[0], box.CM[0] * 0.5])
+    box.PTYPE = np.array([[1, 3], [4, 2]], dtype=np.int32)
+    ctrl = md.SimMDCtrl(RU=np.full((2, 2), 4.4002), NB_RM=np.full((2, 2), 4.7002), NB_MXNBS=64)
+    ft = md.MDForceTable()
+    md.Register_ForceTableProc(1, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne")
+    md.Register_ForceTableProc(2, ft, "msm_pot_ackland_ww_nn", "msm_pot_ackland_ww_ne")
+    md.Register_ForceTableProc(3, ft, "msm_pot_wfs_nn", "msm_pot_ackland_ww_ne")
+    md.Register_ForceTableProc(4, ft, lambda r: (0.0, 0.0), "msm_pot_wfs_ne")
+    md.Create_Interaction_ForceTable(box, ctrl, ft)
+    say("asymmetric kpair")
+    dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    dev.calc_force()
+    _, _, fp, _, _ = dev.get_atoms()
+    fh, eh, _, ddh, _ = oracle_half(orc, box, ctrl, ft)
+    assert np.max(np.abs(fp - fh)) <= F_TOL
+    assert np.max(np.abs(dev.get_den() - ddh)) <= 1e-13
+    dev.close()
+
+
+def test_inactive_atoms(orc, p0):
+    box, ctrl, ft = p0
+    st = box.STATU.copy()
+    st[[3, 100, 4000]] = 0
+    say("inactive atoms")
+    dev = device_for(box, ctrl, ft)
+    dev.set_atoms(box.XP, box.XP1, box.ITYP, st)
+    dev.build_neighbors()
+    dev.calc_force()
+    dev.calc_epot()
+    _, _, fp, _, _ = dev.get_atoms()
+    fo, do, eo = oracle_full(orc, box, ctrl, ft, statu=st)
+    assert np.max(np.abs(fp - fo)) <= F_TOL
+    assert np.all(fp[[3, 100, 4000]] == 0)
+    assert abs(dev.get_epot().sum() - eo.sum()) <= E_REL * abs(eo.sum())
+    dev.close()
+
+
+def test_integrator_kernels(orc, p0):
+    box, ctrl, ft = p0
+    say("integrator")
+    dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    dev.calc_force()
+    x0, v0, f0, st0, d0 = dev.get_atoms()
+    dev.predictor(ctrl.H)
+    x1, v1, _, st1, d1 = dev.get_atoms()
+    ox, ov, od, ost = orc.predictor(x0, v0, f0, d0, st0, box.ITYP - 1, box.CM, ctrl.H,
+                                    box.BOXLOW, box.ZL, ctrl.IFPD)
+    assert np.max(np.abs(x1 - ox)) <= 1e-13
+    assert np.max(np.abs(v1 - ov)) <= 1e-16
+    assert np.max(np.abs(d1 - od)) <= 1e-13
+    assert np.array_equal(st1 & 0xFFFF, ost & 0xFFFF)
+    dev.correction(ctrl.H)
+    _, v2, _, st2, _ = dev.get_atoms()
+    ov2, _ = orc.correction(v1, f0, st1, box.ITYP - 1, box.CM, ctrl.H)
+    assert np.max(np.abs(v2 - ov2)) <= 1e-16
+    dev.close()
+
+
+def test_md_run_matches_oracle(orc, p0):
+    """100 NVE steps, rebuild every 10 (the 8192-atom W benchmark case)."""
+    box, ctrl, ft = p0
+    say("md run 100 steps")
+    dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    dev.calc_force()
+    dev.run(100, ctrl.NB_UPTAB, 0, ctrl.H)
+    dev.calc_epot()
+    x, v, fp, st, dis = dev.get_atoms()
+    ep = dev.get_epot()
+    t = orc.from_mdlib(ft)
+    ox, ov, of, od, ost, oep, _, _ = orc.md_run(box.XP, box.XP1, box.STATU, box.ITYP - 1, box.CM,
+                                                ctrl.H, 100, ctrl.NB_UPTAB, 0, box.BOXLOW, box.ZL,
+                                                ctrl.IFPD, ctrl.NB_RM, ctrl.RU, ctrl.NB_MXNBS, t)
+    dx = x - ox
+    dx -= np.round(dx / box.ZL) * box.ZL
+    assert np.max(np.abs(dx)) <= 1e-9
+    assert np.max(np.abs(v - ov)) <= 1e-11
+    assert np.max(np.abs(fp - of)) <= 1e-8
+    assert np.max(np.abs(dis - od)) <= 1e-9
+    # per-snapshot parity: the oracle evaluated on the GPU's own final positions
+    fh, eh, *_ = oracle_half(orc, box, ctrl, ft, xp=x)
+    assert np.max(np.abs(fp - fh)) <= F_TOL
+    assert abs(ep.sum() - eh.sum()) <= E_REL * abs(eh.sum())
+    # NVE: energy drift small
+    ek0 = 0.5 * box.CM[0] * np.sum(box.XP1 ** 2)
+    dev2 = device_for(box, ctrl, ft)
+    dev2.build_neighbors()
+    dev2.calc_force()
+    dev2.calc_epot()
+    e0 = dev2.get_epot().sum() + ek0
+    e1 = ep.sum() + 0.5 * box.CM[0] * np.sum(v ** 2)
+    assert abs(e1 - e0) <= 1e-4 * abs(e0)
+    dev.close()
+    dev2.close()
+
+
+def test_run_is_deterministic(p0):
+    box, ctrl, ft = p0
+    say("determinism")
+    outs = []
+    for _ in range(2):
+        dev = device_for(box, ctrl, ft)
+        dev.build_neighbors()
+        dev.calc_force()
+        dev.run(30, ctrl.NB_UPTAB, 0, ctrl.H)
+        outs.append(dev.get_atoms())
+        dev.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_uneven_multibox_equals_single_boxes(p0):
+    _, ctrl, _ = p0
+    boxes = [cfg.make_bcc_box(c, cfg.W_A, 0.05, 500.0, 10 + k, n_vac=k, n_dumbbell=k % 2)
+             for k, c in enumerate([(5, 6, 7), (9, 9, 9), (6, 4, 5)])]
+    ft = cfg.w_forcetable(boxes[0], ctrl)
+    say("multibox")
+    dev = device_for(boxes, ctrl, ft)
+    dev.build_neighbors()
+    dev.calc_force()
+    dev.run(25, ctrl.NB_UPTAB, 0, ctrl.H)
+    dev.calc_epot()
+    xb, vb, fb, _, _ = dev.get_atoms()
+    eb = dev.get_box_epot()
+    dev.close()
+    off = 0
+    for k, b in enumerate(boxes):
+        d1 = device_for(b, ctrl, ft)
+        d1.build_neighbors()
+        d1.calc_force()
+        d1.run(25, ctrl.NB_UPTAB, 0, ctrl.H)
+        d1.calc_epot()
+        x1, v1, f1, _, _ = d1.get_atoms()
+        s = slice(off, off + b.NPRT)
+        assert np.array_equal(xb[s], x1) and np.array_equal(fb[s], f1)
+        assert eb[k] == d1.get_box_epot()[0]
+        off += b.NPRT
+        d1.close()
+
+
+def test_errors_are_reported_not_fatal(p0):
+    box, ctrl, ft = p0
+    say("error paths")
+    small = cfg.w_ctrl(mx=8)
+    dev = device_for(box, small, ft)
+    with pytest.raises(MSMError) as ei:
+        dev.build_neighbors()
+    assert ei.value.code == 4
+    dev.close()
+    b2 = cfg.make_bcc_box((6, 6, 6), cfg.W_A, 0.0, 0.0, 1)
+    b2.XP[0, 2] = b2.BOXLOW[2] + b2.ZL[2] + 3.0
+    c2 = cfg.w_ctrl()
+    c2.IFPD = (1, 1, 0)
+    dev = device_for(b2, c2, cfg.w_forcetable(b2, c2))
+    with pytest.raises(MSMError) as ei:
+        dev.build_neighbors()
+    assert ei.value.code == 5
+    dev.close()
+
+
+def test_million_atom_snapshot(orc):
+    """The 80^3 BCC benchmark box (0.5% vacancies): list sets, forces and energy against the
+    oracle at step 0 and after 20 steps (oracle on the GPU's snapshot)."""
+    box = cfg.make_bcc_box((80, 80, 80), cfg.W_A, 0.05, 600.0, 1000, vacancy_frac=0.005)
+    ctrl = cfg.w_ctrl(mx=64)
+    ft = cfg.w_forcetable(box, ctrl)
+    say(f"1M snapshot n={box.NPRT}")
+    dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    dev.calc_force()
+    dev.calc_epot()
+    kv, _ = dev.get_neighbors()
+    _, _, fp, _, _ = dev.get_atoms()
+    fh, eh, *_ = oracle_half(orc, box, ctrl, ft)
+    assert np.max(np.abs(fp - fh)) <= F_TOL
+    assert abs(dev.get_epot().sum() - eh.sum()) <= E_REL * abs(eh.sum())
+    okv, _ = orc.nlist("full", box.XP, box.ITYP - 1, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
+                       ctrl.NB_RM, ctrl.NB_MXNBS)
+    assert np.array_equal(kv, okv)
+    dev.run(20, ctrl.NB_UPTAB, 0, ctrl.H)
+    dev.calc_epot()
+    x, v, fp, _, _ = dev.get_atoms()
+    fh, eh, *_ = oracle_half(orc, box, ctrl, ft, xp=x)
+    assert np.max(np.abs(fp - fh)) <= F_TOL
+    assert abs(dev.get_epot().sum() - eh.sum()) <= E_REL * abs(eh.sum())
+    dev.close()

@@ -1,0 +1,350 @@
+"""Pins of the CPU oracle against the reference's own exported tables, closed forms, brute force
+and invariants chosen so that a plausible slip (wrong grid, dropped factor 1/2, sign, swapped
+KTAB/KTAB1, wrong image shift) fails at least one of them.  CPU only."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def load_rows(name):
+    head = {}
+    rows = []
+    with open(os.path.join(GOLD, name)) as fh:
+        for line in fh:
+            if line.startswith("#"):
+                p = line[1:].split()
+                if len(p) == 2:
+                    head[p[0]] = float(p[1])
+                continue
+            rows.append([float(x) for x in line.split()])
+    return head, np.array(rows)
+
+
+def close(a, b, rel, absol=0.0):
+    return np.all(np.abs(a - b) <= rel * np.abs(b) + absol)
+
+
+# ---------------------------------------------------------------------------- tables ----------
+
+def test_pair_table_matches_reference_export(orc):
+    """Ackland-Thetford W-W table, NTAB=10000, Rmax=10 A, against the reference's .pair export
+    (10 significant digits)."""
+    head, rows = load_rows("w_ackland_pair_rows.txt")
+    t = orc.Table(0, 1, 1, 10000, 10.0, np.zeros((1, 1)))
+    t.fill_pair(0, "orc_pot_ackland_ww_nn", "orc_pot_ackland_ww_ne")
+    idx = rows[:, 0].astype(int)
+    r = (idx / t.csi) ** 2
+    assert close(r, rows[:, 1], 2e-9)
+    scale = np.max(np.abs(rows[:, 2:]), axis=0)
+    assert close(2.0 * t.potr[0, idx], rows[:, 2], 2e-9, 1e-12 * scale[0])
+    assert close(t.fpotr[0, idx], rows[:, 3], 2e-9, 1e-12 * scale[1])
+    assert close(t.potb[0, idx], rows[:, 4], 2e-9, 1e-12 * scale[2])
+    assert close(t.fpotb[0, idx], rows[:, 5], 2e-9, 1e-12 * scale[3])
+    # the pads beyond the grid are zero
+    assert t.potr[0, 0] == 0 and t.potr[0, -1] == 0
+
+
+def test_embedding_grid_matches_reference_export(orc):
+    """RHOD = RHOSCAL * max POTB / NEMBD with the maximum over all tables (table #5 row 1), grid
+    rho_i = (i-1) RHOD, F = -sqrt(rho)."""
+    phead, _ = load_rows("w_ackland_pair_rows.txt")
+    head, rows = load_rows("w_fs_embd_rows.txt")
+    rhod = 20.0 * phead["RHO5_row1"] / 10000
+    t = orc.Table(1, 1, 1, 10, 1.0, np.zeros((1, 1)), nkind1=1, nembd=10000)
+    t.fill_embd(0, "orc_emb_fs", rhod)
+    idx = rows[:, 0].astype(int)
+    assert close((idx - 1) * rhod, rows[:, 1], 2e-8, 1e-300)
+    assert close(t.fembd[0, idx], rows[:, 2], 2e-8, 1e-12)
+
+
+def test_rhomx_tracks_the_running_maximum(orc):
+    t = orc.Table(0, 1, 2, 2000, 4.4002, np.zeros((1, 1)))
+    t.fill_pair(0, "orc_pot_wfs_nn", "orc_pot_wfs_ne")
+    first = t.rhomx
+    assert first == pytest.approx(20.0 * t.potb[0, 1:-1].max(), rel=1e-15)
+    t.fill_pair(1, "orc_pot_ackland_ww_nn", "orc_pot_ackland_ww_ne")
+    assert t.rhomx == pytest.approx(20.0 * max(t.potb[0].max(), t.potb[1].max()), rel=1e-15)
+
+
+# ---------------------------------------------------------------------- neighbour lists ---------
+
+def bcc(n, a, sigma, seed):
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(*(np.arange(n),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = np.concatenate([g, g + 0.5]) * a
+    ln = np.full(3, n * a)
+    lo = -0.5 * ln
+    x = x + lo + rng.normal(0, sigma, x.shape)
+    x = lo + np.mod(x - lo, ln)
+    return x, lo, ln
+
+
+def pairs(kv, ind):
+    s = set()
+    for i in range(len(kv)):
+        for j in ind[i, : kv[i]]:
+            s.add((i, int(j)))
+    return s
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.1])
+def test_cell_lists_equal_brute_force(orc, sigma):
+    x, lo, ln = bcc(6, 3.165, sigma, 7)
+    n = len(x)
+    it = np.zeros(n, np.int32)
+    st = np.ones(n, np.int32)
+    rnl = 4.7002
+    kvh, inh = orc.nlist("half", x, it, st, lo, ln, (1, 1, 1), [[rnl]], 64)
+    kvf, inf = orc.nlist("full", x, it, st, lo, ln, (1, 1, 1), [[rnl]], 64)
+    kvb, inb = orc.nlist("brute", x, it, st, lo, ln, (1, 1, 1), [[rnl]], 64)
+    full, brute = pairs(kvf, inf), pairs(kvb, inb)
+    half = pairs(kvh, inh)
+    # half + mirrored half == full; every unordered pair appears once in the half list
+    assert half | {(j, i) for (i, j) in half} == full
+    assert len(half) * 2 == len(full)
+    # cell list == brute force except pairs within rounding of the cutoff (none here)
+    assert full == brute
+    if sigma == 0.0:
+        assert np.all(kvf == 26) and kvh.sum() == 13 * n
+
+
+def test_neighbour_overflow_and_out_of_box_are_errors(orc):
+    x, lo, ln = bcc(4, 3.165, 0.0, 1)
+    n = len(x)
+    it = np.zeros(n, np.int32)
+    st = np.ones(n, np.int32)
+    with pytest.raises(RuntimeError):
+        orc.nlist("full", x, it, st, lo, ln, (1, 1, 1), [[4.7002]], 10)
+    x2 = x.copy()
+    x2[0, 2] = lo[2] + ln[2] + 5.0
+    with pytest.raises(RuntimeError):
+        orc.nlist("full", x2, it, st, lo, ln, (1, 1, 0), [[4.7002]], 64)
+    # flagged out-of-box atoms are skipped instead
+    st2 = st.copy()
+    st2[0] = orc.STATU_OUTOFBOX
+    kv, _ = orc.nlist("full", x2, it, st2, lo, ln, (1, 1, 0), [[4.7002]], 64)
+    assert kv[0] == 0
+
+
+# ------------------------------------------------------------------------------- forces ---------
+
+def w_table(orc, ntab, pot_type=0, nembd=0):
+    t = orc.Table(pot_type, 1, 1, ntab, 4.4002, np.zeros((1, 1)), nkind1=1 if pot_type else 0,
+                  nembd=nembd)
+    t.fill_pair(0, "orc_pot_wfs_nn", "orc_pot_wfs_ne")
+    if pot_type:
+        t.fill_embd(0, "orc_emb_fs", t.rhomx / nembd)
+    return t
+
+
+def fs_eval(orc, x, lo, ln, t, ifpd=(1, 1, 1), ru=4.4002, rnl=4.7002, it=None):
+    n = len(x)
+    it = np.zeros(n, np.int32) if it is None else it
+    st = np.ones(n, np.int32)
+    kv, ind = orc.nlist("half", x, it, st, lo, ln, ifpd, np.full((t.ngroup, t.ngroup), rnl), 96)
+    return orc.force_fs_half(x, it, kv, ind, ln, ifpd, np.full((t.ngroup, t.ngroup), ru), t)
+
+
+def w_closed(r):
+    v = np.where(r <= 3.25, (r - 3.25) ** 2 * (47.1346 - 33.7666 * r + 6.2542 * r * r), 0.0)
+    rho = np.where(r <= 4.4002, 1.8964 ** 2 * (r - 4.4002) ** 2, 0.0)
+    return v, rho
+
+
+def test_perfect_lattice_energy_and_zero_force(orc):
+    """Perfect BCC: F = 0 and E/atom = 1/2 sum_s n_s V(r_s) - sqrt(sum_s n_s rho(r_s))."""
+    a = 3.165
+    x, lo, ln = bcc(5, a, 0.0, 0)
+    t = w_table(orc, 200000)
+    fp, ep, den, dden, vt = fs_eval(orc, x, lo, ln, t)
+    assert np.max(np.abs(fp)) < 1e-9
+    shells = [(a * math.sqrt(3) / 2, 8), (a, 6), (a * math.sqrt(2), 12)]
+    r = np.array([s[0] for s in shells])
+    nsh = np.array([s[1] for s in shells])
+    v, rho = w_closed(r)
+    e_atom = 0.5 * np.sum(nsh * v) - math.sqrt(np.sum(nsh * rho))
+    assert np.allclose(ep, e_atom, rtol=1e-8)
+    assert np.allclose(den, np.sum(nsh * rho), rtol=1e-8)
+    assert np.allclose(dden, -0.5 / np.sqrt(np.sum(nsh * rho)), rtol=1e-8)
+
+
+def test_forces_are_minus_energy_gradient(orc):
+    x, lo, ln = bcc(4, 3.165, 0.12, 3)
+    t = w_table(orc, 400000)
+    fp, ep, *_ = fs_eval(orc, x, lo, ln, t)
+    assert np.max(np.abs(fp.sum(axis=0))) < 1e-10
+    h = 1e-4
+    for i in (0, 17, 63):
+        for k in range(3):
+            xp = x.copy()
+            xp[i, k] += h
+            xm = x.copy()
+            xm[i, k] -= h
+            ep_p = fs_eval(orc, xp, lo, ln, t)[1].sum()
+            ep_m = fs_eval(orc, xm, lo, ln, t)[1].sum()
+            fd = -(ep_p - ep_m) / (2 * h)
+            assert fd == pytest.approx(fp[i, k], rel=2e-5, abs=2e-5)
+
+
+def alloy_table(orc, ntab):
+    """Two groups with asymmetric density channels: KPAIR = [[0, 2], [3, 1]] (TGT<-SOR), so the
+    KTAB / KTAB1 pairing of the force matters."""
+    kpair = np.array([[0, 2], [3, 1]])
+    t = orc.Table(0, 2, 4, ntab, 4.4002, kpair)
+    t.fill_pair(0, "orc_pot_wfs_nn", "orc_pot_wfs_ne")
+    t.fill_pair(1, "orc_pot_ackland_ww_nn", "orc_pot_ackland_ww_ne")
+    t.fill_pair(2, "orc_pot_wfs_nn", "orc_pot_ackland_ww_ne")
+    t.fill_pair(3, "orc_pot_wfs_nn", "orc_pot_wfs_ne")
+    # make channel 3 differ from channel 2 so that rho(i<-j) != rho(j<-i)
+    t.potb[3] *= 0.6
+    t.fpotb[3] *= 0.6
+    return t
+
+
+def test_two_species_forces_are_minus_energy_gradient(orc):
+    x, lo, ln = bcc(4, 3.165, 0.1, 5)
+    it = (np.arange(len(x)) % 2).astype(np.int32)
+    t = alloy_table(orc, 400000)
+    fp, ep, *_ = fs_eval(orc, x, lo, ln, t, it=it)
+    assert np.max(np.abs(fp.sum(axis=0))) < 1e-10
+    h = 1e-4
+    for i in (1, 30):
+        for k in range(3):
+            xp = x.copy()
+            xp[i, k] += h
+            xm = x.copy()
+            xm[i, k] -= h
+            fd = -(fs_eval(orc, xp, lo, ln, t, it=it)[1].sum()
+                   - fs_eval(orc, xm, lo, ln, t, it=it)[1].sum()) / (2 * h)
+            assert fd == pytest.approx(fp[i, k], rel=2e-5, abs=2e-5)
+
+
+def test_half_list_equals_device_kernel_restatement(orc):
+    """CALFORCE_FS_Force_Table (half list, Newton) == PRECALFOR/CALFORCE/CALEPOT (full list)."""
+    x, lo, ln = bcc(6, 3.165, 0.08, 11)
+    n = len(x)
+    t = w_table(orc, 5000)
+    it = np.zeros(n, np.int32)
+    st = np.ones(n, np.int32)
+    fp, ep, den, dden, vt = fs_eval(orc, x, lo, ln, t)
+    kv, ind = orc.nlist("full", x, it, st, lo, ln, (1, 1, 1), [[4.7002]], 64)
+    fp2, dd2, ep2 = orc.force_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t)
+    assert np.max(np.abs(fp - fp2)) < 1e-12
+    assert np.max(np.abs(dden - dd2)) < 1e-15
+    assert abs(ep.sum() - ep2.sum()) < 1e-12 * abs(ep.sum())
+
+
+def test_eam_with_sqrt_embedding_matches_fs(orc):
+    x, lo, ln = bcc(5, 3.165, 0.05, 2)
+    n = len(x)
+    it = np.zeros(n, np.int32)
+    st = np.ones(n, np.int32)
+    fs = w_table(orc, 5000)
+    eam = w_table(orc, 5000, pot_type=1, nembd=20000)
+    kv, ind = orc.nlist("full", x, it, st, lo, ln, (1, 1, 1), [[4.7002]], 64)
+    f1, d1, e1 = orc.force_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, fs)
+    f2, d2, e2 = orc.force_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, eam)
+    assert np.allclose(d1, d2, rtol=1e-5)
+    assert np.allclose(e1, e2, rtol=1e-6)
+    assert np.max(np.abs(f1 - f2)) < 1e-4
+
+
+def test_inactive_atoms_follow_device_kernel_semantics(orc):
+    x, lo, ln = bcc(4, 3.165, 0.05, 4)
+    n = len(x)
+    it = np.zeros(n, np.int32)
+    st = np.ones(n, np.int32)
+    st[5] = 0
+    t = w_table(orc, 5000)
+    kv, ind = orc.nlist("full", x, it, st, lo, ln, (1, 1, 1), [[4.7002]], 64)
+    fp, dd, ep = orc.force_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t)
+    assert np.all(fp[5] == 0) and dd[5] == 0 and ep[5] == 0
+    assert np.any(fp[ind[5, :kv[5]]] != 0)
+
+
+# --------------------------------------------------------------------------- integrator ---------
+
+def test_integrator_constant_force_closed_form(orc):
+    n = 10
+    rng = np.random.default_rng(0)
+    x0 = rng.uniform(-1, 1, (n, 3))
+    v0 = rng.normal(0, 0.01, (n, 3))
+    f = rng.normal(0, 0.5, (n, 3))
+    cm = np.array([2.5])
+    it = np.zeros(n, np.int32)
+    st = np.ones(n, np.int32)
+    lo, ln = np.full(3, -5000.0), np.full(3, 10000.0)
+    h = 0.7
+    x, v, d = x0.copy(), v0.copy(), np.zeros((n, 3))
+    nsteps = 25
+    for _ in range(nsteps):
+        x, v, d, st = orc.predictor(x, v, f, d, st, it, cm, h, lo, ln, (1, 1, 1))
+        v, _ = orc.correction(v, f, st, it, cm, h)
+    tt = nsteps * h
+    assert np.allclose(x, x0 + v0 * tt + 0.5 * f / cm[0] * tt * tt, rtol=1e-12, atol=1e-12)
+    assert np.allclose(v, v0 + f / cm[0] * tt, rtol=1e-12, atol=1e-14)
+    assert np.allclose(d, x - x0, atol=1e-12)
+
+
+def test_integrator_wraps_and_flags(orc):
+    x = np.array([[4.99, 0.0, 0.0], [0.0, 0.0, 4.99]])
+    v = np.array([[0.1, 0, 0], [0, 0, 0.1]])
+    f = np.zeros((2, 3))
+    st = np.ones(2, np.int32)
+    x2, v2, d2, st2 = orc.predictor(x, v, f, np.zeros((2, 3)), st, np.zeros(2, np.int32),
+                                    np.array([1.0]), 1.0, np.full(3, -5.0), np.full(3, 10.0),
+                                    (1, 1, 0))
+    assert x2[0, 0] == pytest.approx(-4.91)
+    assert st2[0] & 1
+    assert st2[1] == 65536 | 131072  # OUTOFBOX | REFLECT, no longer active
+
+
+def test_nve_energy_is_conserved(orc):
+    """100 fs of NVE on 432 W atoms at 600 K: |E(t) - E(0)| / |E| small (checks step order,
+    half kicks and rebuild cadence together)."""
+    import msmpscu_amd.configs as cfg
+    box = cfg.make_bcc_box((6, 6, 6), 3.165, 0.05, 600.0, 9)
+    t = w_table(orc, 5000)
+    it = box.ITYP - 1
+    res0 = orc.md_run(box.XP, box.XP1, box.STATU, it, box.CM, 1.0, 0, 10, 0, box.BOXLOW, box.ZL,
+                      (1, 1, 1), [[4.7002]], [[4.4002]], 64, t)
+    ek0 = 0.5 * box.CM[0] * np.sum(box.XP1 ** 2)
+    e0 = res0[5].sum() + ek0
+    x, v, fp, d, st, ep, kv, ind = orc.md_run(box.XP, box.XP1, box.STATU, it, box.CM, 1.0, 100,
+                                              10, 0, box.BOXLOW, box.ZL, (1, 1, 1), [[4.7002]],
+                                              [[4.4002]], 64, t)
+    e1 = ep.sum() + 0.5 * box.CM[0] * np.sum(v ** 2)
+    assert abs(e1 - e0) < 2e-4 * abs(e0)
+    assert np.max(np.abs(d)) < 0.5
+
+
+# --------------------------------------------------------------------------------- RNG ----------
+
+def test_drand32_is_lecuyer_combined_generator(orc):
+    s = (12345, 67890)
+    m1, m2 = 2147483563, 2147483399
+    a, b = s
+    for _ in range(2000):
+        val, s = orc.drand32(s)
+        a = (40014 * a) % m1
+        b = (40692 * b) % m2
+        z = a - b
+        if z < 1:
+            z += m1 - 1
+        assert s == (a, b)
+        assert val == z / 2.0 ** 31
+
+
+def test_thermalize_zero_momentum_and_temperature(orc):
+    n = 20000
+    it = np.zeros(n, np.int32)
+    cm = np.array([183.84 * 103.6427])
+    kb = 8.617333262e-5
+    v = orc.thermalize(it, cm, 600.0, kb, (4321, 8765))
+    assert np.max(np.abs(v.sum(axis=0))) < 1e-12 * n
+    temp = cm[0] * np.sum(v ** 2) / (3 * n * kb)
+    assert temp == pytest.approx(600.0, rel=0.03)
