@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing-pass", action="store_true")
+    ap.add_argument("--force-path", type=int, default=None,
+                    help="density/force kernels: 0 LDS bricks, 1/2/4/8 global list with that "
+                         "many lanes per atom (default: library default)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -101,6 +104,9 @@ def main():
     track = not args.no_dis
     _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_TRACK_DIS, 1 if track else 0),
                "msm_set_option")
+    if args.force_path is not None:
+        _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_FORCE_PATH, args.force_path),
+                   "msm_set_option")
     dev.build_neighbors()
     dev.calc_force()
     tot, kmax = dev.list_stats()

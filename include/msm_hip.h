@@ -190,6 +190,12 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
  * HIP events around each launch and accumulates per-class device time. */
 #define MSM_OPT_TRACK_DIS     1
 #define MSM_OPT_KERNEL_TIMING 2
+#define MSM_OPT_NLIST_KERNEL  3  /* 0 (default): LDS-staged cell bricks; 1: per-atom global scan */
+#define MSM_OPT_FORCE_PATH    4  /* density/force passes: 0 LDS bricks; 1, 2, 4, 8: global list
+                                    with that many lanes per atom (default 2) */
+#define MSM_OPT_COMPACT_LIST  5  /* 1 (default): on the global-list path with <= 4 lanes per atom,
+                                    the density pass records the pairs inside the force cutoff and
+                                    the force pass visits only those */
 int msm_set_option(msm_sim* sim, int option, int value);
 
 /* Kernel classes for msm_kernel_times. */

@@ -89,6 +89,9 @@ SIGNATURES = {
 
 MSM_OPT_TRACK_DIS = 1
 MSM_OPT_KERNEL_TIMING = 2
+MSM_OPT_NLIST_KERNEL = 3
+MSM_OPT_FORCE_PATH = 4
+MSM_OPT_COMPACT_LIST = 5
 KCLASS = ["density", "force", "rebuild", "predictor", "nlist"]
 
 _lib = None

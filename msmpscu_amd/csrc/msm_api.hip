@@ -83,6 +83,9 @@ struct msm_sim {
     int sort_cap = 0;
     int* kvois = nullptr;
     int* indi = nullptr;
+    unsigned short* sidx = nullptr;
+    int* cidx = nullptr;  // in-cutoff pairs of the current step (density -> force pass)
+    int* ckv = nullptr;
     size_t indi_cap = 0;
     bool has_list = false;
 
@@ -95,6 +98,13 @@ struct msm_sim {
     // options
     int track_dis = 1;
     int timing = 0;
+    int nlist_global = 0;  // 1: per-atom global-memory list kernel instead of LDS bricks
+    int force_path = 2;    // 0: LDS bricks; 1, 2, 4, 8: global list with that many lanes per atom
+    int compact = 0;       // global-list path: force pass reads the density pass's in-cutoff list
+
+    // LDS-staged list bricks
+    int nbricks = 0;
+    int* d_brick_box = nullptr;
 
     // per-class kernel timing (MSM_OPT_KERNEL_TIMING)
     std::vector<hipEvent_t> evpool;
@@ -184,7 +194,12 @@ ListDev list_view(msm_sim* s) {
     ListDev l{};
     l.kvois = s->kvois;
     l.indi = s->indi;
-    l.stride = s->n_alloc;
+    l.sidx = s->sidx;
+    const bool cmp = s->compact && s->force_path > 0 && s->force_path <= 4;
+    l.cidx = cmp ? s->cidx : nullptr;
+    l.ckv = cmp ? s->ckv : nullptr;
+    l.stride = msm::kListRow;
+    l.mx = s->mx;
     return l;
 }
 
@@ -281,20 +296,98 @@ void enqueue_rebuild(msm_sim* s) {
     msm::launch_gather(s->n, s->n_alloc, s->sb.perm, a_src, s->set[src].gid, a_dst,
                        s->set[dst].gid, s->sb.key, s->set[dst].cell, s->stream);
     TimedScope tl(s, MSM_KCLASS_NLIST);
-    msm::launch_nlist(s->n, s->pos[p1], s->set[dst].ityp, s->set[dst].statu, s->set[dst].boxid,
-                      s->set[dst].cell, s->sb.start, s->d_boxes, s->ngroup, s->d_rnl2, s->mx,
-                      s->kvois, s->indi, s->n_alloc, s->d_err, s->stream);
+    if (s->nlist_global)
+        msm::launch_nlist(s->n, s->pos[p1], s->set[dst].ityp, s->set[dst].statu,
+                          s->set[dst].boxid, s->set[dst].cell, s->sb.start, s->d_boxes, s->ngroup,
+                          s->d_rnl2, s->mx, s->kvois, s->indi, msm::kListRow, s->d_err, s->stream);
+    else
+        msm::launch_nlist_bricks(s->nbricks, s->d_brick_box, s->n, nc, s->pos[p1],
+                                 s->set[dst].ityp, s->sb.start, s->d_boxes, s->ngroup, s->d_rnl2,
+                                 s->mx, s->kvois, s->indi, s->sidx, msm::kListRow, s->d_err,
+                                 s->stream);
     s->scur = dst;
     s->pcur = p1;
 }
 
 bool multibox(const msm_sim* s) { return s->boxes.size() > 1; }
 
+// Brick shape of the LDS-staged list build, per box: the largest brick (at most 4 cells a side)
+// whose halo, at the box's typical occupancy of non-empty cells, fits the LDS capacity.  The
+// occupancy comes from binning the uploaded positions on the host.  Bricks that overflow at run
+// time fall back to the global-memory scan, so the choice affects speed only.
+int plan_bricks(msm_sim* s, const double* xp, int n) {
+    std::vector<int> brick_box;
+    int off = 0, aoff = 0;
+    for (size_t bidx = 0; bidx < s->boxes.size(); ++bidx) {
+        BoxDev& b = s->boxes[bidx];
+        const int nc = b.ncell[0] * b.ncell[1] * b.ncell[2];
+        std::vector<int> cnt(nc, 0);
+        for (int i = aoff; i < aoff + b.natoms; ++i) {
+            int c3[3];
+            for (int k = 0; k < 3; ++k) {
+                const double x = xp[i + static_cast<size_t>(k) * n];
+                int c = static_cast<int>((x - b.lo[k]) / b.len[k] * b.ncell[k]);
+                c3[k] = std::min(std::max(c, 0), b.ncell[k] - 1);
+            }
+            ++cnt[c3[0] + b.ncell[0] * (c3[1] + b.ncell[1] * c3[2])];
+        }
+        aoff += b.natoms;
+        long long occ = 0, nonempty = 0;
+        for (int c : cnt)
+            if (c > 0) {
+                occ += c;
+                ++nonempty;
+            }
+        const double est = nonempty ? 1.3 * static_cast<double>(occ) / nonempty + 1.0 : 1.0;
+        int best[3] = {1, 1, 1};
+        int best_home = 0;
+        for (int bz = 1; bz <= 4; ++bz)
+            for (int by = 1; by <= 4; ++by)
+                for (int bx = 1; bx <= 4; ++bx) {
+                    const int e[3] = {std::min(bx, b.ncell[0]), std::min(by, b.ncell[1]),
+                                      std::min(bz, b.ncell[2])};
+                    const int halo = (e[0] + 2) * (e[1] + 2) * (e[2] + 2);
+                    const int home = e[0] * e[1] * e[2];
+                    if (halo * est > msm::kNlCap || home * est > 4 * msm::kBlock) continue;
+                    if (home > best_home || (home == best_home && e[0] > best[0])) {
+                        best_home = home;
+                        best[0] = e[0];
+                        best[1] = e[1];
+                        best[2] = e[2];
+                    }
+                }
+        int nb = 1;
+        for (int k = 0; k < 3; ++k) {
+            b.bdim[k] = best[k];
+            b.nbrick[k] = (b.ncell[k] + best[k] - 1) / best[k];
+            nb *= b.nbrick[k];
+        }
+        b.brick_off = off;
+        for (int k = 0; k < nb; ++k) brick_box.push_back(static_cast<int>(bidx));
+        off += nb;
+    }
+    s->nbricks = off;
+    int rc = dalloc(s, &s->d_brick_box, brick_box.size());
+    if (rc != MSM_OK) return rc;
+    HIPCHK(s, hipMemcpy(s->d_brick_box, brick_box.data(), brick_box.size() * sizeof(int),
+                        hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->d_boxes, s->boxes.data(), s->boxes.size() * sizeof(BoxDev),
+                        hipMemcpyHostToDevice));
+    return MSM_OK;
+}
+
 void enqueue_density(msm_sim* s) {
     TimedScope ts(s, MSM_KCLASS_DENSITY);
     AtomsDev a = atoms_view(s, s->scur, s->pcur);
-    msm::launch_density(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), s->d_err,
-                        s->stream);
+    if (s->force_path > 0)
+        msm::launch_density_tpa(s->force_path, a, list_view(s), s->tab, s->boxes[0], s->d_boxes,
+                                multibox(s), s->d_err, s->stream);
+    else if (s->nlist_global)
+        msm::launch_density(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
+                            s->d_err, s->stream);
+    else
+        msm::launch_density_bricks(s->nbricks, s->d_brick_box, a, list_view(s), s->tab,
+                                   s->d_boxes, s->sb.start, s->ncell_total, s->d_err, s->stream);
 }
 
 // mode 0: force only; 1: force + corrector; 2: force + corrector + next predictor (flips pos).
@@ -308,8 +401,15 @@ void enqueue_force(msm_sim* s, int mode, int write_f, double h) {
     sp.mode = mode;
     sp.write_f = write_f;
     sp.track_dis = s->track_dis;
-    msm::launch_force(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), sp,
-                      s->stream);
+    if (s->force_path > 0)
+        msm::launch_force_tpa(s->force_path, a, list_view(s), s->tab, s->boxes[0], s->d_boxes,
+                              multibox(s), sp, s->stream);
+    else if (s->nlist_global)
+        msm::launch_force(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), sp,
+                          s->stream);
+    else
+        msm::launch_force_bricks(s->nbricks, s->d_brick_box, a, list_view(s), s->tab, s->d_boxes,
+                                 s->sb.start, s->ncell_total, sp, s->stream);
     if (mode == 2) s->pcur = 1 - s->pcur;
 }
 
@@ -431,6 +531,7 @@ int msm_destroy(msm_sim* s) {
     dfree(&s->d_m4);
     dfree(&s->d_rnl2);
     dfree(&s->d_boxes);
+    dfree(&s->d_brick_box);
     dfree(&s->sb.key);
     dfree(&s->sb.slot);
     dfree(&s->sb.count);
@@ -439,6 +540,9 @@ int msm_destroy(msm_sim* s) {
     dfree(&s->sb.scan_tmp);
     dfree(&s->kvois);
     dfree(&s->indi);
+    dfree(&s->sidx);
+    dfree(&s->cidx);
+    dfree(&s->ckv);
     dfree(&s->d_err);
     dfree(&s->d_ebox);
     dfree(&s->d_stats);
@@ -604,6 +708,7 @@ int msm_set_boxes(msm_sim* s, int nbox, const int* natoms, const double* lo, con
             if (!(d.len[k] > 0.0)) return set_err(s, MSM_ERR_ARG, "box length must be positive");
             d.hlen[k] = d.len[k] * 0.5;
             d.ifpd[k] = ifpd[3 * b + k] ? 1 : 0;
+            d.ilen[k] = d.ifpd[k] ? 1.0 / d.len[k] : 0.0;
             // NCELL = max(3, int(L / max(NB_RM) - eps)), MD_NeighborsList.F90:474-483
             int c = static_cast<int>(d.len[k] / (1.0 * rmax) - 0.0001);
             if (c < 3) c = 3;
@@ -670,6 +775,9 @@ int msm_set_atoms(msm_sim* s, int n, const double* xp, const double* xp1, const 
     if (rc == MSM_OK) rc = dalloc(s, &s->kvois, na);
     const size_t icap = static_cast<size_t>(na) * s->mx;
     if (rc == MSM_OK) rc = dalloc(s, &s->indi, icap);
+    if (rc == MSM_OK) rc = dalloc(s, &s->sidx, icap);
+    if (rc == MSM_OK) rc = dalloc(s, &s->cidx, icap);
+    if (rc == MSM_OK) rc = dalloc(s, &s->ckv, static_cast<size_t>(na) * 8);
     if (rc != MSM_OK) {
         free_atoms(s);
         return rc;
@@ -707,6 +815,8 @@ int msm_set_atoms(msm_sim* s, int n, const double* xp, const double* xp1, const 
     HIPCHK(s, hipMemcpy(z.gid, hg.data(), na * sizeof(int), hipMemcpyHostToDevice));
     std::vector<double> hz(na, 0.0);
     HIPCHK(s, hipMemcpy(s->epot, hz.data(), na * sizeof(double), hipMemcpyHostToDevice));
+    rc = plan_bricks(s, xp, n);
+    if (rc != MSM_OK) return rc;
     s->has_atoms = true;
     s->has_list = false;
     return MSM_OK;
@@ -824,7 +934,7 @@ int msm_get_neighbors(msm_sim* s, int* kvois, int* indi, int ld) {
         dfree(&kv);
         return rc;
     }
-    msm::launch_unsort_list(s->kvois, s->indi, s->n_alloc, s->set[s->scur].gid, n, s->mx, kv, ix,
+    msm::launch_unsort_list(s->kvois, s->indi, msm::kListRow, s->set[s->scur].gid, n, s->mx, kv, ix,
                             ld, s->stream);
     hipError_t e = hipMemcpyAsync(kvois, kv, n * sizeof(int), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess)
@@ -923,6 +1033,20 @@ int msm_set_option(msm_sim* s, int option, int value) {
             return MSM_OK;
         case MSM_OPT_KERNEL_TIMING:
             s->timing = value ? 1 : 0;
+            return MSM_OK;
+        case MSM_OPT_NLIST_KERNEL:
+            s->nlist_global = value ? 1 : 0;
+            clear_graphs(s);
+            return MSM_OK;
+        case MSM_OPT_FORCE_PATH:
+            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+                return set_err(s, MSM_ERR_ARG, "force path must be 0, 1, 2, 4 or 8");
+            s->force_path = value;
+            clear_graphs(s);
+            return MSM_OK;
+        case MSM_OPT_COMPACT_LIST:
+            s->compact = value ? 1 : 0;
+            clear_graphs(s);
             return MSM_OK;
         default:
             return set_err(s, MSM_ERR_ARG, "unknown option");

@@ -1,0 +1,562 @@
+// msm_bricks.hip -- LDS-staged kernels over bricks of cells: neighbour-list build, density pass
+// and force pass (with the fused integrator).
+//
+// Atoms are cell-sorted, so each cell is a contiguous index range.  A brick is at most 4x4x4
+// cells of one box; its halo is the brick plus one cell layer (periodic images resolved), at most
+// 6x6x6 = 216 cells.  One 256-thread workgroup per brick copies the halo atoms into LDS with
+// coalesced loads (consecutive threads read consecutive atoms of a cell run) and then every home
+// atom works out of LDS.  The list build records each neighbour both as a caller-visible sorted
+// index (INDI) and as its 16-bit LDS slot in the home brick's halo; the density and force passes
+// rebuild the same halo (cell ranges do not change between rebuilds), read the slot list and
+// never gather from global memory.  A brick whose halo exceeds kNlCap atoms is processed from
+// global memory instead (same results, slower).
+//
+// Reference semantics: list predicate MD_NeighborsList.F90:396-633; density, force, integrator
+// as in msm_kernels.hip (MD_FS/EAM_ForceTable_GPU.F90 PRECALFOR/CALFORCE, MD_DiffScheme_GPU.F90).
+#include "msm_device.h"
+#include "msm_internal.h"
+
+namespace msm {
+
+namespace {
+
+struct BrickInfo {
+    int c0[3], e[3], H[3];
+    int nh, nhc;
+};
+
+__device__ __forceinline__ void brick_info(const BoxDev& B, int bi, BrickInfo& g) {
+    int loc = bi - B.brick_off;
+    const int kx = loc % B.nbrick[0];
+    loc /= B.nbrick[0];
+    const int ky = loc % B.nbrick[1];
+    const int kz = loc / B.nbrick[1];
+    g.c0[0] = kx * B.bdim[0];
+    g.c0[1] = ky * B.bdim[1];
+    g.c0[2] = kz * B.bdim[2];
+    for (int k = 0; k < 3; ++k) {
+        g.e[k] = min(B.bdim[k], B.ncell[k] - g.c0[k]);
+        g.H[k] = g.e[k] + 2;
+    }
+    g.nh = g.H[0] * g.H[1] * g.H[2];
+    g.nhc = g.e[0] * g.e[1] * g.e[2];
+}
+
+// Halo table of a brick (all threads of the workgroup call it): global cell of each halo cell
+// (-1 outside a free boundary), image shift code (-1/0/+1 per direction, when hsh != nullptr),
+// exclusive prefix of atom counts hoff[0..nh] and of home-cell counts homeoff[0..nhc].  Returns
+// the number of halo atoms; LDS tables are valid on return.
+__device__ int brick_halo(const BoxDev& B, const BrickInfo& g, const int* __restrict__ start,
+                          int* hcell, signed char* hsh, int* hoff, int* homeoff, int* scan_lds) {
+    const int tid = threadIdx.x;
+    int cnt = 0;
+    if (tid < g.nh) {
+        const int h3[3] = {tid % g.H[0], (tid / g.H[0]) % g.H[1], tid / (g.H[0] * g.H[1])};
+        int g3[3];
+        bool ok = true;
+        for (int k = 0; k < 3; ++k) {
+            int c = g.c0[k] - 1 + h3[k];
+            signed char sh = 0;
+            if (B.ifpd[k]) {
+                if (c < 0) {
+                    c += B.ncell[k];
+                    sh = -1;
+                } else if (c >= B.ncell[k]) {
+                    c -= B.ncell[k];
+                    sh = 1;
+                }
+            }
+            ok = ok && c >= 0 && c < B.ncell[k];
+            g3[k] = c;
+            if (hsh) hsh[3 * tid + k] = sh;
+        }
+        const int gc = ok ? B.cell_off + g3[0] + B.ncell[0] * (g3[1] + B.ncell[1] * g3[2]) : -1;
+        hcell[tid] = gc;
+        cnt = ok ? start[gc + 1] - start[gc] : 0;
+    }
+    const int incl = block_incl_scan(cnt, scan_lds);
+    if (tid < g.nh) hoff[tid + 1] = incl;
+    if (tid == 0) hoff[0] = 0;
+    __syncthreads();
+    // prefix over the home cells (x fastest), one thread per home cell
+    int hc = 0;
+    if (tid < g.nhc) {
+        const int ix = tid % g.e[0], iy = (tid / g.e[0]) % g.e[1], iz = tid / (g.e[0] * g.e[1]);
+        const int h = (ix + 1) + g.H[0] * ((iy + 1) + g.H[1] * (iz + 1));
+        hc = hoff[h + 1] - hoff[h];
+    }
+    const int hincl = block_incl_scan(hc, scan_lds);
+    if (tid < g.nhc) homeoff[tid + 1] = hincl;
+    if (tid == 0) homeoff[0] = 0;
+    __syncthreads();
+    return hoff[g.nh];
+}
+
+// Visit every halo slot s of a staged brick with its sorted atom index q and halo cell h.  Halo
+// rows (fixed y, z) are contiguous runs of slots; waves take rows round-robin and lanes take
+// consecutive slots, so the global loads of a row are coalesced, and a lane finds its cell by a
+// short forward scan within the row (at most 6 cells).
+template <typename F>
+__device__ __forceinline__ void for_halo_slots(const BrickInfo& g, const int* __restrict__ start,
+                                               const int* hcell, const int* hoff, F&& f) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nrows = g.H[1] * g.H[2];
+    for (int r = wave; r < nrows; r += kBlock / 64) {
+        const int h0 = r * g.H[0];
+        const int s1 = hoff[h0 + g.H[0]];
+        for (int s = hoff[h0] + lane; s < s1; s += 64) {
+            int h = h0;
+            while (hoff[h + 1] <= s) ++h;
+            f(s, start[hcell[h]] + (s - hoff[h]), h);
+        }
+    }
+}
+
+// Home atom number t of the brick -> halo cell index h and offset within that cell.
+__device__ __forceinline__ void home_atom(int t, const BrickInfo& g, const int* homeoff, int& h,
+                                          int& off) {
+    int lo = 0, hi = g.nhc;  // homeoff[lo] <= t < homeoff[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (homeoff[mid] <= t) lo = mid; else hi = mid;
+    }
+    const int ix = lo % g.e[0], iy = (lo / g.e[0]) % g.e[1], iz = lo / (g.e[0] * g.e[1]);
+    h = (ix + 1) + g.H[0] * ((iy + 1) + g.H[1] * (iz + 1));
+    off = t - homeoff[lo];
+}
+
+// Neighbours processed per group in the density and force passes.
+constexpr int kUnroll = 8;
+
+__device__ __forceinline__ double shift_of(signed char c, double len) {
+    return c ? (c > 0 ? len : -len) : 0.0;
+}
+
+struct BrickLds {
+    int hoff[kNlMaxHalo + 1];
+    int hcell[kNlMaxHalo];
+    int homeoff[65];
+    int scan[kBlock / 64];
+};
+
+}  // namespace
+
+// --------------------------------------------------------------------------------------------
+// list build
+// --------------------------------------------------------------------------------------------
+
+template <int MT>
+__global__ __launch_bounds__(kBlock) void k_nlist_bricks(
+    const int* __restrict__ brick_box, const double4* __restrict__ pos,
+    const int* __restrict__ ityp, const int* __restrict__ start,
+    const BoxDev* __restrict__ boxes, int ngroup, const double* __restrict__ rnl2, int mx,
+    int* __restrict__ kvois, int* __restrict__ indi, unsigned short* __restrict__ sidx,
+    int stride, int* err) {
+#pragma clang fp contract(off)
+    __shared__ double lx[kNlCap], ly[kNlCap], lz[kNlCap];
+    __shared__ int lg[kNlCap];
+    __shared__ int lt[MT ? kNlCap : 1];
+    __shared__ signed char hsh[kNlMaxHalo * 3];
+    __shared__ double srnl2[kMaxGroup * kMaxGroup];
+    __shared__ BrickLds L;
+
+    const int tid = threadIdx.x;
+    const int bi = xcd_block(gridDim.x);
+    const BoxDev& B = boxes[brick_box[bi]];
+    BrickInfo g;
+    brick_info(B, bi, g);
+    if constexpr (MT)
+        for (int k = tid; k < ngroup * ngroup; k += kBlock) srnl2[k] = rnl2[k];
+    const int total = brick_halo(B, g, start, L.hcell, hsh, L.hoff, L.homeoff, L.scan);
+    const bool staged = total <= kNlCap;
+    if (staged) {
+        for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int h) {
+            const double4 pq = pos[q];
+            // the image shift is added once, as XP(J) + CXYZ in MD_NeighborsList.F90:582
+            lx[s] = pq.x + shift_of(hsh[3 * h], B.len[0]);
+            ly[s] = pq.y + shift_of(hsh[3 * h + 1], B.len[1]);
+            lz[s] = pq.z + shift_of(hsh[3 * h + 2], B.len[2]);
+            lg[s] = q;
+            if constexpr (MT) lt[s] = ityp[q];
+        });
+    }
+    __syncthreads();
+    const int nhome = L.homeoff[g.nhc];
+    const double rsingle = rnl2[0];
+    for (int t = tid; t < nhome; t += kBlock) {
+        int h, off;
+        home_atom(t, g, L.homeoff, h, off);
+        if (!staged) {
+            const int p = start[L.hcell[h]] + off;
+            nlist_atom_global(p, L.hcell[h] - B.cell_off, B, pos, ityp, start, ngroup, rnl2, mx,
+                              kvois, indi, stride, err);
+            continue;
+        }
+        const int si = L.hoff[h] + off;
+        const int p = lg[si];
+        const double xi = lx[si], yi = ly[si], zi = lz[si];
+        const int ti = MT ? lt[si] : 0;
+        int c = 0;
+        for (int s = 0; s < 27; ++s) {
+            const int hn = h + c_nix[s] + g.H[0] * (c_niy[s] + g.H[1] * c_niz[s]);
+            const int qe = L.hoff[hn + 1];
+            // candidates in groups of four (LDS loads of a group issued together), taken in order
+            for (int q0 = L.hoff[hn]; q0 < qe; q0 += 4) {
+                bool hit[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const bool valid = q0 + u < qe && q0 + u != si;
+                    const int q = valid ? q0 + u : si;
+                    // XPT(I) - (XP(J) + CXYZ), squared and summed left to right, no contraction
+                    const double dx = xi - lx[q];
+                    const double dy = yi - ly[q];
+                    const double dz = zi - lz[q];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const double lim = MT ? srnl2[ti * ngroup + lt[q]] : rsingle;
+                    hit[u] = valid && r2 < lim;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (hit[u]) {
+                        if (c < mx) {
+                            const size_t a = list_base(p, mx) + static_cast<size_t>(c) * stride;
+                            indi[a] = lg[q0 + u];
+                            sidx[a] = static_cast<unsigned short>(q0 + u);
+                        }
+                        ++c;
+                    }
+                }
+            }
+        }
+        if (c > mx) {
+            atomicOr(&err[kErrOverflow], 1);
+            c = mx;
+        }
+        kvois[p] = c;
+    }
+}
+
+__global__ void k_zero_range(int* __restrict__ p, const int* __restrict__ first, int n) {
+    const int i = *first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void launch_nlist_bricks(int nbricks, const int* brick_box, int n, int ncell_total,
+                         const double4* pos, const int* ityp, const int* start,
+                         const BoxDev* boxes, int ngroup, const double* rnl2, int mx, int* kvois,
+                         int* indi, unsigned short* sidx, int stride, int* err, hipStream_t s) {
+    if (n <= 0) return;
+    // atoms in the out-of-box bucket have no neighbours
+    hipLaunchKernelGGL(k_zero_range, dim3(grid_for(n)), dim3(kBlock), 0, s, kvois,
+                       start + ncell_total, n);
+    if (nbricks <= 0) return;
+    if (ngroup > 1)
+        hipLaunchKernelGGL(k_nlist_bricks<1>, dim3(nbricks), dim3(kBlock), 0, s, brick_box, pos,
+                           ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx, stride, err);
+    else
+        hipLaunchKernelGGL(k_nlist_bricks<0>, dim3(nbricks), dim3(kBlock), 0, s, brick_box, pos,
+                           ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx, stride, err);
+}
+
+// --------------------------------------------------------------------------------------------
+// density pass
+// --------------------------------------------------------------------------------------------
+
+template <int MT, int EAM>
+__global__ __launch_bounds__(kBlock) void k_density_bricks(const int* __restrict__ brick_box,
+                                                           AtomsDev a, ListDev l, TabDev t,
+                                                           const BoxDev* __restrict__ boxes,
+                                                           const int* __restrict__ start,
+                                                           int* err) {
+    __shared__ double lx[kNlCap], ly[kNlCap], lz[kNlCap];
+    __shared__ int lt[MT ? kNlCap : 1];
+    __shared__ BrickLds L;
+
+    const int tid = threadIdx.x;
+    const int bi = xcd_block(gridDim.x);
+    const BoxDev& B = boxes[brick_box[bi]];
+    BrickInfo g;
+    brick_info(B, bi, g);
+    const int total = brick_halo(B, g, start, L.hcell, nullptr, L.hoff, L.homeoff, L.scan);
+    const bool staged = total <= kNlCap;
+    if (staged) {
+        for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int) {
+            // x, y, z only: w (F') of other atoms is being written by this pass
+            const double2 xy = *reinterpret_cast<const double2*>(&a.pos[q]);
+            lx[s] = xy.x;
+            ly[s] = xy.y;
+            lz[s] = a.pos[q].z;
+            if constexpr (MT) lt[s] = a.ityp[q];
+        });
+    }
+    __syncthreads();
+    const int nhome = L.homeoff[g.nhc];
+    for (int tt = tid; tt < nhome; tt += kBlock) {
+        int h, off;
+        home_atom(tt, g, L.homeoff, h, off);
+        const int p = start[L.hcell[h]] + off;
+        const int si = L.hoff[h] + off;
+        double den0 = 0.0;
+        if ((a.statu[p] & STATU_ACTIVE) == STATU_ACTIVE) {
+            const int ti = MT ? a.ityp[p] : 0;
+            const int kv = l.kvois[p];
+            if (staged) {
+                // Neighbours in groups of kUnroll: the group's slot loads are issued together
+                // (one memory latency per group), and the body is branch-free -- the table
+                // index is clamped for pairs beyond the cutoff and their term is selected away
+                // -- so the group's table loads are in flight together as well.  Terms are
+                // still added in list order.
+                const double xi = lx[si], yi = ly[si], zi = lz[si];
+                const unsigned short* __restrict__ li = l.sidx + list_base(p, l.mx);
+                const int kmax = t.ntab2 - 1;
+                for (int base = 0; base < kv; base += kUnroll) {
+                    int qs[kUnroll];
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u)
+                        qs[u] = (base + u < kv) ? li[static_cast<size_t>(base + u) * l.stride] : si;
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        const int q = qs[u];
+                        const double sx = min_image(xi - lx[q], B.len[0], B.ilen[0]);
+                        const double sy = min_image(yi - ly[q], B.len[1], B.ilen[1]);
+                        const double sz = min_image(zi - lz[q], B.len[2], B.ilen[2]);
+                        const double r2 = sx * sx + sy * sy + sz * sz;
+                        const int kind = MT ? t.kpair[ti * t.ngroup + lt[q]] : t.kpair[0];
+                        double inv_r;
+                        const double sk = table_coord(r2, t.csi, inv_r);
+                        const int kk = min(static_cast<int>(sk), kmax);
+                        const double2 e = t.b2[kind * t.ntab2 + kk];
+                        const double v = e.x + (sk - kk) * e.y;
+                        den0 = den0 + ((base + u < kv && r2 <= t.ru2max) ? v : 0.0);
+                    }
+                }
+            } else {
+                const double4 pi = a.pos[p];
+                const int* __restrict__ li = l.indi + list_base(p, l.mx);
+                for (int iw = 0; iw < kv; ++iw) {
+                    const int j = li[static_cast<size_t>(iw) * l.stride];
+                    const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
+                    const double qz = a.pos[j].z;
+                    const double sx = min_image(pi.x - qxy.x, B.len[0], B.ilen[0]);
+                    const double sy = min_image(pi.y - qxy.y, B.len[1], B.ilen[1]);
+                    const double sz = min_image(pi.z - qz, B.len[2], B.ilen[2]);
+                    const double r2 = sx * sx + sy * sy + sz * sz;
+                    if (r2 <= t.ru2max) {
+                        const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
+                        double inv_r;
+                        const double sk = table_coord(r2, t.csi, inv_r);
+                        const int kk = static_cast<int>(sk);
+                        const double2 e = t.b2[kind * t.ntab2 + kk];
+                        den0 = den0 + (e.x + (sk - kk) * e.y);
+                    }
+                }
+            }
+            if constexpr (!EAM) {
+                if (den0 > 0.0) den0 = -0.5 / sqrt(den0);
+            } else {
+                if (den0 > 0.0) {
+                    const double sk = den0 / t.rhod + 1.0;
+                    int kk = static_cast<int>(sk + 0.000001);
+                    if (kk > t.nembd) {
+                        atomicOr(&err[kErrEmbd], 1);
+                        kk = t.nembd;
+                    }
+                    const double4 m = t.m4[t.kembd[ti] * t.nembd2 + kk];
+                    den0 = m.z + (sk - kk) * m.w;
+                }
+            }
+        }
+        a.pos[p].w = den0;
+    }
+}
+
+// --------------------------------------------------------------------------------------------
+// force pass (+ corrector of this step and predictor of the next)
+// --------------------------------------------------------------------------------------------
+
+template <int MT, int MODE>
+__global__ __launch_bounds__(kBlock) void k_force_bricks(const int* __restrict__ brick_box,
+                                                         AtomsDev a, ListDev l, TabDev t,
+                                                         const BoxDev* __restrict__ boxes,
+                                                         const int* __restrict__ start,
+                                                         StepParams sp) {
+    __shared__ double4 lp[kNlCap];
+    __shared__ int lt[MT ? kNlCap : 1];
+    __shared__ BrickLds L;
+
+    const int tid = threadIdx.x;
+    const int bi = xcd_block(gridDim.x);
+    const BoxDev& B = boxes[brick_box[bi]];
+    BrickInfo g;
+    brick_info(B, bi, g);
+    const int total = brick_halo(B, g, start, L.hcell, nullptr, L.hoff, L.homeoff, L.scan);
+    const bool staged = total <= kNlCap;
+    if (staged) {
+        for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int) {
+            lp[s] = a.pos[q];  // x, y, z and F'
+            if constexpr (MT) lt[s] = a.ityp[q];
+        });
+    }
+    __syncthreads();
+    const int nhome = L.homeoff[g.nhc];
+    for (int tt = tid; tt < nhome; tt += kBlock) {
+        int h, off;
+        home_atom(tt, g, L.homeoff, h, off);
+        const int p = start[L.hcell[h]] + off;
+        const int st = a.statu[p];
+        const bool active = (st & STATU_ACTIVE) == STATU_ACTIVE;
+        const double4 pi = staged ? lp[L.hoff[h] + off] : a.pos[p];
+        const int ti = MT ? a.ityp[p] : 0;
+        double fpx = 0.0, fpy = 0.0, fpz = 0.0;
+        if (active) {
+            const double denki = pi.w;
+            const int kv = l.kvois[p];
+            if (staged) {
+                // grouped, branch-free neighbour loop (see the density pass)
+                const unsigned short* __restrict__ li = l.sidx + list_base(p, l.mx);
+                const int si = L.hoff[h] + off;
+                const int kmax = t.ntab2 - 1;
+                for (int base = 0; base < kv; base += kUnroll) {
+                    int qs[kUnroll];
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u)
+                        qs[u] = (base + u < kv) ? li[static_cast<size_t>(base + u) * l.stride] : si;
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        const int q = qs[u];
+                        const double4 pj = lp[q];
+                        const double sx = min_image(pi.x - pj.x, B.len[0], B.ilen[0]);
+                        const double sy = min_image(pi.y - pj.y, B.len[1], B.ilen[1]);
+                        const double sz = min_image(pi.z - pj.z, B.len[2], B.ilen[2]);
+                        const double r2 = sx * sx + sy * sy + sz * sz;
+                        int kind, kind1;
+                        if constexpr (MT) {
+                            const int tj = lt[q];
+                            kind = t.kpair[ti * t.ngroup + tj];
+                            kind1 = t.kpair[tj * t.ngroup + ti];
+                        } else {
+                            kind = kind1 = t.kpair[0];
+                        }
+                        double inv_r;
+                        const double sk = table_coord(r2, t.csi, inv_r);
+                        const int kk = min(static_cast<int>(sk), kmax);
+                        const double dk = sk - kk;
+                        const double4 f = t.f4[kind * t.ntab2 + kk];
+                        double fb1;
+                        if constexpr (MT) {
+                            const double4 f1 = t.f4[kind1 * t.ntab2 + kk];
+                            fb1 = f1.z + dk * f1.w;
+                        } else {
+                            fb1 = f.z + dk * f.w;
+                        }
+                        const double ft = (f.x + dk * f.y) * (inv_r * inv_r) +
+                                          ((f.z + dk * f.w) * denki + fb1 * pj.w) * inv_r;
+                        const double fortot = (base + u < kv && r2 <= t.ru2max) ? ft : 0.0;
+                        fpx = fpx + fortot * sx;
+                        fpy = fpy + fortot * sy;
+                        fpz = fpz + fortot * sz;
+                    }
+                }
+            } else
+            for (int iw = 0; iw < kv; ++iw) {
+                const size_t a_iw = list_base(p, l.mx) + static_cast<size_t>(iw) * l.stride;
+                const int j = l.indi[a_iw];
+                const double4 pj = a.pos[j];
+                int tj = 0;
+                if constexpr (MT) tj = a.ityp[j];
+                const double sx = min_image(pi.x - pj.x, B.len[0], B.ilen[0]);
+                const double sy = min_image(pi.y - pj.y, B.len[1], B.ilen[1]);
+                const double sz = min_image(pi.z - pj.z, B.len[2], B.ilen[2]);
+                const double r2 = sx * sx + sy * sy + sz * sz;
+                if (r2 <= t.ru2max) {
+                    int kind, kind1;
+                    if constexpr (MT) {
+                        kind = t.kpair[ti * t.ngroup + tj];
+                        kind1 = t.kpair[tj * t.ngroup + ti];
+                    } else {
+                        kind = kind1 = t.kpair[0];
+                    }
+                    double inv_r;
+                    const double sk = table_coord(r2, t.csi, inv_r);
+                    const int kk = static_cast<int>(sk);
+                    const double dk = sk - kk;
+                    const double4 f = t.f4[kind * t.ntab2 + kk];
+                    double fb1;
+                    if constexpr (MT) {
+                        const double4 f1 = t.f4[kind1 * t.ntab2 + kk];
+                        fb1 = f1.z + dk * f1.w;
+                    } else {
+                        fb1 = f.z + dk * f.w;
+                    }
+                    // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL)
+                    const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
+                                          ((f.z + dk * f.w) * denki + fb1 * pj.w) * inv_r;
+                    fpx = fpx + fortot * sx;
+                    fpy = fpy + fortot * sy;
+                    fpz = fpz + fortot * sz;
+                }
+            }
+        }
+        if (sp.write_f) {
+            a.fx[p] = fpx;
+            a.fy[p] = fpy;
+            a.fz[p] = fpz;
+        }
+        if constexpr (MODE >= 1)
+            integrate_atom<MODE>(a, t, B, sp, p, st, active, pi, ti, fpx, fpy, fpz);
+    }
+}
+
+// Atoms of the out-of-box bucket belong to no brick: F' = 0, no force, position carried over.
+__global__ void k_outside_atoms(AtomsDev a, const int* __restrict__ first, int write_f,
+                                int copy_pos, int clear_den) {
+    const int p = *first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    if (clear_den) a.pos[p].w = 0.0;
+    if (write_f) {
+        a.fx[p] = 0.0;
+        a.fy[p] = 0.0;
+        a.fz[p] = 0.0;
+    }
+    if (copy_pos) {
+        const double4 x = a.pos[p];
+        a.pos_out[p] = make_double4(x.x, x.y, x.z, 0.0);
+    }
+}
+
+void launch_density_bricks(int nbricks, const int* brick_box, const AtomsDev& a,
+                           const ListDev& l, const TabDev& t, const BoxDev* boxes,
+                           const int* start, int ncell_total, int* err, hipStream_t s) {
+    if (a.n <= 0) return;
+    hipLaunchKernelGGL(k_outside_atoms, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a,
+                       start + ncell_total, 0, 0, 1);
+    if (nbricks <= 0) return;
+    const dim3 gr(nbricks), bl(kBlock);
+    const bool mt = t.ngroup > 1, eam = t.pot_type != 0;
+#define MSM_DB(MT, E) \
+    hipLaunchKernelGGL((k_density_bricks<MT, E>), gr, bl, 0, s, brick_box, a, l, t, boxes, start, err)
+    if (mt) { if (eam) MSM_DB(1, 1); else MSM_DB(1, 0); }
+    else    { if (eam) MSM_DB(0, 1); else MSM_DB(0, 0); }
+#undef MSM_DB
+}
+
+void launch_force_bricks(int nbricks, const int* brick_box, const AtomsDev& a, const ListDev& l,
+                         const TabDev& t, const BoxDev* boxes, const int* start, int ncell_total,
+                         const StepParams& sp, hipStream_t s) {
+    if (a.n <= 0) return;
+    hipLaunchKernelGGL(k_outside_atoms, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a,
+                       start + ncell_total, sp.write_f, sp.mode == 2 ? 1 : 0, 0);
+    if (nbricks <= 0) return;
+    const dim3 gr(nbricks), bl(kBlock);
+    const bool mt = t.ngroup > 1;
+#define MSM_FB(MT, M) \
+    hipLaunchKernelGGL((k_force_bricks<MT, M>), gr, bl, 0, s, brick_box, a, l, t, boxes, start, sp)
+    if (mt) {
+        if (sp.mode == 0) MSM_FB(1, 0); else if (sp.mode == 1) MSM_FB(1, 1); else MSM_FB(1, 2);
+    } else {
+        if (sp.mode == 0) MSM_FB(0, 0); else if (sp.mode == 1) MSM_FB(0, 1); else MSM_FB(0, 2);
+    }
+#undef MSM_FB
+}
+
+}  // namespace msm

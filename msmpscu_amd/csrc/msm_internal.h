@@ -25,12 +25,19 @@ enum ErrSlot { kErrOverflow = 0, kErrOutOfBox = 1, kErrEmbd = 2, kErrSlots = 4 }
 
 struct BoxDev {
     double lo[3], len[3], hlen[3];
+    double ilen[3];  // 1/len for periodic directions, 0 otherwise (branch-free minimum image)
     int ifpd[3];
     int ncell[3];
     int cell_off;  // first global cell id of this box
     int natoms;    // atoms assigned to this box
-    int pad;
+    int bdim[3];   // cells per brick of the LDS-staged list build
+    int nbrick[3]; // bricks per direction
+    int brick_off; // first global brick id of this box
 };
+
+// LDS-staged list build: halo atoms per workgroup and halo cells per brick (bricks <= 4^3).
+constexpr int kNlCap = 1536;
+constexpr int kNlMaxHalo = 216;
 
 struct TabDev {
     const double2* b2;  // density table {POTB, dPOTB}, nkind x (ntab+2)
@@ -60,9 +67,18 @@ struct AtomsDev {
 
 struct ListDev {
     const int* kvois;
-    const int* indi;
-    int stride;
+    const int* indi;            // neighbour sorted index, slot-major
+    const unsigned short* sidx; // neighbour LDS slot in the home brick's halo (brick kernels)
+    int* cidx;                  // in-cutoff pairs of the current step (global-list path), or null
+    int* ckv;                   // their count per lane (n_alloc * lanes per atom)
+    int stride;                 // distance between consecutive slots of one atom (kListRow)
+    int mx;                     // slots per atom (MXKVOIS)
 };
+
+// Neighbour lists are stored in blocks of kListRow atoms: slot iw of atom p lives at
+// (p / kListRow) * kListRow * MXKVOIS + iw * kListRow + p % kListRow, so the slot loads of a wave
+// are contiguous and all slots of a wave's atoms share one small address range.
+constexpr int kListRow = 64;
 
 struct StepParams {
     double h, hs2, h2s2;
@@ -93,6 +109,25 @@ void launch_nlist(int n, const double4* pos, const int* ityp, const int* statu, 
                   const int* cell, const int* start, const BoxDev* boxes, int ngroup,
                   const double* rnl2, int mx, int* kvois, int* indi, int stride, int* err,
                   hipStream_t s);
+// LDS-staged variants over bricks of cells (msm_bricks.hip); out-of-box atoms (sentinel bucket)
+// get KVOIS = 0, F' = 0, F = 0 and keep their position.
+void launch_nlist_bricks(int nbricks, const int* brick_box, int n, int ncell_total,
+                         const double4* pos, const int* ityp, const int* start,
+                         const BoxDev* boxes, int ngroup, const double* rnl2, int mx, int* kvois,
+                         int* indi, unsigned short* sidx, int stride, int* err, hipStream_t s);
+void launch_density_bricks(int nbricks, const int* brick_box, const AtomsDev& a,
+                           const ListDev& l, const TabDev& t, const BoxDev* boxes,
+                           const int* start, int ncell_total, int* err, hipStream_t s);
+void launch_force_bricks(int nbricks, const int* brick_box, const AtomsDev& a, const ListDev& l,
+                         const TabDev& t, const BoxDev* boxes, const int* start, int ncell_total,
+                         const StepParams& sp, hipStream_t s);
+// Global-list passes with tpa (1, 2, 4 or 8) lanes per atom (msm_tpa.hip).
+void launch_density_tpa(int tpa, const AtomsDev& a, const ListDev& l, const TabDev& t,
+                        const BoxDev& box0, const BoxDev* boxes, int multibox, int* err,
+                        hipStream_t s);
+void launch_force_tpa(int tpa, const AtomsDev& a, const ListDev& l, const TabDev& t,
+                      const BoxDev& box0, const BoxDev* boxes, int multibox, const StepParams& sp,
+                      hipStream_t s);
 void launch_density(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                     const BoxDev* boxes, int multibox, int* err, hipStream_t s);
 void launch_force(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,

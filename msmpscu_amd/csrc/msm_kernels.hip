@@ -15,46 +15,10 @@
 //   energy           MD_FS_ForceTable_GPU.F90:1451-1609, MD_EAM_ForceTable_GPU.F90:1470-1636
 //   integrator       MD_DiffScheme_GPU.F90:242-384 (Predictor_KERNEL0), :686-758 (Correction_KERNEL)
 #include "msm_internal.h"
+#include "msm_device.h"
 
 namespace msm {
 
-namespace {
-
-constexpr int STATU_ACTIVE = 1;
-constexpr int STATU_FIXPOSX = 2, STATU_FIXPOSY = 4, STATU_FIXPOSZ = 8;
-constexpr int STATU_FIXVELX = 16, STATU_FIXVELY = 32, STATU_FIXVELZ = 64;
-constexpr int STATU_OUTOFBOX = 65536, STATU_REFLECT = 131072, STATU_TRANSMIT = 262144,
-              STATU_PASSBOUND = 524288;
-
-// Workgroup id -> logical block so that XCD x (receiving ids x, x+8, ...) gets the contiguous
-// logical range [x*q + min(x,r), ...) of a bijection over [0, nblk).
-__device__ __forceinline__ int xcd_block(int nblk) {
-    const int b = blockIdx.x;
-    const int q = nblk >> 3, r = nblk & 7;
-    const int x = b & 7, k = b >> 3;
-    const int base = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
-    return base + k;
-}
-
-__device__ __forceinline__ int atom_index() {
-    return xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-}
-
-inline unsigned grid_for(int n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
-
-// Minimum image as the device kernels write it: if periodic and |s| > L/2 then s - sign(L, s).
-__device__ __forceinline__ double min_image(double s, double len, double hlen, int pd) {
-    return (pd && fabs(s) > hlen) ? s - copysign(len, s) : s;
-}
-
-template <int MB>
-__device__ __forceinline__ const BoxDev& box_of(const BoxDev& box0, const BoxDev* boxes,
-                                                const int* boxid, int i) {
-    if constexpr (MB) return boxes[boxid[i]];
-    return box0;
-}
-
-}  // namespace
 
 // --------------------------------------------------------------------------------------------
 // utilities
@@ -114,32 +78,6 @@ void launch_cell_assign(const double4* pos, const int* statu, const int* boxid,
 // Exclusive scan of int: block-level inclusive scans (wave64 shuffles + LDS), one-block scan of
 // the block totals, then a fix-up pass.
 constexpr int kScanBlock = 1024;
-
-__device__ __forceinline__ int wave_incl_scan(int v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int t = __shfl_up(v, d, 64);
-        if (lane >= d) v += t;
-    }
-    return v;
-}
-
-__device__ __forceinline__ int block_incl_scan(int v, int* lds) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    v = wave_incl_scan(v);
-    if (lane == 63) lds[wid] = v;
-    __syncthreads();
-    if (wid == 0) {
-        int w = (lane < (int)(blockDim.x >> 6)) ? lds[lane] : 0;
-        w = wave_incl_scan(w);
-        if (lane < (int)(blockDim.x >> 6)) lds[lane] = w;
-    }
-    __syncthreads();
-    if (wid > 0) v += lds[wid - 1];
-    __syncthreads();
-    return v;
-}
 
 __global__ void k_scan_blocks(const int* __restrict__ in, int* __restrict__ out,
                               int* __restrict__ sums, int n) {
@@ -248,21 +186,12 @@ void launch_gather(int n, int n_alloc, const int* perm, const AtomsDev& src, con
                        gid_dst, key, cell_dst);
 }
 
-// Full-stencil order of MD_NeighborsList.F90:427-429 (entry 0 = own cell).
-__constant__ int c_nix[27] = {0, -1, -1, -1, 0, 0, -1, 1, -1, 0, 1, -1, 0, 1,
-                              1, 1, 1, 0, 0, -1, 1, -1, 0, 1, -1, 0, 1};
-__constant__ int c_niy[27] = {0, 0, -1, 1, 1, 0, 0, 0, -1, -1, -1, 1, 1, 1,
-                              0, 1, -1, -1, 0, 0, 0, -1, -1, -1, 1, 1, 1};
-__constant__ int c_niz[27] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1,
-                              0, 0, 0, 0, -1, -1, -1, -1, -1, -1, -1, -1, -1};
-
 __global__ __launch_bounds__(kBlock) void k_nlist(
     int n, const double4* __restrict__ pos, const int* __restrict__ ityp,
     const int* __restrict__ statu, const int* __restrict__ boxid, const int* __restrict__ cell,
     const int* __restrict__ start, const BoxDev* __restrict__ boxes, int ngroup,
     const double* __restrict__ rnl2, int mx, int* __restrict__ kvois, int* __restrict__ indi,
     int stride, int* err) {
-#pragma clang fp contract(off)
     const int p = atom_index();
     if (p >= n) return;
     if ((statu[p] & STATU_OUTOFBOX) == STATU_OUTOFBOX) {
@@ -270,51 +199,8 @@ __global__ __launch_bounds__(kBlock) void k_nlist(
         return;
     }
     const BoxDev& b = boxes[boxid[p]];
-    const int lc = cell[p] - b.cell_off;
-    const int ix = lc % b.ncell[0];
-    const int iy = (lc / b.ncell[0]) % b.ncell[1];
-    const int iz = lc / (b.ncell[0] * b.ncell[1]);
-    const double4 xi = pos[p];
-    const int ti = ityp[p];
-    int cnt = 0;
-    for (int s = 0; s < 27; ++s) {
-        int j3[3] = {ix + c_nix[s], iy + c_niy[s], iz + c_niz[s]};
-        double cx[3] = {0.0, 0.0, 0.0};
-        bool skip = false;
-        for (int k = 0; k < 3; ++k) {
-            if (b.ifpd[k]) {
-                if (j3[k] >= b.ncell[k]) {
-                    j3[k] = 0;
-                    cx[k] = b.len[k];
-                } else if (j3[k] < 0) {
-                    j3[k] = b.ncell[k] - 1;
-                    cx[k] = -b.len[k];
-                }
-            }
-            skip = skip || j3[k] < 0 || j3[k] >= b.ncell[k];
-        }
-        if (skip) continue;
-        const int g = b.cell_off + j3[0] + b.ncell[0] * (j3[1] + b.ncell[1] * j3[2]);
-        const int qe = start[g + 1];
-        for (int q = start[g]; q < qe; ++q) {
-            if (q == p) continue;
-            const double4 xj = pos[q];
-            // XPT(I) - (XP(J) + CXYZ), squared and summed left to right, no contraction
-            const double dx = xi.x - (xj.x + cx[0]);
-            const double dy = xi.y - (xj.y + cx[1]);
-            const double dz = xi.z - (xj.z + cx[2]);
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < rnl2[ti * ngroup + ityp[q]]) {
-                if (cnt < mx) indi[p + static_cast<size_t>(cnt) * stride] = q;
-                ++cnt;
-            }
-        }
-    }
-    if (cnt > mx) {
-        atomicOr(&err[kErrOverflow], 1);
-        cnt = mx;
-    }
-    kvois[p] = cnt;
+    nlist_atom_global(p, cell[p] - b.cell_off, b, pos, ityp, start, ngroup, rnl2, mx, kvois, indi,
+                      stride, err);
 }
 
 void launch_nlist(int n, const double4* pos, const int* ityp, const int* statu, const int* boxid,
@@ -342,19 +228,21 @@ __global__ __launch_bounds__(kBlock) void k_density(AtomsDev a, ListDev l, TabDe
         const double pz = a.pos[i].z;
         const int ti = MT ? a.ityp[i] : 0;
         const int kv = l.kvois[i];
-        const int* __restrict__ li = l.indi + i;
+        const int* __restrict__ li = l.indi + list_base(i, l.mx);
+        int jn = kv > 0 ? li[0] : 0;
         for (int iw = 0; iw < kv; ++iw) {
-            const int j = li[static_cast<size_t>(iw) * l.stride];
+            const int j = jn;
+            if (iw + 1 < kv) jn = li[static_cast<size_t>(iw + 1) * l.stride];
             const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
             const double qz = a.pos[j].z;
-            const double sx = min_image(pxy.x - qxy.x, b.len[0], b.hlen[0], b.ifpd[0]);
-            const double sy = min_image(pxy.y - qxy.y, b.len[1], b.hlen[1], b.ifpd[1]);
-            const double sz = min_image(pz - qz, b.len[2], b.hlen[2], b.ifpd[2]);
+            const double sx = min_image(pxy.x - qxy.x, b.len[0], b.ilen[0]);
+            const double sy = min_image(pxy.y - qxy.y, b.len[1], b.ilen[1]);
+            const double sz = min_image(pz - qz, b.len[2], b.ilen[2]);
             const double r2 = sx * sx + sy * sy + sz * sz;
             if (r2 <= t.ru2max) {
                 const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
-                const double r = sqrt(r2);
-                const double sk = sqrt(r) * t.csi;
+                double inv_r;
+                const double sk = table_coord(r2, t.csi, inv_r);
                 const int kk = static_cast<int>(sk);
                 const double2 e = t.b2[kind * t.ntab2 + kk];
                 den0 = den0 + (e.x + (sk - kk) * e.y);
@@ -413,13 +301,15 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomsDev a, ListDev l, TabDev 
     if (active) {
         const double denki = pi.w;
         const int kv = l.kvois[i];
-        const int* __restrict__ li = l.indi + i;
+        const int* __restrict__ li = l.indi + list_base(i, l.mx);
+        int jn = kv > 0 ? li[0] : 0;
         for (int iw = 0; iw < kv; ++iw) {
-            const int j = li[static_cast<size_t>(iw) * l.stride];
+            const int j = jn;
+            if (iw + 1 < kv) jn = li[static_cast<size_t>(iw + 1) * l.stride];
             const double4 pj = a.pos[j];
-            const double sx = min_image(pi.x - pj.x, b.len[0], b.hlen[0], b.ifpd[0]);
-            const double sy = min_image(pi.y - pj.y, b.len[1], b.hlen[1], b.ifpd[1]);
-            const double sz = min_image(pi.z - pj.z, b.len[2], b.hlen[2], b.ifpd[2]);
+            const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
+            const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
+            const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
             const double r2 = sx * sx + sy * sy + sz * sz;
             if (r2 <= t.ru2max) {
                 int kind, kind1;
@@ -430,8 +320,8 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomsDev a, ListDev l, TabDev 
                 } else {
                     kind = kind1 = t.kpair[0];
                 }
-                const double r = sqrt(r2);
-                const double sk = sqrt(r) * t.csi;
+                double inv_r;
+                const double sk = table_coord(r2, t.csi, inv_r);
                 const int kk = static_cast<int>(sk);
                 const double dk = sk - kk;
                 const double4 f = t.f4[kind * t.ntab2 + kk];
@@ -442,8 +332,9 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomsDev a, ListDev l, TabDev 
                 } else {
                     fb1 = f.z + dk * f.w;
                 }
-                const double fortot =
-                    (f.x + dk * f.y) / r2 + ((f.z + dk * f.w) * denki + fb1 * pj.w) / r;
+                // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL :811-813)
+                const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
+                                      ((f.z + dk * f.w) * denki + fb1 * pj.w) * inv_r;
                 fpx = fpx + fortot * sx;
                 fpy = fpy + fortot * sy;
                 fpz = fpz + fortot * sz;
@@ -455,63 +346,8 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomsDev a, ListDev l, TabDev 
         a.fy[i] = fpy;
         a.fz[i] = fpz;
     }
-    if constexpr (MODE >= 1) {
-        if (active) {
-#pragma clang fp contract(off)
-            const double cm0 = t.cm[MT ? ti : 0];
-            const double ax = fpx / cm0, ay = fpy / cm0, az = fpz / cm0;
-            double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-            // Correction_KERNEL
-            if (!(st & (STATU_FIXVELX | STATU_FIXPOSX))) vx = vx + sp.hs2 * ax;
-            if (!(st & (STATU_FIXVELY | STATU_FIXPOSY))) vy = vy + sp.hs2 * ay;
-            if (!(st & (STATU_FIXVELZ | STATU_FIXPOSZ))) vz = vz + sp.hs2 * az;
-            if constexpr (MODE == 2) {
-                // Predictor_KERNEL0 of the next step with the same forces
-                int stat = st;
-                double x = pi.x, y = pi.y, z = pi.z;
-                double dx = sp.h * vx + sp.h2s2 * ax;
-                if (stat & STATU_FIXPOSX) dx = 0.0;
-                x = x + dx;
-                if (b.ifpd[0]) {
-                    if (x > b.lo[0] + b.len[0]) { x = x - b.len[0]; stat |= STATU_PASSBOUND; }
-                    else if (x < b.lo[0])       { x = x + b.len[0]; stat |= STATU_PASSBOUND; }
-                }
-                double dy = sp.h * vy + sp.h2s2 * ay;
-                if (stat & STATU_FIXPOSY) dy = 0.0;
-                y = y + dy;
-                if (b.ifpd[1]) {
-                    if (y > b.lo[1] + b.len[1]) { y = y - b.len[1]; stat |= STATU_PASSBOUND; }
-                    else if (y < b.lo[1])       { y = y + b.len[1]; stat |= STATU_PASSBOUND; }
-                }
-                double dz = sp.h * vz + sp.h2s2 * az;
-                if (stat & STATU_FIXPOSZ) dz = 0.0;
-                z = z + dz;
-                if (b.ifpd[2]) {
-                    if (z > b.lo[2] + b.len[2]) { z = z - b.len[2]; stat |= STATU_PASSBOUND; }
-                    else if (z < b.lo[2])       { z = z + b.len[2]; stat |= STATU_PASSBOUND; }
-                }
-                if (!(stat & (STATU_FIXVELX | STATU_FIXPOSX))) vx = vx + sp.hs2 * ax;
-                if (!(stat & (STATU_FIXVELY | STATU_FIXPOSY))) vy = vy + sp.hs2 * ay;
-                if (!(stat & (STATU_FIXVELZ | STATU_FIXPOSZ))) vz = vz + sp.hs2 * az;
-                if (sp.track_dis) {
-                    a.dx[i] = a.dx[i] + dx;
-                    a.dy[i] = a.dy[i] + dy;
-                    a.dz[i] = a.dz[i] + dz;
-                }
-                if (x > b.lo[0] + b.len[0] || y > b.lo[1] + b.len[1] || z > b.lo[2] + b.len[2])
-                    stat = STATU_OUTOFBOX | STATU_REFLECT;
-                else if (x < b.lo[0] || y < b.lo[1] || z < b.lo[2])
-                    stat = STATU_OUTOFBOX | STATU_TRANSMIT;
-                if (stat != st) a.statu[i] = stat;
-                a.pos_out[i] = make_double4(x, y, z, 0.0);
-            }
-            a.vx[i] = vx;
-            a.vy[i] = vy;
-            a.vz[i] = vz;
-        } else if constexpr (MODE == 2) {
-            a.pos_out[i] = make_double4(pi.x, pi.y, pi.z, 0.0);
-        }
-    }
+    if constexpr (MODE >= 1)
+        integrate_atom<MODE>(a, t, b, sp, i, st, active, pi, ti, fpx, fpy, fpz);
 }
 
 void launch_force(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
@@ -552,20 +388,20 @@ __global__ __launch_bounds__(kBlock) void k_epot(AtomsDev a, ListDev l, TabDev t
         const int ti = a.ityp[i];
         const int kv = l.kvois[i];
         for (int iw = 0; iw < kv; ++iw) {
-            const int j = l.indi[i + static_cast<size_t>(iw) * l.stride];
+            const int j = l.indi[list_base(i, l.mx) + static_cast<size_t>(iw) * l.stride];
             const double4 pj = a.pos[j];
-            const double sx = min_image(pi.x - pj.x, b.len[0], b.hlen[0], b.ifpd[0]);
-            const double sy = min_image(pi.y - pj.y, b.len[1], b.hlen[1], b.ifpd[1]);
-            const double sz = min_image(pi.z - pj.z, b.len[2], b.hlen[2], b.ifpd[2]);
+            const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
+            const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
+            const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
             const double r2 = sx * sx + sy * sy + sz * sz;
             if (r2 <= t.ru2max) {
                 const int kind = t.kpair[ti * t.ngroup + a.ityp[j]];
-                const double r = sqrt(r2);
-                const double sk = sqrt(r) * t.csi;
+                double inv_r;
+                const double sk = table_coord(r2, t.csi, inv_r);
                 const int kk = static_cast<int>(sk);
                 const double dk = sk - kk;
                 const double4 e = t.e4[kind * t.ntab2 + kk];
-                er0 = er0 + (e.x + dk * e.y) / r;
+                er0 = er0 + (e.x + dk * e.y) * inv_r;
                 den0 = den0 + (e.z + dk * e.w);
             }
         }
@@ -718,7 +554,7 @@ __global__ void k_unsort_list(const int* __restrict__ kvois, const int* __restri
     const int kv = kvois[p];
     kv_out[g] = kv;
     for (int iw = 0; iw < kv && iw < mx; ++iw)
-        indi_out[g + static_cast<size_t>(iw) * ld] = gid[indi[p + static_cast<size_t>(iw) * stride]] + 1;
+        indi_out[g + static_cast<size_t>(iw) * ld] = gid[indi[list_base(p, mx) + static_cast<size_t>(iw) * stride]] + 1;
 }
 
 void launch_unsort_d(const double* src, double* dst, const int* gid, int n, hipStream_t s) {

@@ -88,6 +88,50 @@ def test_neighbour_list_bit_exact(orc, p0, case):
     dev.close()
 
 
+def _list_both_kernels(box, ctrl, ft):
+    import msmpscu_amd._lib as L
+    out = []
+    for mode in (0, 1):
+        dev = device_for(box, ctrl, ft)
+        L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_NLIST_KERNEL, mode), "msm_set_option")
+        dev.build_neighbors()
+        out.append(dev.get_neighbors())
+        dev.close()
+    return out
+
+
+@pytest.mark.parametrize("case", ["small_ncell3", "dense_blob", "slab"])
+def test_brick_list_equals_global_scan(orc, case):
+    """The LDS-staged brick build (including bricks that overflow LDS and take the global path)
+    gives the identical list, in identical order, to the per-atom global-memory scan."""
+    if case == "small_ncell3":
+        box = cfg.make_bcc_box((4, 4, 4), cfg.W_A, 0.1, 300.0, 3)
+        ctrl = cfg.w_ctrl(mx=64)
+    elif case == "dense_blob":
+        box = cfg.make_bcc_box((12, 12, 12), cfg.W_A, 0.05, 0.0, 4)
+        rng = np.random.default_rng(4)
+        blob = rng.normal(0.0, 1.2, (600, 3))
+        x = np.concatenate([box.XP, blob])
+        n = x.shape[0]
+        box = md.SimMDBox(NPRT=n, NGROUP=1, ZL=box.ZL, BOXLOW=box.BOXLOW,
+                          XP=np.asfortranarray(x), XP1=np.zeros((n, 3)),
+                          ITYP=np.ones(n, np.int32), STATU=np.ones(n, np.int32), CM=box.CM,
+                          PTYPE=box.PTYPE)
+        ctrl = cfg.w_ctrl(mx=800)
+    else:
+        box = cfg.make_cuni_slab((6, 6, 6), 3.57, 300.0, 9)
+        ctrl = cfg.cuni_ctrl()
+    ft = cfg.w_forcetable(box, ctrl) if case != "slab" else cfg.cuni_forcetable(box, ctrl)
+    say(f"brick vs global list: {case} n={box.NPRT}")
+    (kv0, ind0), (kv1, ind1) = _list_both_kernels(box, ctrl, ft)
+    assert np.array_equal(kv0, kv1)
+    for i in range(len(kv0)):
+        assert np.array_equal(ind0[i, : kv0[i]], ind1[i, : kv1[i]])
+    okv, oind = orc.nlist("full", box.XP, box.ITYP - 1, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
+                          ctrl.NB_RM, ctrl.NB_MXNBS)
+    assert pairset(kv0, ind0, True) == pairset(okv, oind, False)
+
+
 @pytest.mark.parametrize("pot_type", ["FS_TYPE", "EAM_TYPE"])
 def test_force_energy_vs_oracle(orc, p0, pot_type):
     box, ctrl, _ = p0
@@ -138,7 +182,8 @@ def test_two_species_slab(orc):
 
 
 def test_asymmetric_density_channels(orc):
-    """KPAIR(i,j) != KPAIR(j,i) with different density tables: checks the KTAB/KTAB1 pairing."""
+    """KPAIR(i,j) != KPAIR(j,i) with different density tables (the pair term stays symmetric, as
+    any energy requires): checks the KTAB/KTAB1 pairing."""
     box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.08, 300.0, 8)
     box.NGROUP = 2
     box.ITYP = (np.arange(box.NPRT) % 2 + 1).astype(np.int32)
@@ -149,7 +194,7 @@ def test_asymmetric_density_channels(orc):
     md.Register_ForceTableProc(1, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne")
     md.Register_ForceTableProc(2, ft, "msm_pot_ackland_ww_nn", "msm_pot_ackland_ww_ne")
     md.Register_ForceTableProc(3, ft, "msm_pot_wfs_nn", "msm_pot_ackland_ww_ne")
-    md.Register_ForceTableProc(4, ft, lambda r: (0.0, 0.0), "msm_pot_wfs_ne")
+    md.Register_ForceTableProc(4, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne")
     md.Create_Interaction_ForceTable(box, ctrl, ft)
     say("asymmetric kpair")
     dev = device_for(box, ctrl, ft)
@@ -329,4 +374,29 @@ def test_million_atom_snapshot(orc):
     fh, eh, *_ = oracle_half(orc, box, ctrl, ft, xp=x)
     assert np.max(np.abs(fp - fh)) <= F_TOL
     assert abs(dev.get_epot().sum() - eh.sum()) <= E_REL * abs(eh.sum())
+    dev.close()
+
+
+@pytest.mark.parametrize("path,compact", [(0, 1), (1, 0), (1, 1), (2, 0), (2, 1), (4, 1),
+                                          (8, 1)])
+def test_force_paths_agree(orc, p0, path, compact):
+    """Every density/force kernel variant (LDS bricks; global list with 1-8 lanes per atom, with
+    and without the compacted in-cutoff list) meets the parity bar on the same box, before and
+    after a 15-step run."""
+    import msmpscu_amd._lib as L
+    box, ctrl, ft = p0
+    say(f"force path {path} compact {compact}")
+    dev = device_for(box, ctrl, ft)
+    L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
+    L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_COMPACT_LIST, compact), "msm_set_option")
+    dev.build_neighbors()
+    dev.calc_force()
+    _, _, fp, _, _ = dev.get_atoms()
+    fh, eh, _, ddh, _ = oracle_half(orc, box, ctrl, ft)
+    assert np.max(np.abs(fp - fh)) <= F_TOL
+    assert np.max(np.abs(dev.get_den() - ddh)) <= 1e-13
+    dev.run(15, ctrl.NB_UPTAB, 0, ctrl.H)
+    x, _, fp, _, _ = dev.get_atoms()
+    fh, *_ = oracle_half(orc, box, ctrl, ft, xp=x)
+    assert np.max(np.abs(fp - fh)) <= F_TOL
     dev.close()
