@@ -75,7 +75,15 @@ def main():
     ap.add_argument("--no-timing-pass", action="store_true")
     ap.add_argument("--force-path", type=int, default=None,
                     help="density/force kernels: 0 LDS bricks, 1/2/4/8 global list with that "
-                         "many lanes per atom (default: library default)")
+                         "many lanes per atom, 3 two lanes with lane-pair gathers (default: "
+                         "library default)")
+    ap.add_argument("--compact", type=int, default=None,
+                    help="1: force pass visits only the pairs the density pass found inside the "
+                         "cutoff (global-list paths 1, 2, 4)")
+    ap.add_argument("--table-lds", type=int, default=None,
+                    help="1: table knots from an LDS window (one table kind, paths 1, 2, 4)")
+    ap.add_argument("--lt-shape", default=None,
+                    help="LDS-table passes: density block,wgs,force block,wgs (e.g. 1024,1,1024,1)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -107,6 +115,17 @@ def main():
     if args.force_path is not None:
         _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_FORCE_PATH, args.force_path),
                    "msm_set_option")
+    if args.compact is not None:
+        _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_COMPACT_LIST, args.compact),
+                   "msm_set_option")
+    if args.table_lds is not None:
+        _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_TABLE_LDS, args.table_lds),
+                   "msm_set_option")
+    if args.lt_shape:
+        vals = [int(v) for v in args.lt_shape.split(",")]
+        for opt, v in zip((_lib.MSM_OPT_LT_DENSITY_BLOCK, _lib.MSM_OPT_LT_DENSITY_WGS,
+                           _lib.MSM_OPT_LT_FORCE_BLOCK, _lib.MSM_OPT_LT_FORCE_WGS), vals):
+            _lib.check(_lib.lib().msm_set_option(dev.h, opt, v), "msm_set_option")
     dev.build_neighbors()
     dev.calc_force()
     tot, kmax = dev.list_stats()

@@ -192,10 +192,22 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
 #define MSM_OPT_KERNEL_TIMING 2
 #define MSM_OPT_NLIST_KERNEL  3  /* 0 (default): LDS-staged cell bricks; 1: per-atom global scan */
 #define MSM_OPT_FORCE_PATH    4  /* density/force passes: 0 LDS bricks; 1, 2, 4, 8: global list
-                                    with that many lanes per atom (default 2) */
-#define MSM_OPT_COMPACT_LIST  5  /* 1 (default): on the global-list path with <= 4 lanes per atom,
+                                    with that many lanes per atom (default 2); 3: two lanes per
+                                    atom, each 32-byte record gathered by a lane pair */
+#define MSM_OPT_COMPACT_LIST  5  /* 0 (default); 1: on the global-list path with 1, 2 or 4 lanes
+                                    per atom,
                                     the density pass records the pairs inside the force cutoff and
                                     the force pass visits only those */
+#define MSM_OPT_TABLE_LDS     6  /* 1 (default): with one table kind on the global-list paths
+                                    1, 2, 4 (no compaction), the passes read the table's knots
+                                    from a window held in LDS (persistent grid) */
+/* Shape of the LDS-table passes: threads per workgroup (256, 512, 1024; default 1024) and
+ * workgroups per CU (1..16, default 1; capped by what fits); the LDS window is the CU's LDS
+ * divided by the workgroups per CU. */
+#define MSM_OPT_LT_DENSITY_BLOCK 7
+#define MSM_OPT_LT_DENSITY_WGS   8
+#define MSM_OPT_LT_FORCE_BLOCK   9
+#define MSM_OPT_LT_FORCE_WGS     10
 int msm_set_option(msm_sim* sim, int option, int value);
 
 /* Kernel classes for msm_kernel_times. */

@@ -92,6 +92,11 @@ MSM_OPT_KERNEL_TIMING = 2
 MSM_OPT_NLIST_KERNEL = 3
 MSM_OPT_FORCE_PATH = 4
 MSM_OPT_COMPACT_LIST = 5
+MSM_OPT_TABLE_LDS = 6
+MSM_OPT_LT_DENSITY_BLOCK = 7
+MSM_OPT_LT_DENSITY_WGS = 8
+MSM_OPT_LT_FORCE_BLOCK = 9
+MSM_OPT_LT_FORCE_WGS = 10
 KCLASS = ["density", "force", "rebuild", "predictor", "nlist"]
 
 _lib = None

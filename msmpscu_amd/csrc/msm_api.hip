@@ -99,7 +99,10 @@ struct msm_sim {
     int track_dis = 1;
     int timing = 0;
     int nlist_global = 0;  // 1: per-atom global-memory list kernel instead of LDS bricks
-    int force_path = 2;    // 0: LDS bricks; 1, 2, 4, 8: global list with that many lanes per atom
+    int force_path = 2;    // 0: LDS bricks; 1, 2, 4, 8: global list with that many lanes per atom;
+                           // 3: two lanes per atom with lane-pair gathers
+    int table_lds = 1;     // single-kind table knots in LDS (global-list paths 1, 2, 4)
+    msm::LtShape lt{1024, 1, 1024, 1};  // block / workgroups per CU of the LDS-table passes
     int compact = 0;       // global-list path: force pass reads the density pass's in-cutoff list
 
     // LDS-staged list bricks
@@ -195,7 +198,7 @@ ListDev list_view(msm_sim* s) {
     l.kvois = s->kvois;
     l.indi = s->indi;
     l.sidx = s->sidx;
-    const bool cmp = s->compact && s->force_path > 0 && s->force_path <= 4;
+    const bool cmp = s->compact && s->force_path > 0 && s->force_path <= 4 && s->force_path != 3;
     l.cidx = cmp ? s->cidx : nullptr;
     l.ckv = cmp ? s->ckv : nullptr;
     l.stride = msm::kListRow;
@@ -376,10 +379,22 @@ int plan_bricks(msm_sim* s, const double* xp, int n) {
     return MSM_OK;
 }
 
+// LDS-table passes: one table kind, global list with 1, 2 or 4 lanes per atom, no compaction.
+bool use_lds_table(const msm_sim* s) {
+    return s->table_lds && s->tab.ngroup == 1 &&
+           (s->force_path == 1 || s->force_path == 2 || s->force_path == 4) && !s->compact;
+}
+
 void enqueue_density(msm_sim* s) {
     TimedScope ts(s, MSM_KCLASS_DENSITY);
     AtomsDev a = atoms_view(s, s->scur, s->pcur);
-    if (s->force_path > 0)
+    if (use_lds_table(s))
+        msm::launch_density_lt(s->force_path, s->lt, a, list_view(s), s->tab, s->boxes[0],
+                               s->d_boxes, multibox(s), s->d_err, s->stream);
+    else if (s->force_path == 3)
+        msm::launch_density_pair(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
+                                 s->d_err, s->stream);
+    else if (s->force_path > 0)
         msm::launch_density_tpa(s->force_path, a, list_view(s), s->tab, s->boxes[0], s->d_boxes,
                                 multibox(s), s->d_err, s->stream);
     else if (s->nlist_global)
@@ -401,7 +416,13 @@ void enqueue_force(msm_sim* s, int mode, int write_f, double h) {
     sp.mode = mode;
     sp.write_f = write_f;
     sp.track_dis = s->track_dis;
-    if (s->force_path > 0)
+    if (use_lds_table(s))
+        msm::launch_force_lt(s->force_path, s->lt, a, list_view(s), s->tab, s->boxes[0],
+                             s->d_boxes, multibox(s), sp, s->stream);
+    else if (s->force_path == 3)
+        msm::launch_force_pair(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), sp,
+                               s->stream);
+    else if (s->force_path > 0)
         msm::launch_force_tpa(s->force_path, a, list_view(s), s->tab, s->boxes[0], s->d_boxes,
                               multibox(s), sp, s->stream);
     else if (s->nlist_global)
@@ -1039,13 +1060,31 @@ int msm_set_option(msm_sim* s, int option, int value) {
             clear_graphs(s);
             return MSM_OK;
         case MSM_OPT_FORCE_PATH:
-            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
-                return set_err(s, MSM_ERR_ARG, "force path must be 0, 1, 2, 4 or 8");
+            if (value < 0 || value > 8 || (value > 4 && value != 8))
+                return set_err(s, MSM_ERR_ARG, "force path must be 0, 1, 2, 3, 4 or 8");
             s->force_path = value;
             clear_graphs(s);
             return MSM_OK;
         case MSM_OPT_COMPACT_LIST:
             s->compact = value ? 1 : 0;
+            clear_graphs(s);
+            return MSM_OK;
+        case MSM_OPT_TABLE_LDS:
+            s->table_lds = value ? 1 : 0;
+            clear_graphs(s);
+            return MSM_OK;
+        case MSM_OPT_LT_DENSITY_BLOCK:
+        case MSM_OPT_LT_FORCE_BLOCK:
+            if (value != 256 && value != 512 && value != 1024)
+                return set_err(s, MSM_ERR_ARG, "LDS-table block must be 256, 512 or 1024");
+            (option == MSM_OPT_LT_DENSITY_BLOCK ? s->lt.block_d : s->lt.block_f) = value;
+            clear_graphs(s);
+            return MSM_OK;
+        case MSM_OPT_LT_DENSITY_WGS:
+        case MSM_OPT_LT_FORCE_WGS:
+            if (value < 1 || value > 16)
+                return set_err(s, MSM_ERR_ARG, "LDS-table workgroups per CU must be 1..16");
+            (option == MSM_OPT_LT_DENSITY_WGS ? s->lt.wgs_d : s->lt.wgs_f) = value;
             clear_graphs(s);
             return MSM_OK;
         default:

@@ -128,6 +128,30 @@ void launch_density_tpa(int tpa, const AtomsDev& a, const ListDev& l, const TabD
 void launch_force_tpa(int tpa, const AtomsDev& a, const ListDev& l, const TabDev& t,
                       const BoxDev& box0, const BoxDev* boxes, int multibox, const StepParams& sp,
                       hipStream_t s);
+// Global-list passes with the single table kind's knots in LDS (msm_ldstab.hip): a window of
+// the top wf (force, {FPOTR, FPOTB}) / wd (density, POTB) knots starting at k0f / k0d, as large
+// as the LDS share of one workgroup allows; persistent grids of block_* threads and wgs_*
+// workgroups per CU.
+constexpr int kLtMaxBlock = 1024;
+struct LtWin {
+    int k0f, wf, k0d, wd;
+};
+struct LtShape {
+    int block_d, wgs_d, block_f, wgs_f;
+};
+LtWin lt_window(const TabDev& t, int wgs_d, int wgs_f);
+void launch_density_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDev& l,
+                       const TabDev& t, const BoxDev& box0, const BoxDev* boxes, int multibox,
+                       int* err, hipStream_t s);
+void launch_force_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDev& l,
+                     const TabDev& t, const BoxDev& box0, const BoxDev* boxes, int multibox,
+                     const StepParams& sp, hipStream_t s);
+// Two lanes per atom with 32-byte records gathered by lane pairs (msm_pair.hip); the compacted
+// list is not used there.
+void launch_density_pair(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                         const BoxDev* boxes, int multibox, int* err, hipStream_t s);
+void launch_force_pair(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                       const BoxDev* boxes, int multibox, const StepParams& sp, hipStream_t s);
 void launch_density(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                     const BoxDev* boxes, int multibox, int* err, hipStream_t s);
 void launch_force(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,

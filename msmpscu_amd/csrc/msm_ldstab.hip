@@ -1,0 +1,379 @@
+// msm_ldstab.hip -- density and force passes with the interpolation table held in LDS.
+//
+// The per-pair table lookups of the global-list passes are scattered 16/32-byte loads from a
+// 80-160 KB table; together with the neighbour positions they overflow the 32 KB L1 and feed the
+// vector-memory pipeline twice per pair.  Here each workgroup first copies a window of knots of
+// the (single) table kind into LDS -- values only, {FPOTR, FPOTB} for the force pass (16 B/knot)
+// and POTB for the density pass (8 B/knot) -- and the lerp reads knots KK and KK+1 from there:
+//   T(KK) + DK * (T(KK+1) - T(KK))      (MD_FS_ForceTable_GPU.F90 PRECALFOR/CALFORCE kernels)
+// which is the same arithmetic as the reference.  The window is the top of the table (large
+// separations, where every neighbour lies); a rare pair below the window (r under about 1 A for
+// the W tables) is flagged and its term added after its group of neighbours, from the knots of
+// the global table.
+//
+// The grid is persistent: a chosen number of workgroups per CU (capped by what fits), each
+// filling its LDS window once and then walking a contiguous range of atom chunks (contiguous per
+// XCD), so the waves sharing a CU's L1 work on one compact region of the cell-sorted atoms.  Lanes per atom
+// (TPA), list traversal and the finishing lane are those of msm_tpa.hip.
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "msm_device.h"
+#include "msm_internal.h"
+
+namespace msm {
+
+namespace {
+
+constexpr int kGroup = 8;
+
+template <int TPA>
+__device__ __forceinline__ double lane_sum(double v) {
+#pragma unroll
+    for (int d = TPA / 2; d >= 1; d >>= 1) v = v + __shfl_xor(v, d, 64);
+    return v;
+}
+
+// chunk range [c0, c1) of this workgroup
+__device__ __forceinline__ void chunk_range(int nchunks, int& c0, int& c1) {
+    const int nblk = gridDim.x;
+    const int L = xcd_block(nblk);
+    c0 = static_cast<int>(static_cast<long long>(L) * nchunks / nblk);
+    c1 = static_cast<int>(static_cast<long long>(L + 1) * nchunks / nblk);
+}
+
+}  // namespace
+
+template <int MB, int EAM, int TPA>
+__global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev l, TabDev t,
+                                                                BoxDev box0,
+                                                                const BoxDev* __restrict__ boxes,
+                                                                LtWin w, int nchunks, int* err) {
+    extern __shared__ double s_pb[];
+    const double2* __restrict__ b2k = t.b2 + static_cast<size_t>(t.kpair[0]) * t.ntab2;
+    for (int k = threadIdx.x; k < w.wd; k += blockDim.x) s_pb[k] = b2k[w.k0d + k].x;
+    __syncthreads();
+    int c0, c1;
+    chunk_range(nchunks, c0, c1);
+    const int kmax = t.ntab2 - 2;
+    for (int c = c0; c < c1; ++c) {
+        const int gt = c * blockDim.x + threadIdx.x;
+        const int i = gt / TPA, sub = gt % TPA;
+        const bool have = i < a.n;
+        const bool active = have && (a.statu[i] & STATU_ACTIVE) == STATU_ACTIVE;
+        double den0 = 0.0;
+        if (active) {
+            const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
+            const double2 pxy = *reinterpret_cast<const double2*>(&a.pos[i]);
+            const double pz = a.pos[i].z;
+            const int kv = l.kvois[i];
+            const int* __restrict__ li = l.indi + list_base(i, l.mx);
+            for (int base = sub; base < kv; base += TPA * kGroup) {
+                int js[kGroup];
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) {
+                    const int iw = base + u * TPA;
+                    js[u] = iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
+                }
+                unsigned fix = 0;
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) {
+                    const int j = js[u];
+                    const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
+                    const double qz = a.pos[j].z;
+                    const double sx = min_image(pxy.x - qxy.x, b.len[0], b.ilen[0]);
+                    const double sy = min_image(pxy.y - qxy.y, b.len[1], b.ilen[1]);
+                    const double sz = min_image(pz - qz, b.len[2], b.ilen[2]);
+                    const double r2 = sx * sx + sy * sy + sz * sz;
+                    double inv_r;
+                    const double sk = table_coord(r2, t.csi, inv_r);
+                    const int kk = min(static_cast<int>(sk), kmax);
+                    const bool in = base + u * TPA < kv && r2 <= t.ru2max;
+                    const int kl = kk - w.k0d;
+                    if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+                    const int q = max(kl, 0);
+                    const double p0 = s_pb[q], p1 = s_pb[q + 1];
+                    const double v = p0 + (sk - kk) * (p1 - p0);
+                    den0 = den0 + ((in && kl >= 0) ? v : 0.0);
+                }
+                if (fix) {
+#pragma unroll
+                    for (int u = 0; u < kGroup; ++u) {
+                        if (!(fix & (1u << u))) continue;
+                        const int j = li[static_cast<size_t>(base + u * TPA) * l.stride];
+                        const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
+                        const double qz = a.pos[j].z;
+                        const double sx = min_image(pxy.x - qxy.x, b.len[0], b.ilen[0]);
+                        const double sy = min_image(pxy.y - qxy.y, b.len[1], b.ilen[1]);
+                        const double sz = min_image(pz - qz, b.len[2], b.ilen[2]);
+                        const double r2 = sx * sx + sy * sy + sz * sz;
+                        double inv_r;
+                        const double sk = table_coord(r2, t.csi, inv_r);
+                        const int kk = min(static_cast<int>(sk), kmax);
+                        const double2 e = b2k[kk];
+                        den0 = den0 + (e.x + (sk - kk) * e.y);
+                    }
+                }
+            }
+        }
+        den0 = lane_sum<TPA>(den0);
+        if (sub != 0 || !have) continue;
+        if (active) {
+            if constexpr (!EAM) {
+                if (den0 > 0.0) den0 = -0.5 / sqrt(den0);
+            } else {
+                if (den0 > 0.0) {
+                    const double sk = den0 / t.rhod + 1.0;
+                    int kk = static_cast<int>(sk + 0.000001);
+                    if (kk > t.nembd) {
+                        atomicOr(&err[kErrEmbd], 1);
+                        kk = t.nembd;
+                    }
+                    const double4 m = t.m4[t.kembd[0] * t.nembd2 + kk];
+                    den0 = m.z + (sk - kk) * m.w;
+                }
+            }
+        }
+        a.pos[i].w = den0;
+    }
+}
+
+template <int MB, int MODE, int TPA>
+__global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l, TabDev t,
+                                                            BoxDev box0,
+                                                            const BoxDev* __restrict__ boxes,
+                                                            StepParams sp, LtWin w, int nchunks) {
+    extern __shared__ double2 s_ft[];
+    const double4* __restrict__ f4k = t.f4 + static_cast<size_t>(t.kpair[0]) * t.ntab2;
+    for (int k = threadIdx.x; k < w.wf; k += blockDim.x) {
+        const double4 f = f4k[w.k0f + k];
+        s_ft[k] = make_double2(f.x, f.z);
+    }
+    __syncthreads();
+    int c0, c1;
+    chunk_range(nchunks, c0, c1);
+    const int kmax = t.ntab2 - 2;
+    for (int c = c0; c < c1; ++c) {
+        const int gt = c * blockDim.x + threadIdx.x;
+        const int i = gt / TPA, sub = gt % TPA;
+        const bool have = i < a.n;
+        const int st = have ? a.statu[i] : 0;
+        const bool active = have && (st & STATU_ACTIVE) == STATU_ACTIVE;
+        const double4 pi = have ? a.pos[i] : make_double4(0, 0, 0, 0);
+        const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, have ? i : 0);
+        double fpx = 0.0, fpy = 0.0, fpz = 0.0;
+        if (active) {
+            const double denki = pi.w;
+            const int kv = l.kvois[i];
+            const int* __restrict__ li = l.indi + list_base(i, l.mx);
+            for (int base = sub; base < kv; base += TPA * kGroup) {
+                int js[kGroup];
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) {
+                    const int iw = base + u * TPA;
+                    js[u] = iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
+                }
+                unsigned fix = 0;
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) {
+                    const int j = js[u];
+                    const double4 pj = a.pos[j];
+                    const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
+                    const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
+                    const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
+                    const double r2 = sx * sx + sy * sy + sz * sz;
+                    double inv_r;
+                    const double sk = table_coord(r2, t.csi, inv_r);
+                    const int kk = min(static_cast<int>(sk), kmax);
+                    const double dk = sk - kk;
+                    const bool in = base + u * TPA < kv && r2 <= t.ru2max;
+                    const int kl = kk - w.k0f;
+                    if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+                    const int q = max(kl, 0);
+                    const double2 t0 = s_ft[q], t1 = s_ft[q + 1];
+                    // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r, KTAB = KTAB1 here
+                    const double fb = t0.y + dk * (t1.y - t0.y);
+                    const double ft = (t0.x + dk * (t1.x - t0.x)) * (inv_r * inv_r) +
+                                      (fb * denki + fb * pj.w) * inv_r;
+                    const double fortot = (in && kl >= 0) ? ft : 0.0;
+                    fpx = fpx + fortot * sx;
+                    fpy = fpy + fortot * sy;
+                    fpz = fpz + fortot * sz;
+                }
+                if (fix) {
+                    // rare close pairs: the same term with the knots of the global table
+#pragma unroll
+                    for (int u = 0; u < kGroup; ++u) {
+                        if (!(fix & (1u << u))) continue;
+                        const double4 pj = a.pos[li[static_cast<size_t>(base + u * TPA) * l.stride]];
+                        const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
+                        const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
+                        const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
+                        const double r2 = sx * sx + sy * sy + sz * sz;
+                        double inv_r;
+                        const double sk = table_coord(r2, t.csi, inv_r);
+                        const int kk = min(static_cast<int>(sk), kmax);
+                        const double dk = sk - kk;
+                        const double4 f = f4k[kk];
+                        const double fb = f.z + dk * f.w;
+                        const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
+                                              (fb * denki + fb * pj.w) * inv_r;
+                        fpx = fpx + fortot * sx;
+                        fpy = fpy + fortot * sy;
+                        fpz = fpz + fortot * sz;
+                    }
+                }
+            }
+        }
+        fpx = lane_sum<TPA>(fpx);
+        fpy = lane_sum<TPA>(fpy);
+        fpz = lane_sum<TPA>(fpz);
+        if (sub != 0 || !have) continue;
+        if (sp.write_f) {
+            a.fx[i] = fpx;
+            a.fy[i] = fpy;
+            a.fz[i] = fpz;
+        }
+        if constexpr (MODE >= 1)
+            integrate_atom<MODE>(a, t, b, sp, i, st, active, pi, 0, fpx, fpy, fpz);
+    }
+}
+
+namespace {
+
+int device_attr(hipDeviceAttribute_t attr, int fallback) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fallback;
+    if (hipDeviceGetAttribute(&v, attr, dev) != hipSuccess || v <= 0) return fallback;
+    return v;
+}
+
+struct DevLimits {
+    int cus = 0, lds_cu = 0, lds_block = 0;
+};
+
+const DevLimits& dev_limits() {
+    static DevLimits lim[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    DevLimits& d = lim[dev];
+    if (d.cus == 0) {
+        d.lds_cu = device_attr(hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, 65536);
+        d.lds_block = device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock, 65536);
+        d.cus = device_attr(hipDeviceAttributeMultiprocessorCount, 256);
+    }
+    return d;
+}
+
+// Persistent grid: min(requested, co-resident) workgroups per CU of kernel k (cached per kernel,
+// block and LDS size) times the CU count, capped by the work.
+unsigned persistent_grid(const void* k, int block, size_t lds, int wgs, int nchunks) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, size_t>, int> cache;
+    int per_cu = 0;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find({k, block, lds});
+        if (it != cache.end()) per_cu = it->second;
+    }
+    if (per_cu == 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k, block, lds) != hipSuccess || v <= 0)
+            v = 1;
+        per_cu = v;
+        std::lock_guard<std::mutex> g(mu);
+        cache[{k, block, lds}] = v;
+    }
+    if (wgs > 0 && wgs < per_cu) per_cu = wgs;
+    const long long g = static_cast<long long>(per_cu) * dev_limits().cus;
+    return static_cast<unsigned>(g < nchunks ? g : nchunks);
+}
+
+// LDS bytes one workgroup may take when wgs of them share a CU.
+int lds_budget(int wgs) {
+    const DevLimits& d = dev_limits();
+    int b = d.lds_cu / (wgs > 0 ? wgs : 1);
+    if (b > d.lds_block) b = d.lds_block;
+    return b & ~255;
+}
+
+}  // namespace
+
+LtWin lt_window(const TabDev& t, int wgs_d, int wgs_f) {
+    LtWin w{};
+    const int bf = lds_budget(wgs_f) / 16, bd = lds_budget(wgs_d) / 8;
+    w.wf = t.ntab2 < bf ? t.ntab2 : bf;
+    w.k0f = t.ntab2 - w.wf;
+    w.wd = t.ntab2 < bd ? t.ntab2 : bd;
+    w.k0d = t.ntab2 - w.wd;
+    return w;
+}
+
+void launch_density_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDev& l,
+                       const TabDev& t, const BoxDev& box0, const BoxDev* boxes, int multibox,
+                       int* err, hipStream_t s) {
+    if (a.n <= 0) return;
+    const LtWin w = lt_window(t, sh.wgs_d, sh.wgs_f);
+    const size_t lds = static_cast<size_t>(w.wd) * sizeof(double);
+    const long long th = static_cast<long long>(a.n) * tpa;
+    const int nchunks = static_cast<int>((th + sh.block_d - 1) / sh.block_d);
+    const bool eam = t.pot_type != 0;
+#define MSM_DL(MB, E, P)                                                                     \
+    do {                                                                                     \
+        auto k = k_density_lt<MB, E, P>;                                                     \
+        const unsigned g = persistent_grid(reinterpret_cast<const void*>(k), sh.block_d,     \
+                                           lds, sh.wgs_d, nchunks);                          \
+        hipLaunchKernelGGL(k, dim3(g), dim3(sh.block_d), lds, s, a, l, t, box0, boxes, w,    \
+                           nchunks, err);                                                    \
+    } while (0)
+#define MSM_DLP(MB, E)                          \
+    do {                                        \
+        if (tpa == 1) MSM_DL(MB, E, 1);         \
+        else if (tpa == 2) MSM_DL(MB, E, 2);    \
+        else MSM_DL(MB, E, 4);                  \
+    } while (0)
+    if (multibox) {
+        if (eam) MSM_DLP(1, 1); else MSM_DLP(1, 0);
+    } else {
+        if (eam) MSM_DLP(0, 1); else MSM_DLP(0, 0);
+    }
+#undef MSM_DLP
+#undef MSM_DL
+}
+
+void launch_force_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDev& l,
+                     const TabDev& t, const BoxDev& box0, const BoxDev* boxes, int multibox,
+                     const StepParams& sp, hipStream_t s) {
+    if (a.n <= 0) return;
+    const LtWin w = lt_window(t, sh.wgs_d, sh.wgs_f);
+    const size_t lds = static_cast<size_t>(w.wf) * sizeof(double2);
+    const long long th = static_cast<long long>(a.n) * tpa;
+    const int nchunks = static_cast<int>((th + sh.block_f - 1) / sh.block_f);
+#define MSM_FL(MB, M, P)                                                                     \
+    do {                                                                                     \
+        auto k = k_force_lt<MB, M, P>;                                                       \
+        const unsigned g = persistent_grid(reinterpret_cast<const void*>(k), sh.block_f,     \
+                                           lds, sh.wgs_f, nchunks);                          \
+        hipLaunchKernelGGL(k, dim3(g), dim3(sh.block_f), lds, s, a, l, t, box0, boxes, sp,   \
+                           w, nchunks);                                                      \
+    } while (0)
+#define MSM_FLP(MB, M)                          \
+    do {                                        \
+        if (tpa == 1) MSM_FL(MB, M, 1);         \
+        else if (tpa == 2) MSM_FL(MB, M, 2);    \
+        else MSM_FL(MB, M, 4);                  \
+    } while (0)
+#define MSM_FLM(MB)                             \
+    do {                                        \
+        if (sp.mode == 0) MSM_FLP(MB, 0);       \
+        else if (sp.mode == 1) MSM_FLP(MB, 1);  \
+        else MSM_FLP(MB, 2);                    \
+    } while (0)
+    if (multibox) MSM_FLM(1); else MSM_FLM(0);
+#undef MSM_FLM
+#undef MSM_FLP
+#undef MSM_FL
+}
+
+}  // namespace msm

@@ -68,10 +68,13 @@ __global__ __launch_bounds__(kBlock) void k_density_tpa(AtomsDev a, ListDev l, T
                 double inv_r;
                 const double sk = table_coord(r2, t.csi, inv_r);
                 const int kk = min(static_cast<int>(sk), kmax);
-                const double2 e = t.b2[kind * t.ntab2 + kk];
-                const double v = e.x + (sk - kk) * e.y;
                 const bool in = base + u * TPA < kv && r2 <= t.ru2max;
-                den0 = den0 + (in ? v : 0.0);
+                double v = 0.0;
+                if (in) {  // knots are fetched for pairs inside the cutoff only
+                    const double2 e = t.b2[kind * t.ntab2 + kk];
+                    v = e.x + (sk - kk) * e.y;
+                }
+                den0 = den0 + v;
                 if (in && l.cidx) {
                     // this step's in-cutoff pairs, in list order, for the force pass
                     l.cidx[list_base(i, l.mx) + static_cast<size_t>(sub + TPA * nin) * l.stride] = j;
@@ -150,18 +153,21 @@ __global__ __launch_bounds__(kBlock) void k_force_tpa(AtomsDev a, ListDev l, Tab
                 const double sk = table_coord(r2, t.csi, inv_r);
                 const int kk = min(static_cast<int>(sk), kmax);
                 const double dk = sk - kk;
-                const double4 f = t.f4[kind * t.ntab2 + kk];
-                double fb1;
-                if constexpr (MT) {
-                    const double4 f1 = t.f4[kind1 * t.ntab2 + kk];
-                    fb1 = f1.z + dk * f1.w;
-                } else {
-                    fb1 = f.z + dk * f.w;
+                double fortot = 0.0;
+                if (base + u * TPA < kv && r2 <= t.ru2max) {
+                    // knots are fetched for pairs inside the cutoff only
+                    const double4 f = t.f4[kind * t.ntab2 + kk];
+                    double fb1;
+                    if constexpr (MT) {
+                        const double4 f1 = t.f4[kind1 * t.ntab2 + kk];
+                        fb1 = f1.z + dk * f1.w;
+                    } else {
+                        fb1 = f.z + dk * f.w;
+                    }
+                    // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL)
+                    fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
+                             ((f.z + dk * f.w) * denki + fb1 * pj.w) * inv_r;
                 }
-                // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL)
-                const double ft = (f.x + dk * f.y) * (inv_r * inv_r) +
-                                  ((f.z + dk * f.w) * denki + fb1 * pj.w) * inv_r;
-                const double fortot = (base + u * TPA < kv && r2 <= t.ru2max) ? ft : 0.0;
                 fpx = fpx + fortot * sx;
                 fpy = fpy + fortot * sy;
                 fpz = fpz + fortot * sz;

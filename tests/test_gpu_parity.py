@@ -377,18 +377,20 @@ def test_million_atom_snapshot(orc):
     dev.close()
 
 
-@pytest.mark.parametrize("path,compact", [(0, 1), (1, 0), (1, 1), (2, 0), (2, 1), (4, 1),
-                                          (8, 1)])
-def test_force_paths_agree(orc, p0, path, compact):
+@pytest.mark.parametrize("path,compact,lds", [(0, 1, 0), (1, 0, 0), (1, 1, 0), (2, 0, 0),
+                                              (2, 1, 0), (3, 0, 0), (4, 1, 0), (8, 1, 0),
+                                              (1, 0, 1), (2, 0, 1), (4, 0, 1)])
+def test_force_paths_agree(orc, p0, path, compact, lds):
     """Every density/force kernel variant (LDS bricks; global list with 1-8 lanes per atom, with
-    and without the compacted in-cutoff list) meets the parity bar on the same box, before and
-    after a 15-step run."""
+    and without the compacted in-cutoff list, with the table in global memory or in LDS) meets
+    the parity bar on the same box, before and after a 15-step run."""
     import msmpscu_amd._lib as L
     box, ctrl, ft = p0
-    say(f"force path {path} compact {compact}")
+    say(f"force path {path} compact {compact} lds table {lds}")
     dev = device_for(box, ctrl, ft)
     L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
     L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_COMPACT_LIST, compact), "msm_set_option")
+    L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS, lds), "msm_set_option")
     dev.build_neighbors()
     dev.calc_force()
     _, _, fp, _, _ = dev.get_atoms()
@@ -400,3 +402,105 @@ def test_force_paths_agree(orc, p0, path, compact):
     fh, *_ = oracle_half(orc, box, ctrl, ft, xp=x)
     assert np.max(np.abs(fp - fh)) <= F_TOL
     dev.close()
+
+
+def _asym_case():
+    box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.08, 300.0, 8)
+    box.NGROUP = 2
+    box.ITYP = (np.arange(box.NPRT) % 2 + 1).astype(np.int32)
+    box.CM = np.array([box.CM[0], box.CM[0] * 0.5])
+    box.PTYPE = np.array([[1, 3], [4, 2]], dtype=np.int32)
+    ctrl = md.SimMDCtrl(RU=np.full((2, 2), 4.4002), NB_RM=np.full((2, 2), 4.7002), NB_MXNBS=64)
+    ft = md.MDForceTable()
+    md.Register_ForceTableProc(1, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne")
+    md.Register_ForceTableProc(2, ft, "msm_pot_ackland_ww_nn", "msm_pot_ackland_ww_ne")
+    md.Register_ForceTableProc(3, ft, "msm_pot_wfs_nn", "msm_pot_ackland_ww_ne")
+    md.Register_ForceTableProc(4, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne")
+    md.Create_Interaction_ForceTable(box, ctrl, ft)
+    return [box], ctrl, ft
+
+
+@pytest.mark.parametrize("variant", ["pair", "lds"])
+@pytest.mark.parametrize("case", ["fs", "eam", "asym", "inactive", "multibox", "slab"])
+def test_kernel_variant_matches_two_lane_path(p0, case, variant):
+    """The lane-pair gather kernels (force path 3) and the LDS-table kernels split each atom's
+    list over two lanes as the two-lane kernels do (the compiler may contract multiply-adds
+    differently, so agreement is to rounding): F' and forces to the parity bar, and positions
+    after a 12-step run to 1e-12 A, over single and multiple species (the LDS table serves one
+    kind; several kinds take the global table), FS and EAM, inactive atoms, uneven boxes and a
+    slab."""
+    import msmpscu_amd._lib as L
+    box, ctrl, ft = p0
+    boxes, statu = [box], None
+    if case == "eam":
+        ft = cfg.w_forcetable(box, ctrl, pot_type="EAM_TYPE")
+    elif case == "asym":
+        boxes, ctrl, ft = _asym_case()
+    elif case == "inactive":
+        statu = box.STATU.copy()
+        statu[[3, 100, 4000, 4001]] = 0
+    elif case == "multibox":
+        boxes = [cfg.make_bcc_box(c, cfg.W_A, 0.05, 500.0, 30 + k, n_vac=k, n_dumbbell=k % 2)
+                 for k, c in enumerate([(5, 6, 7), (9, 9, 9), (3, 4, 5)])]
+        ft = cfg.w_forcetable(boxes[0], ctrl)
+    elif case == "slab":
+        b = cfg.make_cuni_slab((6, 6, 6), 3.57, 600.0, 5)
+        ctrl = cfg.cuni_ctrl()
+        ft = cfg.cuni_forcetable(b, ctrl)
+        boxes = [b]
+    say(f"{variant} vs two lanes: {case}")
+    outs = []
+    for k in range(2):
+        dev = device_for(boxes, ctrl, ft)
+        path = 3 if (k == 1 and variant == "pair") else 2
+        L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
+        L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS,
+                                       int(k == 1 and variant == "lds")), "msm_set_option")
+        if statu is not None:
+            dev.set_atoms(box.XP, box.XP1, box.ITYP, statu)
+        dev.build_neighbors()
+        dev.calc_force()
+        r = [dev.get_den()] + list(dev.get_atoms())
+        dev.run(12, ctrl.NB_UPTAB, 0, ctrl.H)
+        r += list(dev.get_atoms())
+        outs.append(r)
+        dev.close()
+    den2, x2, v2, f2 = outs[0][0], outs[0][1], outs[0][2], outs[0][3]
+    den3, x3, v3, f3 = outs[1][0], outs[1][1], outs[1][2], outs[1][3]
+    assert np.max(np.abs(den3 - den2)) <= 1e-13
+    assert np.max(np.abs(f3 - f2)) <= F_TOL
+    assert np.array_equal(x3, x2)
+    assert np.max(np.abs(outs[1][6] - outs[0][6])) <= 1e-12
+    assert np.max(np.abs(outs[1][8] - outs[0][8])) <= F_TOL
+
+
+@pytest.mark.parametrize("pot_type", ["FS_TYPE", "EAM_TYPE"])
+def test_lds_table_window_fallback(orc, pot_type):
+    """A 12000-knot table leaves the bottom of both tables outside the LDS windows (force window
+    starts near r = 2.7 A, density window near 1.45 A), and a few atoms are pushed to 1.15-1.4 A
+    from a neighbour: the pairs below the windows take the global-table fallback and every
+    result still meets the parity bar against the oracle."""
+    import msmpscu_amd._lib as L
+    box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.05, 300.0, 77)
+    d = np.array([cfg.W_A / 2, cfg.W_A / 2, cfg.W_A / 2])
+    for i, frac in [(10, 0.55), (200, 0.5), (555, 0.58)]:
+        # atom i toward its neighbour at +d: distance |d| * (1 - frac), 1.15-1.4 A
+        box.XP[i] = box.XP[i] + frac * d
+    box.XP = np.asfortranarray(cfg.wrap_into(box.XP, box.BOXLOW, box.ZL, (1, 1, 1)))
+    ctrl = cfg.w_ctrl(ntab=12000)
+    ft = cfg.w_forcetable(box, ctrl, pot_type=pot_type)
+    say(f"lds window fallback {pot_type}")
+    outs = []
+    for lds in (0, 1):
+        dev = device_for(box, ctrl, ft)
+        L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS, lds), "msm_set_option")
+        dev.build_neighbors()
+        dev.calc_force()
+        _, _, fp, _, _ = dev.get_atoms()
+        outs.append((fp, dev.get_den()))
+        dev.close()
+    fo, do, _ = oracle_full(orc, box, ctrl, ft)
+    for fp, den in outs:
+        assert np.max(np.abs(fp - fo)) <= F_TOL * max(1.0, np.max(np.abs(fo)) / 10.0)
+        assert np.max(np.abs(den - do)) <= 1e-13 * max(1.0, np.max(np.abs(do)))
+    assert np.max(np.abs(outs[1][0] - outs[0][0])) <= F_TOL
