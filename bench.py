@@ -62,6 +62,64 @@ def cpu_baseline(box, ctrl, ft, nsteps: int):
     return box.NPRT * nsteps / dt, dt
 
 
+def make_device(boxes, ctrl, ft, pot):
+    import msmpscu_amd.mdlib as md
+    fc = md.MDForceClassGPU(ForceTable=ft)
+    md.Register_ForceClass(pot, fc)
+    md.Initialize_Globle_Variables_DEV(boxes, ctrl, fc, device=0)
+    md.Init_Forcetable_Dev(boxes, ctrl, fc)
+    return fc.Device
+
+
+def run_batch(nbox: int, steps: int, warmup: int, pot: str, tune: bool):
+    """Uneven batch: nbox W boxes (2000-60000 atoms, log-uniform; 0-5 vacancies, 0-3 dumbbells,
+    300-1200 K) run as one multibox simulation (one launch sequence for all boxes, kernel
+    variant chosen by the autotuner on the batch) against the same boxes run one after another,
+    one simulation and launch sequence each, with the library defaults."""
+    import msmpscu_amd.configs as cfg
+    boxes = cfg.make_batch(nbox)
+    ntot = sum(b.NPRT for b in boxes)
+    ctrl = cfg.w_ctrl(mx=64)
+    ft = cfg.w_forcetable(boxes[0], ctrl, pot_type=pot)
+    log(f"batch: {nbox} boxes, {ntot} atoms")
+    dev = make_device(boxes, ctrl, ft, pot)
+    dev.build_neighbors()
+    dev.calc_force()
+    choice = dev.autotune(3) if tune else None
+    log(f"batch autotune: {choice}")
+    dev.run(warmup, ctrl.NB_UPTAB, 0, ctrl.H)
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.run(steps, ctrl.NB_UPTAB, warmup, ctrl.H)
+    dev.sync()
+    tb = time.perf_counter() - t0
+    dev.close()
+    rate_b = ntot * steps / tb
+    log(f"batched: {rate_b:.4e} atom-steps/s ({1e3 * tb / steps:.4f} ms/step)")
+    devs = []
+    for b in boxes:
+        d = make_device([b], ctrl, ft, pot)
+        d.build_neighbors()
+        d.calc_force()
+        d.run(warmup, ctrl.NB_UPTAB, 0, ctrl.H)
+        d.sync()
+        devs.append(d)
+    t0 = time.perf_counter()
+    for d in devs:
+        d.run(steps, ctrl.NB_UPTAB, warmup, ctrl.H)
+        d.sync()
+    ts = time.perf_counter() - t0
+    for d in devs:
+        d.close()
+    rate_s = ntot * steps / ts
+    log(f"one box at a time: {rate_s:.4e} atom-steps/s; ratio {rate_b / rate_s:.2f}")
+    return {"value": round(rate_b, 1), "unit": "atom-steps/s", "per_box_value": round(rate_s, 1),
+            "ratio": round(rate_b / rate_s, 3), "boxes": nbox, "atoms": ntot, "steps": steps,
+            "warmup": warmup, "autotune": choice,
+            "workload": f"{nbox} W boxes, 2000-60000 atoms log-uniform, 0-5 vacancies, 0-3 "
+                        "<110> dumbbells, 300-1200 K, FS 5000-pt tables, rebuild every 10"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +142,10 @@ def main():
                     help="1: table knots from an LDS window (one table kind, paths 1, 2, 4)")
     ap.add_argument("--lt-shape", default=None,
                     help="LDS-table passes: density block,wgs,force block,wgs (e.g. 1024,1,1024,1)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch measurement")
+    ap.add_argument("--batch-boxes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -188,6 +250,11 @@ def main():
                   "achieved_GBs": round(step_bytes * value / 1e9, 2),
                   "frac": round(step_bytes * value / 1e9 / HBM_PEAK_GBS, 4)}
 
+    dev.close()
+    batch = None
+    if not args.no_batch:
+        batch = run_batch(args.batch_boxes, args.batch_steps, 10, args.pot, not args.no_autotune)
+
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: oracle serial half-list MD, {args.cpu_steps} steps on the same box")
@@ -219,10 +286,10 @@ def main():
         "roofline": roof,
         "step_model": step_model,
         "kernel_times": kt,
+        "batched": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    dev.close()
 
 
 if __name__ == "__main__":

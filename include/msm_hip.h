@@ -210,6 +210,16 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
 #define MSM_OPT_LT_FORCE_WGS     10
 int msm_set_option(msm_sim* sim, int option, int value);
 
+/* Kernel-variant autotuner (no counterpart in the reference, whose kernel shapes are fixed at
+ * compile time).  On the current atoms and list it times `reps` rebuilds with each list-build
+ * kernel and `reps` density + force passes (no integration) of every applicable variant of the
+ * density/force kernels (lanes per atom, LDS table and its workgroup shape, lane-pair gather,
+ * LDS bricks), sets the options to the fastest, and recomputes F' and F with it.  choice[8]
+ * (may be NULL) receives {force path, table LDS, density block, density workgroups per CU, force
+ * block, force workgroups per CU, list kernel, 0}; best_ms[2] (may be NULL) the chosen
+ * density+force and rebuild times in ms.  Positions and velocities are unchanged. */
+int msm_autotune(msm_sim* sim, int reps, int* choice, double* best_ms);
+
 /* Kernel classes for msm_kernel_times. */
 #define MSM_KCLASS_DENSITY   0
 #define MSM_KCLASS_FORCE     1  /* force pass with the fused corrector/predictor */

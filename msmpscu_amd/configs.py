@@ -195,3 +195,20 @@ def bcc_cells_for(natoms: int) -> tuple:
             if best is None or err < best[0]:
                 best = (err, (nx, ny, nz))
     return best[1]
+
+
+def make_batch(nbox: int = 256, lo: int = 2000, hi: int = 60000, seed: int = 4242,
+               sigma: float = 0.05) -> List[SimMDBox]:
+    """Uneven batch of W boxes: sizes log-uniform in [lo, hi] atoms (near-cubic BCC cell
+    counts), each with 0-5 vacancies, 0-3 <110> dumbbells and a temperature drawn from
+    U(300, 1200) K, every box seeded on its own."""
+    rng = np.random.default_rng(seed)
+    sizes = batch_sizes(nbox, lo, hi, seed)
+    boxes = []
+    for k, na in enumerate(sizes):
+        nv = int(rng.integers(0, 6))
+        nd = int(rng.integers(0, 4))
+        temp = float(rng.uniform(300.0, 1200.0))
+        boxes.append(make_bcc_box(bcc_cells_for(na), W_A, sigma, temp, seed * 1000 + k,
+                                  n_vac=nv, n_dumbbell=nd))
+    return boxes

@@ -1108,6 +1108,98 @@ int msm_kernel_times(msm_sim* s, double* ms, long long* count, int nclass, int r
     return MSM_OK;
 }
 
+int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (reps < 1 || reps > 1000) return set_err(s, MSM_ERR_ARG, "reps must be 1..1000");
+    (void)hipSetDevice(s->device);
+    struct Cand {
+        int fp, lds, bd, wd, bf, wf;
+    };
+    std::vector<Cand> cands;
+    if (s->tab.ngroup == 1) {
+        for (int fp : {2, 1, 4}) {
+            cands.push_back({fp, 1, 1024, 1, 1024, 1});
+            cands.push_back({fp, 1, 256, 4, 256, 4});
+        }
+        cands.push_back({2, 1, 512, 2, 512, 2});
+    }
+    for (int fp : {2, 1, 3, 4, 8, 0}) cands.push_back({fp, 0, s->lt.block_d, s->lt.wgs_d,
+                                                       s->lt.block_f, s->lt.wgs_f});
+    const int saved_timing = s->timing;
+    s->timing = 0;
+    s->compact = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(s, hipEventCreate(&e0));
+    HIPCHK(s, hipEventCreate(&e1));
+    auto time_it = [&](auto&& body, float& ms) -> int {
+        body();  // warm-up (code object load, occupancy query)
+        HIPCHK(s, hipEventRecord(e0, s->stream));
+        for (int r = 0; r < reps; ++r) body();
+        HIPCHK(s, hipEventRecord(e1, s->stream));
+        HIPCHK(s, hipEventSynchronize(e1));
+        HIPCHK(s, hipEventElapsedTime(&ms, e0, e1));
+        ms /= static_cast<float>(reps);
+        return MSM_OK;
+    };
+    clear_flags(s);
+    // list-build kernel (a rebuild of unchanged positions reproduces the same order and list)
+    float best_nl = 0.f;
+    int nl_choice = s->nlist_global;
+    for (int g : {0, 1}) {
+        s->nlist_global = g;
+        float ms = 0.f;
+        rc = time_it([&] { enqueue_rebuild(s); }, ms);
+        if (rc != MSM_OK) break;
+        if (g == 0 || ms < best_nl) {
+            best_nl = ms;
+            nl_choice = g;
+        }
+    }
+    s->nlist_global = nl_choice;
+    if (rc == MSM_OK) {
+        enqueue_rebuild(s);
+    }
+    // density + force passes
+    float best = 0.f;
+    size_t bi = 0;
+    for (size_t c = 0; c < cands.size() && rc == MSM_OK; ++c) {
+        s->force_path = cands[c].fp;
+        s->table_lds = cands[c].lds;
+        s->lt = msm::LtShape{cands[c].bd, cands[c].wd, cands[c].bf, cands[c].wf};
+        float ms = 0.f;
+        rc = time_it([&] {
+            enqueue_density(s);
+            enqueue_force(s, 0, 1, 0.0);
+        }, ms);
+        if (rc == MSM_OK && (c == 0 || ms < best)) {
+            best = ms;
+            bi = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->timing = saved_timing;
+    if (rc != MSM_OK) return rc;
+    const Cand& w = cands[bi];
+    s->force_path = w.fp;
+    s->table_lds = w.lds;
+    s->lt = msm::LtShape{w.bd, w.wd, w.bf, w.wf};
+    clear_graphs(s);
+    // leave F' and F as the chosen kernels compute them
+    enqueue_density(s);
+    enqueue_force(s, 0, 1, 0.0);
+    if (choice) {
+        const int c[8] = {w.fp, w.lds, w.bd, w.wd, w.bf, w.wf, s->nlist_global, 0};
+        for (int k = 0; k < 8; ++k) choice[k] = c[k];
+    }
+    if (best_ms) {
+        best_ms[0] = best;
+        best_ms[1] = best_nl;
+    }
+    return read_flags(s);
+}
+
 int msm_list_stats(msm_sim* s, long long* total, int* maxk) {
     if (!s || !total || !maxk) return MSM_ERR_ARG;
     if (!s->has_list) return set_err(s, MSM_ERR_STATE, "no neighbour list");

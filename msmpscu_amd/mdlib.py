@@ -401,6 +401,27 @@ class DeviceMD:
     def sync(self):
         check(lib().msm_sync(self.h), "msm_sync", self.h)
 
+    def set_option(self, option: int, value: int):
+        check(lib().msm_set_option(self.h, int(option), int(value)), "msm_set_option", self.h)
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Times the kernel variants on the current atoms and list and keeps the fastest
+        (msm_autotune); returns the choice and the chosen per-pass times."""
+        ch = np.zeros(8, dtype=np.int32)
+        ms = np.zeros(2)
+        check(lib().msm_autotune(self.h, int(reps), iptr(ch), dptr(ms)), "msm_autotune", self.h)
+        return {"force_path": int(ch[0]), "table_lds": int(ch[1]),
+                "lt_shape": [int(v) for v in ch[2:6]], "nlist_global": int(ch[6]),
+                "density_force_ms": float(ms[0]), "rebuild_ms": float(ms[1])}
+
+    def kernel_times(self, reset: bool = True) -> dict:
+        from ._lib import KCLASS
+        ms = np.zeros(len(KCLASS))
+        cnt = np.zeros(len(KCLASS), dtype=np.int64)
+        check(lib().msm_kernel_times(self.h, dptr(ms), cnt.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                     len(KCLASS), 1 if reset else 0), "msm_kernel_times", self.h)
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KCLASS)}
+
     # ---- copy-out ----
     def get_atoms(self):
         n = self.n

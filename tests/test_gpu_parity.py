@@ -504,3 +504,58 @@ def test_lds_table_window_fallback(orc, pot_type):
         assert np.max(np.abs(fp - fo)) <= F_TOL * max(1.0, np.max(np.abs(fo)) / 10.0)
         assert np.max(np.abs(den - do)) <= 1e-13 * max(1.0, np.max(np.abs(do)))
     assert np.max(np.abs(outs[1][0] - outs[0][0])) <= F_TOL
+
+
+def test_autotune_keeps_state_and_parity(orc, p0):
+    """The autotuner times kernel variants on the live state: positions and velocities stay as
+    they were, the chosen variant's forces meet the parity bar, and a run afterwards matches the
+    oracle's forces on the resulting snapshot."""
+    box, ctrl, ft = p0
+    say("autotune")
+    dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    dev.calc_force()
+    x0, v0, _, _, _ = dev.get_atoms()
+    choice = dev.autotune(2)
+    say(f"autotune choice {choice}")
+    x1, v1, f1, _, _ = dev.get_atoms()
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+    fh, *_ = oracle_half(orc, box, ctrl, ft)
+    assert np.max(np.abs(f1 - fh)) <= F_TOL
+    dev.run(12, ctrl.NB_UPTAB, 0, ctrl.H)
+    x, _, fp, _, _ = dev.get_atoms()
+    fh, *_ = oracle_half(orc, box, ctrl, ft, xp=x)
+    assert np.max(np.abs(fp - fh)) <= F_TOL
+    dev.close()
+
+
+def test_uneven_batch_matches_single_boxes():
+    """A small uneven batch (sizes 500-4000, vacancies, dumbbells, 300-1200 K) run as one
+    multibox simulation after autotuning agrees with each box run alone: positions after 20
+    steps to 1e-12 A and per-box energies to 1e-12 relative."""
+    ctrl = cfg.w_ctrl(mx=64)
+    boxes = cfg.make_batch(9, 500, 4000, seed=17)
+    ft = cfg.w_forcetable(boxes[0], ctrl)
+    say(f"uneven batch n={[b.NPRT for b in boxes]}")
+    dev = device_for(boxes, ctrl, ft)
+    dev.build_neighbors()
+    dev.calc_force()
+    dev.autotune(2)
+    dev.run(20, ctrl.NB_UPTAB, 0, ctrl.H)
+    dev.calc_epot()
+    xb, _, _, _, _ = dev.get_atoms()
+    eb = dev.get_box_epot()
+    dev.close()
+    off = 0
+    for k, b in enumerate(boxes):
+        d1 = device_for(b, ctrl, ft)
+        d1.build_neighbors()
+        d1.calc_force()
+        d1.run(20, ctrl.NB_UPTAB, 0, ctrl.H)
+        d1.calc_epot()
+        x1, _, _, _, _ = d1.get_atoms()
+        assert np.max(np.abs(xb[off:off + b.NPRT] - x1)) <= 1e-12
+        e1 = d1.get_box_epot()[0]
+        assert abs(eb[k] - e1) <= 1e-12 * abs(e1)
+        off += b.NPRT
+        d1.close()
