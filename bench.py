@@ -120,6 +120,27 @@ def run_batch(nbox: int, steps: int, warmup: int, pot: str, tune: bool):
                         "<110> dumbbells, 300-1200 K, FS 5000-pt tables, rebuild every 10"}
 
 
+def run_adjoint(reps: int):
+    """Coefficient gradients of the force/energy matching loss on 64 strained 1024-atom W
+    boxes with EAM tables (3 x 5000 spline coefficients): full gradients per second (forward
+    + msm_adjoint + spline transpose, tables rebuilt from the coefficients every time) and the
+    cost of one gradient relative to one forward evaluation."""
+    from msmpscu_amd import adjoint as adj
+    boxes, ctrl, ft = adj.make_d_config()
+    fm = adj.ForceMatch(boxes, ctrl, ft)
+    c0 = fm.model.fit(ft)
+    fm.set_targets(adj.perturbed(c0, 0.01, seed=5))
+    tm = fm.timing(c0, reps)
+    fm.close()
+    log(f"adjoint: {tm}")
+    return {"value": round(tm["grad_per_s"], 2), "unit": "grad/s",
+            "cost_vs_forward": round(tm["cost_ratio"], 3),
+            "forward_ms": round(1e3 * tm["forward_s"], 3), "grad_ms": round(1e3 * tm["grad_s"], 3),
+            "workload": "64 W boxes x 1024 atoms (8^3 bcc, sigma 0.1 A, diagonal strain up to "
+                        "2%), EAM 5000-pt tables, gradient of mean|dF|^2 + 0.1 mean(dE/atom)^2 "
+                        "wrt 3 x 5000 B-spline coefficients, FP64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +167,8 @@ def main():
     ap.add_argument("--batch-boxes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient measurement")
+    ap.add_argument("--adjoint-reps", type=int, default=20)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -255,6 +278,10 @@ def main():
     if not args.no_batch:
         batch = run_batch(args.batch_boxes, args.batch_steps, 10, args.pot, not args.no_autotune)
 
+    grad = None
+    if not args.no_adjoint:
+        grad = run_adjoint(args.adjoint_reps)
+
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: oracle serial half-list MD, {args.cpu_steps} steps on the same box")
@@ -287,6 +314,7 @@ def main():
         "step_model": step_model,
         "kernel_times": kt,
         "batched": batch,
+        "gradient": grad,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -210,6 +210,19 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
 #define MSM_OPT_LT_FORCE_WGS     10
 int msm_set_option(msm_sim* sim, int option, int value);
 
+/* Adjoint (reverse mode) of the force and energy evaluation with respect to the table knots,
+ * for one atom type (one table kind).  Given dL/dF (dldf, 3*N doubles in the caller's atom order
+ * and FP(N,3) layout) and dL/dE per box (dlde, one double per box, E_b = the box sum of
+ * msm_get_epot), writes dL/d(knot) for every knot i = 1..NTAB of POTR, FPOTR, POTB, FPOTB and,
+ * for EAM tables, i = 1..NEMBD of FEMBD and DFEMBD (same layouts as msm_set_force_table; the
+ * embedding arrays may be NULL for FS tables).  Uses the current positions and neighbour list;
+ * recomputes rho and F' itself.  The per-pair terms are summed with LDS atomics, so repeated
+ * calls agree to rounding.  Corresponds to the C-ABI item "msm_adjoint(h, dLdF, dLdE,
+ * out_grad_tables)" of the hot-path boundary; the reference has no derivative code. */
+int msm_adjoint(msm_sim* sim, const double* dldf, const double* dlde, double* g_potr,
+                double* g_fpotr, double* g_potb, double* g_fpotb, double* g_fembd,
+                double* g_dfembd);
+
 /* Kernel-variant autotuner (no counterpart in the reference, whose kernel shapes are fixed at
  * compile time).  On the current atoms and list it times `reps` rebuilds with each list-build
  * kernel and `reps` density + force passes (no integration) of every applicable variant of the

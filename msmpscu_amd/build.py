@@ -36,7 +36,7 @@ def _run(cmd, cwd=None):
 
 
 def build_library() -> str:
-    srcs = [os.path.join(CSRC, f) for f in ("msm_api.hip", "msm_kernels.hip", "msm_bricks.hip", "msm_tpa.hip", "msm_pair.hip", "msm_ldstab.hip",
+    srcs = [os.path.join(CSRC, f) for f in ("msm_api.hip", "msm_kernels.hip", "msm_bricks.hip", "msm_tpa.hip", "msm_pair.hip", "msm_ldstab.hip", "msm_adjoint.hip",
                                              "msm_host.cpp")]
     tmp = LIB + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,

@@ -105,6 +105,11 @@ struct msm_sim {
     msm::LtShape lt{1024, 1, 1024, 1};  // block / workgroups per CU of the LDS-table passes
     int compact = 0;       // global-list path: force pass reads the density pass's in-cutoff list
 
+    // adjoint scratch (allocated on first use)
+    double *adj_dldf = nullptr, *adj_gbox = nullptr, *adj_g = nullptr, *adj_rho = nullptr,
+           *adj_lam = nullptr, *adj_part = nullptr, *adj_out = nullptr;
+    size_t adj_cap_n = 0, adj_cap_part = 0, adj_cap_out = 0, adj_cap_box = 0;
+
     // LDS-staged list bricks
     int nbricks = 0;
     int* d_brick_box = nullptr;
@@ -541,6 +546,9 @@ int msm_destroy(msm_sim* s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     clear_graphs(s);
     free_atoms(s);
+    for (double** p : {&s->adj_dldf, &s->adj_gbox, &s->adj_g, &s->adj_rho, &s->adj_lam,
+                       &s->adj_part, &s->adj_out})
+        dfree(p);
     for (auto& p : s->pending) {
         (void)hipEventDestroy(std::get<1>(p));
         (void)hipEventDestroy(std::get<2>(p));
@@ -1198,6 +1206,92 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
         best_ms[1] = best_nl;
     }
     return read_flags(s);
+}
+
+int msm_adjoint(msm_sim* s, const double* dldf, const double* dlde, double* g_potr,
+                double* g_fpotr, double* g_potb, double* g_fpotb, double* g_fembd,
+                double* g_dfembd) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (!dldf || !dlde || !g_potr || !g_fpotr || !g_potb || !g_fpotb)
+        return set_err(s, MSM_ERR_ARG, "null argument");
+    const bool eam = s->tab.pot_type != 0;
+    if (eam && (!g_fembd || !g_dfembd))
+        return set_err(s, MSM_ERR_ARG, "EAM tables need the embedding gradient arrays");
+    if (s->tab.ngroup != 1)
+        return set_err(s, MSM_ERR_ARG, "the adjoint supports one atom type (one table kind)");
+    (void)hipSetDevice(s->device);
+    const int n = s->n, nbox = static_cast<int>(s->boxes.size());
+    const int nt = s->tab.ntab2, ne = eam ? s->tab.nembd2 : 0;
+    const size_t nb1 = 3 * static_cast<size_t>(nt), nb2 = nt + 2 * static_cast<size_t>(ne);
+    const size_t stride = std::max(nb1, nb2);
+    const size_t npart = stride * msm::adjoint_workgroups(n);
+    if (s->adj_cap_n < static_cast<size_t>(s->n_alloc)) {
+        dfree(&s->adj_dldf);
+        dfree(&s->adj_g);
+        dfree(&s->adj_rho);
+        dfree(&s->adj_lam);
+        s->adj_cap_n = 0;
+        rc = dalloc(s, &s->adj_dldf, 3 * static_cast<size_t>(s->n_alloc));
+        if (rc == MSM_OK) rc = dalloc(s, &s->adj_g, 3 * static_cast<size_t>(s->n_alloc));
+        if (rc == MSM_OK) rc = dalloc(s, &s->adj_rho, s->n_alloc);
+        if (rc == MSM_OK) rc = dalloc(s, &s->adj_lam, s->n_alloc);
+        if (rc != MSM_OK) return rc;
+        s->adj_cap_n = s->n_alloc;
+    }
+    if (s->adj_cap_part < npart) {
+        dfree(&s->adj_part);
+        s->adj_cap_part = 0;
+        if ((rc = dalloc(s, &s->adj_part, npart)) != MSM_OK) return rc;
+        s->adj_cap_part = npart;
+    }
+    if (s->adj_cap_out < nb1 + nb2) {
+        dfree(&s->adj_out);
+        s->adj_cap_out = 0;
+        if ((rc = dalloc(s, &s->adj_out, nb1 + nb2)) != MSM_OK) return rc;
+        s->adj_cap_out = nb1 + nb2;
+    }
+    if (s->adj_cap_box < static_cast<size_t>(nbox)) {
+        dfree(&s->adj_gbox);
+        s->adj_cap_box = 0;
+        if ((rc = dalloc(s, &s->adj_gbox, nbox)) != MSM_OK) return rc;
+        s->adj_cap_box = nbox;
+    }
+    HIPCHK(s, hipMemcpyAsync(s->adj_dldf, dldf, 3 * static_cast<size_t>(n) * sizeof(double),
+                             hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->adj_gbox, dlde, nbox * sizeof(double), hipMemcpyHostToDevice,
+                             s->stream));
+    msm::AdjDev d{};
+    d.g = s->adj_g;
+    d.gbox = s->adj_gbox;
+    d.rho = s->adj_rho;
+    d.lam = s->adj_lam;
+    d.part = s->adj_part;
+    d.part_stride = stride;
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_adjoint(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
+                        s->adj_dldf, s->set[s->scur].gid, d, s->adj_out, s->adj_out + nb1,
+                        s->stream);
+    std::vector<double> h(nb1 + nb2);
+    HIPCHK(s, hipMemcpyAsync(h.data(), s->adj_out, h.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    // knot i of a table (row i = 1..NTAB of the caller's layout) is bin i
+    const int ntab = nt - 2;
+    for (int i = 1; i <= ntab; ++i) {
+        g_fpotr[i - 1] = h[i];
+        g_fpotb[i - 1] = h[nt + i];
+        g_potr[i - 1] = h[2 * nt + i];
+        g_potb[i - 1] = h[nb1 + i];
+    }
+    if (eam) {
+        const int nembd = ne - 2;
+        for (int i = 1; i <= nembd; ++i) {
+            g_fembd[i - 1] = h[nb1 + nt + i];
+            g_dfembd[i - 1] = h[nb1 + nt + ne + i];
+        }
+    }
+    return MSM_OK;
 }
 
 int msm_list_stats(msm_sim* s, long long* total, int* maxk) {

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_density_bricks(const int* __restrict
                         const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
                         double inv_r;
                         const double sk = table_coord(r2, t.csi, inv_r);
-                        const int kk = static_cast<int>(sk);
+                        const int kk = min(static_cast<int>(sk), t.ntab2 - 1);
                         const double2 e = t.b2[kind * t.ntab2 + kk];
                         den0 = den0 + (e.x + (sk - kk) * e.y);
                     }
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_force_bricks(const int* __restrict__
                     }
                     double inv_r;
                     const double sk = table_coord(r2, t.csi, inv_r);
-                    const int kk = static_cast<int>(sk);
+                    const int kk = min(static_cast<int>(sk), t.ntab2 - 1);
                     const double dk = sk - kk;
                     const double4 f = t.f4[kind * t.ntab2 + kk];
                     double fb1;

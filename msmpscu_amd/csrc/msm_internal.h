@@ -146,6 +146,21 @@ void launch_density_lt(int tpa, const LtShape& sh, const AtomsDev& a, const List
 void launch_force_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDev& l,
                      const TabDev& t, const BoxDev& box0, const BoxDev* boxes, int multibox,
                      const StepParams& sp, hipStream_t s);
+// Reverse-mode table gradients (msm_adjoint.hip): scratch of the adjoint passes.
+struct AdjDev {
+    double* g;           // dL/dF in the sorted order, 3 x n (zero for inactive atoms)
+    const double* gbox;  // dL/dE per box
+    double* rho;         // rho_k
+    double* lam;         // dL/dF'_k
+    double* part;        // per-workgroup bins, part_stride doubles each
+    size_t part_stride;
+};
+int adjoint_workgroups(int n);
+// out_pairs: 3 x ntab2 knot gradients [FPOTR | FPOTB | POTR]; out_dens: ntab2 + 2 x nembd2
+// [POTB | FEMBD | DFEMBD] (embedding parts for EAM only).
+void launch_adjoint(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                    const BoxDev* boxes, int multibox, const double* dldf, const int* gid,
+                    AdjDev d, double* out_pairs, double* out_dens, hipStream_t s);
 // Two lanes per atom with 32-byte records gathered by lane pairs (msm_pair.hip); the compacted
 // list is not used there.
 void launch_density_pair(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,

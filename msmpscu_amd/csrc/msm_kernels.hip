@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_density(AtomsDev a, ListDev l, TabDe
                 const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
                 double inv_r;
                 const double sk = table_coord(r2, t.csi, inv_r);
-                const int kk = static_cast<int>(sk);
+                const int kk = min(static_cast<int>(sk), t.ntab2 - 1);
                 const double2 e = t.b2[kind * t.ntab2 + kk];
                 den0 = den0 + (e.x + (sk - kk) * e.y);
             }
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_force(AtomsDev a, ListDev l, TabDev 
                 }
                 double inv_r;
                 const double sk = table_coord(r2, t.csi, inv_r);
-                const int kk = static_cast<int>(sk);
+                const int kk = min(static_cast<int>(sk), t.ntab2 - 1);
                 const double dk = sk - kk;
                 const double4 f = t.f4[kind * t.ntab2 + kk];
                 double fb1;
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_epot(AtomsDev a, ListDev l, TabDev t
                 const int kind = t.kpair[ti * t.ngroup + a.ityp[j]];
                 double inv_r;
                 const double sk = table_coord(r2, t.csi, inv_r);
-                const int kk = static_cast<int>(sk);
+                const int kk = min(static_cast<int>(sk), t.ntab2 - 1);
                 const double dk = sk - kk;
                 const double4 e = t.e4[kind * t.ntab2 + kk];
                 er0 = er0 + (e.x + dk * e.y) * inv_r;

@@ -13,6 +13,8 @@ Parity status of each restated function (how it is pinned):
   orc_force_fs_half    pinned: perfect-lattice F = 0, sum F = 0, F = -dE/dx (central FD),
                        analytic lattice energy, full-list (device-kernel) restatement agreement
   orc_force_full       pinned: as above, plus FS == EAM with F = -sqrt(rho) tabulated
+  orc_adjoint_full     pinned: central finite differences of orc_force_full (FS and EAM, every
+                       table, inactive atoms), linearity of the pair-table gradients
   orc_predictor/corr.  pinned: closed-form free flight and constant-force motion, NVE drift
   orc_drand32          pinned: exact modular arithmetic of L'Ecuyer's generator
   orc_thermalize_mc    pinned: zero net momentum, temperature statistics
@@ -218,6 +220,33 @@ def force_full(xp, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table: Table):
     if rc != 0:
         raise RuntimeError(f"oracle force_full error {rc}")
     return fp.reshape(3, n).T.copy(), dden, ep
+
+
+def adjoint_full(xp, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table: Table, dldf, dlde,
+                 grads=None):
+    """Reverse mode of force_full for one table kind: adds dL/d(knot) for G = dL/dFP (N,3) and
+    g = dL/dE (E = sum of EPOT) into grads (dict of padded rows; created when None)."""
+    n = int(np.asarray(ityp0).shape[0])
+    nt, ne = table.ntab + 2, table.nembd + 2
+    if grads is None:
+        grads = {k: np.zeros(nt) for k in ("potr", "fpotr", "potb", "fpotb")}
+        grads.update({k: np.zeros(ne) for k in ("fembd", "dfembd")})
+    x = _xp(xp)
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    st = np.ascontiguousarray(statu, dtype=np.int32)
+    kv = np.ascontiguousarray(kvois, dtype=np.int32)
+    ind = np.ascontiguousarray(np.asarray(indi, dtype=np.int32).T.ravel())
+    g = _xp(dldf)
+    emb = table.pot_type != 0
+    rc = lib().orc_adjoint_full(n, _d(x), _i(it), _i(st), _i(kv), _i(ind), _d(_v3(ln)),
+                                _i(_v3(ifpd, np.int32)), C.c_double(ru2max),
+                                C.byref(table.struct()), _d(g), C.c_double(dlde),
+                                _d(grads["potr"]), _d(grads["fpotr"]), _d(grads["potb"]),
+                                _d(grads["fpotb"]), _d(grads["fembd"]) if emb else None,
+                                _d(grads["dfembd"]) if emb else None)
+    if rc != 0:
+        raise RuntimeError(f"oracle adjoint error {rc}")
+    return grads
 
 
 def predictor(xp, xp1, fp, dis, statu, ityp0, cm, h, lo, ln, ifpd):

@@ -518,6 +518,153 @@ int orc_force_full(int n, const double* xp, const int* ityp, const int* statu, c
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* adjoint of the full-list force and energy (one table kind)                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+/* F'(rho) and dF'/drho of the density pass; E_emb(rho) and dE_emb/drho of the energy pass, with
+ * the knot (kk) and weight (w = sk - kk) of each EAM lookup. */
+static void emb_terms(const orc_ftable* t, double rho, double* fp, double* dfp, double* de,
+                      int* kf, double* wf, int* ke, double* we) {
+    *kf = -1;
+    *ke = -1;
+    if (t->pot_type == 0) {
+        if (rho > 0.0) {
+            double sr = sqrt(rho);
+            *fp = -0.5 / sr;
+            *dfp = 0.25 / (rho * sr);
+            *de = -0.5 / sr;
+        } else {
+            *fp = rho;
+            *dfp = 1.0;
+            *de = 0.0;
+        }
+        return;
+    }
+    double sk = rho / t->rhod + 1.0;
+    int kk = (int)(sk + 0.000001);
+    if (kk < 0) kk = 0;
+    if (kk > t->nembd) kk = t->nembd;
+    int k1 = t->kembd[0];
+    *de = (ETAB(t->fembd, k1, kk + 1) - ETAB(t->fembd, k1, kk)) / t->rhod;
+    *ke = kk;
+    *we = sk - (double)kk;
+    if (rho > 0.0) {
+        *fp = ETAB(t->dfembd, k1, kk) + (sk - kk) * (ETAB(t->dfembd, k1, kk + 1) - ETAB(t->dfembd, k1, kk));
+        *dfp = (ETAB(t->dfembd, k1, kk + 1) - ETAB(t->dfembd, k1, kk)) / t->rhod;
+        *kf = kk;
+        *wf = sk - (double)kk;
+    } else {
+        *fp = rho;
+        *dfp = 1.0;
+    }
+}
+
+int orc_adjoint_full(int n, const double* xp, const int* ityp, const int* statu,
+                     const int* kvois, const int* indi, const double len[3], const int ifpd[3],
+                     double ru2max, const orc_ftable* t, const double* dldf, double dlde,
+                     double* g_potr, double* g_fpotr, double* g_potb, double* g_fpotb,
+                     double* g_fembd, double* g_dfembd) {
+    (void)ityp;
+    if (t->nkind != 1 || t->ngroup != 1) return ORC_ERR_ARG;
+    if (t->pot_type != 0 && (!g_fembd || !g_dfembd || t->nkind1 != 1)) return ORC_ERR_ARG;
+    double hbox[3] = {0.5 * len[0], 0.5 * len[1], 0.5 * len[2]};
+    double* rho = (double*)calloc((size_t)n, sizeof(double));
+    double* fpr = (double*)calloc((size_t)n, sizeof(double));
+    double* lam = (double*)calloc((size_t)n, sizeof(double));
+    double* g = (double*)calloc((size_t)3 * n, sizeof(double));
+    if (!rho || !fpr || !lam || !g) {
+        free(rho); free(fpr); free(lam); free(g);
+        return ORC_ERR_NOMEM;
+    }
+#define ACT(i) ((statu[i] & ORC_STATU_ACTIVE) == ORC_STATU_ACTIVE)
+#define PAIR_GEOM(i, j)                                                                   \
+    double s[3];                                                                          \
+    for (int k = 0; k < 3; ++k)                                                           \
+        s[k] = min_image(xp[i + k * n] - xp[j + k * n], len[k], hbox[k], ifpd[k]);        \
+    double r2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    /* G of inactive atoms does not reach the tables (their force is identically zero) */
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) g[i + k * n] = ACT(i) ? dldf[i + k * n] : 0.0;
+    /* forward density and F' */
+    for (int i = 0; i < n; ++i) {
+        if (!ACT(i)) continue;
+        double den0 = 0.0;
+        for (int iw = 0; iw < kvois[i]; ++iw) {
+            int j = indi[i + (size_t)iw * n];
+            PAIR_GEOM(i, j)
+            if (r2 <= ru2max) {
+                double r = sqrt(r2), sk = sqrt(r) * t->csi;
+                int kk = (int)sk;
+                den0 = den0 + (TAB(t->potb, 0, kk) + (sk - kk) * (TAB(t->potb, 0, kk + 1) - TAB(t->potb, 0, kk)));
+            }
+        }
+        rho[i] = den0;
+        double dfp, de, wf, we;
+        int kf, ke;
+        emb_terms(t, den0, &fpr[i], &dfp, &de, &kf, &wf, &ke, &we);
+    }
+    /* pair terms and lambda */
+    for (int i = 0; i < n; ++i) {
+        if (!ACT(i)) continue;
+        for (int iw = 0; iw < kvois[i]; ++iw) {
+            int j = indi[i + (size_t)iw * n];
+            PAIR_GEOM(i, j)
+            if (!(r2 <= ru2max)) continue;
+            double r = sqrt(r2), sk = sqrt(r) * t->csi;
+            int kk = (int)sk;
+            double dk = sk - kk;
+            double bp = TAB(t->fpotb, 0, kk) + dk * (TAB(t->fpotb, 0, kk + 1) - TAB(t->fpotb, 0, kk));
+            double ai = g[i] * s[0] + g[i + n] * s[1] + g[i + 2 * n] * s[2];
+            double aj = g[j] * s[0] + g[j + n] * s[1] + g[j + 2 * n] * s[2];
+            double w;
+            w = ai / r2;
+            g_fpotr[kk] += w * (1.0 - dk);
+            g_fpotr[kk + 1] += w * dk;
+            w = ai * (fpr[i] + fpr[j]) / r;
+            g_fpotb[kk] += w * (1.0 - dk);
+            g_fpotb[kk + 1] += w * dk;
+            w = dlde / r;
+            g_potr[kk] += w * (1.0 - dk);
+            g_potr[kk + 1] += w * dk;
+            lam[i] += (ai - aj) * bp / r;
+        }
+    }
+    /* density terms: mu_i = lambda_i dF'/drho + g dE/drho */
+    for (int i = 0; i < n; ++i) {
+        if (!ACT(i)) continue;
+        double fp, dfp, de, wf = 0.0, we = 0.0;
+        int kf, ke;
+        emb_terms(t, rho[i], &fp, &dfp, &de, &kf, &wf, &ke, &we);
+        if (t->pot_type != 0) {
+            g_fembd[ke] += dlde * (1.0 - we);
+            g_fembd[ke + 1] += dlde * we;
+            if (kf >= 0) {
+                g_dfembd[kf] += lam[i] * (1.0 - wf);
+                g_dfembd[kf + 1] += lam[i] * wf;
+            }
+        }
+        double mu = lam[i] * dfp + dlde * de;
+        for (int iw = 0; iw < kvois[i]; ++iw) {
+            int j = indi[i + (size_t)iw * n];
+            PAIR_GEOM(i, j)
+            if (!(r2 <= ru2max)) continue;
+            double r = sqrt(r2), sk = sqrt(r) * t->csi;
+            int kk = (int)sk;
+            double dk = sk - kk;
+            g_potb[kk] += mu * (1.0 - dk);
+            g_potb[kk + 1] += mu * dk;
+        }
+    }
+#undef PAIR_GEOM
+#undef ACT
+    free(rho);
+    free(fpr);
+    free(lam);
+    free(g);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 /* integrator: MD_SwopeScheme.F90:105-205 (predictor), :233-280 (correction)                   */
 /* ------------------------------------------------------------------------------------------ */
 

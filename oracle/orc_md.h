@@ -53,6 +53,7 @@ extern "C" {
 #define ORC_ERR_OVERFLOW   1   /* more neighbours than mxkvois */
 #define ORC_ERR_OUTOFBOX   2   /* active atom outside the cell grid */
 #define ORC_ERR_ARG        3
+#define ORC_ERR_NOMEM      4
 
 typedef void (*orc_pairfn)(const double* r, double* v, double* dv);
 
@@ -118,6 +119,16 @@ int orc_force_fs_half(int n, const double* xp, const int* ityp, const int* kvois
 int orc_force_full(int n, const double* xp, const int* ityp, const int* statu, const int* kvois,
                    const int* indi, int mxkvois, const double len[3], const int ifpd[3],
                    double ru2max, const orc_ftable* t, double* fp, double* dden, double* epot);
+
+/* Reverse mode of orc_force_full for one table kind: given G = dL/dFP (N,3) and g = dL/dE
+ * (E = sum of EPOT over the atoms), accumulates dL/d(knot) into the padded rows
+ * g_potr, g_fpotr, g_potb, g_fpotb (ntab+2 each) and, for EAM, g_fembd, g_dfembd (nembd+2
+ * each; may be NULL for FS).  Outputs are added to, not overwritten. */
+int orc_adjoint_full(int n, const double* xp, const int* ityp, const int* statu,
+                     const int* kvois, const int* indi, const double len[3], const int ifpd[3],
+                     double ru2max, const orc_ftable* t, const double* dldf, double dlde,
+                     double* g_potr, double* g_fpotr, double* g_potb, double* g_fpotb,
+                     double* g_fembd, double* g_dfembd);
 
 /* ---------------- integrator ---------------- */
 void orc_predictor(int n, double* xp, double* xp1, const double* fp, double* dis, int* statu,
