@@ -120,6 +120,38 @@ def run_batch(nbox: int, steps: int, warmup: int, pot: str, tune: bool):
                         "<110> dumbbells, 300-1200 K, FS 5000-pt tables, rebuild every 10"}
 
 
+def run_slab(ncell: int, steps: int, warmup: int, tune: bool):
+    """Cu-Ni TB-SMA slab: FCC ncell^3 cells (4 ncell^3 atoms), random 50/50, periodic in x and
+    y with 20 A of vacuum above and below, 1200 K, cutoff 6.0 A (skin 0.3), dt 2 fs, list
+    rebuilt every 5 steps; three pair and three density table kinds (Cu-Cu, Ni-Ni, Cu-Ni)."""
+    import msmpscu_amd.configs as cfg
+    box = cfg.make_cuni_slab((ncell, ncell, ncell), 3.57, 1200.0, 31)
+    ctrl = cfg.cuni_ctrl()
+    ft = cfg.cuni_forcetable(box, ctrl)
+    log(f"slab: {box.NPRT} atoms")
+    dev = make_device([box], ctrl, ft, "FS_TYPE")
+    dev.build_neighbors()
+    dev.calc_force()
+    tot, kmax = dev.list_stats()
+    choice = dev.autotune(2) if tune else None
+    log(f"slab autotune: {choice}")
+    dev.run(warmup, ctrl.NB_UPTAB, 0, ctrl.H)
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.run(steps, ctrl.NB_UPTAB, warmup, ctrl.H)
+    dev.sync()
+    dt = time.perf_counter() - t0
+    dev.close()
+    rate = box.NPRT * steps / dt
+    log(f"slab: {rate:.4e} atom-steps/s ({1e3 * dt / steps:.4f} ms/step)")
+    return {"value": round(rate, 1), "unit": "atom-steps/s", "atoms": box.NPRT, "steps": steps,
+            "ms_per_step": round(1e3 * dt / steps, 4), "list_mean": round(tot / box.NPRT, 3),
+            "list_max": kmax, "autotune": choice,
+            "workload": f"Cu-Ni TB-SMA slab, fcc {ncell}^3 cells, 50/50 random, IFPD (1,1,0) "
+                        "with 20 A vacuum, 1200 K, cutoff 6.0 A, skin 0.3 A, dt 2 fs, rebuild "
+                        "every 5 steps"}
+
+
 def run_adjoint(reps: int):
     """Coefficient gradients of the force/energy matching loss on 64 strained 1024-atom W
     boxes with EAM tables (3 x 5000 spline coefficients): full gradients per second (forward
@@ -168,6 +200,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient measurement")
+    ap.add_argument("--no-slab", action="store_true", help="skip the Cu-Ni slab measurement")
+    ap.add_argument("--slab-ncell", type=int, default=100)
+    ap.add_argument("--slab-steps", type=int, default=500)
     ap.add_argument("--adjoint-reps", type=int, default=20)
     args = ap.parse_args()
 
@@ -278,6 +313,10 @@ def main():
     if not args.no_batch:
         batch = run_batch(args.batch_boxes, args.batch_steps, 10, args.pot, not args.no_autotune)
 
+    slab = None
+    if not args.no_slab:
+        slab = run_slab(args.slab_ncell, args.slab_steps, 10, not args.no_autotune)
+
     grad = None
     if not args.no_adjoint:
         grad = run_adjoint(args.adjoint_reps)
@@ -315,6 +354,7 @@ def main():
         "kernel_times": kt,
         "batched": batch,
         "gradient": grad,
+        "slab": slab,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -384,10 +384,13 @@ int plan_bricks(msm_sim* s, const double* xp, int n) {
     return MSM_OK;
 }
 
-// LDS-table passes: one table kind, global list with 1, 2 or 4 lanes per atom, no compaction.
+// LDS-table passes: global list with 1, 2 or 4 lanes per atom, no compaction, and an LDS window
+// that holds at least the upper half of every table kind in use.
 bool use_lds_table(const msm_sim* s) {
-    return s->table_lds && s->tab.ngroup == 1 &&
-           (s->force_path == 1 || s->force_path == 2 || s->force_path == 4) && !s->compact;
+    if (!s->table_lds || s->compact) return false;
+    if (s->force_path != 1 && s->force_path != 2 && s->force_path != 4) return false;
+    const msm::LtWin w = msm::lt_window(s->tab, s->lt.wgs_d, s->lt.wgs_f);
+    return 2 * w.wf >= s->tab.ntab2 && 2 * w.wd >= s->tab.ntab2;
 }
 
 void enqueue_density(msm_sim* s) {
@@ -622,6 +625,7 @@ int msm_set_force_table(msm_sim* s, int pot_type, int ngroup, int nkind, int nta
     t.pot_type = pot_type;
     t.ngroup = ngroup;
     t.ntab2 = ntab2;
+    t.nkind = nkind;
     t.csi = csi;
     for (int i = 0; i < ngroup; ++i)
         for (int j = 0; j < ngroup; ++j) {
@@ -1125,7 +1129,7 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
         int fp, lds, bd, wd, bf, wf;
     };
     std::vector<Cand> cands;
-    if (s->tab.ngroup == 1) {
+    if (msm::lt_window(s->tab, 1, 1).wf * 2 >= s->tab.ntab2) {
         for (int fp : {2, 1, 4}) {
             cands.push_back({fp, 1, 1024, 1, 1024, 1});
             cands.push_back({fp, 1, 256, 4, 256, 4});

@@ -45,6 +45,7 @@ struct TabDev {
     const double4* e4;  // energy table {POTR, dPOTR, POTB, dPOTB}
     const double4* m4;  // embedding {FEMBD, dFEMBD, DFEMBD, dDFEMBD}, nkind1 x (nembd+2)
     int ntab2, nembd2, nembd, ngroup;
+    int nkind;          // table kinds (rows of b2/f4/e4)
     double csi, rhod, ru2max;
     int pot_type;
     int kpair[kMaxGroup * kMaxGroup];  // 0-based kinds, [ti*ngroup + tj]

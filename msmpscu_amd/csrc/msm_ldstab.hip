@@ -1,10 +1,11 @@
 // msm_ldstab.hip -- density and force passes with the interpolation table held in LDS.
 //
 // The per-pair table lookups of the global-list passes are scattered 16/32-byte loads from a
-// 80-160 KB table; together with the neighbour positions they overflow the 32 KB L1 and feed the
-// vector-memory pipeline twice per pair.  Here each workgroup first copies a window of knots of
-// the (single) table kind into LDS -- values only, {FPOTR, FPOTB} for the force pass (16 B/knot)
-// and POTB for the density pass (8 B/knot) -- and the lerp reads knots KK and KK+1 from there:
+// 80-160 KB table per kind; together with the neighbour positions they overflow the 32 KB L1 and
+// feed the vector-memory pipeline twice per pair.  Here each workgroup first copies a window of
+// knots of the table kinds in use into LDS -- values only, {FPOTR, FPOTB} for the force pass
+// (16 B/knot/kind) and POTB for the density pass (8 B/knot/kind) -- and the lerp reads knots KK
+// and KK+1 from there:
 //   T(KK) + DK * (T(KK+1) - T(KK))      (MD_FS_ForceTable_GPU.F90 PRECALFOR/CALFORCE kernels)
 // which is the same arithmetic as the reference.  The window is the top of the table (large
 // separations, where every neighbour lies); a rare pair below the window (r under about 1 A for
@@ -45,14 +46,19 @@ __device__ __forceinline__ void chunk_range(int nchunks, int& c0, int& c1) {
 
 }  // namespace
 
-template <int MB, int EAM, int TPA>
+// Knot rows in LDS: kind-major, w knots per kind from knot k0 (all kinds of the table for
+// several atom types, the one kind in use for a single type).
+template <int MB, int MT, int EAM, int TPA>
 __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev l, TabDev t,
-                                                                BoxDev box0,
-                                                                const BoxDev* __restrict__ boxes,
-                                                                LtWin w, int nchunks, int* err) {
+                                                            BoxDev box0,
+                                                            const BoxDev* __restrict__ boxes,
+                                                            LtWin w, int nchunks, int* err) {
     extern __shared__ double s_pb[];
-    const double2* __restrict__ b2k = t.b2 + static_cast<size_t>(t.kpair[0]) * t.ntab2;
-    for (int k = threadIdx.x; k < w.wd; k += blockDim.x) s_pb[k] = b2k[w.k0d + k].x;
+    const int nk = MT ? t.nkind : 1;
+    for (int k = threadIdx.x; k < nk * w.wd; k += blockDim.x) {
+        const int kd = MT ? k / w.wd : t.kpair[0], q = MT ? k % w.wd : k;
+        s_pb[k] = t.b2[static_cast<size_t>(kd) * t.ntab2 + w.k0d + q].x;
+    }
     __syncthreads();
     int c0, c1;
     chunk_range(nchunks, c0, c1);
@@ -63,10 +69,12 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
         const bool have = i < a.n;
         const bool active = have && (a.statu[i] & STATU_ACTIVE) == STATU_ACTIVE;
         double den0 = 0.0;
+        int ti = 0;
         if (active) {
             const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
             const double2 pxy = *reinterpret_cast<const double2*>(&a.pos[i]);
             const double pz = a.pos[i].z;
+            ti = MT ? a.ityp[i] : 0;
             const int kv = l.kvois[i];
             const int* __restrict__ li = l.indi + list_base(i, l.mx);
             for (int base = sub; base < kv; base += TPA * kGroup) {
@@ -86,18 +94,20 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
                     const double sy = min_image(pxy.y - qxy.y, b.len[1], b.ilen[1]);
                     const double sz = min_image(pz - qz, b.len[2], b.ilen[2]);
                     const double r2 = sx * sx + sy * sy + sz * sz;
+                    const int row = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] * w.wd : 0;
                     double inv_r;
                     const double sk = table_coord(r2, t.csi, inv_r);
                     const int kk = min(static_cast<int>(sk), kmax);
                     const bool in = base + u * TPA < kv && r2 <= t.ru2max;
                     const int kl = kk - w.k0d;
                     if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
-                    const int q = max(kl, 0);
+                    const int q = row + max(kl, 0);
                     const double p0 = s_pb[q], p1 = s_pb[q + 1];
                     const double v = p0 + (sk - kk) * (p1 - p0);
                     den0 = den0 + ((in && kl >= 0) ? v : 0.0);
                 }
                 if (fix) {
+                    // rare close pairs: the same term with the knots of the global table
 #pragma unroll
                     for (int u = 0; u < kGroup; ++u) {
                         if (!(fix & (1u << u))) continue;
@@ -108,10 +118,11 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
                         const double sy = min_image(pxy.y - qxy.y, b.len[1], b.ilen[1]);
                         const double sz = min_image(pz - qz, b.len[2], b.ilen[2]);
                         const double r2 = sx * sx + sy * sy + sz * sz;
+                        const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
                         double inv_r;
                         const double sk = table_coord(r2, t.csi, inv_r);
                         const int kk = min(static_cast<int>(sk), kmax);
-                        const double2 e = b2k[kk];
+                        const double2 e = t.b2[static_cast<size_t>(kind) * t.ntab2 + kk];
                         den0 = den0 + (e.x + (sk - kk) * e.y);
                     }
                 }
@@ -130,7 +141,7 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
                         atomicOr(&err[kErrEmbd], 1);
                         kk = t.nembd;
                     }
-                    const double4 m = t.m4[t.kembd[0] * t.nembd2 + kk];
+                    const double4 m = t.m4[t.kembd[ti] * t.nembd2 + kk];
                     den0 = m.z + (sk - kk) * m.w;
                 }
             }
@@ -139,15 +150,16 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
     }
 }
 
-template <int MB, int MODE, int TPA>
+template <int MB, int MT, int MODE, int TPA>
 __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l, TabDev t,
-                                                            BoxDev box0,
-                                                            const BoxDev* __restrict__ boxes,
-                                                            StepParams sp, LtWin w, int nchunks) {
+                                                          BoxDev box0,
+                                                          const BoxDev* __restrict__ boxes,
+                                                          StepParams sp, LtWin w, int nchunks) {
     extern __shared__ double2 s_ft[];
-    const double4* __restrict__ f4k = t.f4 + static_cast<size_t>(t.kpair[0]) * t.ntab2;
-    for (int k = threadIdx.x; k < w.wf; k += blockDim.x) {
-        const double4 f = f4k[w.k0f + k];
+    const int nk = MT ? t.nkind : 1;
+    for (int k = threadIdx.x; k < nk * w.wf; k += blockDim.x) {
+        const int kd = MT ? k / w.wf : t.kpair[0], q = MT ? k % w.wf : k;
+        const double4 f = t.f4[static_cast<size_t>(kd) * t.ntab2 + w.k0f + q];
         s_ft[k] = make_double2(f.x, f.z);
     }
     __syncthreads();
@@ -162,6 +174,7 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
         const bool active = have && (st & STATU_ACTIVE) == STATU_ACTIVE;
         const double4 pi = have ? a.pos[i] : make_double4(0, 0, 0, 0);
         const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, have ? i : 0);
+        const int ti = (MT && have) ? a.ityp[i] : 0;
         double fpx = 0.0, fpy = 0.0, fpz = 0.0;
         if (active) {
             const double denki = pi.w;
@@ -183,6 +196,12 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
                     const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
                     const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
                     const double r2 = sx * sx + sy * sy + sz * sz;
+                    int row = 0, row1 = 0;
+                    if constexpr (MT) {
+                        const int tj = a.ityp[j];
+                        row = t.kpair[ti * t.ngroup + tj] * w.wf;
+                        row1 = t.kpair[tj * t.ngroup + ti] * w.wf;
+                    }
                     double inv_r;
                     const double sk = table_coord(r2, t.csi, inv_r);
                     const int kk = min(static_cast<int>(sk), kmax);
@@ -191,11 +210,16 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
                     const int kl = kk - w.k0f;
                     if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
                     const int q = max(kl, 0);
-                    const double2 t0 = s_ft[q], t1 = s_ft[q + 1];
-                    // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r, KTAB = KTAB1 here
+                    const double2 t0 = s_ft[row + q], t1 = s_ft[row + q + 1];
                     const double fb = t0.y + dk * (t1.y - t0.y);
+                    double fb1 = fb;
+                    if constexpr (MT) {
+                        const double b0 = s_ft[row1 + q].y, b1 = s_ft[row1 + q + 1].y;
+                        fb1 = b0 + dk * (b1 - b0);
+                    }
+                    // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL)
                     const double ft = (t0.x + dk * (t1.x - t0.x)) * (inv_r * inv_r) +
-                                      (fb * denki + fb * pj.w) * inv_r;
+                                      (fb * denki + fb1 * pj.w) * inv_r;
                     const double fortot = (in && kl >= 0) ? ft : 0.0;
                     fpx = fpx + fortot * sx;
                     fpy = fpy + fortot * sy;
@@ -206,19 +230,27 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
 #pragma unroll
                     for (int u = 0; u < kGroup; ++u) {
                         if (!(fix & (1u << u))) continue;
-                        const double4 pj = a.pos[li[static_cast<size_t>(base + u * TPA) * l.stride]];
+                        const int j = li[static_cast<size_t>(base + u * TPA) * l.stride];
+                        const double4 pj = a.pos[j];
                         const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
                         const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
                         const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
                         const double r2 = sx * sx + sy * sy + sz * sz;
+                        int kind = t.kpair[0], kind1 = t.kpair[0];
+                        if constexpr (MT) {
+                            const int tj = a.ityp[j];
+                            kind = t.kpair[ti * t.ngroup + tj];
+                            kind1 = t.kpair[tj * t.ngroup + ti];
+                        }
                         double inv_r;
                         const double sk = table_coord(r2, t.csi, inv_r);
                         const int kk = min(static_cast<int>(sk), kmax);
                         const double dk = sk - kk;
-                        const double4 f = f4k[kk];
-                        const double fb = f.z + dk * f.w;
+                        const double4 f = t.f4[static_cast<size_t>(kind) * t.ntab2 + kk];
+                        const double4 f1 = t.f4[static_cast<size_t>(kind1) * t.ntab2 + kk];
                         const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
-                                              (fb * denki + fb * pj.w) * inv_r;
+                                              ((f.z + dk * f.w) * denki +
+                                               (f1.z + dk * f1.w) * pj.w) * inv_r;
                         fpx = fpx + fortot * sx;
                         fpy = fpy + fortot * sy;
                         fpz = fpz + fortot * sz;
@@ -236,7 +268,7 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
             a.fz[i] = fpz;
         }
         if constexpr (MODE >= 1)
-            integrate_atom<MODE>(a, t, b, sp, i, st, active, pi, 0, fpx, fpy, fpz);
+            integrate_atom<MODE>(a, t, b, sp, i, st, active, pi, ti, fpx, fpy, fpz);
     }
 }
 
@@ -302,7 +334,8 @@ int lds_budget(int wgs) {
 
 LtWin lt_window(const TabDev& t, int wgs_d, int wgs_f) {
     LtWin w{};
-    const int bf = lds_budget(wgs_f) / 16, bd = lds_budget(wgs_d) / 8;
+    const int nk = t.ngroup > 1 ? t.nkind : 1;
+    const int bf = lds_budget(wgs_f) / (16 * nk), bd = lds_budget(wgs_d) / (8 * nk);
     w.wf = t.ntab2 < bf ? t.ntab2 : bf;
     w.k0f = t.ntab2 - w.wf;
     w.wd = t.ntab2 < bd ? t.ntab2 : bd;
@@ -315,13 +348,14 @@ void launch_density_lt(int tpa, const LtShape& sh, const AtomsDev& a, const List
                        int* err, hipStream_t s) {
     if (a.n <= 0) return;
     const LtWin w = lt_window(t, sh.wgs_d, sh.wgs_f);
-    const size_t lds = static_cast<size_t>(w.wd) * sizeof(double);
+    const bool mt = t.ngroup > 1;
+    const size_t lds = static_cast<size_t>(w.wd) * (mt ? t.nkind : 1) * sizeof(double);
     const long long th = static_cast<long long>(a.n) * tpa;
     const int nchunks = static_cast<int>((th + sh.block_d - 1) / sh.block_d);
     const bool eam = t.pot_type != 0;
 #define MSM_DL(MB, E, P)                                                                     \
     do {                                                                                     \
-        auto k = k_density_lt<MB, E, P>;                                                     \
+        auto k = mt ? k_density_lt<MB, 1, E, P> : k_density_lt<MB, 0, E, P>;                 \
         const unsigned g = persistent_grid(reinterpret_cast<const void*>(k), sh.block_d,     \
                                            lds, sh.wgs_d, nchunks);                          \
         hipLaunchKernelGGL(k, dim3(g), dim3(sh.block_d), lds, s, a, l, t, box0, boxes, w,    \
@@ -347,12 +381,13 @@ void launch_force_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDe
                      const StepParams& sp, hipStream_t s) {
     if (a.n <= 0) return;
     const LtWin w = lt_window(t, sh.wgs_d, sh.wgs_f);
-    const size_t lds = static_cast<size_t>(w.wf) * sizeof(double2);
+    const bool mt = t.ngroup > 1;
+    const size_t lds = static_cast<size_t>(w.wf) * (mt ? t.nkind : 1) * sizeof(double2);
     const long long th = static_cast<long long>(a.n) * tpa;
     const int nchunks = static_cast<int>((th + sh.block_f - 1) / sh.block_f);
 #define MSM_FL(MB, M, P)                                                                     \
     do {                                                                                     \
-        auto k = k_force_lt<MB, M, P>;                                                       \
+        auto k = mt ? k_force_lt<MB, 1, M, P> : k_force_lt<MB, 0, M, P>;                     \
         const unsigned g = persistent_grid(reinterpret_cast<const void*>(k), sh.block_f,     \
                                            lds, sh.wgs_f, nchunks);                          \
         hipLaunchKernelGGL(k, dim3(g), dim3(sh.block_f), lds, s, a, l, t, box0, boxes, sp,   \
