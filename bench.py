@@ -45,6 +45,22 @@ def survey_step_bytes(kmean: float, n_rb: int) -> float:
     return 2 * (4 * kmean + 4) + 48 + 16 + 24 + 120 + (100 + 4 * kmean) / n_rb
 
 
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of a pass of the default workload, as measured with FETCH_SIZE /
+    WRITE_SIZE (scaled by a copy kernel of known size) and recorded in profiles/."""
+    import re
+    path = os.path.join(ROOT, "profiles", "p1_hbm_traffic.md")
+    try:
+        with open(path) as fh:
+            m = re.search(r"traffic_bytes_per_launch (.*)", fh.read())
+    except OSError:
+        return None
+    if not m:
+        return None
+    vals = dict(kv.split("=") for kv in m.group(1).split())
+    return int(vals[kernel]) if kernel in vals else None
+
+
 def cpu_baseline(box, ctrl, ft, nsteps: int):
     import numpy as np
 
@@ -204,7 +220,22 @@ def main():
     ap.add_argument("--slab-ncell", type=int, default=100)
     ap.add_argument("--slab-steps", type=int, default=500)
     ap.add_argument("--adjoint-reps", type=int, default=20)
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device five times (HBM counter "
+                         "calibration for profiling) and exit")
     args = ap.parse_args()
+
+    if args.calib_copy:
+        import torch
+        nbytes = args.calib_copy << 20
+        src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        src.fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": nbytes, "copies": 5}), flush=True)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "
@@ -298,10 +329,13 @@ def main():
         avg_s = kt[dom]["avg_us"] * 1e-6
         bytes_launch = kb[dom] * box.NPRT
         achieved = bytes_launch / avg_s / 1e9
+        default_p1 = nc == 80 and args.pot == "FS_TYPE"
+        traffic = profiled_traffic(dom) if default_p1 else None
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "bytes_per_launch": int(bytes_launch),
-                "avg_launch_us": round(kt[dom]["avg_us"], 3)}
+                "traffic": traffic, "bytes_per_launch": int(bytes_launch),
+                "avg_launch_us": round(kt[dom]["avg_us"], 3),
+                "traffic_source": "profiles/p1_hbm_traffic.md" if traffic else None}
 
     step_bytes = survey_step_bytes(kmean, ctrl.NB_UPTAB)
     step_model = {"bytes_per_atom_step": round(step_bytes, 1),
