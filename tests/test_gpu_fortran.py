@@ -1,0 +1,59 @@
+"""The Fortran drop-in layer (msmpscu_amd/fortran: MDLIB-named routines over the C-ABI) driven by
+a Fortran program: forces and energy of the W box at the start and after 100 steps (one
+per-step call, then a fused run) against the oracle on the same positions."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import msmpscu_amd.configs as cfg
+from msmpscu_amd.mdlib import SimMDBox
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "msmpscu_amd",
+                      "fortran", "fs_w_driver")
+F_TOL = 1e-10
+E_REL = 1e-12
+
+
+def _box(xp):
+    n = xp.shape[0]
+    nc = round((n / 2) ** (1 / 3))
+    ln = np.full(3, nc * cfg.W_A)
+    return SimMDBox(NPRT=n, NGROUP=1, ZL=ln, BOXLOW=-0.5 * ln, XP=np.asfortranarray(xp),
+                    XP1=np.zeros((n, 3), order="F"), ITYP=np.ones(n, dtype=np.int32),
+                    STATU=np.ones(n, dtype=np.int32), CM=np.array([cfg.W_MASS * cfg.AMU]),
+                    PTYPE=np.array([[1]], dtype=np.int32), SYMB=["W"])
+
+
+def test_fortran_driver_against_oracle(orc, tmp_path):
+    if not os.path.exists(DRIVER):
+        pytest.fail("fs_w_driver is not built (msmpscu_amd.build.build_fortran)")
+    out = tmp_path / "w.txt"
+    print("[gpu-test] fortran driver 100 steps", flush=True)
+    sys.stdout.flush()
+    r = subprocess.run([DRIVER, str(out), "100", "16"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = out.read_text().splitlines()
+    n = int(lines[0].split()[0])
+    e0 = float(lines[0].split()[1])
+    start = np.array([[float(v) for v in l.split()] for l in lines[1:1 + n]])
+    end = np.array([[float(v) for v in l.split()] for l in lines[1 + n:1 + 2 * n]])
+    e1 = float(lines[1 + 2 * n].split()[3])
+    ctrl = cfg.w_ctrl()
+    for blk, e in ((start, e0), (end, e1)):
+        box = _box(blk[:, :3])
+        ft = cfg.w_forcetable(box, ctrl)
+        t = orc.from_mdlib(ft)
+        it = box.ITYP - 1
+        kv, ind = orc.nlist("half", box.XP, it, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
+                            ctrl.NB_RM, ctrl.NB_MXNBS)
+        fo, eo, *_ = orc.force_fs_half(box.XP, it, kv, ind, box.ZL, ctrl.IFPD, ctrl.RU, t)
+        assert np.max(np.abs(blk[:, 3:] - fo)) <= F_TOL
+        assert abs(e - eo.sum()) <= E_REL * abs(eo.sum())
+    # the run moved the atoms
+    assert np.max(np.abs(end[:, :3] - start[:, :3])) > 1e-3

@@ -132,3 +132,16 @@ def test_workload_builders():
     assert np.all(np.diff(s.ITYP) >= 0)
     sizes = cfg.batch_sizes(256, 2000, 60000, 5)
     assert min(sizes) >= 2000 and max(sizes) <= 60000
+
+
+def test_fortran_bindings_name_header_symbols():
+    """Every bind(C, name=...) of the Fortran drop-in layer names a function that
+    include/msm_hip.h declares (the Fortran interfaces bind nothing the library lacks)."""
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = open(os.path.join(root, "msmpscu_amd", "fortran", "MD_ForceLib_HIP.F90")).read()
+    hdr = open(os.path.join(root, "include", "msm_hip.h")).read()
+    names = set(re.findall(r'bind\(C,\s*name="(\w+)"\)', src))
+    assert len(names) >= 20
+    declared = set(re.findall(r"\b(msm_\w+)\s*\(", hdr))
+    assert names <= declared, sorted(names - declared)
