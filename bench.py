@@ -300,6 +300,21 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     log(f"{value:.4e} atom-steps/s, {ms_per_step:.4f} ms/step")
 
+    # the same steps including the host boundary: upload of the atoms, first list and forces,
+    # the steps, and the copy-out of positions, velocities and forces
+    xh, vh, _, sth, _ = dev.get_atoms()
+    dev.sync()
+    tp0 = time.perf_counter()
+    dev.set_atoms(xh, vh, box.ITYP, sth)
+    dev.build_neighbors()
+    dev.calc_force()
+    dev.run(args.steps, ctrl.NB_UPTAB, step0, ctrl.H)
+    dev.get_atoms()
+    tp1 = time.perf_counter()
+    step0 += args.steps
+    pcie_value = box.NPRT * args.steps / (tp1 - tp0)
+    log(f"with host upload/download: {pcie_value:.4e} atom-steps/s")
+
     # per-kernel device time from HIP events on the library's stream, same step count
     kt = {}
     if not args.no_timing_pass:
@@ -359,10 +374,20 @@ def main():
     if not args.no_cpu:
         log(f"cpu baseline: oracle serial half-list MD, {args.cpu_steps} steps on the same box")
         rate, dt = cpu_baseline(box, ctrl, ft, args.cpu_steps)
+        model = "unknown CPU"
+        try:
+            with open("/proc/cpuinfo") as fh:
+                for line in fh:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
         cpu = {"value": round(rate, 1), "unit": "atom-steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_steps} MD steps (incl. one list rebuild) of the same "
                          f"{box.NPRT}-atom box, {dt:.1f} s, serial C restatement of the "
-                         "reference host path (half list, Newton's 3rd law)"}
+                         "reference host path (half list, Newton's 3rd law), gcc -O2 "
+                         f"-ffp-contract=off, one thread of {os.cpu_count()} on {model}"}
         log(f"cpu {rate:.4e} atom-steps/s")
 
     out = {
@@ -383,6 +408,7 @@ def main():
                                "rebuild every 10 steps, skin 0.3 A",
                    "atoms": box.NPRT, "list_mean": round(kmean, 3), "list_max": kmax,
                    "dis_tracking": track, "parallelism": "single GPU"},
+        "pcie_inclusive_value": round(pcie_value, 1),
         "roofline": roof,
         "step_model": step_model,
         "kernel_times": kt,
