@@ -220,6 +220,8 @@ def main():
     ap.add_argument("--slab-ncell", type=int, default=100)
     ap.add_argument("--slab-steps", type=int, default=500)
     ap.add_argument("--adjoint-reps", type=int, default=20)
+    ap.add_argument("--cell-brick", type=int, default=None,
+                    help="cell numbering: bricks of N^3 cells contiguous (1 = row-major)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device five times (HBM counter "
                          "calibration for profiling) and exit")
@@ -271,6 +273,9 @@ def main():
                    "msm_set_option")
     if args.table_lds is not None:
         _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_TABLE_LDS, args.table_lds),
+                   "msm_set_option")
+    if args.cell_brick is not None:
+        _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_CELL_BRICK, args.cell_brick),
                    "msm_set_option")
     if args.lt_shape:
         vals = [int(v) for v in args.lt_shape.split(",")]

@@ -190,7 +190,9 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
  * HIP events around each launch and accumulates per-class device time. */
 #define MSM_OPT_TRACK_DIS     1
 #define MSM_OPT_KERNEL_TIMING 2
-#define MSM_OPT_NLIST_KERNEL  3  /* 0 (default): LDS-staged cell bricks; 1: per-atom global scan */
+#define MSM_OPT_NLIST_KERNEL  3  /* 0 (default): LDS-staged cell bricks, small or wide by cell
+                                    occupancy; 1: per-atom global scan; 2: small bricks (one lane
+                                    per atom); 3: wide bricks (several lanes per atom) */
 #define MSM_OPT_FORCE_PATH    4  /* density/force passes: 0 LDS bricks; 1, 2, 4, 8: global list
                                     with that many lanes per atom (default 2); 3: two lanes per
                                     atom, each 32-byte record gathered by a lane pair */
@@ -208,6 +210,10 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
 #define MSM_OPT_LT_DENSITY_WGS   8
 #define MSM_OPT_LT_FORCE_BLOCK   9
 #define MSM_OPT_LT_FORCE_WGS     10
+/* Cell numbering (and so the order of the cell-sorted atoms): bricks of value^3 cells are
+ * contiguous (default 1 = row-major).  Takes effect at the next msm_build_neighbors, which is
+ * then required. */
+#define MSM_OPT_CELL_BRICK       11
 int msm_set_option(msm_sim* sim, int option, int value);
 
 /* Adjoint (reverse mode) of the force and energy evaluation with respect to the table knots,
@@ -229,8 +235,9 @@ int msm_adjoint(msm_sim* sim, const double* dldf, const double* dlde, double* g_
  * density/force kernels (lanes per atom, LDS table and its workgroup shape, lane-pair gather,
  * LDS bricks), sets the options to the fastest, and recomputes F' and F with it.  choice[8]
  * (may be NULL) receives {force path, table LDS, density block, density workgroups per CU, force
- * block, force workgroups per CU, list kernel, 0}; best_ms[2] (may be NULL) the chosen
- * density+force and rebuild times in ms.  Positions and velocities are unchanged. */
+ * block, force workgroups per CU, list kernel (MSM_OPT_NLIST_KERNEL value), 0}; best_ms[2] (may
+ * be NULL) the chosen density+force and rebuild times in ms.  Positions and velocities are
+ * unchanged. */
 int msm_autotune(msm_sim* sim, int reps, int* choice, double* best_ms);
 
 /* Kernel classes for msm_kernel_times. */

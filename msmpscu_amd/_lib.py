@@ -99,6 +99,7 @@ MSM_OPT_LT_DENSITY_BLOCK = 7
 MSM_OPT_LT_DENSITY_WGS = 8
 MSM_OPT_LT_FORCE_BLOCK = 9
 MSM_OPT_LT_FORCE_WGS = 10
+MSM_OPT_CELL_BRICK = 11
 KCLASS = ["density", "force", "rebuild", "predictor", "nlist"]
 
 _lib = None

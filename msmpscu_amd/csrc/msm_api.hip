@@ -98,10 +98,12 @@ struct msm_sim {
     // options
     int track_dis = 1;
     int timing = 0;
-    int nlist_global = 0;  // 1: per-atom global-memory list kernel instead of LDS bricks
+    int nlist_kind = 0;    // list build: 0 LDS bricks sized by occupancy, 1 per-atom global
+                           // scan, 2 small LDS bricks, 3 wide LDS bricks
     int force_path = 2;    // 0: LDS bricks; 1, 2, 4, 8: global list with that many lanes per atom;
                            // 3: two lanes per atom with lane-pair gathers
     int table_lds = 1;     // single-kind table knots in LDS (global-list paths 1, 2, 4)
+    int cell_brick = 1;    // cells are numbered in bricks of cell_brick^3 (1: row-major)
     msm::LtShape lt{1024, 1, 1024, 1};  // block / workgroups per CU of the LDS-table passes
     int compact = 0;       // global-list path: force pass reads the density pass's in-cutoff list
 
@@ -112,6 +114,10 @@ struct msm_sim {
 
     // LDS-staged list bricks
     int nbricks = 0;
+    int nl_lanes = 1;  // lanes per home atom of the list build
+    int nl_cap = 64;   // halo atoms staged per list-build workgroup
+    int nl_wide = 0;   // bricks planned for the wide list kernel
+    std::vector<double> occ;  // mean atoms per non-empty cell, per box (at upload)
     int* d_brick_box = nullptr;
 
     // per-class kernel timing (MSM_OPT_KERNEL_TIMING)
@@ -304,12 +310,14 @@ void enqueue_rebuild(msm_sim* s) {
     msm::launch_gather(s->n, s->n_alloc, s->sb.perm, a_src, s->set[src].gid, a_dst,
                        s->set[dst].gid, s->sb.key, s->set[dst].cell, s->stream);
     TimedScope tl(s, MSM_KCLASS_NLIST);
-    if (s->nlist_global)
+    if (s->nlist_kind == 1)
         msm::launch_nlist(s->n, s->pos[p1], s->set[dst].ityp, s->set[dst].statu,
                           s->set[dst].boxid, s->set[dst].cell, s->sb.start, s->d_boxes, s->ngroup,
                           s->d_rnl2, s->mx, s->kvois, s->indi, msm::kListRow, s->d_err, s->stream);
     else
-        msm::launch_nlist_bricks(s->nbricks, s->d_brick_box, s->n, nc, s->pos[p1],
+        msm::launch_nlist_bricks(s->nl_wide, s->nbricks, s->nl_lanes, s->nl_cap, s->d_brick_box,
+                                 s->n, nc,
+                                 s->pos[p1],
                                  s->set[dst].ityp, s->sb.start, s->d_boxes, s->ngroup, s->d_rnl2,
                                  s->mx, s->kvois, s->indi, s->sidx, msm::kListRow, s->d_err,
                                  s->stream);
@@ -319,13 +327,126 @@ void enqueue_rebuild(msm_sim* s) {
 
 bool multibox(const msm_sim* s) { return s->boxes.size() > 1; }
 
-// Brick shape of the LDS-staged list build, per box: the largest brick (at most 4 cells a side)
-// whose halo, at the box's typical occupancy of non-empty cells, fits the LDS capacity.  The
-// occupancy comes from binning the uploaded positions on the host.  Bricks that overflow at run
-// time fall back to the global-memory scan, so the choice affects speed only.
-int plan_bricks(msm_sim* s, const double* xp, int n) {
+// Brick shape of the LDS-staged list build, per box, and lanes per home atom.  From the box's
+// mean occupancy m of non-empty cells (binning the uploaded positions on the host), a brick of
+// `home` cells with a `halo` of cells costs one workgroup about
+//     ceil(home*m*T / kNlBlock) * 27*m/T   (candidate tests per lane, T lanes per atom)
+//   + 3*halo*m / kNlBlock                  (staging)
+// and serves home*m atoms; its LDS holds halo*(1.3 m + 1) atoms, which sets how many workgroups
+// share a CU.  The largest box picks T in {1, 2, 4, 8}; every box then takes the brick (at most
+// 4 cells a side) with the most atoms per cost per CU.  Bricks that overflow at run time fall
+// back to the global-memory scan, so the choice affects speed only.
+struct BrickChoice {
+    int e[3] = {1, 1, 1};
+    int lanes = 1;
+    int cap = 0;
+    double score = -1.0;
+};
+
+BrickChoice best_brick(const BoxDev& b, double m, int lanes_fixed, int ngroup) {
+    BrickChoice best;
+    const double est = 1.3 * m + 1.0;
+    const double lds_cu = 163840.0, lds_static = 6144.0;
+    for (int lanes : {1, 2, 4, 8}) {
+        if (lanes_fixed > 0 && lanes != lanes_fixed) continue;
+        for (int bz = 1; bz <= 4; ++bz)
+            for (int by = 1; by <= 4; ++by)
+                for (int bx = 1; bx <= 4; ++bx) {
+                    const int e[3] = {std::min(bx, b.ncell[0]), std::min(by, b.ncell[1]),
+                                      std::min(bz, b.ncell[2])};
+                    const int halo = (e[0] + 2) * (e[1] + 2) * (e[2] + 2);
+                    const int home = e[0] * e[1] * e[2];
+                    const int cap = static_cast<int>(std::ceil(halo * est));
+                    if (cap > msm::kNlCap) continue;
+                    const double lds = static_cast<double>(msm::nlist_lds_bytes(cap, ngroup)) +
+                                       lds_static;
+                    const double wgs = std::min(2.0, std::floor(lds_cu / lds));
+                    const double atoms = home * m;
+                    const double rounds = std::ceil(atoms * lanes / msm::kNlBlock);
+                    const double cost = std::max(rounds, 1.0) * 27.0 * m / lanes +
+                                        3.0 * halo * m / msm::kNlBlock;
+                    const double score = wgs * atoms / cost;
+                    if (score > best.score * (1.0 + 1e-9)) {
+                        best.score = score;
+                        best.lanes = lanes;
+                        best.cap = cap;
+                        for (int k = 0; k < 3; ++k) best.e[k] = e[k];
+                    }
+                }
+    }
+    return best;
+}
+
+// Small bricks: the largest brick (at most 4 cells a side) whose halo and home atoms, at
+// 1.3 m + 1 atoms per cell, fit the kBlock-thread kernel's kNlCapPass LDS slots and 4 atoms per
+// thread.
+void small_brick(const BoxDev& b, double m, int e_out[3]) {
+    const double est = 1.3 * m + 1.0;
+    int best_home = 0;
+    e_out[0] = e_out[1] = e_out[2] = 1;
+    for (int bz = 1; bz <= 4; ++bz)
+        for (int by = 1; by <= 4; ++by)
+            for (int bx = 1; bx <= 4; ++bx) {
+                const int e[3] = {std::min(bx, b.ncell[0]), std::min(by, b.ncell[1]),
+                                  std::min(bz, b.ncell[2])};
+                const int halo = (e[0] + 2) * (e[1] + 2) * (e[2] + 2);
+                const int home = e[0] * e[1] * e[2];
+                if (halo * est > msm::kNlCapPass || home * est > 4 * msm::kBlock) continue;
+                if (home > best_home || (home == best_home && e[0] > e_out[0])) {
+                    best_home = home;
+                    for (int k = 0; k < 3; ++k) e_out[k] = e[k];
+                }
+            }
+}
+
+// Bricks for the LDS list build of the current kind (s->nlist_kind; 0 picks wide bricks when
+// the largest box holds 12 or more atoms per non-empty cell) from the occupancies of upload.
+int replan_bricks(msm_sim* s) {
+    if (s->occ.size() != s->boxes.size()) s->occ.assign(s->boxes.size(), 1.0);
+    size_t largest = 0;
+    for (size_t b = 0; b < s->boxes.size(); ++b)
+        if (s->boxes[b].natoms > s->boxes[largest].natoms) largest = b;
+    const int wide = s->nlist_kind == 3 || (s->nlist_kind == 0 && s->occ[largest] >= 12.0);
     std::vector<int> brick_box;
-    int off = 0, aoff = 0;
+    int lanes = 1, cap = 64, off = 0;
+    if (wide) lanes = best_brick(s->boxes[largest], s->occ[largest], 0, s->ngroup).lanes;
+    for (size_t bidx = 0; bidx < s->boxes.size(); ++bidx) {
+        BoxDev& b = s->boxes[bidx];
+        int e[3];
+        if (wide) {
+            const BrickChoice c = best_brick(b, s->occ[bidx], lanes, s->ngroup);
+            cap = std::max(cap, c.cap);
+            for (int k = 0; k < 3; ++k) e[k] = c.e[k];
+        } else {
+            small_brick(b, s->occ[bidx], e);
+        }
+        int nb = 1;
+        for (int k = 0; k < 3; ++k) {
+            b.bdim[k] = e[k];
+            b.nbrick[k] = (b.ncell[k] + e[k] - 1) / e[k];
+            nb *= b.nbrick[k];
+        }
+        b.brick_off = off;
+        for (int k = 0; k < nb; ++k) brick_box.push_back(static_cast<int>(bidx));
+        off += nb;
+    }
+    s->nl_wide = wide;
+    s->nl_lanes = lanes;
+    s->nl_cap = std::min(cap, msm::kNlCap);
+    s->nbricks = off;
+    int rc = dalloc(s, &s->d_brick_box, brick_box.size());
+    if (rc != MSM_OK) return rc;
+    HIPCHK(s, hipMemcpy(s->d_brick_box, brick_box.data(), brick_box.size() * sizeof(int),
+                        hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->d_boxes, s->boxes.data(), s->boxes.size() * sizeof(BoxDev),
+                        hipMemcpyHostToDevice));
+    return MSM_OK;
+}
+
+// Mean atoms per non-empty cell of every box, from the uploaded positions, then the brick plan.
+int plan_bricks(msm_sim* s, const double* xp, int n) {
+    s->occ.assign(s->boxes.size(), 1.0);
+    int aoff = 0;
     for (size_t bidx = 0; bidx < s->boxes.size(); ++bidx) {
         BoxDev& b = s->boxes[bidx];
         const int nc = b.ncell[0] * b.ncell[1] * b.ncell[2];
@@ -346,42 +467,9 @@ int plan_bricks(msm_sim* s, const double* xp, int n) {
                 occ += c;
                 ++nonempty;
             }
-        const double est = nonempty ? 1.3 * static_cast<double>(occ) / nonempty + 1.0 : 1.0;
-        int best[3] = {1, 1, 1};
-        int best_home = 0;
-        for (int bz = 1; bz <= 4; ++bz)
-            for (int by = 1; by <= 4; ++by)
-                for (int bx = 1; bx <= 4; ++bx) {
-                    const int e[3] = {std::min(bx, b.ncell[0]), std::min(by, b.ncell[1]),
-                                      std::min(bz, b.ncell[2])};
-                    const int halo = (e[0] + 2) * (e[1] + 2) * (e[2] + 2);
-                    const int home = e[0] * e[1] * e[2];
-                    if (halo * est > msm::kNlCap || home * est > 4 * msm::kBlock) continue;
-                    if (home > best_home || (home == best_home && e[0] > best[0])) {
-                        best_home = home;
-                        best[0] = e[0];
-                        best[1] = e[1];
-                        best[2] = e[2];
-                    }
-                }
-        int nb = 1;
-        for (int k = 0; k < 3; ++k) {
-            b.bdim[k] = best[k];
-            b.nbrick[k] = (b.ncell[k] + best[k] - 1) / best[k];
-            nb *= b.nbrick[k];
-        }
-        b.brick_off = off;
-        for (int k = 0; k < nb; ++k) brick_box.push_back(static_cast<int>(bidx));
-        off += nb;
+        s->occ[bidx] = nonempty ? static_cast<double>(occ) / nonempty : 1.0;
     }
-    s->nbricks = off;
-    int rc = dalloc(s, &s->d_brick_box, brick_box.size());
-    if (rc != MSM_OK) return rc;
-    HIPCHK(s, hipMemcpy(s->d_brick_box, brick_box.data(), brick_box.size() * sizeof(int),
-                        hipMemcpyHostToDevice));
-    HIPCHK(s, hipMemcpy(s->d_boxes, s->boxes.data(), s->boxes.size() * sizeof(BoxDev),
-                        hipMemcpyHostToDevice));
-    return MSM_OK;
+    return replan_bricks(s);
 }
 
 // LDS-table passes: global list with 1, 2 or 4 lanes per atom, no compaction, and an LDS window
@@ -405,7 +493,7 @@ void enqueue_density(msm_sim* s) {
     else if (s->force_path > 0)
         msm::launch_density_tpa(s->force_path, a, list_view(s), s->tab, s->boxes[0], s->d_boxes,
                                 multibox(s), s->d_err, s->stream);
-    else if (s->nlist_global)
+    else if (s->nlist_kind == 1)
         msm::launch_density(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
                             s->d_err, s->stream);
     else
@@ -433,7 +521,7 @@ void enqueue_force(msm_sim* s, int mode, int write_f, double h) {
     else if (s->force_path > 0)
         msm::launch_force_tpa(s->force_path, a, list_view(s), s->tab, s->boxes[0], s->d_boxes,
                               multibox(s), sp, s->stream);
-    else if (s->nlist_global)
+    else if (s->nlist_kind == 1)
         msm::launch_force(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), sp,
                           s->stream);
     else
@@ -748,6 +836,7 @@ int msm_set_boxes(msm_sim* s, int nbox, const int* natoms, const double* lo, con
             d.ncell[k] = c;
             nc *= c;
         }
+        d.cbs = s->cell_brick;
         d.cell_off = static_cast<int>(cells);
         cells += nc;
         if (cells > (1LL << 30)) return set_err(s, MSM_ERR_ARG, "too many cells");
@@ -1067,10 +1156,18 @@ int msm_set_option(msm_sim* s, int option, int value) {
         case MSM_OPT_KERNEL_TIMING:
             s->timing = value ? 1 : 0;
             return MSM_OK;
-        case MSM_OPT_NLIST_KERNEL:
-            s->nlist_global = value ? 1 : 0;
+        case MSM_OPT_NLIST_KERNEL: {
+            if (value < 0 || value > 3)
+                return set_err(s, MSM_ERR_ARG, "list kernel must be 0, 1, 2 or 3");
+            s->nlist_kind = value;
             clear_graphs(s);
+            if (!s->boxes.empty() && s->has_atoms) {
+                const int rc = replan_bricks(s);
+                if (rc != MSM_OK) return rc;
+                s->has_list = false;  // brick slots of the list follow the plan: rebuild first
+            }
             return MSM_OK;
+        }
         case MSM_OPT_FORCE_PATH:
             if (value < 0 || value > 8 || (value > 4 && value != 8))
                 return set_err(s, MSM_ERR_ARG, "force path must be 0, 1, 2, 3, 4 or 8");
@@ -1083,6 +1180,17 @@ int msm_set_option(msm_sim* s, int option, int value) {
             return MSM_OK;
         case MSM_OPT_TABLE_LDS:
             s->table_lds = value ? 1 : 0;
+            clear_graphs(s);
+            return MSM_OK;
+        case MSM_OPT_CELL_BRICK:
+            if (value < 1 || value > 16)
+                return set_err(s, MSM_ERR_ARG, "cell brick must be 1..16");
+            s->cell_brick = value;
+            for (auto& b : s->boxes) b.cbs = value;
+            if (!s->boxes.empty())
+                HIPCHK(s, hipMemcpy(s->d_boxes, s->boxes.data(), s->boxes.size() * sizeof(BoxDev),
+                                    hipMemcpyHostToDevice));
+            s->has_list = false;  // the sorted order follows the numbering: rebuild first
             clear_graphs(s);
             return MSM_OK;
         case MSM_OPT_LT_DENSITY_BLOCK:
@@ -1157,21 +1265,21 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     clear_flags(s);
     // list-build kernel (a rebuild of unchanged positions reproduces the same order and list)
     float best_nl = 0.f;
-    int nl_choice = s->nlist_global;
-    for (int g : {0, 1}) {
-        s->nlist_global = g;
+    int nl_choice = s->nlist_kind;
+    for (int g : {2, 3, 1}) {
+        s->nlist_kind = g;
+        if ((rc = replan_bricks(s)) != MSM_OK) break;
         float ms = 0.f;
         rc = time_it([&] { enqueue_rebuild(s); }, ms);
         if (rc != MSM_OK) break;
-        if (g == 0 || ms < best_nl) {
+        if (g == 2 || ms < best_nl) {
             best_nl = ms;
             nl_choice = g;
         }
     }
-    s->nlist_global = nl_choice;
-    if (rc == MSM_OK) {
-        enqueue_rebuild(s);
-    }
+    s->nlist_kind = nl_choice;
+    if (rc == MSM_OK) rc = replan_bricks(s);
+    if (rc == MSM_OK) enqueue_rebuild(s);
     // density + force passes
     float best = 0.f;
     size_t bi = 0;
@@ -1202,7 +1310,7 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     enqueue_density(s);
     enqueue_force(s, 0, 1, 0.0);
     if (choice) {
-        const int c[8] = {w.fp, w.lds, w.bd, w.wd, w.bf, w.wf, s->nlist_global, 0};
+        const int c[8] = {w.fp, w.lds, w.bd, w.wd, w.bf, w.wf, s->nlist_kind, 0};
         for (int k = 0; k < 8; ++k) choice[k] = c[k];
     }
     if (best_ms) {

@@ -3,13 +3,14 @@
 //
 // Atoms are cell-sorted, so each cell is a contiguous index range.  A brick is at most 4x4x4
 // cells of one box; its halo is the brick plus one cell layer (periodic images resolved), at most
-// 6x6x6 = 216 cells.  One 256-thread workgroup per brick copies the halo atoms into LDS with
+// 6x6x6 = 216 cells.  One workgroup per brick copies the halo atoms into LDS with
 // coalesced loads (consecutive threads read consecutive atoms of a cell run) and then every home
 // atom works out of LDS.  The list build records each neighbour both as a caller-visible sorted
 // index (INDI) and as its 16-bit LDS slot in the home brick's halo; the density and force passes
 // rebuild the same halo (cell ranges do not change between rebuilds), read the slot list and
-// never gather from global memory.  A brick whose halo exceeds kNlCap atoms is processed from
-// global memory instead (same results, slower).
+// never gather from global memory.  A brick whose halo exceeds the LDS capacity of a kernel
+// (kNlCap for the list build, kNlCapPass for the passes) is processed from global memory
+// instead (same results, slower).
 //
 // Reference semantics: list predicate MD_NeighborsList.F90:396-633; density, force, integrator
 // as in msm_kernels.hip (MD_FS/EAM_ForceTable_GPU.F90 PRECALFOR/CALFORCE, MD_DiffScheme_GPU.F90).
@@ -70,7 +71,7 @@ __device__ int brick_halo(const BoxDev& B, const BrickInfo& g, const int* __rest
             g3[k] = c;
             if (hsh) hsh[3 * tid + k] = sh;
         }
-        const int gc = ok ? B.cell_off + g3[0] + B.ncell[0] * (g3[1] + B.ncell[1] * g3[2]) : -1;
+        const int gc = ok ? B.cell_off + cell_local(B, g3[0], g3[1], g3[2]) : -1;
         hcell[tid] = gc;
         cnt = ok ? start[gc + 1] - start[gc] : 0;
     }
@@ -101,7 +102,7 @@ __device__ __forceinline__ void for_halo_slots(const BrickInfo& g, const int* __
                                                const int* hcell, const int* hoff, F&& f) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nrows = g.H[1] * g.H[2];
-    for (int r = wave; r < nrows; r += kBlock / 64) {
+    for (int r = wave; r < nrows; r += static_cast<int>(blockDim.x >> 6)) {
         const int h0 = r * g.H[0];
         const int s1 = hoff[h0 + g.H[0]];
         for (int s = hoff[h0] + lane; s < s1; s += 64) {
@@ -136,7 +137,7 @@ struct BrickLds {
     int hoff[kNlMaxHalo + 1];
     int hcell[kNlMaxHalo];
     int homeoff[65];
-    int scan[kBlock / 64];
+    int scan[kNlBlock / 64];
 };
 
 }  // namespace
@@ -145,17 +146,20 @@ struct BrickLds {
 // list build
 // --------------------------------------------------------------------------------------------
 
+// Small-brick list build (sparse cells): one kBlock-thread workgroup per brick of at most
+// kNlCapPass halo atoms (several workgroups share a CU), one lane per home atom testing the
+// candidates of each stencil cell four at a time, in the serial order.
 template <int MT>
-__global__ __launch_bounds__(kBlock) void k_nlist_bricks(
+__global__ __launch_bounds__(kBlock) void k_nlist_small(
     const int* __restrict__ brick_box, const double4* __restrict__ pos,
     const int* __restrict__ ityp, const int* __restrict__ start,
     const BoxDev* __restrict__ boxes, int ngroup, const double* __restrict__ rnl2, int mx,
     int* __restrict__ kvois, int* __restrict__ indi, unsigned short* __restrict__ sidx,
     int stride, int* err) {
 #pragma clang fp contract(off)
-    __shared__ double lx[kNlCap], ly[kNlCap], lz[kNlCap];
-    __shared__ int lg[kNlCap];
-    __shared__ int lt[MT ? kNlCap : 1];
+    __shared__ double lx[kNlCapPass], ly[kNlCapPass], lz[kNlCapPass];
+    __shared__ int lg[kNlCapPass];
+    __shared__ int lt[MT ? kNlCapPass : 1];
     __shared__ signed char hsh[kNlMaxHalo * 3];
     __shared__ double srnl2[kMaxGroup * kMaxGroup];
     __shared__ BrickLds L;
@@ -166,9 +170,9 @@ __global__ __launch_bounds__(kBlock) void k_nlist_bricks(
     BrickInfo g;
     brick_info(B, bi, g);
     if constexpr (MT)
-        for (int k = tid; k < ngroup * ngroup; k += kBlock) srnl2[k] = rnl2[k];
+        for (int k = tid; k < ngroup * ngroup; k += blockDim.x) srnl2[k] = rnl2[k];
     const int total = brick_halo(B, g, start, L.hcell, hsh, L.hoff, L.homeoff, L.scan);
-    const bool staged = total <= kNlCap;
+    const bool staged = total <= kNlCapPass;
     if (staged) {
         for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int h) {
             const double4 pq = pos[q];
@@ -183,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_nlist_bricks(
     __syncthreads();
     const int nhome = L.homeoff[g.nhc];
     const double rsingle = rnl2[0];
-    for (int t = tid; t < nhome; t += kBlock) {
+    for (int t = tid; t < nhome; t += blockDim.x) {
         int h, off;
         home_atom(t, g, L.homeoff, h, off);
         if (!staged) {
@@ -236,13 +240,135 @@ __global__ __launch_bounds__(kBlock) void k_nlist_bricks(
     }
 }
 
+
+// Wide-brick list build (dense cells): one kNlBlock-thread workgroup per brick.  The halo (up to `cap` atoms, image shifts added as
+// the single rounding XP(J) + CXYZ of MD_NeighborsList.F90:582) is staged in dynamic LDS; then T
+// adjacent lanes serve one home atom: for each of its 27 stencil cells, in the reference stencil
+// order, the T lanes test kNlUnroll*T consecutive candidates of the cell per round (lane `sub`
+// takes candidates sub, sub+T, ...; their LDS loads are issued together), and wave ballots place
+// the hits in candidate order, so every atom's list comes out in exactly the serial order
+// (stencil cell, then slot) whatever T is.  A brick whose halo exceeds `cap` falls back to the
+// global-memory scan.
+constexpr int kNlUnroll = 4;
+
+template <int MT, int T>
+__global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
+    const int* __restrict__ brick_box, const double4* __restrict__ pos,
+    const int* __restrict__ ityp, const int* __restrict__ start,
+    const BoxDev* __restrict__ boxes, int ngroup, const double* __restrict__ rnl2, int mx,
+    int* __restrict__ kvois, int* __restrict__ indi, unsigned short* __restrict__ sidx,
+    int stride, int* err, int cap) {
+#pragma clang fp contract(off)
+    extern __shared__ double s_nl[];
+    double* lx = s_nl;
+    double* ly = lx + cap;
+    double* lz = ly + cap;
+    int* lg = reinterpret_cast<int*>(lz + cap);
+    int* lt = lg + cap;  // MT only
+    __shared__ signed char hsh[kNlMaxHalo * 3];
+    __shared__ double srnl2[kMaxGroup * kMaxGroup];
+    __shared__ BrickLds L;
+
+    const int tid = threadIdx.x;
+    const int bi = xcd_block(gridDim.x);
+    const BoxDev& B = boxes[brick_box[bi]];
+    BrickInfo g;
+    brick_info(B, bi, g);
+    if constexpr (MT)
+        for (int k = tid; k < ngroup * ngroup; k += blockDim.x) srnl2[k] = rnl2[k];
+    const int total = brick_halo(B, g, start, L.hcell, hsh, L.hoff, L.homeoff, L.scan);
+    const bool staged = total <= cap;
+    if (staged) {
+        for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int h) {
+            const double4 pq = pos[q];
+            lx[s] = pq.x + shift_of(hsh[3 * h], B.len[0]);
+            ly[s] = pq.y + shift_of(hsh[3 * h + 1], B.len[1]);
+            lz[s] = pq.z + shift_of(hsh[3 * h + 2], B.len[2]);
+            lg[s] = q;
+            if constexpr (MT) lt[s] = ityp[q];
+        });
+    }
+    __syncthreads();
+    const int nhome = L.homeoff[g.nhc];
+    const double rsingle = rnl2[0];
+    const int sub = tid % T;
+    const int gbase = (tid & 63) - sub;  // first lane of this atom's group in the wave
+    const unsigned below = (1u << sub) - 1u, gmask = (T == 32) ? ~0u : ((1u << T) - 1u);
+    for (int t0 = 0; t0 < nhome * T; t0 += blockDim.x) {
+        const int a = (t0 + tid) / T;
+        if (a >= nhome) continue;
+        int h, off;
+        home_atom(a, g, L.homeoff, h, off);
+        if (!staged) {
+            if (sub == 0) {
+                const int p = start[L.hcell[h]] + off;
+                nlist_atom_global(p, L.hcell[h] - B.cell_off, B, pos, ityp, start, ngroup, rnl2,
+                                  mx, kvois, indi, stride, err);
+            }
+            continue;
+        }
+        const int si = L.hoff[h] + off;
+        const int p = lg[si];
+        const double xi = lx[si], yi = ly[si], zi = lz[si];
+        const int ti = MT ? lt[si] : 0;
+        const size_t lb = list_base(p, mx);
+        int c = 0;
+        for (int s = 0; s < 27; ++s) {
+            const int hn = h + c_nix[s] + g.H[0] * (c_niy[s] + g.H[1] * c_niz[s]);
+            const int qe = L.hoff[hn + 1];
+            for (int q0 = L.hoff[hn]; q0 < qe; q0 += kNlUnroll * T) {
+                bool hit[kNlUnroll];
+#pragma unroll
+                for (int u = 0; u < kNlUnroll; ++u) {
+                    const int q = q0 + u * T + sub;
+                    const bool valid = q < qe && q != si;
+                    const int qq = valid ? q : si;
+                    // XPT(I) - (XP(J) + CXYZ), squared and summed left to right, no contraction
+                    const double dx = xi - lx[qq];
+                    const double dy = yi - ly[qq];
+                    const double dz = zi - lz[qq];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const double lim = MT ? srnl2[ti * ngroup + lt[qq]] : rsingle;
+                    hit[u] = valid && r2 < lim;
+                }
+#pragma unroll
+                for (int u = 0; u < kNlUnroll; ++u) {
+                    const unsigned grp =
+                        static_cast<unsigned>(__ballot(hit[u]) >> gbase) & gmask;
+                    if (hit[u]) {
+                        const int w = c + __popc(grp & below);
+                        if (w < mx) {
+                            const int q = q0 + u * T + sub;
+                            const size_t at = lb + static_cast<size_t>(w) * stride;
+                            indi[at] = lg[q];
+                            sidx[at] = static_cast<unsigned short>(q);
+                        }
+                    }
+                    c += __popc(grp);
+                }
+            }
+        }
+        if (sub == 0) {
+            if (c > mx) {
+                atomicOr(&err[kErrOverflow], 1);
+                c = mx;
+            }
+            kvois[p] = c;
+        }
+    }
+}
+
+size_t nlist_lds_bytes(int cap, int ngroup) {
+    return static_cast<size_t>(cap) * (3 * sizeof(double) + sizeof(int) + (ngroup > 1 ? sizeof(int) : 0));
+}
+
 __global__ void k_zero_range(int* __restrict__ p, const int* __restrict__ first, int n) {
     const int i = *first + blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
-void launch_nlist_bricks(int nbricks, const int* brick_box, int n, int ncell_total,
-                         const double4* pos, const int* ityp, const int* start,
+void launch_nlist_bricks(int wide, int nbricks, int lanes, int cap, const int* brick_box, int n,
+                         int ncell_total, const double4* pos, const int* ityp, const int* start,
                          const BoxDev* boxes, int ngroup, const double* rnl2, int mx, int* kvois,
                          int* indi, unsigned short* sidx, int stride, int* err, hipStream_t s) {
     if (n <= 0) return;
@@ -250,12 +376,32 @@ void launch_nlist_bricks(int nbricks, const int* brick_box, int n, int ncell_tot
     hipLaunchKernelGGL(k_zero_range, dim3(grid_for(n)), dim3(kBlock), 0, s, kvois,
                        start + ncell_total, n);
     if (nbricks <= 0) return;
-    if (ngroup > 1)
-        hipLaunchKernelGGL(k_nlist_bricks<1>, dim3(nbricks), dim3(kBlock), 0, s, brick_box, pos,
-                           ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx, stride, err);
-    else
-        hipLaunchKernelGGL(k_nlist_bricks<0>, dim3(nbricks), dim3(kBlock), 0, s, brick_box, pos,
-                           ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx, stride, err);
+    if (!wide) {
+        if (ngroup > 1)
+            hipLaunchKernelGGL(k_nlist_small<1>, dim3(nbricks), dim3(kBlock), 0, s, brick_box,
+                               pos, ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx,
+                               stride, err);
+        else
+            hipLaunchKernelGGL(k_nlist_small<0>, dim3(nbricks), dim3(kBlock), 0, s, brick_box,
+                               pos, ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx,
+                               stride, err);
+        return;
+    }
+    const size_t lds = nlist_lds_bytes(cap, ngroup);
+#define MSM_NL(MT, T)                                                                           \
+    hipLaunchKernelGGL((k_nlist_bricks<MT, T>), dim3(nbricks), dim3(kNlBlock), lds, s, brick_box, \
+                       pos, ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx, stride, err, \
+                       cap)
+#define MSM_NLT(MT)                                   \
+    do {                                              \
+        if (lanes >= 8) MSM_NL(MT, 8);                \
+        else if (lanes >= 4) MSM_NL(MT, 4);           \
+        else if (lanes >= 2) MSM_NL(MT, 2);           \
+        else MSM_NL(MT, 1);                           \
+    } while (0)
+    if (ngroup > 1) MSM_NLT(1); else MSM_NLT(0);
+#undef MSM_NLT
+#undef MSM_NL
 }
 
 // --------------------------------------------------------------------------------------------
@@ -268,8 +414,8 @@ __global__ __launch_bounds__(kBlock) void k_density_bricks(const int* __restrict
                                                            const BoxDev* __restrict__ boxes,
                                                            const int* __restrict__ start,
                                                            int* err) {
-    __shared__ double lx[kNlCap], ly[kNlCap], lz[kNlCap];
-    __shared__ int lt[MT ? kNlCap : 1];
+    __shared__ double lx[kNlCapPass], ly[kNlCapPass], lz[kNlCapPass];
+    __shared__ int lt[MT ? kNlCapPass : 1];
     __shared__ BrickLds L;
 
     const int tid = threadIdx.x;
@@ -278,7 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_density_bricks(const int* __restrict
     BrickInfo g;
     brick_info(B, bi, g);
     const int total = brick_halo(B, g, start, L.hcell, nullptr, L.hoff, L.homeoff, L.scan);
-    const bool staged = total <= kNlCap;
+    const bool staged = total <= kNlCapPass;
     if (staged) {
         for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int) {
             // x, y, z only: w (F') of other atoms is being written by this pass
@@ -380,8 +526,8 @@ __global__ __launch_bounds__(kBlock) void k_force_bricks(const int* __restrict__
                                                          const BoxDev* __restrict__ boxes,
                                                          const int* __restrict__ start,
                                                          StepParams sp) {
-    __shared__ double4 lp[kNlCap];
-    __shared__ int lt[MT ? kNlCap : 1];
+    __shared__ double4 lp[kNlCapPass];
+    __shared__ int lt[MT ? kNlCapPass : 1];
     __shared__ BrickLds L;
 
     const int tid = threadIdx.x;
@@ -390,7 +536,7 @@ __global__ __launch_bounds__(kBlock) void k_force_bricks(const int* __restrict__
     BrickInfo g;
     brick_info(B, bi, g);
     const int total = brick_halo(B, g, start, L.hcell, nullptr, L.hoff, L.homeoff, L.scan);
-    const bool staged = total <= kNlCap;
+    const bool staged = total <= kNlCapPass;
     if (staged) {
         for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int) {
             lp[s] = a.pos[q];  // x, y, z and F'

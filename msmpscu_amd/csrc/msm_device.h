@@ -27,6 +27,37 @@ static __device__ __forceinline__ int atom_index() {
 
 static inline unsigned grid_for(int n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
+// Local id of cell (ix, iy, iz) in box b.  Cells are numbered in bricks of cbs^3 cells (x fastest
+// inside a brick; bricks x fastest, then y, then z; the last brick of a direction is cut to the
+// grid), so runs of consecutive cell ids -- and of cell-sorted atoms -- are compact 3-D blocks
+// rather than rows.  cbs = 1 is the plain row-major order.  The numbering is a bijection onto
+// [0, ncell[0]*ncell[1]*ncell[2]).
+__host__ __device__ __forceinline__ int cell_local(const BoxDev& b, int ix, int iy, int iz) {
+    const int bs = b.cbs, nx = b.ncell[0], ny = b.ncell[1], nz = b.ncell[2];
+    const int bx = ix / bs, by = iy / bs, bz = iz / bs;
+    const int lx = ix - bx * bs, ly = iy - by * bs, lz = iz - bz * bs;
+    const int ex = min(bs, nx - bx * bs), ey = min(bs, ny - by * bs), ez = min(bs, nz - bz * bs);
+    return nx * ny * (bz * bs) + nx * (by * bs) * ez + (bx * bs) * ey * ez + lx + ex * (ly + ey * lz);
+}
+
+// Inverse of cell_local.
+__host__ __device__ __forceinline__ void cell_coords(const BoxDev& b, int lc, int& ix, int& iy,
+                                                     int& iz) {
+    const int bs = b.cbs, nx = b.ncell[0], ny = b.ncell[1], nz = b.ncell[2];
+    const int bz = lc / (nx * ny * bs);
+    int r = lc - bz * nx * ny * bs;
+    const int ez = min(bs, nz - bz * bs);
+    const int by = r / (nx * bs * ez);
+    r -= by * nx * bs * ez;
+    const int ey = min(bs, ny - by * bs);
+    const int bx = r / (bs * ey * ez);
+    r -= bx * bs * ey * ez;
+    const int ex = min(bs, nx - bx * bs);
+    ix = bx * bs + r % ex;
+    iy = by * bs + (r / ex) % ey;
+    iz = bz * bs + r / (ex * ey);
+}
+
 // Offset of slot 0 of atom p in a blocked neighbour list (slot iw adds iw * kListRow).
 static __device__ __forceinline__ size_t list_base(int p, int mx) {
     return static_cast<size_t>(p / kListRow) * static_cast<size_t>(mx) * kListRow + (p % kListRow);
@@ -172,9 +203,8 @@ static __device__ void nlist_atom_global(int p, int lc, const BoxDev& b, const d
                                   int* __restrict__ kvois, int* __restrict__ indi, int stride,
                                   int* err) {
 #pragma clang fp contract(off)
-    const int ix = lc % b.ncell[0];
-    const int iy = (lc / b.ncell[0]) % b.ncell[1];
-    const int iz = lc / (b.ncell[0] * b.ncell[1]);
+    int ix, iy, iz;
+    cell_coords(b, lc, ix, iy, iz);
     const double4 xi = pos[p];
     const int ti = ityp[p];
     int cnt = 0;
@@ -195,7 +225,7 @@ static __device__ void nlist_atom_global(int p, int lc, const BoxDev& b, const d
             skip = skip || j3[k] < 0 || j3[k] >= b.ncell[k];
         }
         if (skip) continue;
-        const int g = b.cell_off + j3[0] + b.ncell[0] * (j3[1] + b.ncell[1] * j3[2]);
+        const int g = b.cell_off + cell_local(b, j3[0], j3[1], j3[2]);
         const int qe = start[g + 1];
         for (int q = start[g]; q < qe; ++q) {
             if (q == p) continue;

@@ -33,10 +33,14 @@ struct BoxDev {
     int bdim[3];   // cells per brick of the LDS-staged list build
     int nbrick[3]; // bricks per direction
     int brick_off; // first global brick id of this box
+    int cbs;       // cell ordering: bricks of cbs^3 cells are contiguous (1 = row-major)
 };
 
-// LDS-staged list build: halo atoms per workgroup and halo cells per brick (bricks <= 4^3).
-constexpr int kNlCap = 1536;
+// LDS-staged list build: threads per workgroup, most halo atoms per workgroup (the density and
+// force brick kernels stage at most kNlCapPass of them) and halo cells per brick (bricks <= 4^3).
+constexpr int kNlBlock = 1024;
+constexpr int kNlCap = 4096;
+constexpr int kNlCapPass = 1536;
 constexpr int kNlMaxHalo = 216;
 
 struct TabDev {
@@ -112,7 +116,9 @@ void launch_nlist(int n, const double4* pos, const int* ityp, const int* statu, 
                   hipStream_t s);
 // LDS-staged variants over bricks of cells (msm_bricks.hip); out-of-box atoms (sentinel bucket)
 // get KVOIS = 0, F' = 0, F = 0 and keep their position.
-void launch_nlist_bricks(int nbricks, const int* brick_box, int n, int ncell_total,
+size_t nlist_lds_bytes(int cap, int ngroup);  // dynamic LDS of the list build for cap halo atoms
+void launch_nlist_bricks(int wide, int nbricks, int lanes, int cap, const int* brick_box, int n,
+                         int ncell_total,
                          const double4* pos, const int* ityp, const int* start,
                          const BoxDev* boxes, int ngroup, const double* rnl2, int mx, int* kvois,
                          int* indi, unsigned short* sidx, int stride, int* err, hipStream_t s);

@@ -59,7 +59,7 @@ __global__ void k_cell_assign(const double4* __restrict__ pos, const int* __rest
             in = in && ic[k] >= 0 && ic[k] < b.ncell[k];
         }
         if (in)
-            c = b.cell_off + ic[0] + b.ncell[0] * (ic[1] + b.ncell[1] * ic[2]);
+            c = b.cell_off + cell_local(b, ic[0], ic[1], ic[2]);
         else
             atomicOr(&err[kErrOutOfBox], 1);
     }

@@ -411,7 +411,7 @@ class DeviceMD:
         ms = np.zeros(2)
         check(lib().msm_autotune(self.h, int(reps), iptr(ch), dptr(ms)), "msm_autotune", self.h)
         return {"force_path": int(ch[0]), "table_lds": int(ch[1]),
-                "lt_shape": [int(v) for v in ch[2:6]], "nlist_global": int(ch[6]),
+                "lt_shape": [int(v) for v in ch[2:6]], "nlist_kernel": int(ch[6]),
                 "density_force_ms": float(ms[0]), "rebuild_ms": float(ms[1])}
 
     def kernel_times(self, reset: bool = True) -> dict:

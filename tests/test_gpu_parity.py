@@ -88,11 +88,14 @@ def test_neighbour_list_bit_exact(orc, p0, case):
     dev.close()
 
 
-def _list_both_kernels(box, ctrl, ft):
+def _list_all_kernels(box, ctrl, ft):
+    """Lists from every list kernel (global scan, small bricks, wide bricks, automatic choice)
+    and, for the automatic choice, with cells numbered in 4^3 bricks."""
     import msmpscu_amd._lib as L
     out = []
-    for mode in (0, 1):
+    for mode, cbs in ((1, 1), (2, 1), (3, 1), (0, 1), (0, 4)):
         dev = device_for(box, ctrl, ft)
+        L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_CELL_BRICK, cbs), "msm_set_option")
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_NLIST_KERNEL, mode), "msm_set_option")
         dev.build_neighbors()
         out.append(dev.get_neighbors())
@@ -100,10 +103,12 @@ def _list_both_kernels(box, ctrl, ft):
     return out
 
 
-@pytest.mark.parametrize("case", ["small_ncell3", "dense_blob", "slab"])
+@pytest.mark.parametrize("case", ["small_ncell3", "dense_blob", "slab", "w16"])
 def test_brick_list_equals_global_scan(orc, case):
-    """The LDS-staged brick build (including bricks that overflow LDS and take the global path)
-    gives the identical list, in identical order, to the per-atom global-memory scan."""
+    """The LDS-staged brick builds (small bricks with one lane per atom, wide bricks with
+    several lanes per atom and ballot placement, including bricks that overflow LDS and take the
+    global path) give the identical list, in identical order, to the per-atom global-memory
+    scan; so does the brick-ordered cell numbering."""
     if case == "small_ncell3":
         box = cfg.make_bcc_box((4, 4, 4), cfg.W_A, 0.1, 300.0, 3)
         ctrl = cfg.w_ctrl(mx=64)
@@ -118,15 +123,20 @@ def test_brick_list_equals_global_scan(orc, case):
                           ITYP=np.ones(n, np.int32), STATU=np.ones(n, np.int32), CM=box.CM,
                           PTYPE=box.PTYPE)
         ctrl = cfg.w_ctrl(mx=800)
-    else:
+    elif case == "slab":
         box = cfg.make_cuni_slab((6, 6, 6), 3.57, 300.0, 9)
         ctrl = cfg.cuni_ctrl()
+    else:
+        box = cfg.make_bcc_box((16, 16, 16), cfg.W_A, 0.05, 300.0, 12)
+        ctrl = cfg.w_ctrl(mx=64)
     ft = cfg.w_forcetable(box, ctrl) if case != "slab" else cfg.cuni_forcetable(box, ctrl)
     say(f"brick vs global list: {case} n={box.NPRT}")
-    (kv0, ind0), (kv1, ind1) = _list_both_kernels(box, ctrl, ft)
-    assert np.array_equal(kv0, kv1)
-    for i in range(len(kv0)):
-        assert np.array_equal(ind0[i, : kv0[i]], ind1[i, : kv1[i]])
+    lists = _list_all_kernels(box, ctrl, ft)
+    kv0, ind0 = lists[0]
+    for kv1, ind1 in lists[1:]:
+        assert np.array_equal(kv0, kv1)
+        for i in range(len(kv0)):
+            assert np.array_equal(ind0[i, : kv0[i]], ind1[i, : kv1[i]])
     okv, oind = orc.nlist("full", box.XP, box.ITYP - 1, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
                           ctrl.NB_RM, ctrl.NB_MXNBS)
     assert pairset(kv0, ind0, True) == pairset(okv, oind, False)
