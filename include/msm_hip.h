@@ -170,6 +170,11 @@ int msm_calc_force(msm_sim* sim);
 /* CALEPOT_*_Force_Table2A_DEV: per-atom potential energy (and box totals). */
 int msm_calc_epot(msm_sim* sim);
 /* Predictor_DEV / Correction_DEV (Predictor_KERNEL0 / Correction_KERNEL semantics). */
+/* Virial tensor per box (CALPTENSOR_FS/EAM_Force_Table2A_DEV, MD_FS_ForceTable_GPU.F90:998-1447):
+ * sum over pairs of s (x) s * FORTOT (each pair once), VTENSOR(3,3) column-major at vtensor[9*b],
+ * b = 0..nbox-1; recomputes the density pass and stores the forces as msm_calc_force does.  The
+ * reference's COPYOUT_VIRIALTENSOR returns the mean over boxes; callers average these. */
+int msm_calc_ptensor(msm_sim* sim, double* vtensor);
 int msm_predictor(msm_sim* sim, double h);
 int msm_correction(msm_sim* sim, double h);
 

@@ -1090,6 +1090,45 @@ int msm_calc_epot(msm_sim* s) {
     return read_flags(s);
 }
 
+int msm_calc_ptensor(msm_sim* s, double* vtensor) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (!vtensor) return set_err(s, MSM_ERR_ARG, "null argument");
+    (void)hipSetDevice(s->device);
+    const int nbox = static_cast<int>(s->boxes.size());
+    double* vir = nullptr;
+    double* sums = nullptr;
+    rc = dalloc(s, &vir, 6 * static_cast<size_t>(std::max(s->n, 1)));
+    if (rc == MSM_OK) rc = dalloc(s, &sums, 6 * static_cast<size_t>(nbox));
+    if (rc != MSM_OK) {
+        dfree(&vir);
+        return rc;
+    }
+    clear_flags(s);
+    enqueue_density(s);
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_virial(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), vir,
+                       s->stream);
+    for (int c = 0; c < 6; ++c)
+        msm::launch_box_sum(vir + static_cast<size_t>(c) * s->n, s->sb.start, s->d_boxes, nbox,
+                            sums + static_cast<size_t>(c) * nbox, s->stream);
+    std::vector<double> h(6 * static_cast<size_t>(nbox));
+    hipError_t e = hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    dfree(&vir);
+    dfree(&sums);
+    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    static const int row[6] = {0, 1, 2, 0, 0, 1}, col[6] = {0, 1, 2, 1, 2, 2};
+    for (int b = 0; b < nbox; ++b)
+        for (int c = 0; c < 6; ++c) {
+            const double v = h[static_cast<size_t>(c) * nbox + b];
+            vtensor[9 * b + row[c] + 3 * col[c]] = v;
+            vtensor[9 * b + col[c] + 3 * row[c]] = v;
+        }
+    return read_flags(s);
+}
+
 int msm_predictor(msm_sim* s, double h) {
     int rc = check_ready(s, false);
     if (rc != MSM_OK) return rc;

@@ -189,6 +189,9 @@ void launch_unsort_pos(const double4* src, double* dst, const int* gid, int n, i
                        hipStream_t s);
 void launch_unsort_list(const int* kvois, const int* indi, int stride, const int* gid, int n,
                         int mx, int* kv_out, int* indi_out, int ld, hipStream_t s);
+// Virial tensor per atom (6 components, component-major, stride a.n) and forces.
+void launch_virial(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                   const BoxDev* boxes, int multibox, double* vir, hipStream_t s);
 void launch_box_sum(const double* epot, const int* start, const BoxDev* boxes, int nbox,
                     double* out, hipStream_t s);
 void launch_list_stats(const int* kvois, int n, unsigned long long* total, int* maxk,

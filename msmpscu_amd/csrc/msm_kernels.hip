@@ -575,6 +575,77 @@ void launch_unsort_list(const int* kvois, const int* indi, int stride, const int
                            stride, gid, n, mx, kv_out, indi_out, ld);
 }
 
+// Virial tensor (CALPTENSOR_KERNEL, MD_FS_ForceTable_GPU.F90:998-1272): per active atom i,
+// sum over in-cutoff neighbours of FORTOT/2 * s (x) s (the full list sees each pair twice), in
+// the six distinct components {xx, yy, zz, xy, xz, yz}, stored component-major (stride n); the
+// forces are written as by the force pass.  F' of the density pass is read from pos.w.
+template <int MB, int MT>
+__global__ __launch_bounds__(kBlock) void k_virial(AtomsDev a, ListDev l, TabDev t, BoxDev box0,
+                                                   const BoxDev* __restrict__ boxes,
+                                                   double* __restrict__ vir) {
+    const int i = atom_index();
+    if (i >= a.n) return;
+    double p[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double fpx = 0.0, fpy = 0.0, fpz = 0.0;
+    if ((a.statu[i] & STATU_ACTIVE) == STATU_ACTIVE) {
+        const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
+        const double4 pi = a.pos[i];
+        const int ti = MT ? a.ityp[i] : 0;
+        const int kv = l.kvois[i];
+        for (int iw = 0; iw < kv; ++iw) {
+            const int j = l.indi[list_base(i, l.mx) + static_cast<size_t>(iw) * l.stride];
+            const double4 pj = a.pos[j];
+            const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
+            const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
+            const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            if (!(r2 <= t.ru2max)) continue;
+            int kind = t.kpair[0], kind1 = t.kpair[0];
+            if constexpr (MT) {
+                const int tj = a.ityp[j];
+                kind = t.kpair[ti * t.ngroup + tj];
+                kind1 = t.kpair[tj * t.ngroup + ti];
+            }
+            double inv_r;
+            const double sk = table_coord(r2, t.csi, inv_r);
+            const int kk = min(static_cast<int>(sk), t.ntab2 - 1);
+            const double dk = sk - kk;
+            const double4 f = t.f4[kind * t.ntab2 + kk];
+            const double4 f1 = t.f4[kind1 * t.ntab2 + kk];
+            const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
+                                  ((f.z + dk * f.w) * pi.w + (f1.z + dk * f1.w) * pj.w) * inv_r;
+            fpx = fpx + fortot * sx;
+            fpy = fpy + fortot * sy;
+            fpz = fpz + fortot * sz;
+            const double h = fortot * 0.5;
+            p[0] = p[0] + sx * sx * h;
+            p[1] = p[1] + sy * sy * h;
+            p[2] = p[2] + sz * sz * h;
+            p[3] = p[3] + sx * sy * h;
+            p[4] = p[4] + sx * sz * h;
+            p[5] = p[5] + sy * sz * h;
+        }
+    }
+    a.fx[i] = fpx;
+    a.fy[i] = fpy;
+    a.fz[i] = fpz;
+    for (int c = 0; c < 6; ++c) vir[static_cast<size_t>(c) * a.n + i] = p[c];
+}
+
+void launch_virial(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
+                   const BoxDev* boxes, int multibox, double* vir, hipStream_t s) {
+    if (a.n <= 0) return;
+    const dim3 g(grid_for(a.n)), bl(kBlock);
+    const bool mt = t.ngroup > 1;
+    if (multibox) {
+        if (mt) hipLaunchKernelGGL((k_virial<1, 1>), g, bl, 0, s, a, l, t, box0, boxes, vir);
+        else    hipLaunchKernelGGL((k_virial<1, 0>), g, bl, 0, s, a, l, t, box0, boxes, vir);
+    } else {
+        if (mt) hipLaunchKernelGGL((k_virial<0, 1>), g, bl, 0, s, a, l, t, box0, boxes, vir);
+        else    hipLaunchKernelGGL((k_virial<0, 0>), g, bl, 0, s, a, l, t, box0, boxes, vir);
+    }
+}
+
 // One workgroup per box: fixed-order strided partial sums then a fixed-shape tree, so the total
 // is the same on every run.
 __global__ __launch_bounds__(kBlock) void k_box_sum(const double* __restrict__ epot,

@@ -141,6 +141,13 @@ module MSM_HIP_C
       integer(c_int) :: rc
     end function msm_calc_epot
 
+    function msm_calc_ptensor(sim, vtensor) bind(C, name="msm_calc_ptensor") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: sim
+      real(c_double), intent(out) :: vtensor(*)
+      integer(c_int) :: rc
+    end function msm_calc_ptensor
+
     function msm_predictor(sim, h) bind(C, name="msm_predictor") result(rc)
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: sim
@@ -312,7 +319,8 @@ module MD_ForceLib_HIP
 
   public :: Register_ForceTableProc, Create_Interaction_ForceTable, Register_ForceClass
   public :: Initialize_Globle_Variables_DEV, Init_Forcetable_Dev, Cal_NeighBoreList_DEV
-  public :: CalForce_ForceClass, CalEpot_ForceClass, Predictor_DEV, Correction_DEV
+  public :: CalForce_ForceClass, CalEpot_ForceClass, CalPTensor_ForceClass
+  public :: Predictor_DEV, Correction_DEV
   public :: For_One_Step, Run_Steps_DEV, CopyOut_SimBox_DEV, Copyout_NeighboreList_DEV
   public :: Autotune_DEV, Adjoint_ForceTable_DEV, Clear_Globle_Variables_DEV, MSM_Handle
 
@@ -600,6 +608,28 @@ contains
     integer, optional, intent(out) :: IERR
     call msm_check(msm_calc_epot(gm_SIM), "CalEpot_ForceClass", IERR)
   end subroutine CalEpot_ForceClass
+
+  ! CalPTensor_ForceClass (MD_ForceClass_Register_GPU.F90:684-728): the virial tensor as
+  ! COPYOUT_VIRIALTENSOR returns it (MD_FS_ForceTable_GPU.F90:1415-1447), the mean over the boxes
+  subroutine CalPTensor_ForceClass(SimBox, CtrlParam, ForceClass, VTENSOR, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    type(SimMDCtrl), intent(in) :: CtrlParam
+    type(MDForceClassGPU), intent(in) :: ForceClass
+    real(KINDDF), intent(out) :: VTENSOR(3, 3)
+    integer, optional, intent(out) :: IERR
+    real(c_double), allocatable :: v(:)
+    integer(c_int) :: rc
+    integer :: b
+    allocate(v(9 * gm_NBOX))
+    rc = msm_calc_ptensor(gm_SIM, v)
+    call msm_check(rc, "CalPTensor_ForceClass", IERR)
+    VTENSOR = 0.0d0
+    if (rc /= MSM_OK) return
+    do b = 1, gm_NBOX
+      VTENSOR = VTENSOR + reshape(v(9 * b - 8:9 * b), [3, 3])
+    end do
+    VTENSOR = VTENSOR / dble(gm_NBOX)
+  end subroutine CalPTensor_ForceClass
 
   ! Predictor_DEV / Correction_DEV (MD_DiffScheme_GPU.F90:604-682 / 821-847)
   subroutine Predictor_DEV(ITIME, SimBox, CtrlParam, IERR)

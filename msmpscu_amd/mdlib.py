@@ -389,6 +389,12 @@ class DeviceMD:
     def calc_epot(self):
         check(lib().msm_calc_epot(self.h), "msm_calc_epot", self.h)
 
+    def calc_ptensor(self) -> np.ndarray:
+        """Virial tensor per box, shape (nbox, 3, 3) (msm_calc_ptensor)."""
+        v = np.zeros(9 * self.nbox)
+        check(lib().msm_calc_ptensor(self.h, dptr(v)), "msm_calc_ptensor", self.h)
+        return v.reshape(self.nbox, 3, 3).transpose(0, 2, 1).copy()
+
     def predictor(self, h: float):
         check(lib().msm_predictor(self.h, h), "msm_predictor", self.h)
 
@@ -535,6 +541,12 @@ def CalForce_ForceClass(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGP
 
 def CalEpot_ForceClass(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
     ForceClass.Device.calc_epot()
+
+
+def CalPTensor_ForceClass(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
+    """CalPTensor_ForceClass (MD_ForceClass_Register_GPU.F90:684-728): the virial tensor, as
+    COPYOUT_VIRIALTENSOR returns it -- the mean over the boxes (MD_FS_ForceTable_GPU.F90:1415-1447)."""
+    return ForceClass.Device.calc_ptensor().mean(axis=0)
 
 
 def Predictor_DEV(ITIME: int, SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):

@@ -569,3 +569,75 @@ def test_uneven_batch_matches_single_boxes():
         assert abs(eb[k] - e1) <= 1e-12 * abs(e1)
         off += b.NPRT
         d1.close()
+
+
+def test_virial_fs_matches_oracle(orc, p0):
+    """Virial tensor of the FS box (full list, FORTOT/2 per ordered pair) equals the oracle's
+    half-list VTENSOR (CALFOR, each pair once) to 1e-12 of its largest component, and is
+    symmetric."""
+    box, ctrl, ft = p0
+    say("virial vs oracle")
+    dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    v = dev.calc_ptensor()[0]
+    _, _, fp, _, _ = dev.get_atoms()
+    dev.close()
+    fh, _, _, _, vt = oracle_half(orc, box, ctrl, ft)
+    assert np.max(np.abs(v - vt)) <= 1e-12 * np.max(np.abs(vt))
+    assert np.array_equal(v, v.T)
+    assert np.max(np.abs(fp - fh)) <= F_TOL
+
+
+@pytest.mark.parametrize("pot_type", ["FS_TYPE", "EAM_TYPE"])
+def test_virial_is_minus_strain_derivative_of_energy(pot_type):
+    """W_aa = -dE/d(eps_aa) under a homogeneous diagonal strain of positions and box (central
+    differences of the device energy at eps = 1e-6), for FS and EAM tables.  Energies interpolate
+    POTR/POTB and forces the separately tabulated FPOTR/FPOTB, so the two agree to the tables'
+    interpolation error (about 2e-4 relative here), not to rounding: the bar is 1e-3."""
+    base = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.1, 0.0, 21)
+    ctrl = cfg.w_ctrl(mx=64)
+    ft = cfg.w_forcetable(base, ctrl, pot_type=pot_type)
+    say(f"virial strain derivative {pot_type}")
+
+    def energy(eps):
+        b = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.1, 0.0, 21)
+        f = 1.0 + np.asarray(eps)
+        b.XP = np.asfortranarray(b.XP * f)
+        b.ZL = b.ZL * f
+        b.BOXLOW = -0.5 * b.ZL
+        dev = device_for(b, ctrl, ft)
+        dev.build_neighbors()
+        dev.calc_force()
+        dev.calc_epot()
+        e = dev.get_box_epot()[0]
+        v = dev.calc_ptensor()[0]
+        dev.close()
+        return e, v
+
+    _, v0 = energy([0.0, 0.0, 0.0])
+    h = 1e-6
+    for a in range(3):
+        ep = np.zeros(3)
+        ep[a] = h
+        e_plus, _ = energy(ep)
+        e_minus, _ = energy(-ep)
+        dE = (e_plus - e_minus) / (2 * h)
+        assert abs(v0[a, a] + dE) <= 1e-3 * abs(v0[a, a]), (a, v0[a, a], dE)
+
+
+def test_virial_multibox_equals_single_boxes():
+    ctrl = cfg.w_ctrl(mx=64)
+    boxes = [cfg.make_bcc_box(c, cfg.W_A, 0.08, 0.0, 50 + k, n_vac=k)
+             for k, c in enumerate([(6, 6, 6), (7, 5, 6)])]
+    ft = cfg.w_forcetable(boxes[0], ctrl)
+    say("virial multibox")
+    dev = device_for(boxes, ctrl, ft)
+    dev.build_neighbors()
+    vb = dev.calc_ptensor()
+    dev.close()
+    for k, b in enumerate(boxes):
+        d1 = device_for(b, ctrl, ft)
+        d1.build_neighbors()
+        v1 = d1.calc_ptensor()[0]
+        d1.close()
+        assert np.max(np.abs(vb[k] - v1)) <= 1e-12 * np.max(np.abs(v1))
