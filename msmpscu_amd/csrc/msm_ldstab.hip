@@ -46,6 +46,153 @@ __device__ __forceinline__ void chunk_range(int nchunks, int& c0, int& c1) {
 
 }  // namespace
 
+// Density terms of G list slots of atom i (slots base, base+TPA, ...; slots at or past kv add
+// nothing), added to den0 in slot order.  Knots come from the LDS window s_pb; a pair below the
+// window is flagged and its term added after the group from the global table.
+template <int G, int MT, int TPA>
+__device__ __forceinline__ double density_group(const AtomsDev& a, const ListDev& l,
+                                                const TabDev& t, const BoxDev& b, const LtWin& w,
+                                                const double* s_pb, const int* __restrict__ li,
+                                                int i, int ti, int kv, int base, double2 pxy,
+                                                double pz, double den0) {
+    const int kmax = t.ntab2 - 2;
+    int js[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int iw = base + u * TPA;
+        js[u] = iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
+    }
+    unsigned fix = 0;
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int j = js[u];
+        const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
+        const double qz = a.pos[j].z;
+        const double sx = min_image(pxy.x - qxy.x, b.len[0], b.ilen[0]);
+        const double sy = min_image(pxy.y - qxy.y, b.len[1], b.ilen[1]);
+        const double sz = min_image(pz - qz, b.len[2], b.ilen[2]);
+        const double r2 = sx * sx + sy * sy + sz * sz;
+        const int row = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] * w.wd : 0;
+        double inv_r;
+        const double sk = table_coord(r2, t.csi, inv_r);
+        const int kk = min(static_cast<int>(sk), kmax);
+        const bool in = base + u * TPA < kv && r2 <= t.ru2max;
+        const int kl = kk - w.k0d;
+        if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+        const int q = row + max(kl, 0);
+        const double p0 = s_pb[q], p1 = s_pb[q + 1];
+        const double v = p0 + (sk - kk) * (p1 - p0);
+        den0 = den0 + ((in && kl >= 0) ? v : 0.0);
+    }
+    if (fix) {
+        // rare close pairs: the same term with the knots of the global table
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (!(fix & (1u << u))) continue;
+            const int j = li[static_cast<size_t>(base + u * TPA) * l.stride];
+            const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
+            const double qz = a.pos[j].z;
+            const double sx = min_image(pxy.x - qxy.x, b.len[0], b.ilen[0]);
+            const double sy = min_image(pxy.y - qxy.y, b.len[1], b.ilen[1]);
+            const double sz = min_image(pz - qz, b.len[2], b.ilen[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
+            double inv_r;
+            const double sk = table_coord(r2, t.csi, inv_r);
+            const int kk = min(static_cast<int>(sk), kmax);
+            const double2 e = t.b2[static_cast<size_t>(kind) * t.ntab2 + kk];
+            den0 = den0 + (e.x + (sk - kk) * e.y);
+        }
+    }
+    return den0;
+}
+
+// Force terms of G list slots of atom i (as density_group), added to (fpx, fpy, fpz) in slot
+// order; {FPOTR, FPOTB} knots from the LDS window s_ft.
+template <int G, int MT, int TPA>
+__device__ __forceinline__ void force_group(const AtomsDev& a, const ListDev& l, const TabDev& t,
+                                            const BoxDev& b, const LtWin& w, const double2* s_ft,
+                                            const int* __restrict__ li, int i, int ti, int kv,
+                                            int base, const double4& pi, double denki,
+                                            double& fpx, double& fpy, double& fpz) {
+    const int kmax = t.ntab2 - 2;
+    int js[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int iw = base + u * TPA;
+        js[u] = iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
+    }
+    unsigned fix = 0;
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int j = js[u];
+        const double4 pj = a.pos[j];
+        const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
+        const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
+        const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
+        const double r2 = sx * sx + sy * sy + sz * sz;
+        int row = 0, row1 = 0;
+        if constexpr (MT) {
+            const int tj = a.ityp[j];
+            row = t.kpair[ti * t.ngroup + tj] * w.wf;
+            row1 = t.kpair[tj * t.ngroup + ti] * w.wf;
+        }
+        double inv_r;
+        const double sk = table_coord(r2, t.csi, inv_r);
+        const int kk = min(static_cast<int>(sk), kmax);
+        const double dk = sk - kk;
+        const bool in = base + u * TPA < kv && r2 <= t.ru2max;
+        const int kl = kk - w.k0f;
+        if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+        const int q = max(kl, 0);
+        const double2 t0 = s_ft[row + q], t1 = s_ft[row + q + 1];
+        const double fb = t0.y + dk * (t1.y - t0.y);
+        double fb1 = fb;
+        if constexpr (MT) {
+            const double b0 = s_ft[row1 + q].y, b1 = s_ft[row1 + q + 1].y;
+            fb1 = b0 + dk * (b1 - b0);
+        }
+        // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL)
+        const double ft = (t0.x + dk * (t1.x - t0.x)) * (inv_r * inv_r) +
+                          (fb * denki + fb1 * pj.w) * inv_r;
+        const double fortot = (in && kl >= 0) ? ft : 0.0;
+        fpx = fpx + fortot * sx;
+        fpy = fpy + fortot * sy;
+        fpz = fpz + fortot * sz;
+    }
+    if (fix) {
+        // rare close pairs: the same term with the knots of the global table
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (!(fix & (1u << u))) continue;
+            const int j = li[static_cast<size_t>(base + u * TPA) * l.stride];
+            const double4 pj = a.pos[j];
+            const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
+            const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
+            const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            int kind = t.kpair[0], kind1 = t.kpair[0];
+            if constexpr (MT) {
+                const int tj = a.ityp[j];
+                kind = t.kpair[ti * t.ngroup + tj];
+                kind1 = t.kpair[tj * t.ngroup + ti];
+            }
+            double inv_r;
+            const double sk = table_coord(r2, t.csi, inv_r);
+            const int kk = min(static_cast<int>(sk), kmax);
+            const double dk = sk - kk;
+            const double4 f = t.f4[static_cast<size_t>(kind) * t.ntab2 + kk];
+            const double4 f1 = t.f4[static_cast<size_t>(kind1) * t.ntab2 + kk];
+            const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
+                                  ((f.z + dk * f.w) * denki +
+                                   (f1.z + dk * f1.w) * pj.w) * inv_r;
+            fpx = fpx + fortot * sx;
+            fpy = fpy + fortot * sy;
+            fpz = fpz + fortot * sz;
+        }
+    }
+}
+
 // Knot rows in LDS: kind-major, w knots per kind from knot k0 (all kinds of the table for
 // several atom types, the one kind in use for a single type).
 template <int MB, int MT, int EAM, int TPA>
@@ -77,56 +224,18 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
             ti = MT ? a.ityp[i] : 0;
             const int kv = l.kvois[i];
             const int* __restrict__ li = l.indi + list_base(i, l.mx);
-            for (int base = sub; base < kv; base += TPA * kGroup) {
-                int js[kGroup];
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) {
-                    const int iw = base + u * TPA;
-                    js[u] = iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
-                }
-                unsigned fix = 0;
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) {
-                    const int j = js[u];
-                    const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
-                    const double qz = a.pos[j].z;
-                    const double sx = min_image(pxy.x - qxy.x, b.len[0], b.ilen[0]);
-                    const double sy = min_image(pxy.y - qxy.y, b.len[1], b.ilen[1]);
-                    const double sz = min_image(pz - qz, b.len[2], b.ilen[2]);
-                    const double r2 = sx * sx + sy * sy + sz * sz;
-                    const int row = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] * w.wd : 0;
-                    double inv_r;
-                    const double sk = table_coord(r2, t.csi, inv_r);
-                    const int kk = min(static_cast<int>(sk), kmax);
-                    const bool in = base + u * TPA < kv && r2 <= t.ru2max;
-                    const int kl = kk - w.k0d;
-                    if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
-                    const int q = row + max(kl, 0);
-                    const double p0 = s_pb[q], p1 = s_pb[q + 1];
-                    const double v = p0 + (sk - kk) * (p1 - p0);
-                    den0 = den0 + ((in && kl >= 0) ? v : 0.0);
-                }
-                if (fix) {
-                    // rare close pairs: the same term with the knots of the global table
-#pragma unroll
-                    for (int u = 0; u < kGroup; ++u) {
-                        if (!(fix & (1u << u))) continue;
-                        const int j = li[static_cast<size_t>(base + u * TPA) * l.stride];
-                        const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
-                        const double qz = a.pos[j].z;
-                        const double sx = min_image(pxy.x - qxy.x, b.len[0], b.ilen[0]);
-                        const double sy = min_image(pxy.y - qxy.y, b.len[1], b.ilen[1]);
-                        const double sz = min_image(pz - qz, b.len[2], b.ilen[2]);
-                        const double r2 = sx * sx + sy * sy + sz * sz;
-                        const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
-                        double inv_r;
-                        const double sk = table_coord(r2, t.csi, inv_r);
-                        const int kk = min(static_cast<int>(sk), kmax);
-                        const double2 e = t.b2[static_cast<size_t>(kind) * t.ntab2 + kk];
-                        den0 = den0 + (e.x + (sk - kk) * e.y);
-                    }
-                }
-            }
+            // full groups of kGroup slots, then groups of 4 and single slots for the tail (the
+            // slots of a lane are visited in increasing order either way)
+            int base = sub;
+            for (; base + (kGroup - 1) * TPA < kv; base += TPA * kGroup)
+                den0 = density_group<kGroup, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy,
+                                                      pz, den0);
+            for (; base + 3 * TPA < kv; base += TPA * 4)
+                den0 = density_group<4, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy, pz,
+                                                 den0);
+            for (; base < kv; base += TPA)
+                den0 = density_group<1, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy, pz,
+                                                 den0);
         }
         den0 = lane_sum<TPA>(den0);
         if (sub != 0 || !have) continue;
@@ -180,83 +289,16 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
             const double denki = pi.w;
             const int kv = l.kvois[i];
             const int* __restrict__ li = l.indi + list_base(i, l.mx);
-            for (int base = sub; base < kv; base += TPA * kGroup) {
-                int js[kGroup];
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) {
-                    const int iw = base + u * TPA;
-                    js[u] = iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
-                }
-                unsigned fix = 0;
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) {
-                    const int j = js[u];
-                    const double4 pj = a.pos[j];
-                    const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
-                    const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
-                    const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
-                    const double r2 = sx * sx + sy * sy + sz * sz;
-                    int row = 0, row1 = 0;
-                    if constexpr (MT) {
-                        const int tj = a.ityp[j];
-                        row = t.kpair[ti * t.ngroup + tj] * w.wf;
-                        row1 = t.kpair[tj * t.ngroup + ti] * w.wf;
-                    }
-                    double inv_r;
-                    const double sk = table_coord(r2, t.csi, inv_r);
-                    const int kk = min(static_cast<int>(sk), kmax);
-                    const double dk = sk - kk;
-                    const bool in = base + u * TPA < kv && r2 <= t.ru2max;
-                    const int kl = kk - w.k0f;
-                    if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
-                    const int q = max(kl, 0);
-                    const double2 t0 = s_ft[row + q], t1 = s_ft[row + q + 1];
-                    const double fb = t0.y + dk * (t1.y - t0.y);
-                    double fb1 = fb;
-                    if constexpr (MT) {
-                        const double b0 = s_ft[row1 + q].y, b1 = s_ft[row1 + q + 1].y;
-                        fb1 = b0 + dk * (b1 - b0);
-                    }
-                    // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL)
-                    const double ft = (t0.x + dk * (t1.x - t0.x)) * (inv_r * inv_r) +
-                                      (fb * denki + fb1 * pj.w) * inv_r;
-                    const double fortot = (in && kl >= 0) ? ft : 0.0;
-                    fpx = fpx + fortot * sx;
-                    fpy = fpy + fortot * sy;
-                    fpz = fpz + fortot * sz;
-                }
-                if (fix) {
-                    // rare close pairs: the same term with the knots of the global table
-#pragma unroll
-                    for (int u = 0; u < kGroup; ++u) {
-                        if (!(fix & (1u << u))) continue;
-                        const int j = li[static_cast<size_t>(base + u * TPA) * l.stride];
-                        const double4 pj = a.pos[j];
-                        const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
-                        const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
-                        const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
-                        const double r2 = sx * sx + sy * sy + sz * sz;
-                        int kind = t.kpair[0], kind1 = t.kpair[0];
-                        if constexpr (MT) {
-                            const int tj = a.ityp[j];
-                            kind = t.kpair[ti * t.ngroup + tj];
-                            kind1 = t.kpair[tj * t.ngroup + ti];
-                        }
-                        double inv_r;
-                        const double sk = table_coord(r2, t.csi, inv_r);
-                        const int kk = min(static_cast<int>(sk), kmax);
-                        const double dk = sk - kk;
-                        const double4 f = t.f4[static_cast<size_t>(kind) * t.ntab2 + kk];
-                        const double4 f1 = t.f4[static_cast<size_t>(kind1) * t.ntab2 + kk];
-                        const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
-                                              ((f.z + dk * f.w) * denki +
-                                               (f1.z + dk * f1.w) * pj.w) * inv_r;
-                        fpx = fpx + fortot * sx;
-                        fpy = fpy + fortot * sy;
-                        fpz = fpz + fortot * sz;
-                    }
-                }
-            }
+            int base = sub;
+            for (; base + (kGroup - 1) * TPA < kv; base += TPA * kGroup)
+                force_group<kGroup, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki,
+                                             fpx, fpy, fpz);
+            for (; base + 3 * TPA < kv; base += TPA * 4)
+                force_group<4, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki, fpx,
+                                        fpy, fpz);
+            for (; base < kv; base += TPA)
+                force_group<1, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki, fpx,
+                                        fpy, fpz);
         }
         fpx = lane_sum<TPA>(fpx);
         fpy = lane_sum<TPA>(fpy);
