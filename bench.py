@@ -211,6 +211,10 @@ def main():
                     help="1: table knots from an LDS window (one table kind, paths 1, 2, 4)")
     ap.add_argument("--lt-shape", default=None,
                     help="LDS-table passes: density block,wgs,force block,wgs (e.g. 1024,1,1024,1)")
+    ap.add_argument("--nlist-kernel", type=int, default=None,
+                    help="list build: 0 automatic, 1 global scan, 2 small bricks, 3 wide bricks")
+    ap.add_argument("--brick-lanes", type=int, default=None,
+                    help="lanes per home atom of the LDS-brick passes with the table in LDS")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch measurement")
     ap.add_argument("--batch-boxes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -277,6 +281,12 @@ def main():
     if args.cell_brick is not None:
         _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_CELL_BRICK, args.cell_brick),
                    "msm_set_option")
+    if args.nlist_kernel is not None:
+        _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_NLIST_KERNEL, args.nlist_kernel),
+                   "msm_set_option")
+    if args.brick_lanes is not None:
+        _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_BRICK_LANES, args.brick_lanes),
+                   "msm_set_option")
     if args.lt_shape:
         vals = [int(v) for v in args.lt_shape.split(",")]
         for opt, v in zip((_lib.MSM_OPT_LT_DENSITY_BLOCK, _lib.MSM_OPT_LT_DENSITY_WGS,
@@ -286,7 +296,8 @@ def main():
     dev.calc_force()
     tot, kmax = dev.list_stats()
     kmean = tot / box.NPRT
-    log(f"n={box.NPRT} list mean={kmean:.3f} max={kmax}")
+    pass_kernels = dev.pass_kernel()
+    log(f"n={box.NPRT} list mean={kmean:.3f} max={kmax} pass kernels {pass_kernels}")
 
     log(f"warmup {args.warmup} steps")
     dev.run(args.warmup, ctrl.NB_UPTAB, 0, ctrl.H)
@@ -412,7 +423,8 @@ def main():
                                f"({box.NPRT} atoms), {args.pot} 5000-pt tables, dt 1 fs, "
                                "rebuild every 10 steps, skin 0.3 A",
                    "atoms": box.NPRT, "list_mean": round(kmean, 3), "list_max": kmax,
-                   "dis_tracking": track, "parallelism": "single GPU"},
+                   "dis_tracking": track, "pass_kernels": pass_kernels,
+                   "parallelism": "single GPU"},
         "pcie_inclusive_value": round(pcie_value, 1),
         "roofline": roof,
         "step_model": step_model,

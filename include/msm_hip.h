@@ -205,9 +205,11 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
                                     per atom,
                                     the density pass records the pairs inside the force cutoff and
                                     the force pass visits only those */
-#define MSM_OPT_TABLE_LDS     6  /* 1 (default): with one table kind on the global-list paths
-                                    1, 2, 4 (no compaction), the passes read the table's knots
-                                    from a window held in LDS (persistent grid) */
+#define MSM_OPT_TABLE_LDS     6  /* 1 (default): the passes read the table's knots from a
+                                    window held in LDS (persistent grid) -- on the global-list
+                                    paths 1, 2, 4 (no compaction) and, when the list was built
+                                    by the LDS-brick kernels, on path 0 (then positions and
+                                    knots both come from LDS) */
 /* Shape of the LDS-table passes: threads per workgroup (256, 512, 1024; default 1024) and
  * workgroups per CU (1..16, default 1; capped by what fits); the LDS window is the CU's LDS
  * divided by the workgroups per CU. */
@@ -219,7 +221,19 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
  * contiguous (default 1 = row-major).  Takes effect at the next msm_build_neighbors, which is
  * then required. */
 #define MSM_OPT_CELL_BRICK       11
+/* Lanes per home atom (1 default, 2 or 4) of the LDS-brick passes with the table in LDS. */
+#define MSM_OPT_BRICK_LANES      12
 int msm_set_option(msm_sim* sim, int option, int value);
+/* Read-only: the density/force kernels the current options and list select (MSM_PASS_*). */
+#define MSM_OPT_PASS_KERNEL      13
+#define MSM_PASS_BRICKS      0  /* LDS-staged positions, global table */
+#define MSM_PASS_GLOBAL      1  /* one lane per atom, global list and table */
+#define MSM_PASS_LANES       2  /* several lanes per atom, global list and table */
+#define MSM_PASS_LANE_PAIR   3  /* two lanes per atom, lane-pair gathers */
+#define MSM_PASS_LDS_TABLE   4  /* global list, table window in LDS */
+#define MSM_PASS_BRICK_TABLE 5  /* LDS-staged positions and table window in LDS */
+/* Current value of an option (every MSM_OPT_* above). */
+int msm_get_option(msm_sim* sim, int option, int* value);
 
 /* Adjoint (reverse mode) of the force and energy evaluation with respect to the table knots,
  * for one atom type (one table kind).  Given dL/dF (dldf, 3*N doubles in the caller's atom order
@@ -240,7 +254,8 @@ int msm_adjoint(msm_sim* sim, const double* dldf, const double* dlde, double* g_
  * density/force kernels (lanes per atom, LDS table and its workgroup shape, lane-pair gather,
  * LDS bricks), sets the options to the fastest, and recomputes F' and F with it.  choice[8]
  * (may be NULL) receives {force path, table LDS, density block, density workgroups per CU, force
- * block, force workgroups per CU, list kernel (MSM_OPT_NLIST_KERNEL value), 0}; best_ms[2] (may
+ * block, force workgroups per CU, list kernel (MSM_OPT_NLIST_KERNEL value), brick lanes
+ * (MSM_OPT_BRICK_LANES value)}; best_ms[2] (may
  * be NULL) the chosen density+force and rebuild times in ms.  Positions and velocities are
  * unchanged. */
 int msm_autotune(msm_sim* sim, int reps, int* choice, double* best_ms);

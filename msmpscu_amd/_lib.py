@@ -84,6 +84,7 @@ SIGNATURES = {
     "msm_sync": (C.c_int, [_vp]),
     "msm_list_stats": (C.c_int, [_vp, C.POINTER(C.c_longlong), _ip]),
     "msm_set_option": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "msm_get_option": (C.c_int, [_vp, C.c_int, _ip]),
     "msm_kernel_times": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
     "msm_autotune": (C.c_int, [_vp, C.c_int, _ip, _dp]),
     "msm_calc_ptensor": (C.c_int, [_vp, _dp]),
@@ -101,6 +102,9 @@ MSM_OPT_LT_DENSITY_WGS = 8
 MSM_OPT_LT_FORCE_BLOCK = 9
 MSM_OPT_LT_FORCE_WGS = 10
 MSM_OPT_CELL_BRICK = 11
+MSM_OPT_BRICK_LANES = 12
+MSM_OPT_PASS_KERNEL = 13
+PASS_KERNELS = ["bricks", "global", "lanes", "lane_pair", "lds_table", "brick_table"]
 KCLASS = ["density", "force", "rebuild", "predictor", "nlist"]
 
 _lib = None

@@ -102,7 +102,8 @@ struct msm_sim {
                            // scan, 2 small LDS bricks, 3 wide LDS bricks
     int force_path = 2;    // 0: LDS bricks; 1, 2, 4, 8: global list with that many lanes per atom;
                            // 3: two lanes per atom with lane-pair gathers
-    int table_lds = 1;     // single-kind table knots in LDS (global-list paths 1, 2, 4)
+    int table_lds = 1;     // table knots in LDS (global-list paths 1, 2, 4; brick path 0)
+    int bt_lanes = 1;      // lanes per home atom of the brick passes with the table in LDS
     int cell_brick = 1;    // cells are numbered in bricks of cell_brick^3 (1: row-major)
     msm::LtShape lt{1024, 1, 1024, 1};  // block / workgroups per CU of the LDS-table passes
     int compact = 0;       // global-list path: force pass reads the density pass's in-cutoff list
@@ -214,6 +215,7 @@ ListDev list_view(msm_sim* s) {
     l.ckv = cmp ? s->ckv : nullptr;
     l.stride = msm::kListRow;
     l.mx = s->mx;
+    l.scap = s->nlist_kind == 1 ? 0 : s->nl_wide ? s->nl_cap : msm::kNlCapPass;
     return l;
 }
 
@@ -481,10 +483,25 @@ bool use_lds_table(const msm_sim* s) {
     return 2 * w.wf >= s->tab.ntab2 && 2 * w.wd >= s->tab.ntab2;
 }
 
+// Brick passes with the table in LDS: path 0 over a list built by the brick kernels (its slots
+// index the staged halos), with windows holding at least the top third of every kind in use
+// beside a non-empty staged halo.
+bool use_brick_table(const msm_sim* s) {
+    if (!s->table_lds || s->force_path != 0 || s->nlist_kind == 1) return false;
+    const msm::BtPlan p =
+        msm::bt_plan(s->tab, s->lt, s->nl_wide ? s->nl_cap : msm::kNlCapPass);
+    return 3 * p.w.wf >= s->tab.ntab2 && 3 * p.w.wd >= s->tab.ntab2 && p.cap_d > 0 &&
+           p.cap_f > 0;
+}
+
 void enqueue_density(msm_sim* s) {
     TimedScope ts(s, MSM_KCLASS_DENSITY);
     AtomsDev a = atoms_view(s, s->scur, s->pcur);
-    if (use_lds_table(s))
+    if (use_brick_table(s))
+        msm::launch_density_bt(s->bt_lanes, s->lt, s->nbricks, s->d_brick_box, a, list_view(s),
+                               s->tab, s->d_boxes, s->sb.start, s->ncell_total, s->d_err,
+                               s->stream);
+    else if (use_lds_table(s))
         msm::launch_density_lt(s->force_path, s->lt, a, list_view(s), s->tab, s->boxes[0],
                                s->d_boxes, multibox(s), s->d_err, s->stream);
     else if (s->force_path == 3)
@@ -512,7 +529,10 @@ void enqueue_force(msm_sim* s, int mode, int write_f, double h) {
     sp.mode = mode;
     sp.write_f = write_f;
     sp.track_dis = s->track_dis;
-    if (use_lds_table(s))
+    if (use_brick_table(s))
+        msm::launch_force_bt(s->bt_lanes, s->lt, s->nbricks, s->d_brick_box, a, list_view(s),
+                             s->tab, s->d_boxes, s->sb.start, s->ncell_total, sp, s->stream);
+    else if (use_lds_table(s))
         msm::launch_force_lt(s->force_path, s->lt, a, list_view(s), s->tab, s->boxes[0],
                              s->d_boxes, multibox(s), sp, s->stream);
     else if (s->force_path == 3)
@@ -1232,6 +1252,12 @@ int msm_set_option(msm_sim* s, int option, int value) {
             s->has_list = false;  // the sorted order follows the numbering: rebuild first
             clear_graphs(s);
             return MSM_OK;
+        case MSM_OPT_BRICK_LANES:
+            if (value != 1 && value != 2 && value != 4)
+                return set_err(s, MSM_ERR_ARG, "brick lanes must be 1, 2 or 4");
+            s->bt_lanes = value;
+            clear_graphs(s);
+            return MSM_OK;
         case MSM_OPT_LT_DENSITY_BLOCK:
         case MSM_OPT_LT_FORCE_BLOCK:
             if (value != 256 && value != 512 && value != 1024)
@@ -1245,6 +1271,40 @@ int msm_set_option(msm_sim* s, int option, int value) {
                 return set_err(s, MSM_ERR_ARG, "LDS-table workgroups per CU must be 1..16");
             (option == MSM_OPT_LT_DENSITY_WGS ? s->lt.wgs_d : s->lt.wgs_f) = value;
             clear_graphs(s);
+            return MSM_OK;
+        default:
+            return set_err(s, MSM_ERR_ARG, "unknown option");
+    }
+}
+
+// The density/force kernels enqueue_density / enqueue_force launch.
+static int pass_kernel(const msm_sim* s) {
+    if (use_brick_table(s)) return MSM_PASS_BRICK_TABLE;
+    if (use_lds_table(s)) return MSM_PASS_LDS_TABLE;
+    if (s->force_path == 3) return MSM_PASS_LANE_PAIR;
+    if (s->force_path > 0) return MSM_PASS_LANES;
+    if (s->nlist_kind == 1) return MSM_PASS_GLOBAL;
+    return MSM_PASS_BRICKS;
+}
+
+int msm_get_option(msm_sim* s, int option, int* value) {
+    if (!s || !value) return MSM_ERR_ARG;
+    switch (option) {
+        case MSM_OPT_TRACK_DIS: *value = s->track_dis; return MSM_OK;
+        case MSM_OPT_KERNEL_TIMING: *value = s->timing; return MSM_OK;
+        case MSM_OPT_NLIST_KERNEL: *value = s->nlist_kind; return MSM_OK;
+        case MSM_OPT_FORCE_PATH: *value = s->force_path; return MSM_OK;
+        case MSM_OPT_COMPACT_LIST: *value = s->compact; return MSM_OK;
+        case MSM_OPT_TABLE_LDS: *value = s->table_lds; return MSM_OK;
+        case MSM_OPT_LT_DENSITY_BLOCK: *value = s->lt.block_d; return MSM_OK;
+        case MSM_OPT_LT_DENSITY_WGS: *value = s->lt.wgs_d; return MSM_OK;
+        case MSM_OPT_LT_FORCE_BLOCK: *value = s->lt.block_f; return MSM_OK;
+        case MSM_OPT_LT_FORCE_WGS: *value = s->lt.wgs_f; return MSM_OK;
+        case MSM_OPT_CELL_BRICK: *value = s->cell_brick; return MSM_OK;
+        case MSM_OPT_BRICK_LANES: *value = s->bt_lanes; return MSM_OK;
+        case MSM_OPT_PASS_KERNEL:
+            if (!s->has_table) return set_err(s, MSM_ERR_STATE, "no force table");
+            *value = pass_kernel(s);
             return MSM_OK;
         default:
             return set_err(s, MSM_ERR_ARG, "unknown option");
@@ -1273,18 +1333,19 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     if (reps < 1 || reps > 1000) return set_err(s, MSM_ERR_ARG, "reps must be 1..1000");
     (void)hipSetDevice(s->device);
     struct Cand {
-        int fp, lds, bd, wd, bf, wf;
+        int fp, lds, bd, wd, bf, wf, lanes;
     };
     std::vector<Cand> cands;
     if (msm::lt_window(s->tab, 1, 1).wf * 2 >= s->tab.ntab2) {
         for (int fp : {2, 1, 4}) {
-            cands.push_back({fp, 1, 1024, 1, 1024, 1});
-            cands.push_back({fp, 1, 256, 4, 256, 4});
+            cands.push_back({fp, 1, 1024, 1, 1024, 1, 1});
+            cands.push_back({fp, 1, 256, 4, 256, 4, 1});
         }
-        cands.push_back({2, 1, 512, 2, 512, 2});
+        cands.push_back({2, 1, 512, 2, 512, 2, 1});
     }
     for (int fp : {2, 1, 3, 4, 8, 0}) cands.push_back({fp, 0, s->lt.block_d, s->lt.wgs_d,
-                                                       s->lt.block_f, s->lt.wgs_f});
+                                                       s->lt.block_f, s->lt.wgs_f, 1});
+    const bool brick_list = s->nlist_kind != 1;  // the list-kernel choice below keeps this
     const int saved_timing = s->timing;
     s->timing = 0;
     s->compact = 0;
@@ -1319,6 +1380,16 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     s->nlist_kind = nl_choice;
     if (rc == MSM_OK) rc = replan_bricks(s);
     if (rc == MSM_OK) enqueue_rebuild(s);
+    // brick passes with the table in LDS (they read the slot list of the brick list kernels)
+    if (brick_list && s->nlist_kind != 1)
+        for (const Cand& c : {Cand{0, 1, 512, 2, 512, 2, 1}, Cand{0, 1, 1024, 1, 1024, 1, 2},
+                              Cand{0, 1, 512, 2, 512, 2, 2}, Cand{0, 1, 1024, 1, 1024, 1, 1},
+                              Cand{0, 1, 256, 4, 256, 4, 1}}) {
+            s->lt = msm::LtShape{c.bd, c.wd, c.bf, c.wf};
+            s->force_path = 0;
+            s->table_lds = 1;
+            if (use_brick_table(s)) cands.push_back(c);
+        }
     // density + force passes
     float best = 0.f;
     size_t bi = 0;
@@ -1326,6 +1397,7 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
         s->force_path = cands[c].fp;
         s->table_lds = cands[c].lds;
         s->lt = msm::LtShape{cands[c].bd, cands[c].wd, cands[c].bf, cands[c].wf};
+        s->bt_lanes = cands[c].lanes;
         float ms = 0.f;
         rc = time_it([&] {
             enqueue_density(s);
@@ -1344,12 +1416,13 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     s->force_path = w.fp;
     s->table_lds = w.lds;
     s->lt = msm::LtShape{w.bd, w.wd, w.bf, w.wf};
+    s->bt_lanes = w.lanes;
     clear_graphs(s);
     // leave F' and F as the chosen kernels compute them
     enqueue_density(s);
     enqueue_force(s, 0, 1, 0.0);
     if (choice) {
-        const int c[8] = {w.fp, w.lds, w.bd, w.wd, w.bf, w.wf, s->nlist_kind, 0};
+        const int c[8] = {w.fp, w.lds, w.bd, w.wd, w.bf, w.wf, s->nlist_kind, w.lanes};
         for (int k = 0; k < 8; ++k) choice[k] = c[k];
     }
     if (best_ms) {

@@ -78,6 +78,9 @@ struct ListDev {
     int* ckv;                   // their count per lane (n_alloc * lanes per atom)
     int stride;                 // distance between consecutive slots of one atom (kListRow)
     int mx;                     // slots per atom (MXKVOIS)
+    int scap;                   // brick passes stage a brick (and read sidx) only if its halo
+                                // holds at most this many atoms: the list build wrote sidx for
+                                // those bricks (0 for a list from the global scan)
 };
 
 // Neighbour lists are stored in blocks of kListRow atoms: slot iw of atom p lives at
@@ -128,6 +131,10 @@ void launch_density_bricks(int nbricks, const int* brick_box, const AtomsDev& a,
 void launch_force_bricks(int nbricks, const int* brick_box, const AtomsDev& a, const ListDev& l,
                          const TabDev& t, const BoxDev* boxes, const int* start, int ncell_total,
                          const StepParams& sp, hipStream_t s);
+// Atoms of the out-of-box bucket (from *first to n): F' = 0 (clear_den), F = 0 (write_f), position
+// carried to pos_out (copy_pos).
+void launch_outside_atoms(const AtomsDev& a, const int* first, int write_f, int copy_pos,
+                          int clear_den, hipStream_t s);
 // Global-list passes with tpa (1, 2, 4 or 8) lanes per atom (msm_tpa.hip).
 void launch_density_tpa(int tpa, const AtomsDev& a, const ListDev& l, const TabDev& t,
                         const BoxDev& box0, const BoxDev* boxes, int multibox, int* err,
@@ -153,6 +160,20 @@ void launch_density_lt(int tpa, const LtShape& sh, const AtomsDev& a, const List
 void launch_force_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDev& l,
                      const TabDev& t, const BoxDev& box0, const BoxDev* boxes, int multibox,
                      const StepParams& sp, hipStream_t s);
+// LDS bricks with the table window in LDS (msm_ldstab.hip): tpa (1, 2 or 4) lanes per home
+// atom, persistent grids over the bricks of the list build (whose slot list they read); the
+// staged halo capacity (at most the list build's, l.scap) and the windows share the LDS.
+struct BtPlan {
+    LtWin w;
+    int cap_d, cap_f;  // halo atoms a brick may have to be staged (density, force pass)
+};
+BtPlan bt_plan(const TabDev& t, const LtShape& sh, int scap);
+void launch_density_bt(int tpa, const LtShape& sh, int nbricks, const int* brick_box,
+                       const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev* boxes,
+                       const int* start, int ncell_total, int* err, hipStream_t s);
+void launch_force_bt(int tpa, const LtShape& sh, int nbricks, const int* brick_box,
+                     const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev* boxes,
+                     const int* start, int ncell_total, const StepParams& sp, hipStream_t s);
 // Reverse-mode table gradients (msm_adjoint.hip): scratch of the adjoint passes.
 struct AdjDev {
     double* g;           // dL/dF in the sorted order, 3 x n (zero for inactive atoms)

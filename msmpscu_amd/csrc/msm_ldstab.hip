@@ -16,12 +16,12 @@
 // filling its LDS window once and then walking a contiguous range of atom chunks (contiguous per
 // XCD), so the waves sharing a CU's L1 work on one compact region of the cell-sorted atoms.  Lanes per atom
 // (TPA), list traversal and the finishing lane are those of msm_tpa.hip.
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
 
-#include "msm_device.h"
-#include "msm_internal.h"
+#include "msm_brick.h"
 
 namespace msm {
 
@@ -314,6 +314,363 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
     }
 }
 
+// --------------------------------------------------------------------------------------------
+// LDS bricks with the table window in LDS
+//
+// Each workgroup holds a table window (as above) and walks a contiguous range of bricks (those of
+// one XCD are contiguous, so neighbouring bricks share halo cells in that XCD's L2).  Per brick it
+// stages the halo atoms in LDS with coalesced loads and then every home atom reads its list as
+// 16-bit halo slots: positions and knots both come from LDS, and the only per-pair global load is
+// the 2-byte slot.  A brick whose halo exceeds the staging capacity `cap` takes the global-list groups
+// above (same results).  The list must have been built by the brick list kernels on the same
+// brick plan (the slots are positions in that brick's halo).
+// --------------------------------------------------------------------------------------------
+
+// Density terms of G list slots of the home atom at halo slot si (as density_group, with the
+// neighbour positions read from the staged halo x/y/z arrays).
+template <int G, int MT, int TPA>
+__device__ __forceinline__ double density_group_s(const TabDev& t, const BoxDev& b,
+                                                  const LtWin& w, const double* s_pb,
+                                                  const double* lx, const double* ly,
+                                                  const double* lz, const int* lty,
+                                                  const unsigned short* __restrict__ li,
+                                                  int stride, int si, int ti, int kv, int base,
+                                                  double xi, double yi, double zi, double den0) {
+    const int kmax = t.ntab2 - 2;
+    int qs[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int iw = base + u * TPA;
+        qs[u] = iw < kv ? li[static_cast<size_t>(iw) * stride] : si;
+    }
+    unsigned fix = 0;
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int q = qs[u];
+        const double sx = min_image(xi - lx[q], b.len[0], b.ilen[0]);
+        const double sy = min_image(yi - ly[q], b.len[1], b.ilen[1]);
+        const double sz = min_image(zi - lz[q], b.len[2], b.ilen[2]);
+        const double r2 = sx * sx + sy * sy + sz * sz;
+        const int row = MT ? t.kpair[ti * t.ngroup + lty[q]] * w.wd : 0;
+        double inv_r;
+        const double sk = table_coord(r2, t.csi, inv_r);
+        const int kk = min(static_cast<int>(sk), kmax);
+        const bool in = base + u * TPA < kv && r2 <= t.ru2max;
+        const int kl = kk - w.k0d;
+        if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+        const int k2 = row + max(kl, 0);
+        const double p0 = s_pb[k2], p1 = s_pb[k2 + 1];
+        const double v = p0 + (sk - kk) * (p1 - p0);
+        den0 = den0 + ((in && kl >= 0) ? v : 0.0);
+    }
+    if (fix) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (!(fix & (1u << u))) continue;
+            const int q = qs[u];
+            const double sx = min_image(xi - lx[q], b.len[0], b.ilen[0]);
+            const double sy = min_image(yi - ly[q], b.len[1], b.ilen[1]);
+            const double sz = min_image(zi - lz[q], b.len[2], b.ilen[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            const int kind = MT ? t.kpair[ti * t.ngroup + lty[q]] : t.kpair[0];
+            double inv_r;
+            const double sk = table_coord(r2, t.csi, inv_r);
+            const int kk = min(static_cast<int>(sk), kmax);
+            const double2 e = t.b2[static_cast<size_t>(kind) * t.ntab2 + kk];
+            den0 = den0 + (e.x + (sk - kk) * e.y);
+        }
+    }
+    return den0;
+}
+
+// Force terms of G list slots of the home atom at halo slot si (as force_group, with the
+// neighbour positions and F' read from the staged halo).
+template <int G, int MT, int TPA>
+__device__ __forceinline__ void force_group_s(const TabDev& t, const BoxDev& b, const LtWin& w,
+                                              const double2* s_ft, const double4* lp,
+                                              const int* lty,
+                                              const unsigned short* __restrict__ li, int stride,
+                                              int si, int ti, int kv, int base, const double4& pi,
+                                              double& fpx, double& fpy, double& fpz) {
+    const int kmax = t.ntab2 - 2;
+    int qs[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int iw = base + u * TPA;
+        qs[u] = iw < kv ? li[static_cast<size_t>(iw) * stride] : si;
+    }
+    unsigned fix = 0;
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int q = qs[u];
+        const double4 pj = lp[q];
+        const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
+        const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
+        const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
+        const double r2 = sx * sx + sy * sy + sz * sz;
+        int row = 0, row1 = 0;
+        if constexpr (MT) {
+            const int tj = lty[q];
+            row = t.kpair[ti * t.ngroup + tj] * w.wf;
+            row1 = t.kpair[tj * t.ngroup + ti] * w.wf;
+        }
+        double inv_r;
+        const double sk = table_coord(r2, t.csi, inv_r);
+        const int kk = min(static_cast<int>(sk), kmax);
+        const double dk = sk - kk;
+        const bool in = base + u * TPA < kv && r2 <= t.ru2max;
+        const int kl = kk - w.k0f;
+        if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+        const int k2 = max(kl, 0);
+        const double2 t0 = s_ft[row + k2], t1 = s_ft[row + k2 + 1];
+        const double fb = t0.y + dk * (t1.y - t0.y);
+        double fb1 = fb;
+        if constexpr (MT) {
+            const double b0 = s_ft[row1 + k2].y, b1 = s_ft[row1 + k2 + 1].y;
+            fb1 = b0 + dk * (b1 - b0);
+        }
+        // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL)
+        const double ft = (t0.x + dk * (t1.x - t0.x)) * (inv_r * inv_r) +
+                          (fb * pi.w + fb1 * pj.w) * inv_r;
+        const double fortot = (in && kl >= 0) ? ft : 0.0;
+        fpx = fpx + fortot * sx;
+        fpy = fpy + fortot * sy;
+        fpz = fpz + fortot * sz;
+    }
+    if (fix) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (!(fix & (1u << u))) continue;
+            const double4 pj = lp[qs[u]];
+            const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
+            const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
+            const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            int kind = t.kpair[0], kind1 = t.kpair[0];
+            if constexpr (MT) {
+                const int tj = lty[qs[u]];
+                kind = t.kpair[ti * t.ngroup + tj];
+                kind1 = t.kpair[tj * t.ngroup + ti];
+            }
+            double inv_r;
+            const double sk = table_coord(r2, t.csi, inv_r);
+            const int kk = min(static_cast<int>(sk), kmax);
+            const double dk = sk - kk;
+            const double4 f = t.f4[static_cast<size_t>(kind) * t.ntab2 + kk];
+            const double4 f1 = t.f4[static_cast<size_t>(kind1) * t.ntab2 + kk];
+            const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
+                                  ((f.z + dk * f.w) * pi.w + (f1.z + dk * f1.w) * pj.w) * inv_r;
+            fpx = fpx + fortot * sx;
+            fpy = fpy + fortot * sy;
+            fpz = fpz + fortot * sz;
+        }
+    }
+}
+
+// Slots per group of the staged loops (LDS latency is short; larger groups spill).
+constexpr int kGroupS = 4;
+
+// Dynamic LDS of the brick passes: the knot window, then the staged halo of at most cap atoms
+// (x, y, z arrays for the density pass, double4 records for the force pass; types if MT).
+constexpr long bt_atom_bytes(bool force, bool mt) {
+    return static_cast<long>((force ? sizeof(double4) : 3 * sizeof(double)) +
+                             (mt ? sizeof(int) : 0));
+}
+
+template <int MT, int EAM, int TPA>
+__global__ __launch_bounds__(kLtMaxBlock) void k_density_bt(const int* __restrict__ brick_box,
+                                                            int nbricks, AtomsDev a, ListDev l,
+                                                            TabDev t,
+                                                            const BoxDev* __restrict__ boxes,
+                                                            const int* __restrict__ start,
+                                                            LtWin w, int cap, int* err) {
+    extern __shared__ double s_dyn[];
+    __shared__ BrickLds L;
+    const int nk = MT ? t.nkind : 1;
+    double* s_pb = s_dyn;
+    const int nwin = (nk * w.wd + 1) & ~1;  // keep the halo arrays 16-byte aligned
+    double* lx = s_dyn + nwin;
+    double* ly = lx + cap;
+    double* lz = ly + cap;
+    int* lty = reinterpret_cast<int*>(lz + cap);
+    for (int k = threadIdx.x; k < nk * w.wd; k += blockDim.x) {
+        const int kd = MT ? k / w.wd : t.kpair[0], q = MT ? k % w.wd : k;
+        s_pb[k] = t.b2[static_cast<size_t>(kd) * t.ntab2 + w.k0d + q].x;
+    }
+    int b0, b1;
+    chunk_range(nbricks, b0, b1);
+    for (int bi = b0; bi < b1; ++bi) {
+        __syncthreads();  // the previous brick's halo is no longer read
+        const BoxDev& B = boxes[brick_box[bi]];
+        BrickInfo g;
+        brick_info(B, bi, g);
+        const int total = brick_halo(B, g, start, L.hcell, nullptr, L.hoff, L.homeoff, L.scan);
+        const bool staged = total <= cap;
+        if (staged) {
+            for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int) {
+                // x, y, z only: w (F') of other atoms is being written by this pass
+                const double2 xy = *reinterpret_cast<const double2*>(&a.pos[q]);
+                lx[s] = xy.x;
+                ly[s] = xy.y;
+                lz[s] = a.pos[q].z;
+                if constexpr (MT) lty[s] = a.ityp[q];
+            });
+        }
+        __syncthreads();
+        const int nlanes = L.homeoff[g.nhc] * TPA;
+        for (int t0 = 0; t0 < nlanes; t0 += blockDim.x) {
+            const int tt = t0 + threadIdx.x;
+            const int ha = tt / TPA, sub = tt % TPA;
+            const bool have = tt < nlanes;
+            int p = 0, ti = 0;
+            bool active = false;
+            double den0 = 0.0;
+            if (have) {
+                int h, off;
+                home_atom(ha, g, L.homeoff, h, off);
+                p = start[L.hcell[h]] + off;
+                active = (a.statu[p] & STATU_ACTIVE) == STATU_ACTIVE;
+                if (active) {
+                    ti = MT ? a.ityp[p] : 0;
+                    const int kv = l.kvois[p];
+                    int base = sub;
+                    if (staged) {
+                        const int si = L.hoff[h] + off;
+                        const double xi = lx[si], yi = ly[si], zi = lz[si];
+                        const unsigned short* __restrict__ li = l.sidx + list_base(p, l.mx);
+                        for (; base + (kGroupS - 1) * TPA < kv; base += TPA * kGroupS)
+                            den0 = density_group_s<kGroupS, MT, TPA>(t, B, w, s_pb, lx, ly, lz,
+                                                                     lty, li, l.stride, si, ti, kv,
+                                                                     base, xi, yi, zi, den0);
+                        for (; base < kv; base += TPA)
+                            den0 = density_group_s<1, MT, TPA>(t, B, w, s_pb, lx, ly, lz, lty, li,
+                                                               l.stride, si, ti, kv, base, xi, yi,
+                                                               zi, den0);
+                    } else {
+                        const double2 pxy = *reinterpret_cast<const double2*>(&a.pos[p]);
+                        const double pz = a.pos[p].z;
+                        const int* __restrict__ li = l.indi + list_base(p, l.mx);
+                        for (; base + 3 * TPA < kv; base += TPA * 4)
+                            den0 = density_group<4, MT, TPA>(a, l, t, B, w, s_pb, li, p, ti, kv,
+                                                             base, pxy, pz, den0);
+                        for (; base < kv; base += TPA)
+                            den0 = density_group<1, MT, TPA>(a, l, t, B, w, s_pb, li, p, ti, kv,
+                                                             base, pxy, pz, den0);
+                    }
+                }
+            }
+            den0 = lane_sum<TPA>(den0);
+            if (sub != 0 || !have) continue;
+            if (active) {
+                if constexpr (!EAM) {
+                    if (den0 > 0.0) den0 = -0.5 / sqrt(den0);
+                } else {
+                    if (den0 > 0.0) {
+                        const double sk = den0 / t.rhod + 1.0;
+                        int kk = static_cast<int>(sk + 0.000001);
+                        if (kk > t.nembd) {
+                            atomicOr(&err[kErrEmbd], 1);
+                            kk = t.nembd;
+                        }
+                        const double4 m = t.m4[t.kembd[ti] * t.nembd2 + kk];
+                        den0 = m.z + (sk - kk) * m.w;
+                    }
+                }
+            }
+            a.pos[p].w = den0;
+        }
+    }
+}
+
+template <int MT, int MODE, int TPA>
+__global__ __launch_bounds__(kLtMaxBlock) void k_force_bt(const int* __restrict__ brick_box,
+                                                          int nbricks, AtomsDev a, ListDev l,
+                                                          TabDev t,
+                                                          const BoxDev* __restrict__ boxes,
+                                                          const int* __restrict__ start,
+                                                          StepParams sp, LtWin w, int cap) {
+    extern __shared__ double2 s_dyn2[];
+    __shared__ BrickLds L;
+    const int nk = MT ? t.nkind : 1;
+    double2* s_ft = s_dyn2;
+    double4* lp = reinterpret_cast<double4*>(s_dyn2 + ((nk * w.wf + 1) & ~1));
+    int* lty = reinterpret_cast<int*>(lp + cap);
+    for (int k = threadIdx.x; k < nk * w.wf; k += blockDim.x) {
+        const int kd = MT ? k / w.wf : t.kpair[0], q = MT ? k % w.wf : k;
+        const double4 f = t.f4[static_cast<size_t>(kd) * t.ntab2 + w.k0f + q];
+        s_ft[k] = make_double2(f.x, f.z);
+    }
+    int b0, b1;
+    chunk_range(nbricks, b0, b1);
+    for (int bi = b0; bi < b1; ++bi) {
+        __syncthreads();  // the previous brick's halo is no longer read
+        const BoxDev& B = boxes[brick_box[bi]];
+        BrickInfo g;
+        brick_info(B, bi, g);
+        const int total = brick_halo(B, g, start, L.hcell, nullptr, L.hoff, L.homeoff, L.scan);
+        const bool staged = total <= cap;
+        if (staged) {
+            for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int) {
+                lp[s] = a.pos[q];  // x, y, z and F'
+                if constexpr (MT) lty[s] = a.ityp[q];
+            });
+        }
+        __syncthreads();
+        const int nlanes = L.homeoff[g.nhc] * TPA;
+        for (int t0 = 0; t0 < nlanes; t0 += blockDim.x) {
+            const int tt = t0 + threadIdx.x;
+            const int ha = tt / TPA, sub = tt % TPA;
+            const bool have = tt < nlanes;
+            int p = 0, ti = 0, st = 0;
+            bool active = false;
+            double4 pi = make_double4(0, 0, 0, 0);
+            double fpx = 0.0, fpy = 0.0, fpz = 0.0;
+            if (have) {
+                int h, off;
+                home_atom(ha, g, L.homeoff, h, off);
+                p = start[L.hcell[h]] + off;
+                const int si = L.hoff[h] + off;
+                st = a.statu[p];
+                active = (st & STATU_ACTIVE) == STATU_ACTIVE;
+                pi = staged ? lp[si] : a.pos[p];
+                ti = MT ? a.ityp[p] : 0;
+                if (active) {
+                    const int kv = l.kvois[p];
+                    int base = sub;
+                    if (staged) {
+                        const unsigned short* __restrict__ li = l.sidx + list_base(p, l.mx);
+                        for (; base + (kGroupS - 1) * TPA < kv; base += TPA * kGroupS)
+                            force_group_s<kGroupS, MT, TPA>(t, B, w, s_ft, lp, lty, li, l.stride,
+                                                            si, ti, kv, base, pi, fpx, fpy, fpz);
+                        for (; base < kv; base += TPA)
+                            force_group_s<1, MT, TPA>(t, B, w, s_ft, lp, lty, li, l.stride, si, ti,
+                                                      kv, base, pi, fpx, fpy, fpz);
+                    } else {
+                        const int* __restrict__ li = l.indi + list_base(p, l.mx);
+                        for (; base + 3 * TPA < kv; base += TPA * 4)
+                            force_group<4, MT, TPA>(a, l, t, B, w, s_ft, li, p, ti, kv, base, pi,
+                                                    pi.w, fpx, fpy, fpz);
+                        for (; base < kv; base += TPA)
+                            force_group<1, MT, TPA>(a, l, t, B, w, s_ft, li, p, ti, kv, base, pi,
+                                                    pi.w, fpx, fpy, fpz);
+                    }
+                }
+            }
+            fpx = lane_sum<TPA>(fpx);
+            fpy = lane_sum<TPA>(fpy);
+            fpz = lane_sum<TPA>(fpz);
+            if (sub != 0 || !have) continue;
+            if (sp.write_f) {
+                a.fx[p] = fpx;
+                a.fy[p] = fpy;
+                a.fz[p] = fpz;
+            }
+            if constexpr (MODE >= 1)
+                integrate_atom<MODE>(a, t, B, sp, p, st, active, pi, ti, fpx, fpy, fpz);
+        }
+    }
+}
+
 namespace {
 
 int device_attr(hipDeviceAttribute_t attr, int fallback) {
@@ -451,6 +808,117 @@ void launch_force_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDe
 #undef MSM_FLM
 #undef MSM_FLP
 #undef MSM_FL
+}
+
+}  // namespace msm
+
+namespace msm {
+
+namespace {
+
+// Halo capacity and knot window of one brick pass sharing `budget` bytes: the capacity of the
+// list build (scap), less if needed to leave a window of a third of the table.
+void bt_fit(long budget, long bpa, long bpk, int ntab2, int scap, int& cap, int& wnd) {
+    const long wmin = (ntab2 + 2) / 3;
+    long c = std::min<long>(scap, (budget - (wmin + 2) * bpk) / bpa);
+    if (c < 0) c = 0;
+    const long wv = (budget - c * bpa) / bpk - 2;
+    cap = static_cast<int>(c);
+    wnd = static_cast<int>(std::max(0L, std::min<long>(ntab2, wv)));
+}
+
+}  // namespace
+
+BtPlan bt_plan(const TabDev& t, const LtShape& sh, int scap) {
+    BtPlan p{};
+    const bool mt = t.ngroup > 1;
+    const long nk = mt ? t.nkind : 1;
+    const long fixed = static_cast<long>(sizeof(BrickLds)) + 1024;
+    bt_fit(lds_budget(sh.wgs_d) - fixed, bt_atom_bytes(false, mt), 8 * nk, t.ntab2, scap,
+           p.cap_d, p.w.wd);
+    bt_fit(lds_budget(sh.wgs_f) - fixed, bt_atom_bytes(true, mt), 16 * nk, t.ntab2, scap,
+           p.cap_f, p.w.wf);
+    p.w.k0d = t.ntab2 - p.w.wd;
+    p.w.k0f = t.ntab2 - p.w.wf;
+    return p;
+}
+
+namespace {
+
+size_t bt_lds(const BtPlan& p, const TabDev& t, bool force) {
+    const bool mt = t.ngroup > 1;
+    const int nk = mt ? t.nkind : 1;
+    if (force)
+        return static_cast<size_t>((nk * p.w.wf + 1) & ~1) * sizeof(double2) +
+               static_cast<size_t>(p.cap_f) * bt_atom_bytes(true, mt);
+    return static_cast<size_t>((nk * p.w.wd + 1) & ~1) * sizeof(double) +
+           static_cast<size_t>(p.cap_d) * bt_atom_bytes(false, mt);
+}
+
+}  // namespace
+
+void launch_density_bt(int tpa, const LtShape& sh, int nbricks, const int* brick_box,
+                       const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev* boxes,
+                       const int* start, int ncell_total, int* err, hipStream_t s) {
+    if (a.n <= 0) return;
+    launch_outside_atoms(a, start + ncell_total, 0, 0, 1, s);
+    if (nbricks <= 0) return;
+    const BtPlan pl = bt_plan(t, sh, l.scap);
+    const size_t lds = bt_lds(pl, t, false);
+    const bool mt = t.ngroup > 1, eam = t.pot_type != 0;
+#define MSM_DB(MT, E, P)                                                                       \
+    do {                                                                                       \
+        auto k = k_density_bt<MT, E, P>;                                                       \
+        const unsigned g = persistent_grid(reinterpret_cast<const void*>(k), sh.block_d, lds,  \
+                                           sh.wgs_d, nbricks);                                 \
+        hipLaunchKernelGGL(k, dim3(g), dim3(sh.block_d), lds, s, brick_box, nbricks, a, l, t,  \
+                           boxes, start, pl.w, pl.cap_d, err);                                 \
+    } while (0)
+#define MSM_DBP(MT, E)                          \
+    do {                                        \
+        if (tpa == 1) MSM_DB(MT, E, 1);         \
+        else if (tpa == 2) MSM_DB(MT, E, 2);    \
+        else MSM_DB(MT, E, 4);                  \
+    } while (0)
+    if (mt) { if (eam) MSM_DBP(1, 1); else MSM_DBP(1, 0); }
+    else    { if (eam) MSM_DBP(0, 1); else MSM_DBP(0, 0); }
+#undef MSM_DBP
+#undef MSM_DB
+}
+
+void launch_force_bt(int tpa, const LtShape& sh, int nbricks, const int* brick_box,
+                     const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev* boxes,
+                     const int* start, int ncell_total, const StepParams& sp, hipStream_t s) {
+    if (a.n <= 0) return;
+    launch_outside_atoms(a, start + ncell_total, sp.write_f, sp.mode == 2 ? 1 : 0, 0, s);
+    if (nbricks <= 0) return;
+    const BtPlan pl = bt_plan(t, sh, l.scap);
+    const size_t lds = bt_lds(pl, t, true);
+    const bool mt = t.ngroup > 1;
+#define MSM_FB(MT, M, P)                                                                       \
+    do {                                                                                       \
+        auto k = k_force_bt<MT, M, P>;                                                         \
+        const unsigned g = persistent_grid(reinterpret_cast<const void*>(k), sh.block_f, lds,  \
+                                           sh.wgs_f, nbricks);                                 \
+        hipLaunchKernelGGL(k, dim3(g), dim3(sh.block_f), lds, s, brick_box, nbricks, a, l, t,  \
+                           boxes, start, sp, pl.w, pl.cap_f);                                  \
+    } while (0)
+#define MSM_FBP(MT, M)                          \
+    do {                                        \
+        if (tpa == 1) MSM_FB(MT, M, 1);         \
+        else if (tpa == 2) MSM_FB(MT, M, 2);    \
+        else MSM_FB(MT, M, 4);                  \
+    } while (0)
+#define MSM_FBM(MT)                             \
+    do {                                        \
+        if (sp.mode == 0) MSM_FBP(MT, 0);       \
+        else if (sp.mode == 1) MSM_FBP(MT, 1);  \
+        else MSM_FBP(MT, 2);                    \
+    } while (0)
+    if (mt) MSM_FBM(1); else MSM_FBM(0);
+#undef MSM_FBM
+#undef MSM_FBP
+#undef MSM_FB
 }
 
 }  // namespace msm

@@ -410,6 +410,16 @@ class DeviceMD:
     def set_option(self, option: int, value: int):
         check(lib().msm_set_option(self.h, int(option), int(value)), "msm_set_option", self.h)
 
+    def get_option(self, option: int) -> int:
+        v = np.zeros(1, dtype=np.int32)
+        check(lib().msm_get_option(self.h, int(option), iptr(v)), "msm_get_option", self.h)
+        return int(v[0])
+
+    def pass_kernel(self) -> str:
+        """Name of the density/force kernels the current options select (MSM_PASS_*)."""
+        from ._lib import MSM_OPT_PASS_KERNEL, PASS_KERNELS
+        return PASS_KERNELS[self.get_option(MSM_OPT_PASS_KERNEL)]
+
     def autotune(self, reps: int = 3) -> dict:
         """Times the kernel variants on the current atoms and list and keeps the fastest
         (msm_autotune); returns the choice and the chosen per-pass times."""
@@ -418,6 +428,7 @@ class DeviceMD:
         check(lib().msm_autotune(self.h, int(reps), iptr(ch), dptr(ms)), "msm_autotune", self.h)
         return {"force_path": int(ch[0]), "table_lds": int(ch[1]),
                 "lt_shape": [int(v) for v in ch[2:6]], "nlist_kernel": int(ch[6]),
+                "brick_lanes": int(ch[7]),
                 "density_force_ms": float(ms[0]), "rebuild_ms": float(ms[1])}
 
     def kernel_times(self, reset: bool = True) -> dict:

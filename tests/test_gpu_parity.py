@@ -142,6 +142,39 @@ def test_brick_list_equals_global_scan(orc, case):
     assert pairset(kv0, ind0, True) == pairset(okv, oind, False)
 
 
+@pytest.mark.parametrize("lanes,nlist", [(1, 2), (2, 2), (2, 3)])
+def test_brick_table_overflowing_bricks(orc, lanes, nlist):
+    """A dense blob of 600 extra atoms (Gaussian, 1.2 A wide) in a W box: the bricks around it
+    hold more halo atoms than the LDS-brick passes stage, so they take the global-list groups
+    (with the LDS knot window and its below-window fallback) while the other bricks run from
+    LDS; F' and forces agree with the oracle to the bar scaled by the largest force.  Small
+    list bricks (nlist 2) and wide ones (nlist 3, whose staging capacity the passes follow)."""
+    import msmpscu_amd._lib as L
+    box = cfg.make_bcc_box((12, 12, 12), cfg.W_A, 0.05, 0.0, 4)
+    rng = np.random.default_rng(4)
+    x = np.concatenate([box.XP, rng.normal(0.0, 1.2, (600, 3))])
+    n = x.shape[0]
+    box = md.SimMDBox(NPRT=n, NGROUP=1, ZL=box.ZL, BOXLOW=box.BOXLOW, XP=np.asfortranarray(x),
+                      XP1=np.zeros((n, 3)), ITYP=np.ones(n, np.int32),
+                      STATU=np.ones(n, np.int32), CM=box.CM, PTYPE=box.PTYPE)
+    ctrl = cfg.w_ctrl(mx=800)
+    ft = cfg.w_forcetable(box, ctrl)
+    say(f"brick table, overflowing bricks, lanes {lanes} list kernel {nlist}")
+    dev = device_for(box, ctrl, ft)
+    for opt, v in ((L.MSM_OPT_FORCE_PATH, 0), (L.MSM_OPT_TABLE_LDS, 1),
+                   (L.MSM_OPT_BRICK_LANES, lanes), (L.MSM_OPT_NLIST_KERNEL, nlist)):
+        L.check(L.lib().msm_set_option(dev.h, opt, v), "msm_set_option")
+    assert dev.pass_kernel() == "brick_table"
+    dev.build_neighbors()
+    dev.calc_force()
+    _, _, fp, _, _ = dev.get_atoms()
+    den = dev.get_den()
+    dev.close()
+    fo, do, _ = oracle_full(orc, box, ctrl, ft)
+    assert np.max(np.abs(fp - fo)) <= F_TOL * max(1.0, np.max(np.abs(fo)) / 10.0)
+    assert np.max(np.abs(den - do)) <= 1e-13 * max(1.0, np.max(np.abs(do)))
+
+
 @pytest.mark.parametrize("pot_type", ["FS_TYPE", "EAM_TYPE"])
 def test_force_energy_vs_oracle(orc, p0, pot_type):
     box, ctrl, _ = p0
@@ -387,13 +420,14 @@ def test_million_atom_snapshot(orc):
     dev.close()
 
 
-@pytest.mark.parametrize("path,compact,lds", [(0, 1, 0), (1, 0, 0), (1, 1, 0), (2, 0, 0),
+@pytest.mark.parametrize("path,compact,lds", [(0, 0, 0), (1, 0, 0), (1, 1, 0), (2, 0, 0),
                                               (2, 1, 0), (3, 0, 0), (4, 1, 0), (8, 1, 0),
-                                              (1, 0, 1), (2, 0, 1), (4, 0, 1)])
+                                              (1, 0, 1), (2, 0, 1), (4, 0, 1), (0, 0, 1)])
 def test_force_paths_agree(orc, p0, path, compact, lds):
-    """Every density/force kernel variant (LDS bricks; global list with 1-8 lanes per atom, with
-    and without the compacted in-cutoff list, with the table in global memory or in LDS) meets
-    the parity bar on the same box, before and after a 15-step run."""
+    """Every density/force kernel variant (LDS bricks with the table in global memory or in LDS;
+    global list with 1-8 lanes per atom, with and without the compacted in-cutoff list, with the
+    table in global memory or in LDS) meets the parity bar on the same box, before and after a
+    15-step run."""
     import msmpscu_amd._lib as L
     box, ctrl, ft = p0
     say(f"force path {path} compact {compact} lds table {lds}")
@@ -401,6 +435,9 @@ def test_force_paths_agree(orc, p0, path, compact, lds):
     L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
     L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_COMPACT_LIST, compact), "msm_set_option")
     L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS, lds), "msm_set_option")
+    expect = {0: "brick_table" if lds else "bricks", 3: "lane_pair"}.get(
+        path, "lds_table" if lds else "lanes")
+    assert dev.pass_kernel() == expect
     dev.build_neighbors()
     dev.calc_force()
     _, _, fp, _, _ = dev.get_atoms()
@@ -414,13 +451,14 @@ def test_force_paths_agree(orc, p0, path, compact, lds):
     dev.close()
 
 
-def _asym_case():
+def _asym_case(ntab=5000):
     box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.08, 300.0, 8)
     box.NGROUP = 2
     box.ITYP = (np.arange(box.NPRT) % 2 + 1).astype(np.int32)
     box.CM = np.array([box.CM[0], box.CM[0] * 0.5])
     box.PTYPE = np.array([[1, 3], [4, 2]], dtype=np.int32)
-    ctrl = md.SimMDCtrl(RU=np.full((2, 2), 4.4002), NB_RM=np.full((2, 2), 4.7002), NB_MXNBS=64)
+    ctrl = md.SimMDCtrl(RU=np.full((2, 2), 4.4002), NB_RM=np.full((2, 2), 4.7002), NB_MXNBS=64,
+                        NUMFTABR=ntab)
     ft = md.MDForceTable()
     md.Register_ForceTableProc(1, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne")
     md.Register_ForceTableProc(2, ft, "msm_pot_ackland_ww_nn", "msm_pot_ackland_ww_ne")
@@ -430,7 +468,7 @@ def _asym_case():
     return [box], ctrl, ft
 
 
-@pytest.mark.parametrize("variant", ["pair", "lds"])
+@pytest.mark.parametrize("variant", ["pair", "lds", "bt1", "bt2", "bt4"])
 @pytest.mark.parametrize("case", ["fs", "eam", "asym", "inactive", "multibox", "slab"])
 def test_kernel_variant_matches_two_lane_path(p0, case, variant):
     """The lane-pair gather kernels (force path 3) and the LDS-table kernels split each atom's
@@ -438,14 +476,16 @@ def test_kernel_variant_matches_two_lane_path(p0, case, variant):
     differently, so agreement is to rounding): F' and forces to the parity bar, and positions
     after a 12-step run to 1e-12 A, over single and multiple species (the LDS table serves one
     kind; several kinds take the global table), FS and EAM, inactive atoms, uneven boxes and a
-    slab."""
+    slab.  The bt variants are the LDS-brick passes with the table in LDS (1, 2, 4 lanes per
+    home atom; the two-species cases use 1500-knot tables so that every kind's window fits)."""
     import msmpscu_amd._lib as L
     box, ctrl, ft = p0
     boxes, statu = [box], None
+    bt = variant.startswith("bt")
     if case == "eam":
         ft = cfg.w_forcetable(box, ctrl, pot_type="EAM_TYPE")
     elif case == "asym":
-        boxes, ctrl, ft = _asym_case()
+        boxes, ctrl, ft = _asym_case(1500 if bt else 5000)
     elif case == "inactive":
         statu = box.STATU.copy()
         statu[[3, 100, 4000, 4001]] = 0
@@ -455,7 +495,7 @@ def test_kernel_variant_matches_two_lane_path(p0, case, variant):
         ft = cfg.w_forcetable(boxes[0], ctrl)
     elif case == "slab":
         b = cfg.make_cuni_slab((6, 6, 6), 3.57, 600.0, 5)
-        ctrl = cfg.cuni_ctrl()
+        ctrl = cfg.cuni_ctrl(ntab=1500 if bt else 5000)
         ft = cfg.cuni_forcetable(b, ctrl)
         boxes = [b]
     say(f"{variant} vs two lanes: {case}")
@@ -463,9 +503,16 @@ def test_kernel_variant_matches_two_lane_path(p0, case, variant):
     for k in range(2):
         dev = device_for(boxes, ctrl, ft)
         path = 3 if (k == 1 and variant == "pair") else 2
+        if k == 1 and bt:
+            path = 0
+            L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_BRICK_LANES, int(variant[2:])),
+                    "msm_set_option")
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS,
-                                       int(k == 1 and variant == "lds")), "msm_set_option")
+                                       int(k == 1 and variant in ("lds", "bt1", "bt2", "bt4"))),
+                "msm_set_option")
+        if k == 1 and bt:
+            assert dev.pass_kernel() == "brick_table"
         if statu is not None:
             dev.set_atoms(box.XP, box.XP1, box.ITYP, statu)
         dev.build_neighbors()
@@ -484,12 +531,14 @@ def test_kernel_variant_matches_two_lane_path(p0, case, variant):
     assert np.max(np.abs(outs[1][8] - outs[0][8])) <= F_TOL
 
 
+@pytest.mark.parametrize("path", [2, 0])
 @pytest.mark.parametrize("pot_type", ["FS_TYPE", "EAM_TYPE"])
-def test_lds_table_window_fallback(orc, pot_type):
+def test_lds_table_window_fallback(orc, pot_type, path):
     """A 12000-knot table leaves the bottom of both tables outside the LDS windows (force window
-    starts near r = 2.7 A, density window near 1.45 A), and a few atoms are pushed to 1.15-1.4 A
-    from a neighbour: the pairs below the windows take the global-table fallback and every
-    result still meets the parity bar against the oracle."""
+    of the global-list passes starts near r = 2.7 A, density window near 1.45 A; the LDS-brick
+    passes hold less), and a few atoms are pushed to 1.15-1.4 A from a neighbour: the pairs
+    below the windows take the global-table fallback and every result still meets the parity
+    bar against the oracle."""
     import msmpscu_amd._lib as L
     box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.05, 300.0, 77)
     d = np.array([cfg.W_A / 2, cfg.W_A / 2, cfg.W_A / 2])
@@ -499,11 +548,14 @@ def test_lds_table_window_fallback(orc, pot_type):
     box.XP = np.asfortranarray(cfg.wrap_into(box.XP, box.BOXLOW, box.ZL, (1, 1, 1)))
     ctrl = cfg.w_ctrl(ntab=12000)
     ft = cfg.w_forcetable(box, ctrl, pot_type=pot_type)
-    say(f"lds window fallback {pot_type}")
+    say(f"lds window fallback {pot_type} path {path}")
     outs = []
     for lds in (0, 1):
         dev = device_for(box, ctrl, ft)
+        L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS, lds), "msm_set_option")
+        if lds:
+            assert dev.pass_kernel() == ("lds_table" if path else "brick_table")
         dev.build_neighbors()
         dev.calc_force()
         _, _, fp, _, _ = dev.get_atoms()
