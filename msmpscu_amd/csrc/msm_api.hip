@@ -118,6 +118,8 @@ struct msm_sim {
     int nl_lanes = 1;  // lanes per home atom of the list build
     int nl_cap = 64;   // halo atoms staged per list-build workgroup
     int nl_wide = 0;   // bricks planned for the wide list kernel
+    int list_sidx = 0; // the current list holds halo slots (written for the brick passes)
+    int tuning = 0;    // msm_autotune is running: lists carry halo slots for every candidate
     std::vector<double> occ;  // mean atoms per non-empty cell, per box (at upload)
     int* d_brick_box = nullptr;
 
@@ -215,7 +217,7 @@ ListDev list_view(msm_sim* s) {
     l.ckv = cmp ? s->ckv : nullptr;
     l.stride = msm::kListRow;
     l.mx = s->mx;
-    l.scap = s->nlist_kind == 1 ? 0 : s->nl_wide ? s->nl_cap : msm::kNlCapPass;
+    l.scap = (s->nlist_kind == 1 || !s->list_sidx) ? 0 : s->nl_wide ? s->nl_cap : msm::kNlCapPass;
     return l;
 }
 
@@ -296,7 +298,9 @@ int read_flags(msm_sim* s) {
 
 // Neighbour-list rebuild on the stream (capturable: kernels only).  Reads the current set and
 // position buffer, writes the other set and buffer, then flips both.
-void enqueue_rebuild(msm_sim* s) {
+// keep_f: carry the forces into the new order (not needed inside msm_run, where a force pass
+// follows every rebuild).
+void enqueue_rebuild(msm_sim* s, bool keep_f = true) {
     TimedScope ts(s, MSM_KCLASS_REBUILD);
     const int src = s->scur, dst = 1 - s->scur;
     const int p0 = s->pcur, p1 = 1 - s->pcur;
@@ -310,19 +314,24 @@ void enqueue_rebuild(msm_sim* s) {
     msm::launch_scan_exclusive(s->sb.count, s->sb.start, nc + 2, s->sb.scan_tmp, s->stream);
     msm::launch_scatter_sort(s->n, nc, s->sb, s->set[src].gid, s->stream);
     msm::launch_gather(s->n, s->n_alloc, s->sb.perm, a_src, s->set[src].gid, a_dst,
-                       s->set[dst].gid, s->sb.key, s->set[dst].cell, s->stream);
+                       s->set[dst].gid, s->sb.key, s->set[dst].cell, keep_f ? 1 : 0, s->stream);
     TimedScope tl(s, MSM_KCLASS_NLIST);
-    if (s->nlist_kind == 1)
+    if (s->nlist_kind == 1) {
+        s->list_sidx = 0;
         msm::launch_nlist(s->n, s->pos[p1], s->set[dst].ityp, s->set[dst].statu,
                           s->set[dst].boxid, s->set[dst].cell, s->sb.start, s->d_boxes, s->ngroup,
                           s->d_rnl2, s->mx, s->kvois, s->indi, msm::kListRow, s->d_err, s->stream);
-    else
-        msm::launch_nlist_bricks(s->nl_wide, s->nbricks, s->nl_lanes, s->nl_cap, s->d_brick_box,
-                                 s->n, nc,
+    } else {
+        // halo slots only for the brick passes (a list without them sends those passes to
+        // the global list until the next rebuild)
+        s->list_sidx = s->force_path == 0 || s->tuning;
+        msm::launch_nlist_bricks(s->nl_wide, s->nbricks, s->nl_lanes, s->nl_cap, s->list_sidx,
+                                 s->d_brick_box, s->n, nc,
                                  s->pos[p1],
                                  s->set[dst].ityp, s->sb.start, s->d_boxes, s->ngroup, s->d_rnl2,
                                  s->mx, s->kvois, s->indi, s->sidx, msm::kListRow, s->d_err,
                                  s->stream);
+    }
     s->scur = dst;
     s->pcur = p1;
 }
@@ -553,7 +562,7 @@ void enqueue_force(msm_sim* s, int mode, int write_f, double h) {
 // One segment of the step loop: [rebuild] then nsteps x (density, fused force); the final step
 // of the run applies only the corrector and stores forces.
 void enqueue_segment(msm_sim* s, int rebuild, int nsteps, int last, double h) {
-    if (rebuild) enqueue_rebuild(s);
+    if (rebuild) enqueue_rebuild(s, false);
     for (int k = 0; k < nsteps; ++k) {
         enqueue_density(s);
         const bool final_step = last && (k == nsteps - 1);
@@ -1352,6 +1361,7 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(s, hipEventCreate(&e0));
     HIPCHK(s, hipEventCreate(&e1));
+    s->tuning = 1;  // cleared below on every path after this point
     auto time_it = [&](auto&& body, float& ms) -> int {
         body();  // warm-up (code object load, occupancy query)
         HIPCHK(s, hipEventRecord(e0, s->stream));
@@ -1411,6 +1421,7 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     s->timing = saved_timing;
+    s->tuning = 0;
     if (rc != MSM_OK) return rc;
     const Cand& w = cands[bi];
     s->force_path = w.fp;

@@ -23,9 +23,15 @@ namespace msm {
 // --------------------------------------------------------------------------------------------
 
 // Small-brick list build (sparse cells): one kBlock-thread workgroup per brick of at most
-// kNlCapPass halo atoms (several workgroups share a CU), one lane per home atom testing the
-// candidates of each stencil cell four at a time, in the serial order.
-template <int MT>
+// kNlCapPass halo atoms, one lane per home atom.  The LDS tables stay near 40 KB (a neighbour's
+// sorted index is its cell's base plus its halo slot, not a stored array), so four workgroups
+// share a CU.  The candidates of each stencil cell are tested kNlGroup at a time (their LDS
+// loads issued together) and the hits stored in candidate order.  SIDX: also store each
+// neighbour's halo slot (read by the LDS-brick passes).  Measured alternatives: groups of 8,
+// and hit masks with one store round per hit of the busiest lane -- both slower.
+constexpr int kNlGroup = 4;
+
+template <int MT, int SIDX>
 __global__ __launch_bounds__(kBlock) void k_nlist_small(
     const int* __restrict__ brick_box, const double4* __restrict__ pos,
     const int* __restrict__ ityp, const int* __restrict__ start,
@@ -33,11 +39,12 @@ __global__ __launch_bounds__(kBlock) void k_nlist_small(
     int* __restrict__ kvois, int* __restrict__ indi, unsigned short* __restrict__ sidx,
     int stride, int* err) {
 #pragma clang fp contract(off)
+    // about 40 KB for one atom type, so that four workgroups share a CU
     __shared__ double lx[kNlCapPass], ly[kNlCapPass], lz[kNlCapPass];
-    __shared__ int lg[kNlCapPass];
     __shared__ int lt[MT ? kNlCapPass : 1];
+    __shared__ int hbase[kNlMaxHalo];  // sorted index of halo slot q in cell h: hbase[h] + q
     __shared__ signed char hsh[kNlMaxHalo * 3];
-    __shared__ double srnl2[kMaxGroup * kMaxGroup];
+    __shared__ double srnl2[MT ? kMaxGroup * kMaxGroup : 1];
     __shared__ BrickLds L;
 
     const int tid = threadIdx.x;
@@ -49,6 +56,8 @@ __global__ __launch_bounds__(kBlock) void k_nlist_small(
         for (int k = tid; k < ngroup * ngroup; k += blockDim.x) srnl2[k] = rnl2[k];
     const int total = brick_halo(B, g, start, L.hcell, hsh, L.hoff, L.homeoff, L.scan);
     const bool staged = total <= kNlCapPass;
+    for (int k = tid; k < g.nh; k += blockDim.x)
+        hbase[k] = L.hcell[k] >= 0 ? start[L.hcell[k]] - L.hoff[k] : 0;
     if (staged) {
         for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int h) {
             const double4 pq = pos[q];
@@ -56,7 +65,6 @@ __global__ __launch_bounds__(kBlock) void k_nlist_small(
             lx[s] = pq.x + shift_of(hsh[3 * h], B.len[0]);
             ly[s] = pq.y + shift_of(hsh[3 * h + 1], B.len[1]);
             lz[s] = pq.z + shift_of(hsh[3 * h + 2], B.len[2]);
-            lg[s] = q;
             if constexpr (MT) lt[s] = ityp[q];
         });
     }
@@ -73,18 +81,20 @@ __global__ __launch_bounds__(kBlock) void k_nlist_small(
             continue;
         }
         const int si = L.hoff[h] + off;
-        const int p = lg[si];
+        const int p = hbase[h] + si;
         const double xi = lx[si], yi = ly[si], zi = lz[si];
         const int ti = MT ? lt[si] : 0;
+        int* __restrict__ li = indi + list_base(p, mx);
+        unsigned short* __restrict__ ls = SIDX ? sidx + list_base(p, mx) : nullptr;
         int c = 0;
         for (int s = 0; s < 27; ++s) {
             const int hn = h + c_nix[s] + g.H[0] * (c_niy[s] + g.H[1] * c_niz[s]);
             const int qe = L.hoff[hn + 1];
-            // candidates in groups of four (LDS loads of a group issued together), taken in order
-            for (int q0 = L.hoff[hn]; q0 < qe; q0 += 4) {
-                bool hit[4];
+            const int gb = hbase[hn];
+            for (int q0 = L.hoff[hn]; q0 < qe; q0 += kNlGroup) {
+                bool hit[kNlGroup];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kNlGroup; ++u) {
                     const bool valid = q0 + u < qe && q0 + u != si;
                     const int q = valid ? q0 + u : si;
                     // XPT(I) - (XP(J) + CXYZ), squared and summed left to right, no contraction
@@ -96,12 +106,12 @@ __global__ __launch_bounds__(kBlock) void k_nlist_small(
                     hit[u] = valid && r2 < lim;
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kNlGroup; ++u) {
                     if (hit[u]) {
                         if (c < mx) {
-                            const size_t a = list_base(p, mx) + static_cast<size_t>(c) * stride;
-                            indi[a] = lg[q0 + u];
-                            sidx[a] = static_cast<unsigned short>(q0 + u);
+                            li[c * stride] = gb + q0 + u;
+                            if constexpr (SIDX)
+                                ls[c * stride] = static_cast<unsigned short>(q0 + u);
                         }
                         ++c;
                     }
@@ -127,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_nlist_small(
 // global-memory scan.
 constexpr int kNlUnroll = 4;
 
-template <int MT, int T>
+template <int MT, int T, int SIDX>
 __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
     const int* __restrict__ brick_box, const double4* __restrict__ pos,
     const int* __restrict__ ityp, const int* __restrict__ start,
@@ -217,7 +227,7 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
                             const int q = q0 + u * T + sub;
                             const size_t at = lb + static_cast<size_t>(w) * stride;
                             indi[at] = lg[q];
-                            sidx[at] = static_cast<unsigned short>(q);
+                            if constexpr (SIDX) sidx[at] = static_cast<unsigned short>(q);
                         }
                     }
                     c += __popc(grp);
@@ -243,7 +253,8 @@ __global__ void k_zero_range(int* __restrict__ p, const int* __restrict__ first,
     if (i < n) p[i] = 0;
 }
 
-void launch_nlist_bricks(int wide, int nbricks, int lanes, int cap, const int* brick_box, int n,
+void launch_nlist_bricks(int wide, int nbricks, int lanes, int cap, int write_sidx,
+                         const int* brick_box, int n,
                          int ncell_total, const double4* pos, const int* ityp, const int* start,
                          const BoxDev* boxes, int ngroup, const double* rnl2, int mx, int* kvois,
                          int* indi, unsigned short* sidx, int stride, int* err, hipStream_t s) {
@@ -253,29 +264,28 @@ void launch_nlist_bricks(int wide, int nbricks, int lanes, int cap, const int* b
                        start + ncell_total, n);
     if (nbricks <= 0) return;
     if (!wide) {
-        if (ngroup > 1)
-            hipLaunchKernelGGL(k_nlist_small<1>, dim3(nbricks), dim3(kBlock), 0, s, brick_box,
-                               pos, ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx,
-                               stride, err);
-        else
-            hipLaunchKernelGGL(k_nlist_small<0>, dim3(nbricks), dim3(kBlock), 0, s, brick_box,
-                               pos, ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx,
-                               stride, err);
+#define MSM_NS(MT, S)                                                                         \
+    hipLaunchKernelGGL((k_nlist_small<MT, S>), dim3(nbricks), dim3(kBlock), 0, s, brick_box, pos, \
+                       ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx, stride, err)
+        if (ngroup > 1) { if (write_sidx) MSM_NS(1, 1); else MSM_NS(1, 0); }
+        else            { if (write_sidx) MSM_NS(0, 1); else MSM_NS(0, 0); }
+#undef MSM_NS
         return;
     }
     const size_t lds = nlist_lds_bytes(cap, ngroup);
-#define MSM_NL(MT, T)                                                                           \
-    hipLaunchKernelGGL((k_nlist_bricks<MT, T>), dim3(nbricks), dim3(kNlBlock), lds, s, brick_box, \
-                       pos, ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx, stride, err, \
-                       cap)
-#define MSM_NLT(MT)                                   \
+#define MSM_NL(MT, T, S)                                                                       \
+    hipLaunchKernelGGL((k_nlist_bricks<MT, T, S>), dim3(nbricks), dim3(kNlBlock), lds, s,       \
+                       brick_box, pos, ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx,  \
+                       stride, err, cap)
+#define MSM_NLT(MT, S)                                \
     do {                                              \
-        if (lanes >= 8) MSM_NL(MT, 8);                \
-        else if (lanes >= 4) MSM_NL(MT, 4);           \
-        else if (lanes >= 2) MSM_NL(MT, 2);           \
-        else MSM_NL(MT, 1);                           \
+        if (lanes >= 8) MSM_NL(MT, 8, S);             \
+        else if (lanes >= 4) MSM_NL(MT, 4, S);        \
+        else if (lanes >= 2) MSM_NL(MT, 2, S);        \
+        else MSM_NL(MT, 1, S);                        \
     } while (0)
-    if (ngroup > 1) MSM_NLT(1); else MSM_NLT(0);
+    if (ngroup > 1) { if (write_sidx) MSM_NLT(1, 1); else MSM_NLT(1, 0); }
+    else            { if (write_sidx) MSM_NLT(0, 1); else MSM_NLT(0, 0); }
 #undef MSM_NLT
 #undef MSM_NL
 }

@@ -112,7 +112,8 @@ void launch_cell_assign(const double4* pos, const int* statu, const int* boxid,
 void launch_scan_exclusive(const int* in, int* out, int n, int* tmp, hipStream_t s);
 void launch_scatter_sort(int n, int ncell_total, SortBufs sb, const int* gid, hipStream_t s);
 void launch_gather(int n, int n_alloc, const int* perm, const AtomsDev& src, const int* gid_src,
-                   AtomsDev& dst, int* gid_dst, const int* key, int* cell_dst, hipStream_t s);
+                   AtomsDev& dst, int* gid_dst, const int* key, int* cell_dst, int keep_f,
+                   hipStream_t s);
 void launch_nlist(int n, const double4* pos, const int* ityp, const int* statu, const int* boxid,
                   const int* cell, const int* start, const BoxDev* boxes, int ngroup,
                   const double* rnl2, int mx, int* kvois, int* indi, int stride, int* err,
@@ -120,7 +121,8 @@ void launch_nlist(int n, const double4* pos, const int* ityp, const int* statu, 
 // LDS-staged variants over bricks of cells (msm_bricks.hip); out-of-box atoms (sentinel bucket)
 // get KVOIS = 0, F' = 0, F = 0 and keep their position.
 size_t nlist_lds_bytes(int cap, int ngroup);  // dynamic LDS of the list build for cap halo atoms
-void launch_nlist_bricks(int wide, int nbricks, int lanes, int cap, const int* brick_box, int n,
+void launch_nlist_bricks(int wide, int nbricks, int lanes, int cap, int write_sidx,
+                         const int* brick_box, int n,
                          int ncell_total,
                          const double4* pos, const int* ityp, const int* start,
                          const BoxDev* boxes, int ngroup, const double* rnl2, int mx, int* kvois,

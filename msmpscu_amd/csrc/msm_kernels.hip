@@ -14,6 +14,8 @@
 //   force            MD_FS_ForceTable_GPU.F90:623-806, MD_EAM_ForceTable_GPU.F90:646-829
 //   energy           MD_FS_ForceTable_GPU.F90:1451-1609, MD_EAM_ForceTable_GPU.F90:1470-1636
 //   integrator       MD_DiffScheme_GPU.F90:242-384 (Predictor_KERNEL0), :686-758 (Correction_KERNEL)
+#include <climits>
+
 #include "msm_internal.h"
 #include "msm_device.h"
 
@@ -64,7 +66,24 @@ __global__ void k_cell_assign(const double4* __restrict__ pos, const int* __rest
             atomicOr(&err[kErrOutOfBox], 1);
     }
     key[i] = c;
-    slot[i] = atomicAdd(&count[c], 1);
+    // one atomic per distinct cell of the wave (after the first build the atoms are cell-sorted,
+    // so a wave spans a few cells): the leader of each cell adds the cell's lane count and every
+    // lane takes base + its rank among those lanes.  Slot order inside a cell is made
+    // deterministic afterwards by k_cell_order.
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(1);
+    int my = 0;
+    while (todo) {
+        const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+        const int lc = __shfl(c, leader, 64);
+        const unsigned long long same = __ballot(c == lc) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&count[lc], __popcll(same));
+        base = __shfl(base, leader, 64);
+        if (c == lc) my = base + __popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    slot[i] = my;
 }
 
 void launch_cell_assign(const double4* pos, const int* statu, const int* boxid,
@@ -125,13 +144,47 @@ __global__ void k_scatter(int n, const int* __restrict__ key, const int* __restr
     if (i < n) perm[start[key[i]] + slot[i]] = i;
 }
 
-// One thread per cell: order the cell's atoms by caller index (insertion sort), which makes the
-// sorted order, the list order and every summation order independent of atomic arrival order.
+// One thread per cell: order the cell's atoms by caller index, which makes the sorted order,
+// the list order and every summation order independent of atomic arrival order.  A cell of at
+// most kCellSortReg atoms is loaded in two independent rounds (entries, then their caller
+// indices) and sorted in registers by an odd-even transposition network; a larger cell takes an
+// insertion sort in memory.
+constexpr int kCellSortReg = 16;
+
 __global__ void k_cell_order(int ncells, const int* __restrict__ start, int* __restrict__ perm,
                              const int* __restrict__ gid) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells) return;
     const int b = start[c], e = start[c + 1];
+    const int n = e - b;
+    if (n <= 1) return;
+    if (n <= kCellSortReg) {
+        int v[kCellSortReg], g[kCellSortReg];
+#pragma unroll
+        for (int k = 0; k < kCellSortReg; ++k) v[k] = k < n ? perm[b + k] : 0;
+#pragma unroll
+        for (int k = 0; k < kCellSortReg; ++k) g[k] = k < n ? gid[v[k]] : INT_MAX;
+        auto cswap = [&](int k) {
+            const bool sw = g[k] > g[k + 1];
+            const int g0 = sw ? g[k + 1] : g[k], g1 = sw ? g[k] : g[k + 1];
+            const int v0 = sw ? v[k + 1] : v[k], v1 = sw ? v[k] : v[k + 1];
+            g[k] = g0;
+            g[k + 1] = g1;
+            v[k] = v0;
+            v[k + 1] = v1;
+        };
+#pragma unroll
+        for (int r = 0; r < kCellSortReg / 2; ++r) {  // even phase, then odd phase
+#pragma unroll
+            for (int k = 0; k + 1 < kCellSortReg; k += 2) cswap(k);
+#pragma unroll
+            for (int k = 1; k + 1 < kCellSortReg; k += 2) cswap(k);
+        }
+#pragma unroll
+        for (int k = 0; k < kCellSortReg; ++k)
+            if (k < n) perm[b + k] = v[k];
+        return;
+    }
     for (int a = b + 1; a < e; ++a) {
         const int v = perm[a];
         const int g = gid[v];
@@ -153,6 +206,8 @@ void launch_scatter_sort(int n, int ncell_total, SortBufs sb, const int* gid, hi
                        sb.perm, gid);
 }
 
+// F = 0: forces are not carried (a force pass overwrites them before they are read).
+template <int F>
 __global__ void k_gather(int n, const int* __restrict__ perm, AtomsDev src,
                          const int* __restrict__ gid_src, AtomsDev dst, int* __restrict__ gid_dst,
                          const int* __restrict__ key, int* __restrict__ cell_dst) {
@@ -165,9 +220,11 @@ __global__ void k_gather(int n, const int* __restrict__ perm, AtomsDev src,
     dst.vx[p] = src.vx[i];
     dst.vy[p] = src.vy[i];
     dst.vz[p] = src.vz[i];
-    dst.fx[p] = src.fx[i];
-    dst.fy[p] = src.fy[i];
-    dst.fz[p] = src.fz[i];
+    if constexpr (F) {
+        dst.fx[p] = src.fx[i];
+        dst.fy[p] = src.fy[i];
+        dst.fz[p] = src.fz[i];
+    }
     dst.dx[p] = src.dx[i];
     dst.dy[p] = src.dy[i];
     dst.dz[p] = src.dz[i];
@@ -179,11 +236,16 @@ __global__ void k_gather(int n, const int* __restrict__ perm, AtomsDev src,
 }
 
 void launch_gather(int n, int n_alloc, const int* perm, const AtomsDev& src, const int* gid_src,
-                   AtomsDev& dst, int* gid_dst, const int* key, int* cell_dst, hipStream_t s) {
+                   AtomsDev& dst, int* gid_dst, const int* key, int* cell_dst, int keep_f,
+                   hipStream_t s) {
     (void)n_alloc;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_gather, dim3(grid_for(n)), dim3(kBlock), 0, s, n, perm, src, gid_src, dst,
-                       gid_dst, key, cell_dst);
+    if (keep_f)
+        hipLaunchKernelGGL(k_gather<1>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, perm, src,
+                           gid_src, dst, gid_dst, key, cell_dst);
+    else
+        hipLaunchKernelGGL(k_gather<0>, dim3(grid_for(n)), dim3(kBlock), 0, s, n, perm, src,
+                           gid_src, dst, gid_dst, key, cell_dst);
 }
 
 __global__ __launch_bounds__(kBlock) void k_nlist(
@@ -673,17 +735,34 @@ void launch_box_sum(const double* epot, const int* start, const BoxDev* boxes, i
         hipLaunchKernelGGL(k_box_sum, dim3(nbox), dim3(kBlock), 0, s, epot, start, boxes, out);
 }
 
+// Sum and maximum of KVOIS: grid-stride partial results, reduced per wave and then per
+// workgroup, so that only two atomics per workgroup reach the two global counters.
 __global__ void k_list_stats(const int* __restrict__ kvois, int n, unsigned long long* total,
                              int* maxk) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    int v = (p < n) ? kvois[p] : 0;
-    unsigned long long s = static_cast<unsigned long long>(v);
-    int m = v;
+    __shared__ unsigned long long ws[kBlock / 64];
+    __shared__ int wm[kBlock / 64];
+    unsigned long long s = 0;
+    int m = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const int v = kvois[p];
+        s += static_cast<unsigned long long>(v);
+        m = max(m, v);
+    }
     for (int d = 32; d > 0; d >>= 1) {
         s += __shfl_down(s, d, 64);
         m = max(m, __shfl_down(m, d, 64));
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        ws[w] = s;
+        wm[w] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < static_cast<int>(blockDim.x >> 6); ++k) {
+            s += ws[k];
+            m = max(m, wm[k]);
+        }
         atomicAdd(total, s);
         atomicMax(maxk, m);
     }
@@ -691,7 +770,9 @@ __global__ void k_list_stats(const int* __restrict__ kvois, int n, unsigned long
 
 void launch_list_stats(const int* kvois, int n, unsigned long long* total, int* maxk,
                        hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_list_stats, dim3(grid_for(n)), dim3(kBlock), 0, s, kvois, n, total, maxk);
+    if (n <= 0) return;
+    const unsigned g = grid_for(n) < 1024u ? grid_for(n) : 1024u;
+    hipLaunchKernelGGL(k_list_stats, dim3(g), dim3(kBlock), 0, s, kvois, n, total, maxk);
 }
 
 }  // namespace msm

@@ -182,7 +182,7 @@ def nlist(kind: str, xp, ityp0, statu, lo, ln, ifpd, nb_rm, mx):
     return kv, ind.reshape(mx, n).T
 
 
-def force_fs_half(xp, ityp0, kvois, indi, ln, ifpd, ru, table: Table):
+def force_fs_half(xp, ityp0, kvois, indi, ln, ifpd, ru, table: Table, with_vabs=False):
     n = int(np.asarray(ityp0).shape[0])
     mx = indi.shape[1]
     x = _xp(xp)
@@ -195,12 +195,14 @@ def force_fs_half(xp, ityp0, kvois, indi, ln, ifpd, ru, table: Table):
     den = np.zeros(n)
     dden = np.zeros(n)
     vt = np.zeros(9)
+    va = np.zeros(9)
     rc = lib().orc_force_fs_half(n, _d(x), _i(it), _i(kv), _i(ind), mx, _d(_v3(ln)),
                                  _i(_v3(ifpd, np.int32)), _d(r), C.byref(table.struct()), _d(fp),
-                                 _d(ep), _d(den), _d(dden), _d(vt))
+                                 _d(ep), _d(den), _d(dden), _d(vt), _d(va) if with_vabs else None)
     if rc != 0:
         raise RuntimeError(f"oracle force error {rc}")
-    return fp.reshape(3, n).T.copy(), ep, den, dden, vt.reshape(3, 3).T.copy()
+    out = (fp.reshape(3, n).T.copy(), ep, den, dden, vt.reshape(3, 3).T.copy())
+    return out + (va.reshape(3, 3).T.copy(),) if with_vabs else out
 
 
 def force_full(xp, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table: Table):

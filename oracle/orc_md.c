@@ -333,7 +333,7 @@ double orc_sum(const double* a, int n) {
 int orc_force_fs_half(int n, const double* xp, const int* ityp, const int* kvois, const int* indi,
                       int mxkvois, const double len[3], const int ifpd[3], const double* ru,
                       const orc_ftable* t, double* fp, double* epot, double* den, double* dden,
-                      double* vtensor) {
+                      double* vtensor, double* vabs) {
     (void)mxkvois;
     int ng = t->ngroup;
     double hbox[3] = {0.5 * len[0], 0.5 * len[1], 0.5 * len[2]};
@@ -375,6 +375,8 @@ int orc_force_fs_half(int n, const double* xp, const int* ityp, const int* kvois
     /* CALFOR, :277-395 */
     for (int a = 0; a < 3 * n; ++a) fp[a] = 0.0;
     for (int a = 0; a < 9; ++a) vtensor[a] = 0.0;
+    if (vabs)
+        for (int a = 0; a < 9; ++a) vabs[a] = 0.0;
     for (int i = 0; i < n; ++i) {
         int ti = ityp[i];
         double denki = dden[i];
@@ -403,6 +405,10 @@ int orc_force_fs_half(int n, const double* xp, const int* ityp, const int* kvois
                 }
                 for (int k = 0; k < 3; ++k)
                     for (int k1 = 0; k1 < 3; ++k1) vtensor[k + 3 * k1] += sep[k1] * sep[k] * fortot;
+                if (vabs)
+                    for (int k = 0; k < 3; ++k)
+                        for (int k1 = 0; k1 < 3; ++k1)
+                            vabs[k + 3 * k1] += fabs(sep[k1] * sep[k] * fortot);
             }
         }
     }
@@ -732,7 +738,7 @@ int orc_md_run(int n, double* xp, double* xp1, double* fp, double* dis, int* sta
     if (init) {
         rc = orc_nlist_half(n, xp, ityp, statu, lo, len, ifpd, t->ngroup, nb_rm, mxkvois, kvois, indi);
         if (rc) goto out;
-        rc = orc_force_fs_half(n, xp, ityp, kvois, indi, mxkvois, len, ifpd, ru, t, fp, epot, den, dden, vt);
+        rc = orc_force_fs_half(n, xp, ityp, kvois, indi, mxkvois, len, ifpd, ru, t, fp, epot, den, dden, vt, NULL);
         if (rc) goto out;
     }
     rc = ORC_OK;
@@ -742,7 +748,7 @@ int orc_md_run(int n, double* xp, double* xp1, double* fp, double* dis, int* sta
             rc = orc_nlist_half(n, xp, ityp, statu, lo, len, ifpd, t->ngroup, nb_rm, mxkvois, kvois, indi);
             if (rc) goto out;
         }
-        rc = orc_force_fs_half(n, xp, ityp, kvois, indi, mxkvois, len, ifpd, ru, t, fp, epot, den, dden, vt);
+        rc = orc_force_fs_half(n, xp, ityp, kvois, indi, mxkvois, len, ifpd, ru, t, fp, epot, den, dden, vt, NULL);
         if (rc) goto out;
         orc_correction(n, xp1, fp, statu, ityp, cm, h, NULL);
     }

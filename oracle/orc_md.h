@@ -108,11 +108,13 @@ int orc_nlist_brute(int n, const double* xp, const int* ityp, const int* statu,
 
 /* ---------------- forces ---------------- */
 /* CALFORCE_FS_Force_Table: half list, Newton's third law, per-pair cutoff RU(ti,tj).
- * Outputs FP(N,3), EPOT(N) = ER - sqrt(DEN), DEN(N) (density), DDEN(N) (F'), VTENSOR(3,3). */
+ * Outputs FP(N,3), EPOT(N) = ER - sqrt(DEN), DEN(N) (density), DDEN(N) (F'), VTENSOR(3,3) and,
+ * when vabs is not NULL, the sum of the absolute values of VTENSOR's pair terms per component
+ * (the scale of its rounding error). */
 int orc_force_fs_half(int n, const double* xp, const int* ityp, const int* kvois, const int* indi,
                       int mxkvois, const double len[3], const int ifpd[3], const double* ru,
                       const orc_ftable* t, double* fp, double* epot, double* den, double* dden,
-                      double* vtensor);
+                      double* vtensor, double* vabs);
 /* Full-list restatement of the device kernels PRECALFOR/CALFORCE/CALEPOT (FS or EAM per
  * t->pot_type): single cutoff max(RU^2), only ACTIVE atoms compute, KPAIR/KPAIR^T pairing as
  * in the corrected kernels.  Outputs FP, DEN (F'), EPOT. */
