@@ -207,9 +207,9 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
                                     the force pass visits only those */
 #define MSM_OPT_TABLE_LDS     6  /* 1 (default): the passes read the table's knots from a
                                     window held in LDS (persistent grid) -- on the global-list
-                                    paths 1, 2, 4 (no compaction) and, when the list was built
-                                    by the LDS-brick kernels, on path 0 (then positions and
-                                    knots both come from LDS) */
+                                    paths 1, 2, 3, 4 (no compaction) and, when the list was
+                                    built by the LDS-brick kernels, on path 0 (then positions
+                                    and knots both come from LDS) */
 /* Shape of the LDS-table passes: threads per workgroup (256, 512, 1024; default 1024) and
  * workgroups per CU (1..16, default 1; capped by what fits); the LDS window is the CU's LDS
  * divided by the workgroups per CU. */

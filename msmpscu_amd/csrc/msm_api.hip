@@ -483,11 +483,12 @@ int plan_bricks(msm_sim* s, const double* xp, int n) {
     return replan_bricks(s);
 }
 
-// LDS-table passes: global list with 1, 2 or 4 lanes per atom, no compaction, and an LDS window
-// that holds at least the upper half of every table kind in use.
+// LDS-table passes: global list with 1, 2 or 4 lanes per atom or two lanes paired for 32-byte
+// gathers (path 3), no compaction, and an LDS window that holds at least the upper half of every
+// table kind in use.
 bool use_lds_table(const msm_sim* s) {
     if (!s->table_lds || s->compact) return false;
-    if (s->force_path != 1 && s->force_path != 2 && s->force_path != 4) return false;
+    if (s->force_path < 1 || s->force_path > 4) return false;  // 3: lane-pair gathers
     const msm::LtWin w = msm::lt_window(s->tab, s->lt.wgs_d, s->lt.wgs_f);
     return 2 * w.wf >= s->tab.ntab2 && 2 * w.wd >= s->tab.ntab2;
 }
@@ -1346,7 +1347,7 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     };
     std::vector<Cand> cands;
     if (msm::lt_window(s->tab, 1, 1).wf * 2 >= s->tab.ntab2) {
-        for (int fp : {2, 1, 4}) {
+        for (int fp : {2, 3, 1, 4}) {
             cands.push_back({fp, 1, 1024, 1, 1024, 1, 1});
             cands.push_back({fp, 1, 256, 4, 256, 4, 1});
         }

@@ -22,6 +22,7 @@
 #include <tuple>
 
 #include "msm_brick.h"
+#include "msm_lanepair.h"
 
 namespace msm {
 
@@ -29,11 +30,34 @@ namespace {
 
 constexpr int kGroup = 8;
 
+// TPA = 3 stands for two lanes per atom paired as lanes l and l + 32 (lane-pair gathers).
 template <int TPA>
 __device__ __forceinline__ double lane_sum(double v) {
+    if constexpr (TPA == 3) return v + __shfl_xor(v, 32, 64);
 #pragma unroll
     for (int d = TPA / 2; d >= 1; d >>= 1) v = v + __shfl_xor(v, d, 64);
     return v;
+}
+
+// Slot stride of one lane: lanes per atom.
+template <int TPA>
+constexpr int slot_step() {
+    return TPA == 3 ? 2 : TPA;
+}
+
+// Atom and lane-within-atom of this thread in chunk c: TPA consecutive lanes per atom, or (TPA
+// = 3) lanes l and l + 32 of a wave for atom 32 * wave + l.
+template <int TPA>
+__device__ __forceinline__ void atom_of_lane(int c, int& i, int& sub) {
+    const int gt = c * blockDim.x + threadIdx.x;
+    if constexpr (TPA == 3) {
+        const int lane = threadIdx.x & 63;
+        i = (gt >> 6) * 32 + (lane & 31);
+        sub = lane >> 5;
+    } else {
+        i = gt / TPA;
+        sub = gt % TPA;
+    }
 }
 
 // chunk range [c0, c1) of this workgroup
@@ -54,20 +78,30 @@ __device__ __forceinline__ double density_group(const AtomsDev& a, const ListDev
                                                 const TabDev& t, const BoxDev& b, const LtWin& w,
                                                 const double* s_pb, const int* __restrict__ li,
                                                 int i, int ti, int kv, int base, double2 pxy,
-                                                double pz, double den0) {
+                                                double pz, double den0, int sub = 0) {
+    constexpr int S = slot_step<TPA>();
     const int kmax = t.ntab2 - 2;
     int js[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
-        const int iw = base + u * TPA;
+        const int iw = base + u * S;
         js[u] = iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
     }
     unsigned fix = 0;
 #pragma unroll
     for (int u = 0; u < G; ++u) {
         const int j = js[u];
-        const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
-        const double qz = a.pos[j].z;
+        double2 qxy;
+        double qz;
+        if constexpr (TPA == 3) {
+            double2 h0, h1;
+            pair_gather(a.pos, j, sub, h0, h1);
+            qxy = h0;
+            qz = h1.x;
+        } else {
+            qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
+            qz = a.pos[j].z;
+        }
         const double sx = min_image(pxy.x - qxy.x, b.len[0], b.ilen[0]);
         const double sy = min_image(pxy.y - qxy.y, b.len[1], b.ilen[1]);
         const double sz = min_image(pz - qz, b.len[2], b.ilen[2]);
@@ -76,7 +110,7 @@ __device__ __forceinline__ double density_group(const AtomsDev& a, const ListDev
         double inv_r;
         const double sk = table_coord(r2, t.csi, inv_r);
         const int kk = min(static_cast<int>(sk), kmax);
-        const bool in = base + u * TPA < kv && r2 <= t.ru2max;
+        const bool in = base + u * S < kv && r2 <= t.ru2max;
         const int kl = kk - w.k0d;
         if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
         const int q = row + max(kl, 0);
@@ -89,7 +123,7 @@ __device__ __forceinline__ double density_group(const AtomsDev& a, const ListDev
 #pragma unroll
         for (int u = 0; u < G; ++u) {
             if (!(fix & (1u << u))) continue;
-            const int j = li[static_cast<size_t>(base + u * TPA) * l.stride];
+            const int j = li[static_cast<size_t>(base + u * S) * l.stride];
             const double2 qxy = *reinterpret_cast<const double2*>(&a.pos[j]);
             const double qz = a.pos[j].z;
             const double sx = min_image(pxy.x - qxy.x, b.len[0], b.ilen[0]);
@@ -114,19 +148,27 @@ __device__ __forceinline__ void force_group(const AtomsDev& a, const ListDev& l,
                                             const BoxDev& b, const LtWin& w, const double2* s_ft,
                                             const int* __restrict__ li, int i, int ti, int kv,
                                             int base, const double4& pi, double denki,
-                                            double& fpx, double& fpy, double& fpz) {
+                                            double& fpx, double& fpy, double& fpz, int sub = 0) {
+    constexpr int S = slot_step<TPA>();
     const int kmax = t.ntab2 - 2;
     int js[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
-        const int iw = base + u * TPA;
+        const int iw = base + u * S;
         js[u] = iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
     }
     unsigned fix = 0;
 #pragma unroll
     for (int u = 0; u < G; ++u) {
         const int j = js[u];
-        const double4 pj = a.pos[j];
+        double4 pj;
+        if constexpr (TPA == 3) {
+            double2 h0, h1;
+            pair_gather(a.pos, j, sub, h0, h1);
+            pj = make_double4(h0.x, h0.y, h1.x, h1.y);
+        } else {
+            pj = a.pos[j];
+        }
         const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
         const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
         const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
@@ -141,7 +183,7 @@ __device__ __forceinline__ void force_group(const AtomsDev& a, const ListDev& l,
         const double sk = table_coord(r2, t.csi, inv_r);
         const int kk = min(static_cast<int>(sk), kmax);
         const double dk = sk - kk;
-        const bool in = base + u * TPA < kv && r2 <= t.ru2max;
+        const bool in = base + u * S < kv && r2 <= t.ru2max;
         const int kl = kk - w.k0f;
         if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
         const int q = max(kl, 0);
@@ -165,7 +207,7 @@ __device__ __forceinline__ void force_group(const AtomsDev& a, const ListDev& l,
 #pragma unroll
         for (int u = 0; u < G; ++u) {
             if (!(fix & (1u << u))) continue;
-            const int j = li[static_cast<size_t>(base + u * TPA) * l.stride];
+            const int j = li[static_cast<size_t>(base + u * S) * l.stride];
             const double4 pj = a.pos[j];
             const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
             const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
@@ -211,8 +253,8 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
     chunk_range(nchunks, c0, c1);
     const int kmax = t.ntab2 - 2;
     for (int c = c0; c < c1; ++c) {
-        const int gt = c * blockDim.x + threadIdx.x;
-        const int i = gt / TPA, sub = gt % TPA;
+        int i, sub;
+        atom_of_lane<TPA>(c, i, sub);
         const bool have = i < a.n;
         const bool active = have && (a.statu[i] & STATU_ACTIVE) == STATU_ACTIVE;
         double den0 = 0.0;
@@ -226,16 +268,30 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
             const int* __restrict__ li = l.indi + list_base(i, l.mx);
             // full groups of kGroup slots, then groups of 4 and single slots for the tail (the
             // slots of a lane are visited in increasing order either way)
-            int base = sub;
-            for (; base + (kGroup - 1) * TPA < kv; base += TPA * kGroup)
-                den0 = density_group<kGroup, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy,
-                                                      pz, den0);
-            for (; base + 3 * TPA < kv; base += TPA * 4)
-                den0 = density_group<4, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy, pz,
-                                                 den0);
-            for (; base < kv; base += TPA)
-                den0 = density_group<1, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy, pz,
-                                                 den0);
+            if constexpr (TPA == 3) {
+                // the two lanes of a pair run the same groups (their gathers are shared)
+                int base = 0;
+                for (; base + 2 * kGroup <= kv; base += 2 * kGroup)
+                    den0 = density_group<kGroup, MT, 3>(a, l, t, b, w, s_pb, li, i, ti, kv,
+                                                        base + sub, pxy, pz, den0, sub);
+                for (; base + 8 <= kv; base += 8)
+                    den0 = density_group<4, MT, 3>(a, l, t, b, w, s_pb, li, i, ti, kv, base + sub,
+                                                   pxy, pz, den0, sub);
+                for (; base < kv; base += 2)
+                    den0 = density_group<1, MT, 3>(a, l, t, b, w, s_pb, li, i, ti, kv, base + sub,
+                                                   pxy, pz, den0, sub);
+            } else {
+                int base = sub;
+                for (; base + (kGroup - 1) * TPA < kv; base += TPA * kGroup)
+                    den0 = density_group<kGroup, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base,
+                                                          pxy, pz, den0);
+                for (; base + 3 * TPA < kv; base += TPA * 4)
+                    den0 = density_group<4, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy,
+                                                     pz, den0);
+                for (; base < kv; base += TPA)
+                    den0 = density_group<1, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy,
+                                                     pz, den0);
+            }
         }
         den0 = lane_sum<TPA>(den0);
         if (sub != 0 || !have) continue;
@@ -276,8 +332,8 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
     chunk_range(nchunks, c0, c1);
     const int kmax = t.ntab2 - 2;
     for (int c = c0; c < c1; ++c) {
-        const int gt = c * blockDim.x + threadIdx.x;
-        const int i = gt / TPA, sub = gt % TPA;
+        int i, sub;
+        atom_of_lane<TPA>(c, i, sub);
         const bool have = i < a.n;
         const int st = have ? a.statu[i] : 0;
         const bool active = have && (st & STATU_ACTIVE) == STATU_ACTIVE;
@@ -289,16 +345,30 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
             const double denki = pi.w;
             const int kv = l.kvois[i];
             const int* __restrict__ li = l.indi + list_base(i, l.mx);
-            int base = sub;
-            for (; base + (kGroup - 1) * TPA < kv; base += TPA * kGroup)
-                force_group<kGroup, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki,
-                                             fpx, fpy, fpz);
-            for (; base + 3 * TPA < kv; base += TPA * 4)
-                force_group<4, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki, fpx,
-                                        fpy, fpz);
-            for (; base < kv; base += TPA)
-                force_group<1, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki, fpx,
-                                        fpy, fpz);
+            if constexpr (TPA == 3) {
+                // the two lanes of a pair run the same groups (their gathers are shared)
+                int base = 0;
+                for (; base + 2 * kGroup <= kv; base += 2 * kGroup)
+                    force_group<kGroup, MT, 3>(a, l, t, b, w, s_ft, li, i, ti, kv, base + sub, pi,
+                                               denki, fpx, fpy, fpz, sub);
+                for (; base + 8 <= kv; base += 8)
+                    force_group<4, MT, 3>(a, l, t, b, w, s_ft, li, i, ti, kv, base + sub, pi, denki,
+                                          fpx, fpy, fpz, sub);
+                for (; base < kv; base += 2)
+                    force_group<1, MT, 3>(a, l, t, b, w, s_ft, li, i, ti, kv, base + sub, pi, denki,
+                                          fpx, fpy, fpz, sub);
+            } else {
+                int base = sub;
+                for (; base + (kGroup - 1) * TPA < kv; base += TPA * kGroup)
+                    force_group<kGroup, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi,
+                                                 denki, fpx, fpy, fpz);
+                for (; base + 3 * TPA < kv; base += TPA * 4)
+                    force_group<4, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki,
+                                            fpx, fpy, fpz);
+                for (; base < kv; base += TPA)
+                    force_group<1, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki,
+                                            fpx, fpy, fpz);
+            }
         }
         fpx = lane_sum<TPA>(fpx);
         fpy = lane_sum<TPA>(fpy);
@@ -749,7 +819,7 @@ void launch_density_lt(int tpa, const LtShape& sh, const AtomsDev& a, const List
     const LtWin w = lt_window(t, sh.wgs_d, sh.wgs_f);
     const bool mt = t.ngroup > 1;
     const size_t lds = static_cast<size_t>(w.wd) * (mt ? t.nkind : 1) * sizeof(double);
-    const long long th = static_cast<long long>(a.n) * tpa;
+    const long long th = static_cast<long long>(a.n) * (tpa == 3 ? 2 : tpa);
     const int nchunks = static_cast<int>((th + sh.block_d - 1) / sh.block_d);
     const bool eam = t.pot_type != 0;
 #define MSM_DL(MB, E, P)                                                                     \
@@ -764,6 +834,7 @@ void launch_density_lt(int tpa, const LtShape& sh, const AtomsDev& a, const List
     do {                                        \
         if (tpa == 1) MSM_DL(MB, E, 1);         \
         else if (tpa == 2) MSM_DL(MB, E, 2);    \
+        else if (tpa == 3) MSM_DL(MB, E, 3);    \
         else MSM_DL(MB, E, 4);                  \
     } while (0)
     if (multibox) {
@@ -782,7 +853,7 @@ void launch_force_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDe
     const LtWin w = lt_window(t, sh.wgs_d, sh.wgs_f);
     const bool mt = t.ngroup > 1;
     const size_t lds = static_cast<size_t>(w.wf) * (mt ? t.nkind : 1) * sizeof(double2);
-    const long long th = static_cast<long long>(a.n) * tpa;
+    const long long th = static_cast<long long>(a.n) * (tpa == 3 ? 2 : tpa);
     const int nchunks = static_cast<int>((th + sh.block_f - 1) / sh.block_f);
 #define MSM_FL(MB, M, P)                                                                     \
     do {                                                                                     \
@@ -796,6 +867,7 @@ void launch_force_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDe
     do {                                        \
         if (tpa == 1) MSM_FL(MB, M, 1);         \
         else if (tpa == 2) MSM_FL(MB, M, 2);    \
+        else if (tpa == 3) MSM_FL(MB, M, 3);    \
         else MSM_FL(MB, M, 4);                  \
     } while (0)
 #define MSM_FLM(MB)                             \

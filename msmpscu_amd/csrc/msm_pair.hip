@@ -13,53 +13,14 @@
 //
 // Same semantics as msm_tpa.hip (MD_FS/EAM_ForceTable_GPU.F90 PRECALFOR_KERNEL /
 // CALFORCE_KERNEL; MD_DiffScheme_GPU.F90 Correction/Predictor kernels).
-#include "msm_device.h"
 #include "msm_internal.h"
+#include "msm_lanepair.h"
 
 namespace msm {
 
 namespace {
 
 constexpr int kGroup = 8;
-
-// On entry lo (hi) holds, in every lane, the half `lane >> 5` of the record of the low (high)
-// lane of its pair; on exit lo and hi hold halves 0 and 1 of the lane's own record.
-// permlane32_swap(vdst, vsrc) exchanges lanes 32..63 of vdst with lanes 0..31 of vsrc.
-__device__ __forceinline__ void pair_swap(double& lo, double& hi) {
-    const unsigned long long a = static_cast<unsigned long long>(__double_as_longlong(lo));
-    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(hi));
-    const auto w0 = __builtin_amdgcn_permlane32_swap(static_cast<unsigned>(a),
-                                                     static_cast<unsigned>(b), false, false);
-    const auto w1 = __builtin_amdgcn_permlane32_swap(static_cast<unsigned>(a >> 32),
-                                                     static_cast<unsigned>(b >> 32), false, false);
-    lo = __longlong_as_double(static_cast<long long>(
-        (static_cast<unsigned long long>(w1[0]) << 32) | w0[0]));
-    hi = __longlong_as_double(static_cast<long long>(
-        (static_cast<unsigned long long>(w1[1]) << 32) | w0[1]));
-}
-
-// Index of the low lane's and the high lane's value (both lanes receive both).
-__device__ __forceinline__ void pair_both(int v, int& vlo, int& vhi) {
-    const auto w = __builtin_amdgcn_permlane32_swap(static_cast<unsigned>(v),
-                                                    static_cast<unsigned>(v), false, false);
-    vlo = static_cast<int>(w[0]);
-    vhi = static_cast<int>(w[1]);
-}
-
-// Record r (32 bytes) of a double4 array, gathered by the pair: returns halves 0 and 1 of the
-// lane's own record.
-__device__ __forceinline__ void pair_gather(const double4* base, int r, int sub,
-                                            double2& h0, double2& h1) {
-    int rlo, rhi;
-    pair_both(r, rlo, rhi);
-    const double2* p = reinterpret_cast<const double2*>(base);
-    double2 a = p[2 * static_cast<size_t>(rlo) + sub];
-    double2 b = p[2 * static_cast<size_t>(rhi) + sub];
-    pair_swap(a.x, b.x);
-    pair_swap(a.y, b.y);
-    h0 = a;
-    h1 = b;
-}
 
 __device__ __forceinline__ double pair_sum(double v) {
     // lane l < 32 adds its partner's value (lane l + 32) in the order of the two-lane kernels

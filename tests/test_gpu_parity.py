@@ -453,7 +453,8 @@ def test_million_atom_snapshot(orc):
 
 @pytest.mark.parametrize("path,compact,lds", [(0, 0, 0), (1, 0, 0), (1, 1, 0), (2, 0, 0),
                                               (2, 1, 0), (3, 0, 0), (4, 1, 0), (8, 1, 0),
-                                              (1, 0, 1), (2, 0, 1), (4, 0, 1), (0, 0, 1)])
+                                              (1, 0, 1), (2, 0, 1), (4, 0, 1), (0, 0, 1),
+                                              (3, 0, 1)])
 def test_force_paths_agree(orc, p0, path, compact, lds):
     """Every density/force kernel variant (LDS bricks with the table in global memory or in LDS;
     global list with 1-8 lanes per atom, with and without the compacted in-cutoff list, with the
@@ -466,8 +467,8 @@ def test_force_paths_agree(orc, p0, path, compact, lds):
     L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
     L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_COMPACT_LIST, compact), "msm_set_option")
     L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS, lds), "msm_set_option")
-    expect = {0: "brick_table" if lds else "bricks", 3: "lane_pair"}.get(
-        path, "lds_table" if lds else "lanes")
+    expect = {0: "brick_table" if lds else "bricks",
+              3: "lds_table" if lds else "lane_pair"}.get(path, "lds_table" if lds else "lanes")
     assert dev.pass_kernel() == expect
     dev.build_neighbors()
     dev.calc_force()
@@ -499,12 +500,13 @@ def _asym_case(ntab=5000):
     return [box], ctrl, ft
 
 
-@pytest.mark.parametrize("variant", ["pair", "lds", "bt1", "bt2", "bt4"])
+@pytest.mark.parametrize("variant", ["pair", "lds", "lds_pair", "bt1", "bt2", "bt4"])
 @pytest.mark.parametrize("case", ["fs", "eam", "asym", "inactive", "multibox", "slab"])
 def test_kernel_variant_matches_two_lane_path(p0, case, variant):
-    """The lane-pair gather kernels (force path 3) and the LDS-table kernels split each atom's
-    list over two lanes as the two-lane kernels do (the compiler may contract multiply-adds
-    differently, so agreement is to rounding): F' and forces to the parity bar, and positions
+    """The lane-pair gather kernels (force path 3, with the table in global memory or in LDS) and
+    the LDS-table kernels split each atom's list over two lanes as the two-lane kernels do (the
+    compiler may contract multiply-adds differently, so agreement is to rounding): F' and forces
+    to the parity bar, and positions
     after a 12-step run to 1e-12 A, over single and multiple species (the LDS table serves one
     kind; several kinds take the global table), FS and EAM, inactive atoms, uneven boxes and a
     slab.  The bt variants are the LDS-brick passes with the table in LDS (1, 2, 4 lanes per
@@ -533,17 +535,20 @@ def test_kernel_variant_matches_two_lane_path(p0, case, variant):
     outs = []
     for k in range(2):
         dev = device_for(boxes, ctrl, ft)
-        path = 3 if (k == 1 and variant == "pair") else 2
+        path = 3 if (k == 1 and variant in ("pair", "lds_pair")) else 2
         if k == 1 and bt:
             path = 0
             L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_BRICK_LANES, int(variant[2:])),
                     "msm_set_option")
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS,
-                                       int(k == 1 and variant in ("lds", "bt1", "bt2", "bt4"))),
+                                       int(k == 1 and variant in ("lds", "lds_pair", "bt1", "bt2",
+                                                                  "bt4"))),
                 "msm_set_option")
         if k == 1 and bt:
             assert dev.pass_kernel() == "brick_table"
+        if k == 1 and variant == "lds_pair" and case not in ("asym", "slab"):
+            assert dev.pass_kernel() == "lds_table"
         if statu is not None:
             dev.set_atoms(box.XP, box.XP1, box.ITYP, statu)
         dev.build_neighbors()
