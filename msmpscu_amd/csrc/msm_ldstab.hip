@@ -29,6 +29,10 @@ namespace msm {
 namespace {
 
 constexpr int kGroup = 8;
+// Slots per software-pipelined group of the density and force passes (measured: 8 for the force
+// pass spills and is 25% slower than 6).
+constexpr int kGroupD = 6;
+constexpr int kGroupF = 4;
 
 // TPA = 3 stands for two lanes per atom paired as lanes l and l + 32 (lane-pair gathers).
 template <int TPA>
@@ -235,6 +239,167 @@ __device__ __forceinline__ void force_group(const AtomsDev& a, const ListDev& l,
     }
 }
 
+// Software-pipelined full groups.  js holds the neighbour indices of this group (slots base,
+// base + S, ...); the group's position gathers are issued first, then the index loads of the next
+// group (slots base + G*S, ..., into js), and only then does the arithmetic wait for the gathers,
+// so the next group's index latency hides behind this group's gathers and arithmetic instead of
+// adding a second round trip.  Slots at or past kv read the atom itself and add nothing.
+template <int G, int TPA>
+__device__ __forceinline__ void load_slots(const int* __restrict__ li, int stride, int base,
+                                           int kv, int self, int (&js)[G]) {
+    constexpr int S = slot_step<TPA>();
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int iw = base + u * S;
+        js[u] = iw < kv ? li[static_cast<size_t>(iw) * stride] : self;
+    }
+}
+
+template <int G, int MT, int TPA>
+__device__ __forceinline__ double density_group_pf(const AtomsDev& a, const ListDev& l,
+                                                   const TabDev& t, const BoxDev& b,
+                                                   const LtWin& w, const double* s_pb,
+                                                   const int* __restrict__ li, int i, int ti,
+                                                   int kv, int base, double2 pxy, double pz,
+                                                   double den0, int (&js)[G]) {
+    constexpr int S = slot_step<TPA>();
+    const int kmax = t.ntab2 - 2;
+    double2 qxy[G];
+    double qz[G];
+    int tj[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        qxy[u] = *reinterpret_cast<const double2*>(&a.pos[js[u]]);
+        qz[u] = a.pos[js[u]].z;
+        tj[u] = MT ? a.ityp[js[u]] : 0;
+    }
+    load_slots<G, TPA>(li, l.stride, base + G * S, kv, i, js);
+    unsigned fix = 0;
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const double sx = min_image(pxy.x - qxy[u].x, b.len[0], b.ilen[0]);
+        const double sy = min_image(pxy.y - qxy[u].y, b.len[1], b.ilen[1]);
+        const double sz = min_image(pz - qz[u], b.len[2], b.ilen[2]);
+        const double r2 = sx * sx + sy * sy + sz * sz;
+        const int row = MT ? t.kpair[ti * t.ngroup + tj[u]] * w.wd : 0;
+        double inv_r;
+        const double sk = table_coord(r2, t.csi, inv_r);
+        const int kk = min(static_cast<int>(sk), kmax);
+        const bool in = base + u * S < kv && r2 <= t.ru2max;
+        const int kl = kk - w.k0d;
+        if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+        const int q = row + max(kl, 0);
+        const double p0 = s_pb[q], p1 = s_pb[q + 1];
+        const double v = p0 + (sk - kk) * (p1 - p0);
+        den0 = den0 + ((in && kl >= 0) ? v : 0.0);
+    }
+    if (fix) {
+        // rare close pairs: reload the neighbour (keeping the group's records live would cost
+        // registers) and take the knots of the global table
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (!(fix & (1u << u))) continue;
+            const int j = li[static_cast<size_t>(base + u * S) * l.stride];
+            const double2 q2 = *reinterpret_cast<const double2*>(&a.pos[j]);
+            const double sx = min_image(pxy.x - q2.x, b.len[0], b.ilen[0]);
+            const double sy = min_image(pxy.y - q2.y, b.len[1], b.ilen[1]);
+            const double sz = min_image(pz - a.pos[j].z, b.len[2], b.ilen[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
+            double inv_r;
+            const double sk = table_coord(r2, t.csi, inv_r);
+            const int kk = min(static_cast<int>(sk), kmax);
+            const double2 e = t.b2[static_cast<size_t>(kind) * t.ntab2 + kk];
+            den0 = den0 + (e.x + (sk - kk) * e.y);
+        }
+    }
+    return den0;
+}
+
+template <int G, int MT, int TPA>
+__device__ __forceinline__ void force_group_pf(const AtomsDev& a, const ListDev& l,
+                                               const TabDev& t, const BoxDev& b, const LtWin& w,
+                                               const double2* s_ft, const int* __restrict__ li,
+                                               int i, int ti, int kv, int base, const double4& pi,
+                                               double denki, double& fpx, double& fpy,
+                                               double& fpz, int (&js)[G]) {
+    constexpr int S = slot_step<TPA>();
+    const int kmax = t.ntab2 - 2;
+    double4 pj[G];
+    int tj[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        pj[u] = a.pos[js[u]];
+        tj[u] = MT ? a.ityp[js[u]] : 0;
+    }
+    load_slots<G, TPA>(li, l.stride, base + G * S, kv, i, js);
+    unsigned fix = 0;
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const double sx = min_image(pi.x - pj[u].x, b.len[0], b.ilen[0]);
+        const double sy = min_image(pi.y - pj[u].y, b.len[1], b.ilen[1]);
+        const double sz = min_image(pi.z - pj[u].z, b.len[2], b.ilen[2]);
+        const double r2 = sx * sx + sy * sy + sz * sz;
+        int row = 0, row1 = 0;
+        if constexpr (MT) {
+            row = t.kpair[ti * t.ngroup + tj[u]] * w.wf;
+            row1 = t.kpair[tj[u] * t.ngroup + ti] * w.wf;
+        }
+        double inv_r;
+        const double sk = table_coord(r2, t.csi, inv_r);
+        const int kk = min(static_cast<int>(sk), kmax);
+        const double dk = sk - kk;
+        const bool in = base + u * S < kv && r2 <= t.ru2max;
+        const int kl = kk - w.k0f;
+        if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+        const int q = max(kl, 0);
+        const double2 t0 = s_ft[row + q], t1 = s_ft[row + q + 1];
+        const double fb = t0.y + dk * (t1.y - t0.y);
+        double fb1 = fb;
+        if constexpr (MT) {
+            const double b0 = s_ft[row1 + q].y, b1 = s_ft[row1 + q + 1].y;
+            fb1 = b0 + dk * (b1 - b0);
+        }
+        // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL)
+        const double ft = (t0.x + dk * (t1.x - t0.x)) * (inv_r * inv_r) +
+                          (fb * denki + fb1 * pj[u].w) * inv_r;
+        const double fortot = (in && kl >= 0) ? ft : 0.0;
+        fpx = fpx + fortot * sx;
+        fpy = fpy + fortot * sy;
+        fpz = fpz + fortot * sz;
+    }
+    if (fix) {
+        // rare close pairs: reload the neighbour and take the knots of the global table
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (!(fix & (1u << u))) continue;
+            const int j = li[static_cast<size_t>(base + u * S) * l.stride];
+            const double4 pk = a.pos[j];
+            const double sx = min_image(pi.x - pk.x, b.len[0], b.ilen[0]);
+            const double sy = min_image(pi.y - pk.y, b.len[1], b.ilen[1]);
+            const double sz = min_image(pi.z - pk.z, b.len[2], b.ilen[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            int kind = t.kpair[0], kind1 = t.kpair[0];
+            if constexpr (MT) {
+                const int tk = a.ityp[j];
+                kind = t.kpair[ti * t.ngroup + tk];
+                kind1 = t.kpair[tk * t.ngroup + ti];
+            }
+            double inv_r;
+            const double sk = table_coord(r2, t.csi, inv_r);
+            const int kk = min(static_cast<int>(sk), kmax);
+            const double dk = sk - kk;
+            const double4 f = t.f4[static_cast<size_t>(kind) * t.ntab2 + kk];
+            const double4 f1 = t.f4[static_cast<size_t>(kind1) * t.ntab2 + kk];
+            const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
+                                  ((f.z + dk * f.w) * denki + (f1.z + dk * f1.w) * pk.w) * inv_r;
+            fpx = fpx + fortot * sx;
+            fpy = fpy + fortot * sy;
+            fpz = fpz + fortot * sz;
+        }
+    }
+}
+
 // Knot rows in LDS: kind-major, w knots per kind from knot k0 (all kinds of the table for
 // several atom types, the one kind in use for a single type).
 template <int MB, int MT, int EAM, int TPA>
@@ -282,9 +447,11 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
                                                    pxy, pz, den0, sub);
             } else {
                 int base = sub;
-                for (; base + (kGroup - 1) * TPA < kv; base += TPA * kGroup)
-                    den0 = density_group<kGroup, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base,
-                                                          pxy, pz, den0);
+                int js[kGroupD];
+                load_slots<kGroupD, TPA>(li, l.stride, base, kv, i, js);
+                for (; base + (kGroupD - 1) * TPA < kv; base += TPA * kGroupD)
+                    den0 = density_group_pf<kGroupD, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv,
+                                                              base, pxy, pz, den0, js);
                 for (; base + 3 * TPA < kv; base += TPA * 4)
                     den0 = density_group<4, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy,
                                                      pz, den0);
@@ -359,9 +526,11 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
                                           fpx, fpy, fpz, sub);
             } else {
                 int base = sub;
-                for (; base + (kGroup - 1) * TPA < kv; base += TPA * kGroup)
-                    force_group<kGroup, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi,
-                                                 denki, fpx, fpy, fpz);
+                int js[kGroupF];
+                load_slots<kGroupF, TPA>(li, l.stride, base, kv, i, js);
+                for (; base + (kGroupF - 1) * TPA < kv; base += TPA * kGroupF)
+                    force_group_pf<kGroupF, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi,
+                                                     denki, fpx, fpy, fpz, js);
                 for (; base + 3 * TPA < kv; base += TPA * 4)
                     force_group<4, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki,
                                             fpx, fpy, fpz);
