@@ -86,6 +86,7 @@ struct msm_sim {
     unsigned short* sidx = nullptr;
     int* cidx = nullptr;  // in-cutoff pairs of the current step (density -> force pass)
     int* ckv = nullptr;
+    unsigned long long* cmask = nullptr;  // in-cutoff slot masks per lane (LDS-table passes)
     size_t indi_cap = 0;
     bool has_list = false;
 
@@ -207,14 +208,19 @@ AtomsDev atoms_view(msm_sim* s, int set, int pcur) {
     return a;
 }
 
+bool use_lds_table(const msm_sim* s);
+
 ListDev list_view(msm_sim* s) {
     ListDev l{};
     l.kvois = s->kvois;
     l.indi = s->indi;
     l.sidx = s->sidx;
-    const bool cmp = s->compact && s->force_path > 0 && s->force_path <= 4 && s->force_path != 3;
+    const bool lt = use_lds_table(s);
+    const bool cmp = !lt && s->compact && s->force_path > 0 && s->force_path <= 4 &&
+                     s->force_path != 3;
     l.cidx = cmp ? s->cidx : nullptr;
     l.ckv = cmp ? s->ckv : nullptr;
+    l.cmask = (lt && s->compact) ? s->cmask : nullptr;
     l.stride = msm::kListRow;
     l.mx = s->mx;
     l.scap = (s->nlist_kind == 1 || !s->list_sidx) ? 0 : s->nl_wide ? s->nl_cap : msm::kNlCapPass;
@@ -487,8 +493,12 @@ int plan_bricks(msm_sim* s, const double* xp, int n) {
 // gathers (path 3), no compaction, and an LDS window that holds at least the upper half of every
 // table kind in use.
 bool use_lds_table(const msm_sim* s) {
-    if (!s->table_lds || s->compact) return false;
+    if (!s->table_lds) return false;
     if (s->force_path < 1 || s->force_path > 4) return false;  // 3: lane-pair gathers
+    // compaction: per-lane 64-bit masks of in-cutoff slots (not with lane pairs, whose two
+    // lanes walk their slots together)
+    if (s->compact && (s->force_path == 3 || (s->mx + s->force_path - 1) / s->force_path > 64))
+        return false;
     const msm::LtWin w = msm::lt_window(s->tab, s->lt.wgs_d, s->lt.wgs_f);
     return 2 * w.wf >= s->tab.ntab2 && 2 * w.wd >= s->tab.ntab2;
 }
@@ -693,6 +703,7 @@ int msm_destroy(msm_sim* s) {
     dfree(&s->sidx);
     dfree(&s->cidx);
     dfree(&s->ckv);
+    dfree(&s->cmask);
     dfree(&s->d_err);
     dfree(&s->d_ebox);
     dfree(&s->d_stats);
@@ -930,6 +941,7 @@ int msm_set_atoms(msm_sim* s, int n, const double* xp, const double* xp1, const 
     if (rc == MSM_OK) rc = dalloc(s, &s->sidx, icap);
     if (rc == MSM_OK) rc = dalloc(s, &s->cidx, icap);
     if (rc == MSM_OK) rc = dalloc(s, &s->ckv, static_cast<size_t>(na) * 8);
+    if (rc == MSM_OK) rc = dalloc(s, &s->cmask, static_cast<size_t>(na) * 4);
     if (rc != MSM_OK) {
         free_atoms(s);
         return rc;

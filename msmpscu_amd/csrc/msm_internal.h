@@ -76,6 +76,9 @@ struct ListDev {
     const unsigned short* sidx; // neighbour LDS slot in the home brick's halo (brick kernels)
     int* cidx;                  // in-cutoff pairs of the current step (global-list path), or null
     int* ckv;                   // their count per lane (n_alloc * lanes per atom)
+    unsigned long long* cmask;  // LDS-table passes: per lane (atom * lanes + lane), bit k set if
+                                // the lane's k-th slot is inside the cutoff this step (written
+                                // by the density pass for the force pass), or null
     int stride;                 // distance between consecutive slots of one atom (kListRow)
     int mx;                     // slots per atom (MXKVOIS)
     int scap;                   // brick passes stage a brick (and read sidx) only if its halo

@@ -82,9 +82,11 @@ __device__ __forceinline__ double density_group(const AtomsDev& a, const ListDev
                                                 const TabDev& t, const BoxDev& b, const LtWin& w,
                                                 const double* s_pb, const int* __restrict__ li,
                                                 int i, int ti, int kv, int base, double2 pxy,
-                                                double pz, double den0, int sub = 0) {
+                                                double pz, double den0, int sub = 0,
+                                                unsigned long long* mk = nullptr) {
     constexpr int S = slot_step<TPA>();
     const int kmax = t.ntab2 - 2;
+    const int k0 = base / S;  // the lane's slot number of u = 0 (base = sub + k0 * S)
     int js[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
@@ -115,6 +117,7 @@ __device__ __forceinline__ double density_group(const AtomsDev& a, const ListDev
         const double sk = table_coord(r2, t.csi, inv_r);
         const int kk = min(static_cast<int>(sk), kmax);
         const bool in = base + u * S < kv && r2 <= t.ru2max;
+        if (mk && in) *mk |= 1ull << (k0 + u);
         const int kl = kk - w.k0d;
         if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
         const int q = row + max(kl, 0);
@@ -261,9 +264,11 @@ __device__ __forceinline__ double density_group_pf(const AtomsDev& a, const List
                                                    const LtWin& w, const double* s_pb,
                                                    const int* __restrict__ li, int i, int ti,
                                                    int kv, int base, double2 pxy, double pz,
-                                                   double den0, int (&js)[G]) {
+                                                   double den0, int (&js)[G],
+                                                   unsigned long long* mk = nullptr) {
     constexpr int S = slot_step<TPA>();
     const int kmax = t.ntab2 - 2;
+    const int k0 = base / S;
     double2 qxy[G];
     double qz[G];
     int tj[G];
@@ -286,6 +291,7 @@ __device__ __forceinline__ double density_group_pf(const AtomsDev& a, const List
         const double sk = table_coord(r2, t.csi, inv_r);
         const int kk = min(static_cast<int>(sk), kmax);
         const bool in = base + u * S < kv && r2 <= t.ru2max;
+        if (mk && in) *mk |= 1ull << (k0 + u);
         const int kl = kk - w.k0d;
         if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
         const int q = row + max(kl, 0);
@@ -400,6 +406,117 @@ __device__ __forceinline__ void force_group_pf(const AtomsDev& a, const ListDev&
     }
 }
 
+// Force terms over the slots the density pass found inside the cutoff: m holds the lane's
+// in-cutoff slot numbers as bits (bit k = slot sub + k * S); groups of G of them, software
+// pipelined as force_group_pf.  The pairs skipped are those whose term is zero, so the sums are
+// those of the full walk.
+template <int G, int TPA>
+__device__ __forceinline__ void pop_slots(const int* __restrict__ li, int stride, int sub,
+                                          unsigned long long& m, int self, int (&js)[G],
+                                          int (&ks)[G]) {
+    constexpr int S = slot_step<TPA>();
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        ks[u] = -1;
+        js[u] = self;
+        if (m) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1;
+            ks[u] = sub + k * S;
+            js[u] = li[static_cast<size_t>(ks[u]) * stride];
+        }
+    }
+}
+
+template <int G, int MT, int TPA>
+__device__ __forceinline__ void force_masked(const AtomsDev& a, const ListDev& l, const TabDev& t,
+                                             const BoxDev& b, const LtWin& w,
+                                             const double2* s_ft, const int* __restrict__ li,
+                                             int i, int ti, int sub, unsigned long long m,
+                                             const double4& pi, double denki, double& fpx,
+                                             double& fpy, double& fpz) {
+    const int kmax = t.ntab2 - 2;
+    int js[G], ks[G];
+    pop_slots<G, TPA>(li, l.stride, sub, m, i, js, ks);
+    while (ks[0] >= 0) {
+        double4 pj[G];
+        int tj[G], kc[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            pj[u] = a.pos[js[u]];
+            tj[u] = MT ? a.ityp[js[u]] : 0;
+            kc[u] = ks[u];
+        }
+        pop_slots<G, TPA>(li, l.stride, sub, m, i, js, ks);
+        unsigned fix = 0;
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const double sx = min_image(pi.x - pj[u].x, b.len[0], b.ilen[0]);
+            const double sy = min_image(pi.y - pj[u].y, b.len[1], b.ilen[1]);
+            const double sz = min_image(pi.z - pj[u].z, b.len[2], b.ilen[2]);
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            int row = 0, row1 = 0;
+            if constexpr (MT) {
+                row = t.kpair[ti * t.ngroup + tj[u]] * w.wf;
+                row1 = t.kpair[tj[u] * t.ngroup + ti] * w.wf;
+            }
+            double inv_r;
+            const double sk = table_coord(r2, t.csi, inv_r);
+            const int kk = min(static_cast<int>(sk), kmax);
+            const double dk = sk - kk;
+            const bool in = kc[u] >= 0 && r2 <= t.ru2max;
+            const int kl = kk - w.k0f;
+            if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+            const int q = max(kl, 0);
+            const double2 t0 = s_ft[row + q], t1 = s_ft[row + q + 1];
+            const double fb = t0.y + dk * (t1.y - t0.y);
+            double fb1 = fb;
+            if constexpr (MT) {
+                const double b0 = s_ft[row1 + q].y, b1 = s_ft[row1 + q + 1].y;
+                fb1 = b0 + dk * (b1 - b0);
+            }
+            // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL)
+            const double ft = (t0.x + dk * (t1.x - t0.x)) * (inv_r * inv_r) +
+                              (fb * denki + fb1 * pj[u].w) * inv_r;
+            const double fortot = (in && kl >= 0) ? ft : 0.0;
+            fpx = fpx + fortot * sx;
+            fpy = fpy + fortot * sy;
+            fpz = fpz + fortot * sz;
+        }
+        if (fix) {
+            // rare close pairs: reload the neighbour and take the knots of the global table
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (!(fix & (1u << u))) continue;
+                const int j = li[static_cast<size_t>(kc[u]) * l.stride];
+                const double4 pk = a.pos[j];
+                const double sx = min_image(pi.x - pk.x, b.len[0], b.ilen[0]);
+                const double sy = min_image(pi.y - pk.y, b.len[1], b.ilen[1]);
+                const double sz = min_image(pi.z - pk.z, b.len[2], b.ilen[2]);
+                const double r2 = sx * sx + sy * sy + sz * sz;
+                int kind = t.kpair[0], kind1 = t.kpair[0];
+                if constexpr (MT) {
+                    const int tk = a.ityp[j];
+                    kind = t.kpair[ti * t.ngroup + tk];
+                    kind1 = t.kpair[tk * t.ngroup + ti];
+                }
+                double inv_r;
+                const double sk = table_coord(r2, t.csi, inv_r);
+                const int kk = min(static_cast<int>(sk), kmax);
+                const double dk = sk - kk;
+                const double4 f = t.f4[static_cast<size_t>(kind) * t.ntab2 + kk];
+                const double4 f1 = t.f4[static_cast<size_t>(kind1) * t.ntab2 + kk];
+                const double fortot =
+                    (f.x + dk * f.y) * (inv_r * inv_r) +
+                    ((f.z + dk * f.w) * denki + (f1.z + dk * f1.w) * pk.w) * inv_r;
+                fpx = fpx + fortot * sx;
+                fpy = fpy + fortot * sy;
+                fpz = fpz + fortot * sz;
+            }
+        }
+    }
+}
+
 // Knot rows in LDS: kind-major, w knots per kind from knot k0 (all kinds of the table for
 // several atom types, the one kind in use for a single type).
 template <int MB, int MT, int EAM, int TPA>
@@ -448,16 +565,18 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
             } else {
                 int base = sub;
                 int js[kGroupD];
+                unsigned long long mk = 0;  // this lane's in-cutoff slots (for the force pass)
                 load_slots<kGroupD, TPA>(li, l.stride, base, kv, i, js);
                 for (; base + (kGroupD - 1) * TPA < kv; base += TPA * kGroupD)
                     den0 = density_group_pf<kGroupD, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv,
-                                                              base, pxy, pz, den0, js);
+                                                              base, pxy, pz, den0, js, &mk);
                 for (; base + 3 * TPA < kv; base += TPA * 4)
                     den0 = density_group<4, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy,
-                                                     pz, den0);
+                                                     pz, den0, sub, &mk);
                 for (; base < kv; base += TPA)
                     den0 = density_group<1, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy,
-                                                     pz, den0);
+                                                     pz, den0, sub, &mk);
+                if (l.cmask) l.cmask[static_cast<size_t>(i) * TPA + sub] = mk;
             }
         }
         den0 = lane_sum<TPA>(den0);
@@ -524,6 +643,10 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
                 for (; base < kv; base += 2)
                     force_group<1, MT, 3>(a, l, t, b, w, s_ft, li, i, ti, kv, base + sub, pi, denki,
                                           fpx, fpy, fpz, sub);
+            } else if (l.cmask) {
+                force_masked<kGroupF, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, sub,
+                                               l.cmask[static_cast<size_t>(i) * TPA + sub], pi,
+                                               denki, fpx, fpy, fpz);
             } else {
                 int base = sub;
                 int js[kGroupF];

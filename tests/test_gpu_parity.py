@@ -454,11 +454,12 @@ def test_million_atom_snapshot(orc):
 @pytest.mark.parametrize("path,compact,lds", [(0, 0, 0), (1, 0, 0), (1, 1, 0), (2, 0, 0),
                                               (2, 1, 0), (3, 0, 0), (4, 1, 0), (8, 1, 0),
                                               (1, 0, 1), (2, 0, 1), (4, 0, 1), (0, 0, 1),
-                                              (3, 0, 1)])
+                                              (3, 0, 1), (1, 1, 1), (2, 1, 1), (4, 1, 1)])
 def test_force_paths_agree(orc, p0, path, compact, lds):
     """Every density/force kernel variant (LDS bricks with the table in global memory or in LDS;
-    global list with 1-8 lanes per atom, with and without the compacted in-cutoff list, with the
-    table in global memory or in LDS) meets the parity bar on the same box, before and after a
+    global list with 1-8 lanes per atom, with and without the density pass's in-cutoff record for
+    the force pass -- a compacted list with the global table, per-lane slot masks with the LDS
+    table -- and lane-pair gathers) meets the parity bar on the same box, before and after a
     15-step run."""
     import msmpscu_amd._lib as L
     box, ctrl, ft = p0
@@ -500,7 +501,7 @@ def _asym_case(ntab=5000):
     return [box], ctrl, ft
 
 
-@pytest.mark.parametrize("variant", ["pair", "lds", "lds_pair", "bt1", "bt2", "bt4"])
+@pytest.mark.parametrize("variant", ["pair", "lds", "lds_pair", "lds_mask", "bt1", "bt2", "bt4"])
 @pytest.mark.parametrize("case", ["fs", "eam", "asym", "inactive", "multibox", "slab"])
 def test_kernel_variant_matches_two_lane_path(p0, case, variant):
     """The lane-pair gather kernels (force path 3, with the table in global memory or in LDS) and
@@ -542,12 +543,14 @@ def test_kernel_variant_matches_two_lane_path(p0, case, variant):
                     "msm_set_option")
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS,
-                                       int(k == 1 and variant in ("lds", "lds_pair", "bt1", "bt2",
-                                                                  "bt4"))),
+                                       int(k == 1 and variant in ("lds", "lds_pair", "lds_mask",
+                                                                  "bt1", "bt2", "bt4"))),
                 "msm_set_option")
+        L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_COMPACT_LIST,
+                                       int(k == 1 and variant == "lds_mask")), "msm_set_option")
         if k == 1 and bt:
             assert dev.pass_kernel() == "brick_table"
-        if k == 1 and variant == "lds_pair" and case not in ("asym", "slab"):
+        if k == 1 and variant in ("lds_pair", "lds_mask") and case not in ("asym", "slab"):
             assert dev.pass_kernel() == "lds_table"
         if statu is not None:
             dev.set_atoms(box.XP, box.XP1, box.ITYP, statu)
