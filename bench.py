@@ -213,6 +213,8 @@ def main():
                     help="LDS-table passes: density block,wgs,force block,wgs (e.g. 1024,1,1024,1)")
     ap.add_argument("--nlist-kernel", type=int, default=None,
                     help="list build: 0 automatic, 1 global scan, 2 small bricks, 3 wide bricks")
+    ap.add_argument("--force-lanes", type=int, default=None,
+                    help="lanes per atom of the LDS-table force pass (0: as --force-path)")
     ap.add_argument("--brick-lanes", type=int, default=None,
                     help="lanes per home atom of the LDS-brick passes with the table in LDS")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch measurement")
@@ -283,6 +285,9 @@ def main():
                    "msm_set_option")
     if args.nlist_kernel is not None:
         _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_NLIST_KERNEL, args.nlist_kernel),
+                   "msm_set_option")
+    if args.force_lanes is not None:
+        _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_FORCE_LANES, args.force_lanes),
                    "msm_set_option")
     if args.brick_lanes is not None:
         _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_BRICK_LANES, args.brick_lanes),

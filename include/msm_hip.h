@@ -224,6 +224,9 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
 /* Lanes per home atom (1 default, 2 or 4) of the LDS-brick passes with the table in LDS. */
 #define MSM_OPT_BRICK_LANES      12
 int msm_set_option(msm_sim* sim, int option, int value);
+/* Lanes per atom (1, 2, 4) of the LDS-table force pass when it should differ from the density
+ * pass's (MSM_OPT_FORCE_PATH); 0 (default) follows MSM_OPT_FORCE_PATH. */
+#define MSM_OPT_FORCE_LANES      14
 /* Read-only: the density/force kernels the current options and list select (MSM_PASS_*). */
 #define MSM_OPT_PASS_KERNEL      13
 #define MSM_PASS_BRICKS      0  /* LDS-staged positions, global table */

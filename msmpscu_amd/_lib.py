@@ -104,6 +104,7 @@ MSM_OPT_LT_FORCE_WGS = 10
 MSM_OPT_CELL_BRICK = 11
 MSM_OPT_BRICK_LANES = 12
 MSM_OPT_PASS_KERNEL = 13
+MSM_OPT_FORCE_LANES = 14
 PASS_KERNELS = ["bricks", "global", "lanes", "lane_pair", "lds_table", "brick_table"]
 KCLASS = ["density", "force", "rebuild", "predictor", "nlist"]
 

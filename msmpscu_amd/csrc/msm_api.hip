@@ -105,6 +105,7 @@ struct msm_sim {
                            // 3: two lanes per atom with lane-pair gathers
     int table_lds = 1;     // table knots in LDS (global-list paths 1, 2, 4; brick path 0)
     int bt_lanes = 1;      // lanes per home atom of the brick passes with the table in LDS
+    int force_lanes = 0;   // lanes per atom of the LDS-table force pass (0: as force_path)
     int cell_brick = 1;    // cells are numbered in bricks of cell_brick^3 (1: row-major)
     msm::LtShape lt{1024, 1, 1024, 1};  // block / workgroups per CU of the LDS-table passes
     int compact = 0;       // global-list path: force pass reads the density pass's in-cutoff list
@@ -492,12 +493,19 @@ int plan_bricks(msm_sim* s, const double* xp, int n) {
 // LDS-table passes: global list with 1, 2 or 4 lanes per atom or two lanes paired for 32-byte
 // gathers (path 3), no compaction, and an LDS window that holds at least the upper half of every
 // table kind in use.
+// Lanes per atom of the LDS-table force pass: MSM_OPT_FORCE_LANES when set (not with lane pairs),
+// else those of the density pass.
+int lt_force_lanes(const msm_sim* s) {
+    return (s->force_lanes > 0 && s->force_path != 3) ? s->force_lanes : s->force_path;
+}
+
 bool use_lds_table(const msm_sim* s) {
     if (!s->table_lds) return false;
     if (s->force_path < 1 || s->force_path > 4) return false;  // 3: lane-pair gathers
-    // compaction: per-lane 64-bit masks of in-cutoff slots (not with lane pairs, whose two
-    // lanes walk their slots together)
-    if (s->compact && (s->force_path == 3 || (s->mx + s->force_path - 1) / s->force_path > 64))
+    // compaction: per-lane 64-bit masks of in-cutoff slots, written by the density pass for the
+    // same lanes of the force pass (not with lane pairs, whose two lanes walk their slots together)
+    if (s->compact && (s->force_path == 3 || lt_force_lanes(s) != s->force_path ||
+                       (s->mx + s->force_path - 1) / s->force_path > 64))
         return false;
     const msm::LtWin w = msm::lt_window(s->tab, s->lt.wgs_d, s->lt.wgs_f);
     return 2 * w.wf >= s->tab.ntab2 && 2 * w.wd >= s->tab.ntab2;
@@ -553,7 +561,7 @@ void enqueue_force(msm_sim* s, int mode, int write_f, double h) {
         msm::launch_force_bt(s->bt_lanes, s->lt, s->nbricks, s->d_brick_box, a, list_view(s),
                              s->tab, s->d_boxes, s->sb.start, s->ncell_total, sp, s->stream);
     else if (use_lds_table(s))
-        msm::launch_force_lt(s->force_path, s->lt, a, list_view(s), s->tab, s->boxes[0],
+        msm::launch_force_lt(lt_force_lanes(s), s->lt, a, list_view(s), s->tab, s->boxes[0],
                              s->d_boxes, multibox(s), sp, s->stream);
     else if (s->force_path == 3)
         msm::launch_force_pair(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), sp,
@@ -1280,6 +1288,12 @@ int msm_set_option(msm_sim* s, int option, int value) {
             s->bt_lanes = value;
             clear_graphs(s);
             return MSM_OK;
+        case MSM_OPT_FORCE_LANES:
+            if (value != 0 && value != 1 && value != 2 && value != 4)
+                return set_err(s, MSM_ERR_ARG, "force lanes must be 0, 1, 2 or 4");
+            s->force_lanes = value;
+            clear_graphs(s);
+            return MSM_OK;
         case MSM_OPT_LT_DENSITY_BLOCK:
         case MSM_OPT_LT_FORCE_BLOCK:
             if (value != 256 && value != 512 && value != 1024)
@@ -1324,6 +1338,7 @@ int msm_get_option(msm_sim* s, int option, int* value) {
         case MSM_OPT_LT_FORCE_WGS: *value = s->lt.wgs_f; return MSM_OK;
         case MSM_OPT_CELL_BRICK: *value = s->cell_brick; return MSM_OK;
         case MSM_OPT_BRICK_LANES: *value = s->bt_lanes; return MSM_OK;
+        case MSM_OPT_FORCE_LANES: *value = s->force_lanes; return MSM_OK;
         case MSM_OPT_PASS_KERNEL:
             if (!s->has_table) return set_err(s, MSM_ERR_STATE, "no force table");
             *value = pass_kernel(s);
@@ -1355,7 +1370,7 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     if (reps < 1 || reps > 1000) return set_err(s, MSM_ERR_ARG, "reps must be 1..1000");
     (void)hipSetDevice(s->device);
     struct Cand {
-        int fp, lds, bd, wd, bf, wf, lanes;
+        int fp, lds, bd, wd, bf, wf, lanes, flanes = 0;
     };
     std::vector<Cand> cands;
     if (msm::lt_window(s->tab, 1, 1).wf * 2 >= s->tab.ntab2) {
@@ -1364,6 +1379,7 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
             cands.push_back({fp, 1, 256, 4, 256, 4, 1});
         }
         cands.push_back({2, 1, 512, 2, 512, 2, 1});
+        cands.push_back({2, 1, 1024, 1, 1024, 1, 1, 1});  // force pass with one lane per atom
     }
     for (int fp : {2, 1, 3, 4, 8, 0}) cands.push_back({fp, 0, s->lt.block_d, s->lt.wgs_d,
                                                        s->lt.block_f, s->lt.wgs_f, 1});
@@ -1421,6 +1437,7 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
         s->table_lds = cands[c].lds;
         s->lt = msm::LtShape{cands[c].bd, cands[c].wd, cands[c].bf, cands[c].wf};
         s->bt_lanes = cands[c].lanes;
+        s->force_lanes = cands[c].flanes;
         float ms = 0.f;
         rc = time_it([&] {
             enqueue_density(s);
@@ -1441,6 +1458,7 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     s->table_lds = w.lds;
     s->lt = msm::LtShape{w.bd, w.wd, w.bf, w.wf};
     s->bt_lanes = w.lanes;
+    s->force_lanes = w.flanes;
     clear_graphs(s);
     // leave F' and F as the chosen kernels compute them
     enqueue_density(s);

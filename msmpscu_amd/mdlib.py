@@ -426,9 +426,10 @@ class DeviceMD:
         ch = np.zeros(8, dtype=np.int32)
         ms = np.zeros(2)
         check(lib().msm_autotune(self.h, int(reps), iptr(ch), dptr(ms)), "msm_autotune", self.h)
+        from ._lib import MSM_OPT_FORCE_LANES
         return {"force_path": int(ch[0]), "table_lds": int(ch[1]),
                 "lt_shape": [int(v) for v in ch[2:6]], "nlist_kernel": int(ch[6]),
-                "brick_lanes": int(ch[7]),
+                "brick_lanes": int(ch[7]), "force_lanes": self.get_option(MSM_OPT_FORCE_LANES),
                 "density_force_ms": float(ms[0]), "rebuild_ms": float(ms[1])}
 
     def kernel_times(self, reset: bool = True) -> dict:

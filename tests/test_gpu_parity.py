@@ -501,7 +501,8 @@ def _asym_case(ntab=5000):
     return [box], ctrl, ft
 
 
-@pytest.mark.parametrize("variant", ["pair", "lds", "lds_pair", "lds_mask", "bt1", "bt2", "bt4"])
+@pytest.mark.parametrize("variant", ["pair", "lds", "lds_pair", "lds_mask", "lds_f1", "bt1", "bt2",
+                                     "bt4"])
 @pytest.mark.parametrize("case", ["fs", "eam", "asym", "inactive", "multibox", "slab"])
 def test_kernel_variant_matches_two_lane_path(p0, case, variant):
     """The lane-pair gather kernels (force path 3, with the table in global memory or in LDS) and
@@ -510,7 +511,8 @@ def test_kernel_variant_matches_two_lane_path(p0, case, variant):
     to the parity bar, and positions
     after a 12-step run to 1e-12 A, over single and multiple species (the LDS table serves one
     kind; several kinds take the global table), FS and EAM, inactive atoms, uneven boxes and a
-    slab.  The bt variants are the LDS-brick passes with the table in LDS (1, 2, 4 lanes per
+    slab.  lds_f1 runs the LDS-table force pass with one lane per atom after a two-lane
+    density pass.  The bt variants are the LDS-brick passes with the table in LDS (1, 2, 4 lanes per
     home atom; the two-species cases use 1500-knot tables so that every kind's window fits)."""
     import msmpscu_amd._lib as L
     box, ctrl, ft = p0
@@ -544,13 +546,16 @@ def test_kernel_variant_matches_two_lane_path(p0, case, variant):
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS,
                                        int(k == 1 and variant in ("lds", "lds_pair", "lds_mask",
-                                                                  "bt1", "bt2", "bt4"))),
+                                                                  "lds_f1", "bt1", "bt2", "bt4"))),
                 "msm_set_option")
+        L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_LANES,
+                                       int(k == 1 and variant == "lds_f1")), "msm_set_option")
         L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_COMPACT_LIST,
                                        int(k == 1 and variant == "lds_mask")), "msm_set_option")
         if k == 1 and bt:
             assert dev.pass_kernel() == "brick_table"
-        if k == 1 and variant in ("lds_pair", "lds_mask") and case not in ("asym", "slab"):
+        if k == 1 and variant in ("lds_pair", "lds_mask", "lds_f1") and case not in ("asym",
+                                                                                       "slab"):
             assert dev.pass_kernel() == "lds_table"
         if statu is not None:
             dev.set_atoms(box.XP, box.XP1, box.ITYP, statu)
