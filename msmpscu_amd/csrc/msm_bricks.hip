@@ -149,10 +149,10 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
     double* lx = s_nl;
     double* ly = lx + cap;
     double* lz = ly + cap;
-    int* lg = reinterpret_cast<int*>(lz + cap);
-    int* lt = lg + cap;  // MT only
+    int* lt = reinterpret_cast<int*>(lz + cap);  // MT only
+    __shared__ int hbase[kNlMaxHalo];  // sorted index of halo slot q in cell h: hbase[h] + q
     __shared__ signed char hsh[kNlMaxHalo * 3];
-    __shared__ double srnl2[kMaxGroup * kMaxGroup];
+    __shared__ double srnl2[MT ? kMaxGroup * kMaxGroup : 1];
     __shared__ BrickLds L;
 
     const int tid = threadIdx.x;
@@ -164,13 +164,14 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
         for (int k = tid; k < ngroup * ngroup; k += blockDim.x) srnl2[k] = rnl2[k];
     const int total = brick_halo(B, g, start, L.hcell, hsh, L.hoff, L.homeoff, L.scan);
     const bool staged = total <= cap;
+    for (int k = tid; k < g.nh; k += blockDim.x)
+        hbase[k] = L.hcell[k] >= 0 ? start[L.hcell[k]] - L.hoff[k] : 0;
     if (staged) {
         for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int h) {
             const double4 pq = pos[q];
             lx[s] = pq.x + shift_of(hsh[3 * h], B.len[0]);
             ly[s] = pq.y + shift_of(hsh[3 * h + 1], B.len[1]);
             lz[s] = pq.z + shift_of(hsh[3 * h + 2], B.len[2]);
-            lg[s] = q;
             if constexpr (MT) lt[s] = ityp[q];
         });
     }
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
             continue;
         }
         const int si = L.hoff[h] + off;
-        const int p = lg[si];
+        const int p = hbase[h] + si;
         const double xi = lx[si], yi = ly[si], zi = lz[si];
         const int ti = MT ? lt[si] : 0;
         const size_t lb = list_base(p, mx);
@@ -202,6 +203,7 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
         for (int s = 0; s < 27; ++s) {
             const int hn = h + c_nix[s] + g.H[0] * (c_niy[s] + g.H[1] * c_niz[s]);
             const int qe = L.hoff[hn + 1];
+            const int gb = hbase[hn];
             for (int q0 = L.hoff[hn]; q0 < qe; q0 += kNlUnroll * T) {
                 bool hit[kNlUnroll];
 #pragma unroll
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
                         if (w < mx) {
                             const int q = q0 + u * T + sub;
                             const size_t at = lb + static_cast<size_t>(w) * stride;
-                            indi[at] = lg[q];
+                            indi[at] = gb + q;
                             if constexpr (SIDX) sidx[at] = static_cast<unsigned short>(q);
                         }
                     }
@@ -245,7 +247,7 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
 }
 
 size_t nlist_lds_bytes(int cap, int ngroup) {
-    return static_cast<size_t>(cap) * (3 * sizeof(double) + sizeof(int) + (ngroup > 1 ? sizeof(int) : 0));
+    return static_cast<size_t>(cap) * (3 * sizeof(double) + (ngroup > 1 ? sizeof(int) : 0));
 }
 
 __global__ void k_zero_range(int* __restrict__ p, const int* __restrict__ first, int n) {
