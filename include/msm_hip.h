@@ -175,6 +175,14 @@ int msm_calc_epot(msm_sim* sim);
  * b = 0..nbox-1; recomputes the density pass and stores the forces as msm_calc_force does.  The
  * reference's COPYOUT_VIRIALTENSOR returns the mean over boxes; callers average these. */
 int msm_calc_ptensor(msm_sim* sim, double* vtensor);
+
+/* Per-atom stress (virial part), the reference's CAL_FS/EAM_AtomicStress_KERNEL through
+ * Cal_AtomicStress (MD_FS_ForceTable_GPU.F90:1746-2076, Analysis/AtomicVirialStress_GPU.F90:734-773):
+ * for every active atom i, P_ab = sum over its full list of s_a s_b FORTOT (no factor 1/2; zero
+ * for inactive atoms), written as AVP(N,9) column-major in the caller's atom order with components
+ * P11, P12, P13, P21, P22, P23, P31, P32, P33 at avp[c*N + i].  Recomputes the density pass and
+ * stores the forces as msm_calc_force does; the sum over atoms is twice msm_calc_ptensor's. */
+int msm_calc_atomic_stress(msm_sim* sim, double* avp);
 int msm_predictor(msm_sim* sim, double h);
 int msm_correction(msm_sim* sim, double h);
 

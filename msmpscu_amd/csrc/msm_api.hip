@@ -1179,6 +1179,36 @@ int msm_calc_ptensor(msm_sim* s, double* vtensor) {
     return read_flags(s);
 }
 
+int msm_calc_atomic_stress(msm_sim* s, double* avp) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (!avp) return set_err(s, MSM_ERR_ARG, "null argument");
+    (void)hipSetDevice(s->device);
+    const int n = s->n;
+    if (n == 0) return MSM_OK;
+    double* vir = nullptr;
+    double* out = nullptr;
+    rc = dalloc(s, &vir, 6 * static_cast<size_t>(n));
+    if (rc == MSM_OK) rc = dalloc(s, &out, 9 * static_cast<size_t>(n));
+    if (rc != MSM_OK) {
+        dfree(&vir);
+        return rc;
+    }
+    clear_flags(s);
+    enqueue_density(s);
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_virial(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), vir,
+                       s->stream);
+    msm::launch_unsort_stress(vir, out, s->set[s->scur].gid, n, s->stream);
+    hipError_t e = hipMemcpyAsync(avp, out, 9 * static_cast<size_t>(n) * sizeof(double),
+                                  hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    dfree(&vir);
+    dfree(&out);
+    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    return read_flags(s);
+}
+
 int msm_predictor(msm_sim* s, double h) {
     int rc = check_ready(s, false);
     if (rc != MSM_OK) return rc;

@@ -218,6 +218,8 @@ void launch_unsort_list(const int* kvois, const int* indi, int stride, const int
 // Virial tensor per atom (6 components, component-major, stride a.n) and forces.
 void launch_virial(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                    const BoxDev* boxes, int multibox, double* vir, hipStream_t s);
+// AVP(N, 9) in caller order (column-major) from launch_virial's per-atom halves.
+void launch_unsort_stress(const double* vir, double* avp, const int* gid, int n, hipStream_t s);
 void launch_box_sum(const double* epot, const int* start, const BoxDev* boxes, int nbox,
                     double* out, hipStream_t s);
 void launch_list_stats(const int* kvois, int n, unsigned long long* total, int* maxk,

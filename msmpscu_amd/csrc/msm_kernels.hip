@@ -619,6 +619,28 @@ __global__ void k_unsort_list(const int* __restrict__ kvois, const int* __restri
         indi_out[g + static_cast<size_t>(iw) * ld] = gid[indi[list_base(p, mx) + static_cast<size_t>(iw) * stride]] + 1;
 }
 
+// Per-atom stress AVP(N, 9) in the caller's order from the per-atom virial halves (6 components,
+// component-major, sorted order): AVP(i, 3a+b) = 2 * vir(ab), the full-list sum
+// sum_j s_a s_b FORTOT of CAL_FS_AtomicStress_KERNEL (MD_FS_ForceTable_GPU.F90:1746-1965),
+// components P11, P12, P13, P21, ..., P33.  Doubling is exact, so this is that sum itself.
+__global__ void k_unsort_stress(const double* __restrict__ vir, double* __restrict__ avp,
+                                const int* __restrict__ gid, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    // vir components: xx, yy, zz, xy, xz, yz
+    constexpr int kMap[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+    const size_t i = static_cast<size_t>(gid[p]);
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+        avp[static_cast<size_t>(c) * n + i] = 2.0 * vir[static_cast<size_t>(kMap[c]) * n + p];
+}
+
+void launch_unsort_stress(const double* vir, double* avp, const int* gid, int n, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_unsort_stress, dim3(grid_for(n)), dim3(kBlock), 0, s, vir, avp, gid,
+                           n);
+}
+
 void launch_unsort_d(const double* src, double* dst, const int* gid, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_unsort_d, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, gid, n);
 }

@@ -14,7 +14,7 @@
 module MSM_HIP_C
   use iso_c_binding
   implicit none
-  integer(c_int), parameter :: MSM_OK = 0
+  integer(c_int), parameter :: MSM_OK = 0, MSM_ERR_ARG = 1
   integer(c_int), parameter :: MSM_FS_TYPE = 0, MSM_EAM_TYPE = 1
   integer(c_int), parameter :: MSM_OPT_TRACK_DIS = 1, MSM_OPT_KERNEL_TIMING = 2, &
                                MSM_OPT_NLIST_KERNEL = 3, MSM_OPT_FORCE_PATH = 4
@@ -147,6 +147,13 @@ module MSM_HIP_C
       real(c_double), intent(out) :: vtensor(*)
       integer(c_int) :: rc
     end function msm_calc_ptensor
+
+    function msm_calc_atomic_stress(sim, avp) bind(C, name="msm_calc_atomic_stress") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: sim
+      real(c_double), intent(out) :: avp(*)
+      integer(c_int) :: rc
+    end function msm_calc_atomic_stress
 
     function msm_predictor(sim, h) bind(C, name="msm_predictor") result(rc)
       import :: c_int, c_double, c_ptr
@@ -319,7 +326,7 @@ module MD_ForceLib_HIP
 
   public :: Register_ForceTableProc, Create_Interaction_ForceTable, Register_ForceClass
   public :: Initialize_Globle_Variables_DEV, Init_Forcetable_Dev, Cal_NeighBoreList_DEV
-  public :: CalForce_ForceClass, CalEpot_ForceClass, CalPTensor_ForceClass
+  public :: CalForce_ForceClass, CalEpot_ForceClass, CalPTensor_ForceClass, Cal_AtomicStress
   public :: Predictor_DEV, Correction_DEV
   public :: For_One_Step, Run_Steps_DEV, CopyOut_SimBox_DEV, Copyout_NeighboreList_DEV
   public :: Autotune_DEV, Adjoint_ForceTable_DEV, Clear_Globle_Variables_DEV, MSM_Handle
@@ -630,6 +637,32 @@ contains
     end do
     VTENSOR = VTENSOR / dble(gm_NBOX)
   end subroutine CalPTensor_ForceClass
+
+  ! Cal_AtomicStress (Analysis/AtomicVirialStress_GPU.F90:734-773): the virial part of the atomic
+  ! pressure tensor through CAL_FS/EAM_AtomicStress_KERNEL, hAVP(NPRT, 9) over all boxes in the
+  ! caller's atom order, components P11, P12, P13, P21, ..., P33 (zero for inactive atoms)
+  subroutine Cal_AtomicStress(SimBox, CtrlParam, hAVP, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    type(SimMDCtrl), intent(in) :: CtrlParam
+    real(KINDDF), intent(out) :: hAVP(:, :)
+    integer, optional, intent(out) :: IERR
+    real(c_double), allocatable :: v(:)
+    integer(c_int) :: rc
+    hAVP = 0.0d0
+    if (size(hAVP, 1) < gm_NPRT .or. size(hAVP, 2) < 9) then
+      if (present(IERR)) then
+        IERR = MSM_ERR_ARG
+        return
+      end if
+      write(*, "(A)") "MDPSCU Error: Cal_AtomicStress: hAVP must be at least (NPRT, 9)"
+      stop 1
+    end if
+    allocate(v(9 * max(gm_NPRT, 1)))
+    rc = msm_calc_atomic_stress(gm_SIM, v)
+    call msm_check(rc, "Cal_AtomicStress", IERR)
+    if (rc /= MSM_OK) return
+    hAVP(1:gm_NPRT, 1:9) = reshape(v(1:9 * gm_NPRT), [gm_NPRT, 9])
+  end subroutine Cal_AtomicStress
 
   ! Predictor_DEV / Correction_DEV (MD_DiffScheme_GPU.F90:604-682 / 821-847)
   subroutine Predictor_DEV(ITIME, SimBox, CtrlParam, IERR)

@@ -5,8 +5,10 @@
 !  the end.  Units: eV, Angstrom, fs, masses in eV fs^2 / A^2.
 !
 !  usage: fs_w_driver OUTFILE [NSTEPS [NC]]
-!  OUTFILE: line 1 "N E0 0"; N lines "x y z fx fy fz" of the start; N lines of the end; a last
-!  line "# energies E0 E1" (total potential energy at the start and at the end).
+!  OUTFILE: line 1 "N E0 0"; N lines "x y z fx fy fz" of the start; N lines of the end; a line
+!  "# energies E0 E1" (total potential energy at the start and at the end) and a last line
+!  "# stress_sum S11 S12 ... S33" (per-atom stresses of Cal_AtomicStress summed over the atoms at
+!  the end).
 
 module W_FS_Potential
   use MD_ForceLib_HIP, only: KINDDF
@@ -74,6 +76,7 @@ program fs_w_driver
   integer :: nsteps, nc, n, i, j, k, l, m, u
   integer(c_int) :: seed(2), rc
   real(KINDDF) :: e0, e1, z1, z2, x
+  real(KINDDF), allocatable :: avp(:, :)
 
   if (command_argument_count() < 1) then
     write(*, "(A)") "usage: fs_w_driver OUTFILE [NSTEPS [NC]]"
@@ -172,6 +175,9 @@ program fs_w_driver
     write(u, "(6(ES25.17E3,1X))") box(1)%XP(m, :), box(1)%FP(m, :)
   end do
   write(u, "(A,2(1X,ES25.17E3))") "# energies", e0, e1
+  allocate(avp(n, 9))
+  call Cal_AtomicStress(box, ctrl, avp)
+  write(u, "(A,9(1X,ES25.17E3))") "# stress_sum", (sum(avp(:, m)), m = 1, 9)
   close(u)
   call Clear_Globle_Variables_DEV()
   write(*, "(A,I0,A,ES24.16,A,ES24.16)") "fs_w_driver: n=", n, " E0=", e0, " E1=", e1

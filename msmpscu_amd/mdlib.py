@@ -395,6 +395,13 @@ class DeviceMD:
         check(lib().msm_calc_ptensor(self.h, dptr(v)), "msm_calc_ptensor", self.h)
         return v.reshape(self.nbox, 3, 3).transpose(0, 2, 1).copy()
 
+    def calc_atomic_stress(self) -> np.ndarray:
+        """Per-atom stress AVP(N, 9) in caller order (msm_calc_atomic_stress): components P11,
+        P12, P13, P21, ..., P33 of sum_j s_a s_b FORTOT; Fortran-ordered (N, 9) array."""
+        v = np.zeros(9 * self.n)
+        check(lib().msm_calc_atomic_stress(self.h, dptr(v)), "msm_calc_atomic_stress", self.h)
+        return v.reshape(9, self.n).T.copy(order="F")
+
     def predictor(self, h: float):
         check(lib().msm_predictor(self.h, h), "msm_predictor", self.h)
 
@@ -559,6 +566,12 @@ def CalPTensor_ForceClass(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClass
     """CalPTensor_ForceClass (MD_ForceClass_Register_GPU.F90:684-728): the virial tensor, as
     COPYOUT_VIRIALTENSOR returns it -- the mean over the boxes (MD_FS_ForceTable_GPU.F90:1415-1447)."""
     return ForceClass.Device.calc_ptensor().mean(axis=0)
+
+
+def Cal_AtomicStress(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU) -> np.ndarray:
+    """Cal_AtomicStress (Analysis/AtomicVirialStress_GPU.F90:734-773) through the force class's
+    atomic-stress routine (CAL_FS/EAM_AtomicStress_KERNEL): hAVP(N, 9) in the caller's order."""
+    return ForceClass.Device.calc_atomic_stress()
 
 
 def Predictor_DEV(ITIME: int, SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):

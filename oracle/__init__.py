@@ -13,6 +13,9 @@ Parity status of each restated function (how it is pinned):
   orc_force_fs_half    pinned: perfect-lattice F = 0, sum F = 0, F = -dE/dx (central FD),
                        analytic lattice energy, full-list (device-kernel) restatement agreement
   orc_force_full       pinned: as above, plus FS == EAM with F = -sqrt(rho) tabulated
+  orc_atomic_stress_full pinned: sum over atoms = 2 x the half-list VTENSOR of orc_force_fs_half
+                       (independent loop), equal tensors on every atom of a perfect lattice with
+                       cubic symmetry, zero on inactive atoms, per-atom symmetry
   orc_adjoint_full     pinned: central finite differences of orc_force_full (FS and EAM, every
                        table, inactive atoms), linearity of the pair-table gradients
   orc_predictor/corr.  pinned: closed-form free flight and constant-force motion, NVE drift
@@ -222,6 +225,28 @@ def force_full(xp, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table: Table):
     if rc != 0:
         raise RuntimeError(f"oracle force_full error {rc}")
     return fp.reshape(3, n).T.copy(), dden, ep
+
+
+def atomic_stress_full(xp, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table: Table, dden,
+                       with_abs=False):
+    """Per-atom stress AP (N, 9) of CAL_FS/EAM_AtomicStress_KERNEL given F' (dden from
+    force_full); with_abs also returns the per-component sums of |term|."""
+    n = int(np.asarray(ityp0).shape[0])
+    x = _xp(xp)
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    st = np.ascontiguousarray(statu, dtype=np.int32)
+    kv = np.ascontiguousarray(kvois, dtype=np.int32)
+    ind = np.ascontiguousarray(np.asarray(indi, dtype=np.int32).T.ravel())
+    dd = np.ascontiguousarray(dden, dtype=np.float64)
+    ap = np.zeros(9 * n)
+    aa = np.zeros(9 * n)
+    rc = lib().orc_atomic_stress_full(n, _d(x), _i(it), _i(st), _i(kv), _i(ind), _d(_v3(ln)),
+                                      _i(_v3(ifpd, np.int32)), C.c_double(ru2max),
+                                      C.byref(table.struct()), _d(dd), _d(ap), _d(aa))
+    if rc != 0:
+        raise RuntimeError(f"oracle atomic_stress_full error {rc}")
+    ap = ap.reshape(9, n).T.copy()
+    return (ap, aa.reshape(9, n).T.copy()) if with_abs else ap
 
 
 def adjoint_full(xp, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table: Table, dldf, dlde,

@@ -418,6 +418,55 @@ int orc_force_fs_half(int n, const double* xp, const int* ityp, const int* kvois
     return ORC_OK;
 }
 
+/* CAL_FS/EAM_AtomicStress_KERNEL (MD_FS_ForceTable_GPU.F90:1746-1965, EAM twin): for every
+ * active atom, AP(i, 3a+b) = sum over its full list of SEP_a SEP_b FORTOT (no factor 1/2), with
+ * FORTOT = FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r and the single cutoff RU2; zero for
+ * inactive atoms.  dden is F' as orc_force_full returns it.  ap is (N, 9) column-major; apabs (may
+ * be NULL) receives the sums of the terms' absolute values. */
+int orc_atomic_stress_full(int n, const double* xp, const int* ityp, const int* statu,
+                           const int* kvois, const int* indi, const double len[3],
+                           const int ifpd[3], double ru2max, const orc_ftable* t,
+                           const double* dden, double* ap, double* apabs) {
+    int ng = t->ngroup;
+    double hbox[3] = {0.5 * len[0], 0.5 * len[1], 0.5 * len[2]};
+    for (int i = 0; i < n; ++i) {
+        double p[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if ((statu[i] & ORC_STATU_ACTIVE) == ORC_STATU_ACTIVE) {
+            int ti = ityp[i];
+            for (int iw = 0; iw < kvois[i]; ++iw) {
+                int j = indi[i + (size_t)iw * n];
+                double s[3];
+                for (int k = 0; k < 3; ++k)
+                    s[k] = min_image(xp[i + k * n] - xp[j + k * n], len[k], hbox[k], ifpd[k]);
+                double r2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+                if (r2 <= ru2max) {
+                    int tj = ityp[j];
+                    int ktab = t->kpair[ti * ng + tj], ktab1 = t->kpair[tj * ng + ti];
+                    double r = sqrt(r2), sk = sqrt(r) * t->csi;
+                    int kk = (int)sk;
+                    double dk = sk - kk;
+                    double fortot =
+                        (TAB(t->fpotr, ktab, kk) + dk * (TAB(t->fpotr, ktab, kk + 1) - TAB(t->fpotr, ktab, kk))) / r2 +
+                        ((TAB(t->fpotb, ktab, kk) + dk * (TAB(t->fpotb, ktab, kk + 1) - TAB(t->fpotb, ktab, kk))) * dden[i] +
+                         (TAB(t->fpotb, ktab1, kk) + dk * (TAB(t->fpotb, ktab1, kk + 1) - TAB(t->fpotb, ktab1, kk))) * dden[j]) /
+                            r;
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) {
+                            double v = s[a] * s[b] * fortot;
+                            p[3 * a + b] = p[3 * a + b] + v;
+                            q[3 * a + b] = q[3 * a + b] + fabs(v);
+                        }
+                }
+            }
+        }
+        for (int c = 0; c < 9; ++c) {
+            ap[i + (size_t)c * n] = p[c];
+            if (apabs) apabs[i + (size_t)c * n] = q[c];
+        }
+    }
+    return ORC_OK;
+}
+
 /* EAM embedding lookups, MD_EAM_ForceTable_GPU.F90:535-541 (F') and :1628-1631 (F). */
 static inline double emb_lerp(const orc_ftable* t, const double* tab, int k, double rho, int* err) {
     double sk = rho / t->rhod + 1.0;

@@ -115,6 +115,13 @@ int orc_force_fs_half(int n, const double* xp, const int* ityp, const int* kvois
                       int mxkvois, const double len[3], const int ifpd[3], const double* ru,
                       const orc_ftable* t, double* fp, double* epot, double* den, double* dden,
                       double* vtensor, double* vabs);
+/* CAL_FS/EAM_AtomicStress_KERNEL: per-atom stress AP(N,9) column-major (P11, P12, ..., P33), the
+ * full-list sum of SEP_a SEP_b FORTOT for active atoms (zero otherwise) given F' in dden (as
+ * orc_force_full returns it); apabs (may be NULL) gets the sums of the terms' absolute values. */
+int orc_atomic_stress_full(int n, const double* xp, const int* ityp, const int* statu,
+                           const int* kvois, const int* indi, const double len[3],
+                           const int ifpd[3], double ru2max, const orc_ftable* t,
+                           const double* dden, double* ap, double* apabs);
 /* Full-list restatement of the device kernels PRECALFOR/CALFORCE/CALEPOT (FS or EAM per
  * t->pot_type): single cutoff max(RU^2), only ACTIVE atoms compute, KPAIR/KPAIR^T pairing as
  * in the corrected kernels.  Outputs FP, DEN (F'), EPOT. */

@@ -1,6 +1,7 @@
 """The Fortran drop-in layer (msmpscu_amd/fortran: MDLIB-named routines over the C-ABI) driven by
 a Fortran program: forces and energy of the W box at the start and after 100 steps (one
-per-step call, then a fused run) against the oracle on the same positions."""
+per-step call, then a fused run) against the oracle on the same positions, and the per-atom
+stresses of Cal_AtomicStress summed over the atoms against twice the oracle's virial."""
 import os
 import subprocess
 import sys
@@ -44,6 +45,7 @@ def test_fortran_driver_against_oracle(orc, tmp_path):
     start = np.array([[float(v) for v in l.split()] for l in lines[1:1 + n]])
     end = np.array([[float(v) for v in l.split()] for l in lines[1 + n:1 + 2 * n]])
     e1 = float(lines[1 + 2 * n].split()[3])
+    ssum = np.array([float(v) for v in lines[2 + 2 * n].split()[2:]]).reshape(3, 3)
     ctrl = cfg.w_ctrl()
     for blk, e in ((start, e0), (end, e1)):
         box = _box(blk[:, :3])
@@ -52,8 +54,11 @@ def test_fortran_driver_against_oracle(orc, tmp_path):
         it = box.ITYP - 1
         kv, ind = orc.nlist("half", box.XP, it, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
                             ctrl.NB_RM, ctrl.NB_MXNBS)
-        fo, eo, *_ = orc.force_fs_half(box.XP, it, kv, ind, box.ZL, ctrl.IFPD, ctrl.RU, t)
+        fo, eo, _, _, vt, vabs = orc.force_fs_half(box.XP, it, kv, ind, box.ZL, ctrl.IFPD,
+                                                   ctrl.RU, t, with_vabs=True)
         assert np.max(np.abs(blk[:, 3:] - fo)) <= F_TOL
         assert abs(e - eo.sum()) <= E_REL * abs(eo.sum())
+    # positions were written to 17 significant digits, so the oracle sees them to ~1e-17 relative
+    assert np.all(np.abs(ssum - 2.0 * vt) <= 1e-13 * 2.0 * vabs)
     # the run moved the atoms
     assert np.max(np.abs(end[:, :3] - start[:, :3])) > 1e-3

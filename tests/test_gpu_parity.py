@@ -687,6 +687,52 @@ def test_virial_fs_matches_oracle(orc, p0):
     assert np.max(np.abs(fp - fh)) <= F_TOL
 
 
+@pytest.mark.parametrize("case", ["fs", "eam", "asym", "multibox", "inactive"])
+def test_atomic_stress_matches_oracle(orc, p0, case):
+    """Per-atom stress (msm_calc_atomic_stress, CAL_FS/EAM_AtomicStress_KERNEL) equals the
+    oracle's full-list restatement atom by atom, to 1e-13 of the sum of each component's absolute
+    pair terms; inactive atoms carry none; summed over the atoms of each box it is twice that
+    box's virial from msm_calc_ptensor."""
+    box, ctrl, ft = p0
+    boxes, statu = [box], None
+    if case == "eam":
+        ft = cfg.w_forcetable(box, ctrl, pot_type="EAM_TYPE")
+    elif case == "asym":
+        boxes, ctrl, ft = _asym_case()
+    elif case == "multibox":
+        boxes = [cfg.make_bcc_box(c, cfg.W_A, 0.05, 500.0, 40 + k, n_vac=k)
+                 for k, c in enumerate([(5, 6, 7), (9, 9, 9), (4, 4, 5)])]
+        ft = cfg.w_forcetable(boxes[0], ctrl)
+    elif case == "inactive":
+        statu = box.STATU.copy()
+        statu[[3, 100, 4000]] = 0
+    say(f"atomic stress {case}")
+    dev = device_for(boxes, ctrl, ft)
+    if statu is not None:
+        dev.set_atoms(box.XP, box.XP1, box.ITYP, statu)
+    dev.build_neighbors()
+    avp = dev.calc_atomic_stress()
+    vt = dev.calc_ptensor()
+    dev.close()
+    t = orc.from_mdlib(ft)
+    off = 0
+    for k, b in enumerate(boxes):
+        st = b.STATU if statu is None else statu
+        it = b.ITYP - 1
+        kv, ind = orc.nlist("full", b.XP, it, st, b.BOXLOW, b.ZL, ctrl.IFPD, ctrl.NB_RM,
+                            ctrl.NB_MXNBS)
+        _, dd, _ = orc.force_full(b.XP, it, st, kv, ind, b.ZL, ctrl.IFPD,
+                                  float(np.max(ctrl.RU)) ** 2, t)
+        ap, aabs = orc.atomic_stress_full(b.XP, it, st, kv, ind, b.ZL, ctrl.IFPD,
+                                          float(np.max(ctrl.RU)) ** 2, t, dd, with_abs=True)
+        mine = avp[off:off + b.NPRT]
+        assert np.all(np.abs(mine - ap) <= 1e-13 * aabs + 1e-300)
+        assert np.all(mine[st == 0] == 0)
+        tot = mine.sum(axis=0).reshape(3, 3)
+        assert np.all(np.abs(tot - 2.0 * vt[k]) <= 1e-13 * aabs.sum(axis=0).reshape(3, 3))
+        off += b.NPRT
+
+
 @pytest.mark.parametrize("pot_type", ["FS_TYPE", "EAM_TYPE"])
 def test_virial_is_minus_strain_derivative_of_energy(pot_type):
     """W_aa = -dE/d(eps_aa) under a homogeneous diagonal strain of positions and box (central

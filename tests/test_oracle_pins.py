@@ -266,6 +266,61 @@ def test_inactive_atoms_follow_device_kernel_semantics(orc):
     assert np.any(fp[ind[5, :kv[5]]] != 0)
 
 
+@pytest.mark.parametrize("case", ["w", "alloy"])
+def test_atomic_stress_sums_to_twice_the_half_list_virial(orc, case):
+    """CAL_FS_AtomicStress_KERNEL restatement: summed over atoms it is twice the half-list
+    VTENSOR of CALFOR (a separate loop visiting each pair once), to a few ulps of the sum of the
+    terms' absolute values; each atom's tensor is symmetric, and an inactive atom's is zero.  The
+    alloy has asymmetric density channels (KPAIR = [[0, 2], [3, 1]], symmetric pair parts), so a
+    swapped KTAB/KTAB1 fails, as does a dropped or extra factor 1/2."""
+    x, lo, ln = bcc(6, 3.165, 0.08, 21)
+    n = len(x)
+    if case == "w":
+        t = w_table(orc, 5000)
+        it = np.zeros(n, np.int32)
+    else:
+        t = alloy_table(orc, 5000)
+        it = (np.arange(n) % 2).astype(np.int32)
+    st = np.ones(n, np.int32)
+    rnl = np.full((t.ngroup, t.ngroup), 4.7002)
+    ru = np.full((t.ngroup, t.ngroup), 4.4002)
+    kvh, indh = orc.nlist("half", x, it, st, lo, ln, (1, 1, 1), rnl, 64)
+    _, _, _, _, vt, vabs = orc.force_fs_half(x, it, kvh, indh, ln, (1, 1, 1), ru, t,
+                                             with_vabs=True)
+    kv, ind = orc.nlist("full", x, it, st, lo, ln, (1, 1, 1), rnl, 64)
+    _, dd, _ = orc.force_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t)
+    ap, aabs = orc.atomic_stress_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t, dd,
+                                      with_abs=True)
+    tot = ap.sum(axis=0).reshape(3, 3)
+    assert np.all(np.abs(tot - 2.0 * vt) <= 1e-14 * 2.0 * vabs)
+    assert np.array_equal(ap[:, [1, 2, 5]], ap[:, [3, 6, 7]])
+    # an inactive atom carries no stress, and the others still see it as a neighbour
+    st[7] = 0
+    _, dd, _ = orc.force_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t)
+    ap2 = orc.atomic_stress_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t, dd)
+    assert np.all(ap2[7] == 0) and np.any(ap2[ind[7, :kv[7]]] != 0)
+
+
+def test_atomic_stress_is_uniform_on_a_perfect_lattice(orc):
+    """Perfect BCC W compressed to a = 3.10 A (at 3.165 A the stress nearly vanishes): every
+    atom carries the same stress tensor, diagonal with three equal components (cubic symmetry),
+    and N times it is twice the box virial."""
+    x, lo, ln = bcc(5, 3.10, 0.0, 1)
+    n = len(x)
+    t = w_table(orc, 5000)
+    it = np.zeros(n, np.int32)
+    st = np.ones(n, np.int32)
+    kv, ind = orc.nlist("full", x, it, st, lo, ln, (1, 1, 1), [[4.7002]], 64)
+    _, dd, _ = orc.force_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t)
+    ap = orc.atomic_stress_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t, dd)
+    d = ap[0, 0]
+    assert abs(d) > 1.0
+    assert np.all(np.abs(ap[:, [0, 4, 8]] - d) <= 1e-11 * abs(d))
+    assert np.all(np.abs(ap[:, [1, 2, 3, 5, 6, 7]]) <= 1e-11 * abs(d))
+    _, _, _, _, vt = fs_eval(orc, x, lo, ln, t)
+    assert np.allclose(n * d, 2.0 * vt[0, 0], rtol=1e-11)
+
+
 # --------------------------------------------------------------------------- integrator ---------
 
 def test_integrator_constant_force_closed_form(orc):
