@@ -5,8 +5,8 @@ tests) read like MDLIB code:
 
 * ``MDForceTable``, ``Register_ForceTableProc``, ``New_ForceTable``,
   ``Create_Pairwise_ForceTable``, ``Create_EMBDFUNTable``, ``Create_Interaction_ForceTable``,
-  ``Export_ForceTable``, ``Import_ForceTable``
-  -- MDLIB/sor/Common/MD_TypeDef_ForceTable.F90:117-155, 530-1056, 1151-1230, 1315-1459
+  ``Export_ForceTable``, ``Import_ForceTable``, ``Register_Imported_ForceTable``
+  -- MDLIB/sor/Common/MD_TypeDef_ForceTable.F90:117-155, 530-1056, 1151-1230, 1315-1855
 * ``SimMDBox`` / ``SimMDCtrl`` (the fields this path reads)
   -- MDLIB/sor/Common/MD_TypeDef_SimBox.F90:43-132, MD_TypeDef_SimCtrlParam.F90:56-142
 * ``MDForceClassGPU``, ``Register_ForceClass``, ``Init_Forcetable_Dev``, ``CalForce_ForceClass``,
@@ -77,6 +77,8 @@ class MDForceTable:
     DFEMBD: Optional[np.ndarray] = None
     RHOMX: float = -1.0
     RHOD: float = 1.0
+    R: Optional[np.ndarray] = None    # grid of an imported table (Import_ForceTable)
+    RHO: Optional[np.ndarray] = None  # embedding grid of an imported table
     RHOMULT: float = 20.0
     REGPROCS: Dict[int, _ExtProc] = field(default_factory=dict)
     HASREG: Dict[int, bool] = field(default_factory=dict)
@@ -621,6 +623,9 @@ def Export_ForceTable(fname: str, FTable: MDForceTable, rho_unit: float = 1.0):
         fh.write(f'&POTTYPE "{FTable.PotType.strip()}"\n')
         fh.write(f"&NUMTABLE {len(ids):7d} table IDs: " + "".join(f"{i:5d}" for i in ids) + "\n")
         fh.write(f"&NUMPOINT {FTable.NTAB:7d}\n")
+        title = "".join(f"      r*V{i}(r)[eV*A]        -r*dV{i}/dr[eV]            RHO{i}(r)"
+                        f"            -dRHO{i}/dr[/A] " for i in ids)
+        fh.write(f"&#           Rij(A)      {title}\n")
         for it in range(1, FTable.NTAB + 1):
             t = it * FTable.CSIV
             r = t * t
@@ -636,6 +641,8 @@ def Export_ForceTable(fname: str, FTable: MDForceTable, rho_unit: float = 1.0):
         ids1 = sorted(FTable.FPAIR1) if FTable.FPAIR1 else [1]
         fh.write(f"&NUMTABLE {len(ids1):7d} table IDs: " + "".join(f"{i:5d}" for i in ids1) + "\n")
         fh.write(f"&NUMPOINT {FTable.NEMBD:7d}\n")
+        title = "".join(f"         F{i}(RHO)      dF{i}/d(RHO)" for i in ids1)
+        fh.write(f"&#           RHO       {title}\n")
         for it in range(1, FTable.NEMBD + 1):
             rho = (it - 1) * FTable.RHOD
             cols = []
@@ -648,6 +655,184 @@ def Export_ForceTable(fname: str, FTable: MDForceTable, rho_unit: float = 1.0):
                     f, df = FTable.FEMBD[k, it - 1], FTable.DFEMBD[k, it - 1]
                 cols += [f, df]
             fh.write(f" {it:6d} " + "".join(f"{v:16.8E}" for v in [rho] + cols) + "\n")
+
+
+def _spline_second_derivatives(x: np.ndarray, f: np.ndarray) -> np.ndarray:
+    """Second derivatives of the cubic spline through (x, f) clamped at both ends to the slope of
+    the cubic through the four end points -- SPLID1 with IOP = (5, 5)
+    (LIB/sor/f/MiniUtilities/DINTF2.F:17-189), solved as one tridiagonal system."""
+    n = len(x)
+    if n < 4:
+        raise ValueError("MDPSCU Error: a table needs at least 4 points to be interpolated")
+    a1, a2, a3 = x[0] - x[1], x[0] - x[2], x[0] - x[3]
+    a4, a5, a6 = x[1] - x[2], x[1] - x[3], x[2] - x[3]
+    d0 = (f[0] * (1.0 / a1 + 1.0 / a2 + 1.0 / a3) - a2 * a3 * f[1] / (a1 * a4 * a5)
+          + a1 * a3 * f[2] / (a2 * a4 * a6) - a1 * a2 * f[3] / (a3 * a5 * a6))
+    b1, b2, b3 = x[-1] - x[-4], x[-1] - x[-3], x[-1] - x[-2]
+    b4, b5, b6 = x[-2] - x[-4], x[-2] - x[-3], x[-3] - x[-4]
+    dn = (-b2 * b3 * f[-4] / (b6 * b4 * b1) + b1 * b3 * f[-3] / (b6 * b5 * b2)
+          - b1 * b2 * f[-2] / (b4 * b5 * b3) + f[-1] * (1.0 / b1 + 1.0 / b2 + 1.0 / b3))
+    h = np.diff(x)
+    sl = np.diff(f) / h
+    sub = np.zeros(n)
+    dia = np.zeros(n)
+    sup = np.zeros(n)
+    rhs = np.zeros(n)
+    dia[0], sup[0], rhs[0] = h[0] / 3.0, h[0] / 6.0, sl[0] - d0
+    sub[1:-1], dia[1:-1], sup[1:-1] = h[:-1] / 6.0, (h[:-1] + h[1:]) / 3.0, h[1:] / 6.0
+    rhs[1:-1] = sl[1:] - sl[:-1]
+    sub[-1], dia[-1], rhs[-1] = h[-1] / 6.0, h[-1] / 3.0, dn - sl[-1]
+    # Thomas algorithm (the matrix is diagonally dominant)
+    c = np.zeros(n)
+    d = np.zeros(n)
+    c[0], d[0] = sup[0] / dia[0], rhs[0] / dia[0]
+    for i in range(1, n):
+        m = dia[i] - sub[i] * c[i - 1]
+        c[i] = sup[i] / m
+        d[i] = (rhs[i] - sub[i] * d[i - 1]) / m
+    w = np.zeros(n)
+    w[-1] = d[-1]
+    for i in range(n - 2, -1, -1):
+        w[i] = d[i] - c[i] * w[i + 1]
+    return w
+
+
+def _spline_eval(x: np.ndarray, f: np.ndarray, w: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """Spline value at y (SPLID2, DINTF2.F:191-236): the interval with x[j] <= y < x[j+1], the first
+    interval below x[0] and the last at or above x[-1]."""
+    y = np.asarray(y, dtype=np.float64)
+    j = np.clip(np.searchsorted(x, y, side="right") - 1, 0, len(x) - 2)
+    x0, x1 = x[j], x[j + 1]
+    hk = x1 - x0
+    return ((w[j] * (x1 - y) ** 3 + w[j + 1] * (y - x0) ** 3) / (6.0 * hk)
+            + (f[j + 1] / hk - w[j + 1] * hk / 6.0) * (y - x0)
+            + (f[j] / hk - hk * w[j] / 6.0) * (x1 - y))
+
+
+def Import_ForceTable(fname: str, rho_unit: float = 1.0) -> MDForceTable:
+    """Import_ForceTable (MD_TypeDef_ForceTable.F90:1463-1608): read ``fname.pair`` and
+    ``fname.embd`` as written by Export_ForceTable into a table on the file's own grid: R, and
+    POTR = (r V)/2, FPOTR = -r dV/dr, POTB = RHO/rho_unit, FPOTB = -dRHO/dr/rho_unit per imported
+    table, RHO and FEMBD, DFEMBD per embedding table (eV and Angstrom)."""
+    def parse(path, ncol):
+        head, rows = {}, []
+        with open(path) as fh:
+            in_head = True
+            for line in fh:
+                s = line.strip()
+                if not s or s.startswith("!"):
+                    continue
+                if in_head:
+                    if s.startswith("&#"):
+                        in_head = False
+                    elif s.startswith("&"):
+                        parts = s.split()
+                        head[parts[0].upper()] = parts[1:]
+                    continue
+                rows.append([float(v) for v in s.split()[1:1 + ncol]])
+        if in_head:
+            raise ValueError(f"MDPSCU Error: no table rows (&# line) in {path}")
+        return head, np.array(rows, dtype=np.float64)
+
+    ft = MDForceTable()
+    with open(fname + ".pair") as fh:
+        for line in fh:
+            if line.strip().upper().startswith("&NUMTABLE"):
+                nc = int(line.split()[1])
+                break
+        else:
+            raise ValueError(f"MDPSCU Error: no &NUMTABLE in {fname}.pair")
+    head, rows = parse(fname + ".pair", 1 + 4 * nc)
+    ft.PotType = head.get("&POTTYPE", ['"FS_TYPE"'])[0].strip('"')
+    ft.FPAIR = [int(v) for v in head["&NUMTABLE"][3:3 + nc]]
+    ft.NKIND = nc
+    ft.NTAB = int(head["&NUMPOINT"][0])
+    if rows.shape[0] != ft.NTAB:
+        raise ValueError(f"MDPSCU Error: {fname}.pair holds {rows.shape[0]} rows, "
+                         f"&NUMPOINT says {ft.NTAB}")
+    ft.R = rows[:, 0].copy()
+    cols = rows[:, 1:].reshape(ft.NTAB, nc, 4)
+    ft.POTR = np.asfortranarray(cols[:, :, 0].T * 0.5)
+    ft.FPOTR = np.asfortranarray(cols[:, :, 1].T.copy())
+    ft.POTB = np.asfortranarray(cols[:, :, 2].T / rho_unit)
+    ft.FPOTB = np.asfortranarray(cols[:, :, 3].T / rho_unit)
+    with open(fname + ".embd") as fh:
+        for line in fh:
+            if line.strip().upper().startswith("&NUMTABLE"):
+                nc1 = int(line.split()[1])
+                break
+        else:
+            raise ValueError(f"MDPSCU Error: no &NUMTABLE in {fname}.embd")
+    head, rows = parse(fname + ".embd", 1 + 2 * nc1)
+    ft.FPAIR1 = [int(v) for v in head["&NUMTABLE"][3:3 + nc1]]
+    ft.NKIND1 = nc1
+    ft.NEMBD = int(head["&NUMPOINT"][0])
+    ft.RHO = rows[:, 0] / rho_unit
+    cols = rows[:, 1:].reshape(ft.NEMBD, nc1, 2)
+    ft.FEMBD = np.asfortranarray(cols[:, :, 0].T.copy())
+    ft.DFEMBD = np.asfortranarray(cols[:, :, 1].T.copy())
+    return ft
+
+
+def Register_Imported_ForceTable(SimBox: "SimMDBox", CtrlParam: "SimMDCtrl",
+                                 FTable: MDForceTable, fname: str, rho_unit: float = 1.0):
+    """Register_Imported_ForceTable (MD_TypeDef_ForceTable.F90:1612-1855): import ``fname`` and
+    resample every table the box's PTYPE needs onto the simulation grid (r_k = (k/CSI)^2 from
+    CtrlParam) by the clamped cubic spline in r; zero beyond the imported range.  Embedding
+    tables are resampled on rho_k = (k-1) RHOD with RHOMX = RHO(NEMBD) NEMBD/(NEMBD-1) of the
+    imported grid."""
+    src = Import_ForceTable(fname, rho_unit)
+    n = SimBox.NGROUP
+    for i in range(n):
+        for j in range(n):
+            if int(SimBox.PTYPE[i, j]) not in src.FPAIR:
+                raise ValueError(f"MDPSCU Error: cannot find force table # {int(SimBox.PTYPE[i, j])}"
+                                 f" in lib {fname}.pair")
+        if int(SimBox.PTYPE[i, i]) not in src.FPAIR1:
+            raise ValueError(f"MDPSCU Error: cannot find force table # {int(SimBox.PTYPE[i, i])}"
+                             f" in lib {fname}.embd")
+    for tid in src.FPAIR:
+        if tid > 0 and tid not in FTable.REGPROCS:
+            FTable.REGPROCS[tid] = _ExtProc(tid, "", None, None, None, f"Table # {tid}")
+    FTable.PotType = src.PotType
+    New_ForceTable(SimBox, CtrlParam, FTable)
+    FTable.KPAIR[:, :] = -1
+    trmax = float(np.max(src.R))
+    tt = (np.arange(1, FTable.NTAB + 1) * FTable.CSIV) ** 2
+    inside = tt <= trmax
+    for i in range(n):
+        for j in range(n):
+            iforce = int(SimBox.PTYPE[i, j])
+            k = FTable.FPAIR.index(iforce)
+            if not FTable.HASREG.get(k):
+                k0 = src.FPAIR.index(iforce)
+                for name in ("POTR", "FPOTR", "POTB", "FPOTB"):
+                    fsrc = getattr(src, name)[k0]
+                    w = _spline_second_derivatives(src.R, fsrc)
+                    out = np.zeros(FTable.NTAB)
+                    out[inside] = _spline_eval(src.R, fsrc, w, tt[inside])
+                    getattr(FTable, name)[k, :] = out
+                FTable.HASREG[k] = True
+            FTable.KPAIR[i, j] = k + 1
+    nembd_src = len(src.RHO)
+    FTable.RHOMX = src.RHO[-1] / (nembd_src - 1) * nembd_src
+    FTable.RHOD = FTable.RHOMX / FTable.NEMBD
+    if FTable.RHOD <= 1.0e-64:
+        FTable.RHOD = 1.0
+    if FTable.PotType.strip() == "EAM_TYPE":
+        FTable.KEMBD[:] = -1
+        rho = np.arange(FTable.NEMBD) * FTable.RHOD
+        for i in range(n):
+            iforce = int(SimBox.PTYPE[i, i])
+            k = FTable.FPAIR1.index(iforce)
+            if not FTable.HASREG1.get(k):
+                k0 = src.FPAIR1.index(iforce)
+                for name in ("FEMBD", "DFEMBD"):
+                    fsrc = getattr(src, name)[k0]
+                    w = _spline_second_derivatives(src.RHO, fsrc)
+                    getattr(FTable, name)[k, :] = _spline_eval(src.RHO, fsrc, w, rho)
+                FTable.HASREG1[k] = True
+            FTable.KEMBD[i] = k + 1
 
 
 def read_table_file(path: str):
