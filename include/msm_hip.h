@@ -183,6 +183,28 @@ int msm_calc_ptensor(msm_sim* sim, double* vtensor);
  * P11, P12, P13, P21, P22, P23, P31, P32, P33 at avp[c*N + i].  Recomputes the density pass and
  * stores the forces as msm_calc_force does; the sum over atoms is twice msm_calc_ptensor's. */
 int msm_calc_atomic_stress(msm_sim* sim, double* avp);
+
+/* Thermalisation and quench around the force path (MD_DiffScheme_GPU.F90).
+ * msm_damping: DAMPING_KERNEL (:125-187) -- each velocity component opposite to its force
+ * component is set to zero, and components with FIXPOS set (every atom).
+ * msm_calc_ekin: CALEKIN_KERNEL (:851-904) and Cal_GlobalT_DEV (:1042-1055) per box -- ekin (may be
+ * NULL) receives CM/2 |v|^2 in the caller's order for active atoms with no FIXPOS bit and -1e32
+ * for the others; tbox (may be NULL) the temperature 2 <EK> / (3 kb) of each box over the atoms
+ * that count (kb in the caller's energy unit per K).
+ * msm_scale_velocities: VelScaling_DEV (:1262-1448) -- velocities of active atoms times
+ * sqrt(temp * 3 kb / 2 / <EK>_box), components with FIXPOS or FIXVEL set to zero; MSM_ERR_ARG if a
+ * box has no kinetic energy.
+ * msm_dyn_damp: Do_DynDamp_Forsteps_DEV (:1809-1883) -- up to maxsteps of damping, predictor,
+ * force, per-atom energy, stopping when max |EPOT_i - EPOT_i(previous)| <= mindele (before the
+ * corrector), else corrector; the list is not rebuilt.  steps receives the iteration it stopped
+ * at (maxsteps + 1 if it did not converge), delepot the last max |dEPOT|.
+ * msm_calc_ekin, msm_scale_velocities and msm_dyn_damp need a built neighbour list (the atoms sit
+ * in cell order on the device from the first msm_build_neighbors on). */
+int msm_damping(msm_sim* sim);
+int msm_calc_ekin(msm_sim* sim, double kb, double* ekin, double* tbox);
+int msm_scale_velocities(msm_sim* sim, double temp, double kb);
+int msm_dyn_damp(msm_sim* sim, int maxsteps, double mindele, double h, int* steps,
+                 double* delepot);
 int msm_predictor(msm_sim* sim, double h);
 int msm_correction(msm_sim* sim, double h);
 

@@ -89,6 +89,10 @@ SIGNATURES = {
     "msm_autotune": (C.c_int, [_vp, C.c_int, _ip, _dp]),
     "msm_calc_ptensor": (C.c_int, [_vp, _dp]),
     "msm_calc_atomic_stress": (C.c_int, [_vp, _dp]),
+    "msm_damping": (C.c_int, [_vp]),
+    "msm_calc_ekin": (C.c_int, [_vp, C.c_double, _dp, _dp]),
+    "msm_scale_velocities": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "msm_dyn_damp": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, _ip, _dp]),
     "msm_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
 }
 

@@ -14,11 +14,11 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _lib
-from .mdlib import (MDForceTable, Register_ForceTableProc, SimMDBox, SimMDCtrl,
+from .mdlib import (CP_KB_EV, MDForceTable, Register_ForceTableProc, SimMDBox, SimMDCtrl,
                     Create_Interaction_ForceTable)
 
 AMU = 1.66053906660e-27 * 1.0e10 / 1.602176634e-19  # eV fs^2 / A^2 per amu (103.6427...)
-KB = 8.617333262e-5                                   # eV / K
+KB = CP_KB_EV                                         # eV / K
 W_MASS = 183.84
 CU_MASS = 63.546
 NI_MASS = 58.6934

@@ -1209,6 +1209,140 @@ int msm_calc_atomic_stress(msm_sim* s, double* avp) {
     return read_flags(s);
 }
 
+// Per-box sum and count of the kinetic energies that count (CALEKIN_KERNEL); ek (device, n) keeps
+// the per-atom values.
+static int box_kinetic(msm_sim* s, double* ek, std::vector<double>& sum, std::vector<double>& cnt) {
+    const int nbox = static_cast<int>(s->boxes.size());
+    double* d = nullptr;
+    int rc = dalloc(s, &d, 2 * static_cast<size_t>(nbox));
+    if (rc != MSM_OK) return rc;
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_ekin(a, s->tab, ek, s->stream);
+    msm::launch_box_ekin(ek, s->sb.start, s->d_boxes, nbox, d, d + nbox, s->stream);
+    std::vector<double> h(2 * static_cast<size_t>(nbox));
+    hipError_t e = hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    dfree(&d);
+    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    sum.assign(h.begin(), h.begin() + nbox);
+    cnt.assign(h.begin() + nbox, h.end());
+    return MSM_OK;
+}
+
+int msm_damping(msm_sim* s) {
+    int rc = check_ready(s, false);
+    if (rc != MSM_OK) return rc;
+    (void)hipSetDevice(s->device);
+    msm::launch_damping(atoms_view(s, s->scur, s->pcur), s->stream);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MSM_OK;
+}
+
+int msm_calc_ekin(msm_sim* s, double kb, double* ekin, double* tbox) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (!(kb > 0.0)) return set_err(s, MSM_ERR_ARG, "kb must be positive");
+    (void)hipSetDevice(s->device);
+    double* ek = nullptr;
+    if ((rc = dalloc(s, &ek, std::max(s->n, 1))) != MSM_OK) return rc;
+    std::vector<double> sum, cnt;
+    rc = box_kinetic(s, ek, sum, cnt);
+    if (rc == MSM_OK && ekin) rc = pull_sorted_d(s, ek, -1, ekin);
+    dfree(&ek);
+    if (rc != MSM_OK) return rc;
+    if (tbox)
+        for (size_t b = 0; b < sum.size(); ++b)
+            tbox[b] = cnt[b] > 0.0 ? 2.0 * (sum[b] / cnt[b]) / (3.0 * kb) : 0.0;
+    return MSM_OK;
+}
+
+int msm_scale_velocities(msm_sim* s, double temp, double kb) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (!(kb > 0.0) || !(temp >= 0.0)) return set_err(s, MSM_ERR_ARG, "kb > 0 and temp >= 0");
+    (void)hipSetDevice(s->device);
+    const int nbox = static_cast<int>(s->boxes.size());
+    double* ek = nullptr;
+    double* scal = nullptr;
+    rc = dalloc(s, &ek, std::max(s->n, 1));
+    if (rc == MSM_OK) rc = dalloc(s, &scal, nbox);
+    std::vector<double> sum, cnt;
+    if (rc == MSM_OK) rc = box_kinetic(s, ek, sum, cnt);
+    std::vector<double> f(nbox);
+    for (int b = 0; rc == MSM_OK && b < nbox; ++b) {
+        const double mean = cnt[b] > 0.0 ? sum[b] / cnt[b] : 0.0;
+        if (!(mean > 0.0))
+            rc = set_err(s, MSM_ERR_ARG, "current temperature of a box is zero in scaling velocity");
+        else
+            f[b] = std::sqrt(temp * 3.0 * kb * 0.5 / mean);  // VelScaling_DEV (:1390-1448)
+    }
+    if (rc == MSM_OK) {
+        hipError_t e = hipMemcpyAsync(scal, f.data(), nbox * sizeof(double),
+                                      hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess) {
+            msm::launch_velscale(atoms_view(s, s->scur, s->pcur), scal, s->stream);
+            e = hipStreamSynchronize(s->stream);
+        }
+        if (e != hipSuccess) rc = set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    }
+    dfree(&ek);
+    dfree(&scal);
+    return rc;
+}
+
+int msm_dyn_damp(msm_sim* s, int maxsteps, double mindele, double h, int* steps,
+                 double* delepot) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (maxsteps < 0) return set_err(s, MSM_ERR_ARG, "maxsteps must be >= 0");
+    (void)hipSetDevice(s->device);
+    const int n = s->n;
+    double* epot0 = nullptr;
+    unsigned long long* dmax = nullptr;
+    rc = dalloc(s, &epot0, std::max(n, 1));
+    if (rc == MSM_OK) rc = dalloc(s, &dmax, 1);
+    if (rc != MSM_OK) {
+        dfree(&epot0);
+        return rc;
+    }
+    clear_flags(s);
+    // UpdateEPOT_ForceClass, then copy (Do_DynDamp_Forsteps_DEV, MD_DiffScheme_GPU.F90:1809-1883)
+    msm::launch_epot(atoms_view(s, s->scur, s->pcur), list_view(s), s->tab, s->boxes[0],
+                     s->d_boxes, multibox(s), s->d_err, s->stream);
+    hipError_t e = hipMemcpyAsync(epot0, s->epot, std::max(n, 1) * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s->stream);
+    int it = 1;
+    double del = 0.0;
+    for (; e == hipSuccess && it <= maxsteps; ++it) {
+        AtomsDev a = atoms_view(s, s->scur, s->pcur);
+        msm::launch_damping(a, s->stream);
+        msm::launch_predictor(a, s->tab, s->boxes[0], s->d_boxes, multibox(s), h, s->stream);
+        enqueue_density(s);
+        enqueue_force(s, 0, 1, 0.0);
+        a = atoms_view(s, s->scur, s->pcur);
+        msm::launch_epot(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), s->d_err,
+                         s->stream);
+        unsigned long long bits = 0;
+        e = hipMemsetAsync(dmax, 0, sizeof(unsigned long long), s->stream);
+        if (e != hipSuccess) break;
+        msm::launch_epot_maxdelta(s->epot, epot0, n, dmax, s->stream);
+        e = hipMemcpyAsync(&bits, dmax, sizeof(bits), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) break;
+        std::memcpy(&del, &bits, sizeof(del));
+        if (del <= mindele) break;  // converged: the reference leaves before the corrector
+        msm::launch_correction(a, s->tab, h, s->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    dfree(&epot0);
+    dfree(&dmax);
+    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    if (steps) *steps = it;  // maxsteps + 1 when not converged, as the reference's ITER
+    if (delepot) *delepot = del;
+    return read_flags(s);
+}
+
 int msm_predictor(msm_sim* s, double h) {
     int rc = check_ready(s, false);
     if (rc != MSM_OK) return rc;

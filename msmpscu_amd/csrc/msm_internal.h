@@ -218,6 +218,16 @@ void launch_unsort_list(const int* kvois, const int* indi, int stride, const int
 // Virial tensor per atom (6 components, component-major, stride a.n) and forces.
 void launch_virial(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                    const BoxDev* boxes, int multibox, double* vir, hipStream_t s);
+// Thermalisation and quench (MD_DiffScheme_GPU.F90): DAMPING, per-atom EK (-1e32 for atoms that
+// do not count), per-box sum and count of non-negative EK, velocity scaling by the factor of the
+// atom's box, and max |epot - epot0| (then epot0 = epot; *out zero on entry, a double's bits).
+void launch_damping(const AtomsDev& a, hipStream_t s);
+void launch_ekin(const AtomsDev& a, const TabDev& t, double* ek, hipStream_t s);
+void launch_box_ekin(const double* ek, const int* start, const BoxDev* boxes, int nbox,
+                     double* sum, double* cnt, hipStream_t s);
+void launch_velscale(const AtomsDev& a, const double* scal, hipStream_t s);
+void launch_epot_maxdelta(const double* epot, double* epot0, int n, unsigned long long* out,
+                          hipStream_t s);
 // AVP(N, 9) in caller order (column-major) from launch_virial's per-atom halves.
 void launch_unsort_stress(const double* vir, double* avp, const int* gid, int n, hipStream_t s);
 void launch_box_sum(const double* epot, const int* start, const BoxDev* boxes, int nbox,

@@ -586,6 +586,130 @@ void launch_correction(const AtomsDev& a, const TabDev& t, double h, hipStream_t
 }
 
 // --------------------------------------------------------------------------------------------
+// thermalisation and quench around the force path (MD_DiffScheme_GPU.F90)
+// --------------------------------------------------------------------------------------------
+
+// DAMPING_KERNEL (:125-187): a velocity component opposite to its force component is zeroed, and
+// components with FIXPOS set are zeroed (every atom, as the reference).
+__global__ void k_damping(AtomsDev a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int st = a.statu[i];
+    double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+    if (vx * a.fx[i] < 0.0) vx = 0.0;
+    if (vy * a.fy[i] < 0.0) vy = 0.0;
+    if (vz * a.fz[i] < 0.0) vz = 0.0;
+    if (st & STATU_FIXPOSX) vx = 0.0;
+    if (st & STATU_FIXPOSY) vy = 0.0;
+    if (st & STATU_FIXPOSZ) vz = 0.0;
+    a.vx[i] = vx;
+    a.vy[i] = vy;
+    a.vz[i] = vz;
+}
+
+void launch_damping(const AtomsDev& a, hipStream_t s) {
+    if (a.n > 0) hipLaunchKernelGGL(k_damping, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a);
+}
+
+// CALEKIN_KERNEL (:851-904): EK = CM/2 |v|^2 for active atoms with no FIXPOS bit, else -1e32.
+__global__ void k_ekin(AtomsDev a, TabDev t, double* __restrict__ ek) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int st = a.statu[i];
+    double e = -1.0e32;
+    if ((st & STATU_ACTIVE) == STATU_ACTIVE &&
+        (st & (STATU_FIXPOSX | STATU_FIXPOSY | STATU_FIXPOSZ)) == 0) {
+        const double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+        e = 0.5 * t.cm[a.ityp[i]] * (vx * vx + vy * vy + vz * vz);
+    }
+    ek[i] = e;
+}
+
+void launch_ekin(const AtomsDev& a, const TabDev& t, double* ek, hipStream_t s) {
+    if (a.n > 0) hipLaunchKernelGGL(k_ekin, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a, t, ek);
+}
+
+// One workgroup per box: sum and count of the non-negative EK of the box's atoms (fixed-order
+// strided partial sums, then a fixed-shape tree).
+__global__ __launch_bounds__(kBlock) void k_box_ekin(const double* __restrict__ ek,
+                                                     const int* __restrict__ start,
+                                                     const BoxDev* __restrict__ boxes,
+                                                     double* __restrict__ sum,
+                                                     double* __restrict__ cnt) {
+    __shared__ double ls[kBlock], lc[kBlock];
+    const BoxDev b = boxes[blockIdx.x];
+    const int c0 = b.cell_off, c1 = b.cell_off + b.ncell[0] * b.ncell[1] * b.ncell[2];
+    const int a0 = start[c0], a1 = start[c1];
+    double sv = 0.0, cv = 0.0;
+    for (int i = a0 + threadIdx.x; i < a1; i += kBlock) {
+        const double e = ek[i];
+        if (e >= 0.0) {
+            sv += e;
+            cv += 1.0;
+        }
+    }
+    ls[threadIdx.x] = sv;
+    lc[threadIdx.x] = cv;
+    __syncthreads();
+    for (int d = kBlock / 2; d > 0; d >>= 1) {
+        if (threadIdx.x < d) {
+            ls[threadIdx.x] += ls[threadIdx.x + d];
+            lc[threadIdx.x] += lc[threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sum[blockIdx.x] = ls[0];
+        cnt[blockIdx.x] = lc[0];
+    }
+}
+
+void launch_box_ekin(const double* ek, const int* start, const BoxDev* boxes, int nbox,
+                     double* sum, double* cnt, hipStream_t s) {
+    if (nbox > 0)
+        hipLaunchKernelGGL(k_box_ekin, dim3(nbox), dim3(kBlock), 0, s, ek, start, boxes, sum, cnt);
+}
+
+// VelScaling_KERNEL (:1262-1326): active atoms' velocities times their box's factor; components
+// with FIXPOS or FIXVEL set become zero.
+__global__ void k_velscale(AtomsDev a, const double* __restrict__ scal) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int st = a.statu[i];
+    if ((st & STATU_ACTIVE) != STATU_ACTIVE) return;
+    const double f = scal[a.boxid[i]];
+    a.vx[i] = (st & (STATU_FIXPOSX | STATU_FIXVELX)) ? 0.0 : a.vx[i] * f;
+    a.vy[i] = (st & (STATU_FIXPOSY | STATU_FIXVELY)) ? 0.0 : a.vy[i] * f;
+    a.vz[i] = (st & (STATU_FIXPOSZ | STATU_FIXVELZ)) ? 0.0 : a.vz[i] * f;
+}
+
+void launch_velscale(const AtomsDev& a, const double* scal, hipStream_t s) {
+    if (a.n > 0) hipLaunchKernelGGL(k_velscale, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a, scal);
+}
+
+// max_i |epot[i] - epot0[i]| into *out (bit pattern of a non-negative double, which orders as an
+// unsigned integer); *out must be zero on entry.  Then epot0 = epot.
+__global__ void k_epot_maxdelta(const double* __restrict__ epot, double* __restrict__ epot0,
+                                int n, unsigned long long* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.0;
+    if (i < n) {
+        d = fabs(epot[i] - epot0[i]);
+        epot0[i] = epot[i];
+    }
+    for (int o = 32; o > 0; o >>= 1) d = fmax(d, __shfl_xor(d, o, 64));
+    if ((threadIdx.x & 63) == 0 && d > 0.0)
+        atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(d)));
+}
+
+void launch_epot_maxdelta(const double* epot, double* epot0, int n, unsigned long long* out,
+                          hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_epot_maxdelta, dim3(grid_for(n)), dim3(kBlock), 0, s, epot, epot0, n,
+                           out);
+}
+
+// --------------------------------------------------------------------------------------------
 // copy-out helpers (sorted -> caller order), box totals, list statistics
 // --------------------------------------------------------------------------------------------
 

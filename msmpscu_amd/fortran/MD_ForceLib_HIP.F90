@@ -155,6 +155,38 @@ module MSM_HIP_C
       integer(c_int) :: rc
     end function msm_calc_atomic_stress
 
+    function msm_damping(sim) bind(C, name="msm_damping") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sim
+      integer(c_int) :: rc
+    end function msm_damping
+
+    function msm_calc_ekin(sim, kb, ekin, tbox) bind(C, name="msm_calc_ekin") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: sim
+      real(c_double), value :: kb
+      real(c_double), intent(out) :: ekin(*), tbox(*)
+      integer(c_int) :: rc
+    end function msm_calc_ekin
+
+    function msm_scale_velocities(sim, temp, kb) bind(C, name="msm_scale_velocities") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: sim
+      real(c_double), value :: temp, kb
+      integer(c_int) :: rc
+    end function msm_scale_velocities
+
+    function msm_dyn_damp(sim, maxsteps, mindele, h, steps, delepot) &
+        bind(C, name="msm_dyn_damp") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: sim
+      integer(c_int), value :: maxsteps
+      real(c_double), value :: mindele, h
+      integer(c_int), intent(out) :: steps
+      real(c_double), intent(out) :: delepot
+      integer(c_int) :: rc
+    end function msm_dyn_damp
+
     function msm_predictor(sim, h) bind(C, name="msm_predictor") result(rc)
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: sim
@@ -254,6 +286,7 @@ module MD_ForceLib_HIP
   private
 
   integer, parameter, public :: KINDDF = c_double
+  real(KINDDF), parameter, public :: CP_KB_EV = 8.617333262d-5   ! Boltzmann constant, eV/K
   integer, parameter :: mp_MXFORCE = 64
 
   ! --- MD_TypeDef_ForceTable.F90:46-78: the registered potential routines
@@ -309,6 +342,7 @@ module MD_ForceLib_HIP
     real(KINDDF), allocatable :: RU(:, :), NB_RM(:, :)
     integer :: NB_UPTAB = 10, NB_MXNBS = 800, NUMFTABR = 10000, NUMFTABE = 10000
     real(KINDDF) :: RHOSCAL = 20.0d0, H = 1.0d0
+    real(KINDDF) :: STEEPEST_MiDelE = 1.0d-3   ! eV (MD_TypeDef_SimCtrlParam.F90:217)
     integer :: IFPD(3) = 1
   end type SimMDCtrl
 
@@ -327,7 +361,7 @@ module MD_ForceLib_HIP
   public :: Register_ForceTableProc, Create_Interaction_ForceTable, Register_ForceClass
   public :: Initialize_Globle_Variables_DEV, Init_Forcetable_Dev, Cal_NeighBoreList_DEV
   public :: CalForce_ForceClass, CalEpot_ForceClass, CalPTensor_ForceClass, Cal_AtomicStress
-  public :: Predictor_DEV, Correction_DEV
+  public :: Predictor_DEV, Correction_DEV, Cal_GlobalT_DEV, VelScaling_DEV, Do_DynDamp_Forsteps_DEV
   public :: For_One_Step, Run_Steps_DEV, CopyOut_SimBox_DEV, Copyout_NeighboreList_DEV
   public :: Autotune_DEV, Adjoint_ForceTable_DEV, Clear_Globle_Variables_DEV, MSM_Handle
 
@@ -663,6 +697,55 @@ contains
     if (rc /= MSM_OK) return
     hAVP(1:gm_NPRT, 1:9) = reshape(v(1:9 * gm_NPRT), [gm_NPRT, 9])
   end subroutine Cal_AtomicStress
+
+  ! Cal_GlobalT_DEV (MD_DiffScheme_GPU.F90:1042-1064): 2 <EK> / (3 kB) over the atoms of all boxes
+  ! that count (active, no FIXPOS bit), in K
+  subroutine Cal_GlobalT_DEV(SimBox, CtrlParam, CURT, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    type(SimMDCtrl), intent(in) :: CtrlParam
+    real(KINDDF), intent(out) :: CURT
+    integer, optional, intent(out) :: IERR
+    real(c_double), allocatable :: ek(:), tb(:)
+    integer(c_int) :: rc
+    CURT = 0.0d0
+    allocate(ek(max(gm_NPRT, 1)), tb(max(gm_NBOX, 1)))
+    rc = msm_calc_ekin(gm_SIM, CP_KB_EV, ek, tb)
+    call msm_check(rc, "Cal_GlobalT_DEV", IERR)
+    if (rc /= MSM_OK) return
+    if (count(ek(1:gm_NPRT) >= 0.0d0) > 0) &
+      CURT = 2.0d0 * sum(ek(1:gm_NPRT), mask=(ek(1:gm_NPRT) >= 0.0d0)) / &
+             dble(count(ek(1:gm_NPRT) >= 0.0d0)) / (3.0d0 * CP_KB_EV)
+  end subroutine Cal_GlobalT_DEV
+
+  ! VelScaling_DEV (MD_DiffScheme_GPU.F90:1390-1446): scale the velocities of each box to DT (K)
+  subroutine VelScaling_DEV(SimBox, CtrlParam, DT, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    type(SimMDCtrl), intent(in) :: CtrlParam
+    real(KINDDF), intent(in) :: DT
+    integer, optional, intent(out) :: IERR
+    call msm_check(msm_scale_velocities(gm_SIM, DT, CP_KB_EV), "VelScaling_DEV", IERR)
+  end subroutine VelScaling_DEV
+
+  ! Do_DynDamp_Forsteps_DEV (MD_DiffScheme_GPU.F90:1809-1883): damped dynamics on the current
+  ! neighbour list until the largest per-atom change of EPOT is at most STEEPEST_MiDelE (eV)
+  subroutine Do_DynDamp_Forsteps_DEV(SimBox, CtrlParam, ForceClass, MXNUMSTEPS, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    type(SimMDCtrl), intent(in) :: CtrlParam
+    type(MDForceClassGPU), intent(in) :: ForceClass
+    integer, intent(in) :: MXNUMSTEPS
+    integer, optional, intent(out) :: IERR
+    integer(c_int) :: rc, iter
+    real(c_double) :: delepot
+    rc = msm_dyn_damp(gm_SIM, MXNUMSTEPS, CtrlParam%STEEPEST_MiDelE, CtrlParam%H, iter, delepot)
+    call msm_check(rc, "Do_DynDamp_Forsteps_DEV", IERR)
+    if (rc /= MSM_OK) return
+    if (iter > MXNUMSTEPS) then
+      write(*, "(A, I8)") " MDPSCU WARNING: DnyDamp finished after max steps:  ", iter - 1
+    else
+      write(*, "(A, I8)") " MDPSCU Message: DnyDamp finished after steps:  ", iter
+    end if
+    write(*, "(A, 1PE13.4)") "                 with max energy uncertainty(ev): ", delepot
+  end subroutine Do_DynDamp_Forsteps_DEV
 
   ! Predictor_DEV / Correction_DEV (MD_DiffScheme_GPU.F90:604-682 / 821-847)
   subroutine Predictor_DEV(ITIME, SimBox, CtrlParam, IERR)

@@ -34,6 +34,9 @@ from ._lib import MSM_EAM_TYPE, MSM_FS_TYPE, check, dptr, iptr, lib
 
 PotFn = Union[str, Callable[[float], tuple], None]
 
+CP_KB_EV = 8.617333262e-5  # Boltzmann constant, eV/K
+
+
 
 # ----------------------------------------------------------------------------------------------
 # force tables
@@ -305,6 +308,7 @@ class SimMDCtrl:
     IFPD: Sequence[int] = (1, 1, 1)
     H: float = 1.0
     MULTIBOX: int = 1
+    STEEPEST_MiDelE: float = 1.0e-3  # eV, convergence of the damped dynamics (MD_TypeDef_SimCtrlParam.F90:217)
 
     def __post_init__(self):
         self.RU = np.asfortranarray(np.atleast_2d(self.RU), dtype=np.float64)
@@ -403,6 +407,28 @@ class DeviceMD:
         v = np.zeros(9 * self.n)
         check(lib().msm_calc_atomic_stress(self.h, dptr(v)), "msm_calc_atomic_stress", self.h)
         return v.reshape(9, self.n).T.copy(order="F")
+
+    def damping(self):
+        check(lib().msm_damping(self.h), "msm_damping", self.h)
+
+    def calc_ekin(self, kb: float = CP_KB_EV):
+        """(per-atom EK in caller order, -1e32 for atoms that do not count; temperature per
+        box) -- msm_calc_ekin."""
+        ek = np.zeros(self.n)
+        tb = np.zeros(self.nbox)
+        check(lib().msm_calc_ekin(self.h, kb, dptr(ek), dptr(tb)), "msm_calc_ekin", self.h)
+        return ek, tb
+
+    def scale_velocities(self, temp: float, kb: float = CP_KB_EV):
+        check(lib().msm_scale_velocities(self.h, temp, kb), "msm_scale_velocities", self.h)
+
+    def dyn_damp(self, maxsteps: int, mindele: float, h: float):
+        """Damped dynamics (msm_dyn_damp): returns (iteration it stopped at, last max |dEPOT|)."""
+        st = np.zeros(1, dtype=np.int32)
+        de = np.zeros(1)
+        check(lib().msm_dyn_damp(self.h, int(maxsteps), float(mindele), float(h), iptr(st),
+                                 dptr(de)), "msm_dyn_damp", self.h)
+        return int(st[0]), float(de[0])
 
     def predictor(self, h: float):
         check(lib().msm_predictor(self.h, h), "msm_predictor", self.h)
@@ -574,6 +600,35 @@ def Cal_AtomicStress(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU) 
     """Cal_AtomicStress (Analysis/AtomicVirialStress_GPU.F90:734-773) through the force class's
     atomic-stress routine (CAL_FS/EAM_AtomicStress_KERNEL): hAVP(N, 9) in the caller's order."""
     return ForceClass.Device.calc_atomic_stress()
+
+
+def Cal_GlobalT_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU,
+                    kb: float = CP_KB_EV) -> float:
+    """Cal_GlobalT_DEV (MD_DiffScheme_GPU.F90:1042-1055): 2 <EK> / (3 kB) over all atoms that
+    count (active, no FIXPOS bit)."""
+    ek, _ = ForceClass.Device.calc_ekin(kb)
+    m = ek >= 0.0
+    return 2.0 * float(ek[m].sum()) / float(m.sum()) / (3.0 * kb)
+
+
+def VelScaling_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU, DT: float,
+                   kb: float = CP_KB_EV):
+    """VelScaling_DEV (MD_DiffScheme_GPU.F90:1390-1448): scale each box to temperature DT."""
+    ForceClass.Device.scale_velocities(DT, kb)
+
+
+def Do_DynDamp_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU,
+                            MXNUMSTEPS: int):
+    """Do_DynDamp_Forsteps_DEV (MD_DiffScheme_GPU.F90:1809-1883): damped dynamics until the largest
+    per-atom energy change is at most CtrlParam.STEEPEST_MiDelE; returns (steps, max |dEPOT|)."""
+    it, de = ForceClass.Device.dyn_damp(MXNUMSTEPS, CtrlParam.STEEPEST_MiDelE, CtrlParam.H)
+    if it > MXNUMSTEPS:
+        print(f" MDPSCU WARNING: DnyDamp finished after max steps:  {it - 1}")
+        print(f"                 with max energy uncertainty(ev): {de:13.4E}")
+    else:
+        print(f" MDPSCU Message: DnyDamp finished after steps:  {it}")
+        print(f"                 with max energy uncertainty(ev): {de:13.4E}")
+    return it, de
 
 
 def Predictor_DEV(ITIME: int, SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):

@@ -19,6 +19,9 @@ Parity status of each restated function (how it is pinned):
   orc_adjoint_full     pinned: central finite differences of orc_force_full (FS and EAM, every
                        table, inactive atoms), linearity of the pair-table gradients
   orc_predictor/corr.  pinned: closed-form free flight and constant-force motion, NVE drift
+  orc_damping/ekin/velscale  pinned: component-wise closed forms (v.F < 0 and FIXPOS zeroed,
+                       EK = m v^2 / 2), scaled velocities reach the target temperature exactly
+                       and leave inactive atoms, FIXVEL/FIXPOS components as specified
   orc_drand32          pinned: exact modular arithmetic of L'Ecuyer's generator
   orc_thermalize_mc    pinned: zero net momentum, temperature statistics
 """
@@ -156,7 +159,9 @@ def from_mdlib(ft) -> Table:
 
 
 def _xp(x):
-    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel(order="F"))
+    """(N, 3) -> component-major flat copy (never a view: several oracle routines update it in
+    place, and the caller's array must stay as it was)."""
+    return np.asarray(x, dtype=np.float64).flatten(order="F")
 
 
 def _v3(a, dtype=np.float64):
@@ -297,6 +302,62 @@ def correction(xp1, fp, statu, ityp0, cm, h):
     ek = np.zeros(n)
     lib().orc_correction(n, _d(v), _d(f), _i(st), _i(it), _d(c), C.c_double(h), _d(ek))
     return v.reshape(3, n).T.copy(), ek
+
+
+def damping(xp1, fp, statu):
+    n = int(np.asarray(statu).shape[0])
+    v, f = _xp(xp1), _xp(fp)
+    st = np.ascontiguousarray(statu, dtype=np.int32)
+    lib().orc_damping(n, _d(v), _d(f), _i(st))
+    return v.reshape(3, n).T.copy()
+
+
+def ekin(xp1, statu, ityp0, cm):
+    n = int(np.asarray(statu).shape[0])
+    v = _xp(xp1)
+    st = np.ascontiguousarray(statu, dtype=np.int32)
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    c = np.ascontiguousarray(cm, dtype=np.float64)
+    ek = np.zeros(n)
+    lib().orc_ekin(n, _d(v), _i(st), _i(it), _d(c), _d(ek))
+    return ek
+
+
+def velscale(xp1, statu, ityp0, cm, temp, kb):
+    n = int(np.asarray(statu).shape[0])
+    v = _xp(xp1)
+    st = np.ascontiguousarray(statu, dtype=np.int32)
+    it = np.ascontiguousarray(ityp0, dtype=np.int32)
+    c = np.ascontiguousarray(cm, dtype=np.float64)
+    rc = lib().orc_velscale(n, _d(v), _i(st), _i(it), _d(c), C.c_double(temp), C.c_double(kb))
+    if rc != 0:
+        raise ValueError("oracle velscale: no kinetic energy")
+    return v.reshape(3, n).T.copy()
+
+
+def dyn_damp(xp, xp1, fp, dis, statu, ityp0, cm, h, lo, ln, ifpd, kvois, indi, ru2max,
+             table: Table, maxsteps, mindele):
+    """Do_DynDamp_Forsteps_DEV (MD_DiffScheme_GPU.F90:1809-1883) on the fixed full list
+    (kvois, indi): per iteration damping (DAMPING_KERNEL :125-185), predictor, force and EPOT,
+    exit when max |EPOT - EPOT0| <= mindele (before the corrector), else EPOT0 = EPOT and the
+    corrector.  fp is the force of the entry configuration.  Returns (x, v, f, dis, statu, epot,
+    iteration it stopped at or maxsteps + 1, last max |dEPOT|)."""
+    x, v, f, d = (np.array(a, dtype=np.float64) for a in (xp, xp1, fp, dis))
+    st = np.ascontiguousarray(statu, dtype=np.int32).copy()
+    _, _, ep0 = force_full(x, ityp0, st, kvois, indi, ln, ifpd, ru2max, table)
+    ep = ep0
+    it, de = 1, 0.0
+    while it <= maxsteps:
+        v = damping(v, f, st)
+        x, v, d, st = predictor(x, v, f, d, st, ityp0, cm, h, lo, ln, ifpd)
+        f, _, ep = force_full(x, ityp0, st, kvois, indi, ln, ifpd, ru2max, table)
+        de = float(np.max(np.abs(ep - ep0))) if len(ep) else 0.0
+        if de <= mindele:
+            break
+        ep0 = ep
+        v, _ = correction(v, f, st, ityp0, cm, h)
+        it += 1
+    return x, v, f, d, st, ep, it, de
 
 
 def md_run(xp, xp1, statu, ityp0, cm, h, nsteps, nb_uptab, step0, lo, ln, ifpd, nb_rm, ru, mx,

@@ -775,6 +775,58 @@ void orc_correction(int n, double* xp1, const double* fp, const int* statu, cons
         }
 }
 
+/* DAMPING_KERNEL (MD_DiffScheme_GPU.F90:125-187): a velocity component whose product with the
+ * force component is negative becomes zero, and so does a component with FIXPOS set. */
+void orc_damping(int n, double* xp1, const double* fp, const int* statu) {
+    static const int fixp[3] = {ORC_STATU_FIXPOSX, ORC_STATU_FIXPOSY, ORC_STATU_FIXPOSZ};
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < n; ++j) {
+            size_t a = (size_t)j + (size_t)k * n;
+            if (xp1[a] * fp[a] < 0.0) xp1[a] = 0.0;
+            if (statu[j] & fixp[k]) xp1[a] = 0.0;
+        }
+}
+
+/* CALEKIN_KERNEL (:851-904): EK = CM/2 |v|^2 for active atoms with no FIXPOS bit, else -1e32. */
+void orc_ekin(int n, const double* xp1, const int* statu, const int* ityp, const double* cm,
+              double* ek) {
+    const int fixpos = ORC_STATU_FIXPOSX | ORC_STATU_FIXPOSY | ORC_STATU_FIXPOSZ;
+    for (int j = 0; j < n; ++j) {
+        ek[j] = -1.0e32;
+        if ((statu[j] & ORC_STATU_ACTIVE) == ORC_STATU_ACTIVE && (statu[j] & fixpos) == 0) {
+            double vx = xp1[j], vy = xp1[j + n], vz = xp1[j + 2 * (size_t)n];
+            ek[j] = 0.5 * cm[ityp[j]] * (vx * vx + vy * vy + vz * vz);
+        }
+    }
+}
+
+/* VelScaling_KERNEL (:1262-1326) with the factor of VelScaling_DEV (:1390-1448): velocities of
+ * active atoms times sqrt(temp 3 kb / 2 / <EK>) (mean over the atoms with EK >= 0), components
+ * with FIXPOS or FIXVEL set to zero.  Returns ORC_ERR_ARG when <EK> is not positive. */
+int orc_velscale(int n, double* xp1, const int* statu, const int* ityp, const double* cm,
+                 double temp, double kb) {
+    double* ek = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    double sum = 0.0, cnt = 0.0;
+    orc_ekin(n, xp1, statu, ityp, cm, ek);
+    for (int j = 0; j < n; ++j)
+        if (ek[j] >= 0.0) {
+            sum += ek[j];
+            cnt += 1.0;
+        }
+    free(ek);
+    if (!(cnt > 0.0) || !(sum > 0.0)) return ORC_ERR_ARG;
+    double f = sqrt(temp * 3.0 * kb * 0.5 / (sum / cnt));
+    static const int fix[3] = {ORC_STATU_FIXPOSX | ORC_STATU_FIXVELX, ORC_STATU_FIXPOSY | ORC_STATU_FIXVELY,
+                               ORC_STATU_FIXPOSZ | ORC_STATU_FIXVELZ};
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < n; ++j) {
+            if ((statu[j] & ORC_STATU_ACTIVE) != ORC_STATU_ACTIVE) continue;
+            size_t a = (size_t)j + (size_t)k * n;
+            xp1[a] = (statu[j] & fix[k]) ? 0.0 : xp1[a] * f;
+        }
+    return ORC_OK;
+}
+
 int orc_md_run(int n, double* xp, double* xp1, double* fp, double* dis, int* statu,
                const int* ityp, const double* cm, double h, int nsteps, int nb_uptab, int step0,
                int init, const double lo[3], const double len[3], const int ifpd[3],

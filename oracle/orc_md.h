@@ -143,6 +143,13 @@ int orc_adjoint_full(int n, const double* xp, const int* ityp, const int* statu,
 void orc_predictor(int n, double* xp, double* xp1, const double* fp, double* dis, int* statu,
                    const int* ityp, const double* cm, double h, const double lo[3],
                    const double len[3], const int ifpd[3]);
+/* DAMPING_KERNEL, CALEKIN_KERNEL and VelScaling (MD_DiffScheme_GPU.F90:125-187, 851-904,
+ * 1262-1448) for one box; orc_velscale returns ORC_ERR_ARG when the box has no kinetic energy. */
+void orc_damping(int n, double* xp1, const double* fp, const int* statu);
+void orc_ekin(int n, const double* xp1, const int* statu, const int* ityp, const double* cm,
+              double* ek);
+int orc_velscale(int n, double* xp1, const int* statu, const int* ityp, const double* cm,
+                 double temp, double kb);
 void orc_correction(int n, double* xp1, const double* fp, const int* statu, const int* ityp,
                     const double* cm, double h, double* ekin);
 

@@ -403,3 +403,135 @@ def test_thermalize_zero_momentum_and_temperature(orc):
     assert np.max(np.abs(v.sum(axis=0))) < 1e-12 * n
     temp = cm[0] * np.sum(v ** 2) / (3 * n * kb)
     assert temp == pytest.approx(600.0, rel=0.03)
+
+
+# -------------------------------------------------------------------- thermalisation / quench ---
+
+KB = 8.617333262e-5
+W_CM = 183.84 * 103.6427
+
+
+def _statu_mix(n, rng):
+    """ACTIVE everywhere, then a few atoms with FIXPOS / FIXVEL bits and a few inactive ones."""
+    st = np.ones(n, np.int32)
+    idx = rng.permutation(n)
+    st[idx[:5]] |= 2          # FIXPOSX
+    st[idx[5:10]] |= 4 | 8    # FIXPOSY | FIXPOSZ
+    st[idx[10:15]] |= 16      # FIXVELX
+    st[idx[15:20]] = 0        # inactive
+    return st
+
+
+def test_damping_keeps_only_components_along_the_force(orc):
+    rng = np.random.default_rng(3)
+    n = 400
+    v = rng.normal(0, 1, (n, 3))
+    f = rng.normal(0, 1, (n, 3))
+    f[:7] = 0.0
+    st = _statu_mix(n, rng)
+    v1 = orc.damping(v, f, st)
+    fixpos = np.stack([(st & b) != 0 for b in (2, 4, 8)], axis=1)
+    # every surviving component points along the force, FIXPOS components are gone
+    assert np.all(v1 * f >= 0.0)
+    assert np.all(v1[fixpos] == 0.0)
+    # components that already pointed along the force (or had none) are untouched
+    keep = (v * f >= 0.0) & ~fixpos
+    assert np.array_equal(v1[keep], v[keep])
+    # a second damping changes nothing
+    assert np.array_equal(orc.damping(v1, f, st), v1)
+
+
+def test_ekin_temperature_of_a_maxwell_sample(orc):
+    """Velocities drawn from N(0, kT/m) per component: 2 <EK> / (3 kB) = T within sampling error,
+    and atoms that do not count (inactive or FIXPOS) carry the -1e32 marker."""
+    rng = np.random.default_rng(5)
+    n = 30000
+    temp = 800.0
+    cm = np.array([W_CM, 63.546 * 103.6427])
+    it = rng.integers(0, 2, n).astype(np.int32)
+    v = rng.normal(0, 1, (n, 3)) * np.sqrt(KB * temp / cm[it])[:, None]
+    st = _statu_mix(n, rng)
+    ek = orc.ekin(v, st, it, cm)
+    out = ((st & 1) == 0) | ((st & 14) != 0)
+    assert np.all(ek[out] == -1e32) and np.all(ek[~out] >= 0.0)
+    t = 2.0 * ek[~out].mean() / (3.0 * KB)
+    assert t == pytest.approx(temp, rel=0.02)
+    # a single atom: EK = m |v|^2 / 2 exactly
+    e1 = orc.ekin(np.array([[0.01, -0.02, 0.03]]), np.ones(1, np.int32), np.zeros(1, np.int32),
+                  cm)
+    assert e1[0] == pytest.approx(0.5 * W_CM * 14e-4, rel=1e-15)
+
+
+def test_velocity_scaling_reaches_the_target_temperature(orc):
+    rng = np.random.default_rng(7)
+    n = 2000
+    cm = np.array([W_CM])
+    it = np.zeros(n, np.int32)
+    v = rng.normal(0, 0.003, (n, 3))
+    st = np.ones(n, np.int32)
+    st[:10] = 0
+    v1 = orc.velscale(v, st, it, cm, 450.0, KB)
+    ek = orc.ekin(v1, st, it, cm)
+    assert 2.0 * ek[ek >= 0].mean() / (3.0 * KB) == pytest.approx(450.0, rel=1e-12)
+    # one common factor for every active atom, inactive atoms untouched
+    c = v1[10:] / v[10:]
+    assert np.allclose(c, c.flat[0], rtol=1e-14)
+    assert np.array_equal(v1[:10], v[:10])
+    # FIXVEL / FIXPOS components are set to zero
+    st2 = st.copy()
+    st2[20] |= 16
+    st2[21] |= 2 | 4 | 8
+    v2 = orc.velscale(v, st2, it, cm, 450.0, KB)
+    assert v2[20, 0] == 0.0 and v2[20, 1] != 0.0 and np.all(v2[21] == 0.0)
+    with pytest.raises(ValueError):
+        orc.velscale(np.zeros((n, 3)), st, it, cm, 450.0, KB)
+
+
+def test_dyn_damp_relaxes_to_the_perfect_lattice(orc):
+    """Damped dynamics from a displaced BCC W lattice at rest reaches the perfect-lattice energy
+    E/atom = 1/2 sum_s n_s V(r_s) - sqrt(sum_s n_s rho(r_s)) and near-zero forces."""
+    a = 3.165
+    x, lo, ln = bcc(4, a, 0.04, 11)
+    n = len(x)
+    t = w_table(orc, 100000)
+    it = np.zeros(n, np.int32)
+    st = np.ones(n, np.int32)
+    kv, ind = orc.nlist("full", x, it, st, lo, ln, (1, 1, 1), np.full((1, 1), 4.7002), 96)
+    f0, _, ep0 = orc.force_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t)
+    cm = np.array([W_CM])
+    x1, v1, f1, d1, st1, ep1, steps, de = orc.dyn_damp(
+        x, np.zeros((n, 3)), f0, np.zeros((n, 3)), st, it, cm, 1.0, lo, ln, (1, 1, 1), kv, ind,
+        4.4002 ** 2, t, 3000, 1e-9)
+    shells = [(a * math.sqrt(3) / 2, 8), (a, 6), (a * math.sqrt(2), 12)]
+    r = np.array([s[0] for s in shells])
+    nsh = np.array([s[1] for s in shells])
+    v, rho = w_closed(r)
+    e_atom = 0.5 * np.sum(nsh * v) - math.sqrt(np.sum(nsh * rho))
+    assert steps <= 3000 and de <= 1e-9
+    assert ep0.sum() > n * e_atom + 0.1
+    assert ep1.sum() == pytest.approx(n * e_atom, rel=1e-6)
+    assert np.max(np.abs(f1)) < 1e-2
+    # mindele = 0 never converges: all steps run and the counter ends at maxsteps + 1
+    *_, steps0, _ = orc.dyn_damp(x, np.zeros((n, 3)), f0, np.zeros((n, 3)), st, it, cm, 1.0, lo,
+                                 ln, (1, 1, 1), kv, ind, 4.4002 ** 2, t, 5, 0.0)
+    assert steps0 == 6
+
+
+def test_oracle_integrator_leaves_its_inputs_alone(orc):
+    """The oracle's in-place C routines work on copies, for C- and Fortran-ordered inputs alike."""
+    rng = np.random.default_rng(2)
+    n = 50
+    for order in ("C", "F"):
+        x = np.asarray(rng.uniform(-1, 1, (n, 3)), order=order)
+        v = np.asarray(rng.normal(0, 1, (n, 3)), order=order)
+        f = np.asarray(rng.normal(0, 1, (n, 3)), order=order)
+        keep = [a.copy() for a in (x, v, f)]
+        st = np.ones(n, np.int32)
+        it = np.zeros(n, np.int32)
+        orc.predictor(x, v, f, np.zeros((n, 3), order=order), st, it, np.array([2.0]), 0.5,
+                      np.full(3, -50.0), np.full(3, 100.0), (1, 1, 1))
+        orc.correction(v, f, st, it, np.array([2.0]), 0.5)
+        orc.damping(v, f, st)
+        orc.velscale(v, st, it, np.array([2.0]), 300.0, KB)
+        for a, b in zip((x, v, f), keep):
+            assert np.array_equal(a, b)
