@@ -205,6 +205,23 @@ int msm_calc_ekin(msm_sim* sim, double kb, double* ekin, double* tbox);
 int msm_scale_velocities(msm_sim* sim, double temp, double kb);
 int msm_dyn_damp(msm_sim* sim, int maxsteps, double mindele, double h, int* steps,
                  double* delepot);
+/* Steepest-descent quench on the current neighbour list (MD_SteepestScheme_GPU.F90), lengths in
+ * the caller's unit (the reference passes STEEPEST_MxStep and STEEPEST_MiStep times the lattice
+ * unit RR), energies in the caller's unit.
+ * linesearch = 0: Do_Steepest0_Forsteps_DEV (:20-152) -- moves of alpha_k F, alpha_k from
+ *   <dx, dF> / <dF, dF> of the last two steps (alpha when negative), the largest component capped
+ *   at maxdis; stops when that largest component (before the cap) is <= mindis or the largest
+ *   per-atom energy change is <= mindele.  steps receives the iteration it stopped at (0 when the
+ *   first move is already below mindis, maxsteps + 1 if it did not converge).
+ * linesearch = 1: Do_Steepest1_Forsteps_DEV (:156-260) -- along the normalised force, secant
+ *   steps on F . D (capped at maxdis) until a step is <= mindis, repeated until the per-atom
+ *   energy change of a line search is <= mindele (the reference uses 1e-3 eV); steps receives the
+ *   number of force evaluations of the line searches (> maxsteps if it did not converge).
+ * Positions move by the displacement with the bounds of DevAdd (periodic directions wrap once);
+ * atoms are moved whatever their FIXPOS bits, as in the reference.  delepot receives the last
+ * max |dEPOT|, maxmove the last largest displacement component (or line-search step). */
+int msm_steepest(msm_sim* sim, int maxsteps, int linesearch, double alpha, double maxdis,
+                 double mindis, double mindele, int* steps, double* delepot, double* maxmove);
 int msm_predictor(msm_sim* sim, double h);
 int msm_correction(msm_sim* sim, double h);
 

@@ -111,7 +111,7 @@ def make_bcc_box(ncell: Sequence[int], a: float, sigma: float, temp: float, seed
     ityp = np.ones(n, dtype=np.int32)
     cm = np.array([mass * AMU])
     v = maxwell_velocities(ityp, cm, temp, seed) if temp > 0 else np.zeros((n, 3), order="F")
-    return SimMDBox(NPRT=n, NGROUP=1, ZL=ln, BOXLOW=lo, XP=np.asfortranarray(x), XP1=v,
+    return SimMDBox(NPRT=n, NGROUP=1, ZL=ln, BOXLOW=lo, XP=np.asfortranarray(x), XP1=v, RR=a,
                     ITYP=ityp, STATU=np.ones(n, dtype=np.int32), CM=cm,
                     PTYPE=np.array([[1]], dtype=np.int32), SYMB=["W"])
 
@@ -153,7 +153,7 @@ def make_cuni_slab(ncell: Sequence[int], a: float, temp: float, seed: int,
     x = wrap_into(x, lo, ln, (1, 1, 0))
     cm = np.array([CU_MASS * AMU, NI_MASS * AMU])
     v = maxwell_velocities(ityp, cm, temp, seed) if temp > 0 else np.zeros((n, 3), order="F")
-    return SimMDBox(NPRT=n, NGROUP=2, ZL=ln, BOXLOW=lo, XP=np.asfortranarray(x), XP1=v,
+    return SimMDBox(NPRT=n, NGROUP=2, ZL=ln, BOXLOW=lo, XP=np.asfortranarray(x), XP1=v, RR=a,
                     ITYP=ityp, STATU=np.ones(n, dtype=np.int32), CM=cm,
                     PTYPE=np.array([[1, 3], [3, 2]], dtype=np.int32), SYMB=["Cu", "Ni"])
 

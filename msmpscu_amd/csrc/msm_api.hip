@@ -1343,6 +1343,202 @@ int msm_dyn_damp(msm_sim* s, int maxsteps, double mindele, double h, int* steps,
     return read_flags(s);
 }
 
+namespace {
+
+// Device scratch of one steepest-descent call (freed on every exit) and the small steps it is
+// built from; every value the host branches on is read back with a stream synchronisation.
+struct Quench {
+    msm_sim* s;
+    int n = 0;
+    double* pf = nullptr;     // previous forces (dPreFP) or the unit direction (dDirect)
+    double* dx = nullptr;     // displacement of the next move (dDXP)
+    double* epot0 = nullptr;  // per-atom energy of the reference configuration (dEPOT0)
+    double* part = nullptr;   // partial sums of the reductions
+    double* out = nullptr;    // reduced values
+    unsigned long long* bits = nullptr;
+    hipError_t e = hipSuccess;
+
+    explicit Quench(msm_sim* sim) : s(sim), n(sim->n) {}
+    ~Quench() {
+        dfree(&pf);
+        dfree(&dx);
+        dfree(&epot0);
+        dfree(&part);
+        dfree(&out);
+        dfree(&bits);
+    }
+    int alloc() {
+        const size_t m = static_cast<size_t>(std::max(n, 1));
+        int rc = dalloc(s, &pf, 3 * m);
+        if (rc == MSM_OK) rc = dalloc(s, &dx, 3 * m);
+        if (rc == MSM_OK) rc = dalloc(s, &epot0, m);
+        if (rc == MSM_OK) rc = dalloc(s, &part, 2 * static_cast<size_t>(msm::kQuenchBlocks));
+        if (rc == MSM_OK) rc = dalloc(s, &out, 2);
+        if (rc == MSM_OK) rc = dalloc(s, &bits, 1);
+        return rc;
+    }
+    msm::P3 vec(double* base) const { return {base, base + n, base + 2 * static_cast<size_t>(n)}; }
+    msm::P3 forces() const {
+        AtomsDev a = atoms_view(s, s->scur, s->pcur);
+        return {a.fx, a.fy, a.fz};
+    }
+    bool ok() const { return e == hipSuccess; }
+    // CalForce_ForceClass: density and force passes on the current positions, forces stored
+    void force() {
+        enqueue_density(s);
+        enqueue_force(s, 0, 1, 0.0);
+    }
+    // UpdateEPOT_ForceClass
+    void epot() {
+        msm::launch_epot(atoms_view(s, s->scur, s->pcur), list_view(s), s->tab, s->boxes[0],
+                         s->d_boxes, multibox(s), s->d_err, s->stream);
+    }
+    void copy_epot0() {
+        if (ok())
+            e = hipMemcpyAsync(epot0, s->epot, std::max(n, 1) * sizeof(double),
+                               hipMemcpyDeviceToDevice, s->stream);
+    }
+    double read_bits() {
+        unsigned long long b = 0;
+        if (ok()) e = hipMemcpyAsync(&b, bits, sizeof(b), hipMemcpyDeviceToHost, s->stream);
+        if (ok()) e = hipStreamSynchronize(s->stream);
+        double v = 0.0;
+        std::memcpy(&v, &b, sizeof(v));
+        return ok() ? v : 0.0;
+    }
+    // DevMaxAbsval of a 3-component vector
+    double maxabs(msm::P3 v) {
+        if (ok()) e = hipMemsetAsync(bits, 0, sizeof(unsigned long long), s->stream);
+        if (ok()) msm::launch_maxabs3(v, n, bits, s->stream);
+        return read_bits();
+    }
+    // max |EPOT - EPOT0| over atoms, then EPOT0 = EPOT
+    double maxdelta() {
+        if (ok()) e = hipMemsetAsync(bits, 0, sizeof(unsigned long long), s->stream);
+        if (ok()) msm::launch_epot_maxdelta(s->epot, epot0, n, bits, s->stream);
+        return read_bits();
+    }
+    void read_out(double* h, int k) {
+        if (ok()) e = hipMemcpyAsync(h, out, k * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+        if (ok()) e = hipStreamSynchronize(s->stream);
+    }
+    double dot(msm::P3 u, msm::P3 v) {
+        double r = 0.0;
+        if (ok()) msm::launch_dot3(u, v, n, part, out, s->stream);
+        read_out(&r, 1);
+        return r;
+    }
+    void scale(msm::P3 src, double c, msm::P3 dst) {
+        if (ok()) msm::launch_scale3(src, c, n, dst, s->stream);
+    }
+    void move(msm::P3 d) {
+        if (ok())
+            msm::launch_move3(atoms_view(s, s->scur, s->pcur), d, s->boxes[0], s->d_boxes,
+                              multibox(s), s->stream);
+    }
+};
+
+}  // namespace
+
+int msm_steepest(msm_sim* s, int maxsteps, int linesearch, double alpha, double maxdis,
+                 double mindis, double mindele, int* steps, double* delepot, double* maxmove) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (maxsteps < 0 || !(maxdis > 0.0) || !(mindis >= 0.0) || !(alpha > 0.0))
+        return set_err(s, MSM_ERR_ARG, "maxsteps >= 0, maxdis > 0, mindis >= 0 and alpha > 0");
+    (void)hipSetDevice(s->device);
+    Quench q(s);
+    if ((rc = q.alloc()) != MSM_OK) return rc;
+    clear_flags(s);
+    const msm::P3 DX = q.vec(q.dx), PF = q.vec(q.pf);
+    int it = 0;
+    double del = 0.0, mm = 0.0;
+    if (!linesearch) {
+        // Do_Steepest0_Forsteps_DEV (MD_SteepestScheme_GPU.F90:20-152)
+        q.force();
+        q.scale(q.forces(), 1.0, PF);
+        q.scale(q.forces(), alpha, DX);
+        mm = q.maxabs(DX);
+        bool done = false;
+        if (mm > maxdis)
+            q.scale(DX, maxdis / mm, DX);
+        else if (mm <= mindis)
+            done = true;  // converged at the first step: nothing moves
+        if (!done) {
+            q.epot();
+            q.copy_epot0();
+            q.move(DX);
+            int iflag = 0;
+            for (int i = 1; q.ok() && i <= maxsteps; ++i) {
+                q.force();
+                double d2[2] = {0.0, 0.0};
+                if (q.ok()) msm::launch_dxdf(DX, PF, q.forces(), q.n, q.part, q.out, s->stream);
+                q.read_out(d2, 2);
+                // ALPHA = DOTDXDF / DOTDF, replaced by STEEPEST_Alpha when negative (also when the
+                // forces did not change, where the reference divides by zero)
+                double al = d2[1] > 0.0 ? d2[0] / d2[1] : -1.0;
+                if (al < 0.0) al = alpha;
+                q.scale(q.forces(), al, DX);
+                mm = q.maxabs(DX);
+                if (mm > maxdis) q.scale(DX, maxdis / mm, DX);
+                q.move(DX);
+                if (mm <= mindis) {
+                    iflag = i;
+                    break;
+                }
+                q.epot();
+                del = q.maxdelta();
+                if (del <= mindele) {
+                    iflag = i;
+                    break;
+                }
+                q.scale(q.forces(), 1.0, PF);
+            }
+            it = iflag ? iflag : maxsteps + 1;
+        }
+    } else {
+        // Do_Steepest1_Forsteps_DEV (MD_SteepestScheme_GPU.F90:156-260): line search along the
+        // normalised force
+        const msm::P3 D = PF;
+        double step = maxdis;
+        while (q.ok() && it <= maxsteps) {
+            q.force();
+            q.epot();
+            q.copy_epot0();
+            const double nrm = q.dot(q.forces(), q.forces());
+            if (!(nrm > 0.0)) break;  // no force: already at a stationary point
+            q.scale(q.forces(), 1.0 / std::sqrt(nrm), D);
+            step = maxdis;
+            q.scale(D, step, DX);
+            double pf0 = q.dot(q.forces(), D);
+            while (q.ok() && it <= maxsteps) {
+                q.move(DX);
+                q.force();
+                const double pf1 = q.dot(q.forces(), D);
+                // secant step to the zero of F . D; equal projections (a division by zero in the
+                // reference) take the largest step in the downhill direction
+                step = pf1 != pf0 ? -step * pf1 / (pf1 - pf0)
+                                  : (-step * pf1 >= 0.0 ? maxdis : -maxdis);
+                if (std::fabs(step) > maxdis) step = step > 0.0 ? maxdis : -maxdis;
+                q.scale(D, step, DX);
+                ++it;
+                if (std::fabs(step) <= mindis) break;
+                pf0 = pf1;
+            }
+            q.epot();
+            del = q.maxdelta();
+            mm = std::fabs(step);
+            if (del <= mindele) break;
+        }
+    }
+    if (q.ok()) q.e = hipStreamSynchronize(s->stream);
+    if (!q.ok()) return set_err(s, MSM_ERR_HIP, hipGetErrorString(q.e));
+    if (steps) *steps = it;
+    if (delepot) *delepot = del;
+    if (maxmove) *maxmove = mm;
+    return read_flags(s);
+}
+
 int msm_predictor(msm_sim* s, double h) {
     int rc = check_ready(s, false);
     if (rc != MSM_OK) return rc;

@@ -228,6 +228,21 @@ void launch_box_ekin(const double* ek, const int* start, const BoxDev* boxes, in
 void launch_velscale(const AtomsDev& a, const double* scal, hipStream_t s);
 void launch_epot_maxdelta(const double* epot, double* epot0, int n, unsigned long long* out,
                           hipStream_t s);
+
+// steepest-descent quench helpers (3-component vectors as three arrays of n doubles)
+struct P3 {
+    double* x;
+    double* y;
+    double* z;
+};
+constexpr int kQuenchBlocks = 1024;   // most workgroups of the dot-product reductions
+int quench_blocks(int n);             // workgroups (and partials per component) used for n
+void launch_dot3(P3 u, P3 v, int n, double* part, double* out, hipStream_t s);
+void launch_dxdf(P3 dx, P3 p, P3 f, int n, double* part, double* out, hipStream_t s);
+void launch_scale3(P3 src, double c, int n, P3 dst, hipStream_t s);
+void launch_maxabs3(P3 v, int n, unsigned long long* out, hipStream_t s);
+void launch_move3(const AtomsDev& a, P3 dx, const BoxDev& box0, const BoxDev* boxes, bool mb,
+                  hipStream_t s);
 // AVP(N, 9) in caller order (column-major) from launch_virial's per-atom halves.
 void launch_unsort_stress(const double* vir, double* avp, const int* gid, int n, hipStream_t s);
 void launch_box_sum(const double* epot, const int* start, const BoxDev* boxes, int nbox,

@@ -14,6 +14,7 @@
 //   force            MD_FS_ForceTable_GPU.F90:623-806, MD_EAM_ForceTable_GPU.F90:646-829
 //   energy           MD_FS_ForceTable_GPU.F90:1451-1609, MD_EAM_ForceTable_GPU.F90:1470-1636
 //   integrator       MD_DiffScheme_GPU.F90:242-384 (Predictor_KERNEL0), :686-758 (Correction_KERNEL)
+#include <algorithm>
 #include <climits>
 
 #include "msm_internal.h"
@@ -707,6 +708,140 @@ void launch_epot_maxdelta(const double* epot, double* epot0, int n, unsigned lon
     if (n > 0)
         hipLaunchKernelGGL(k_epot_maxdelta, dim3(grid_for(n)), dim3(kBlock), 0, s, epot, epot0, n,
                            out);
+}
+
+// --------------------------------------------------------------------------------------------
+// steepest-descent quench (MD_SteepestScheme_GPU.F90) on 3-component vectors held as three
+// arrays of n doubles, like the forces; the reductions of DevDot_product / DevMaxAbsVal
+// (MSM_MultiGPU_Basic.F90) are deterministic: fixed grid, fixed-order partial sums, fixed tree
+// --------------------------------------------------------------------------------------------
+
+int quench_blocks(int n) { return std::max(1, std::min(kQuenchBlocks, (n + kBlock - 1) / kBlock)); }
+
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double* __restrict__ part) {
+    __shared__ double ls[NV][kBlock];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) ls[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int d = kBlock / 2; d > 0; d >>= 1) {
+        if (threadIdx.x < d) {
+#pragma unroll
+            for (int c = 0; c < NV; ++c) ls[c][threadIdx.x] += ls[c][threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) part[NV * blockIdx.x + c] = ls[c][0];
+    }
+}
+
+// partial sums of u . v
+__global__ __launch_bounds__(kBlock) void k_dot3(P3 u, P3 v, int n, double* __restrict__ part) {
+    double acc[1] = {0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        acc[0] += u.x[i] * v.x[i] + u.y[i] * v.y[i] + u.z[i] * v.z[i];
+    block_sums<1>(acc, part);
+}
+
+// partial sums of dx . (p - f) and (p - f) . (p - f)  (DOTDXDF and DOTDF of Do_Steepest0)
+__global__ __launch_bounds__(kBlock) void k_dxdf(P3 dx, P3 p, P3 f, int n,
+                                                 double* __restrict__ part) {
+    double acc[2] = {0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double gx = p.x[i] - f.x[i], gy = p.y[i] - f.y[i], gz = p.z[i] - f.z[i];
+        acc[0] += dx.x[i] * gx + dx.y[i] * gy + dx.z[i] * gz;
+        acc[1] += gx * gx + gy * gy + gz * gz;
+    }
+    block_sums<2>(acc, part);
+}
+
+// out[c] = sum of the nblk partials of component c, in block order then a fixed tree
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_sum_parts(const double* __restrict__ part, int nblk,
+                                                      double* __restrict__ out) {
+    double acc[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) acc[c] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kBlock) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) acc[c] += part[NV * b + c];
+    }
+    block_sums<NV>(acc, out);
+}
+
+void launch_dot3(P3 u, P3 v, int n, double* part, double* out, hipStream_t s) {
+    const int nb = quench_blocks(n);
+    hipLaunchKernelGGL(k_dot3, dim3(nb), dim3(kBlock), 0, s, u, v, n, part);
+    hipLaunchKernelGGL(k_sum_parts<1>, dim3(1), dim3(kBlock), 0, s, part, nb, out);
+}
+
+void launch_dxdf(P3 dx, P3 p, P3 f, int n, double* part, double* out, hipStream_t s) {
+    const int nb = quench_blocks(n);
+    hipLaunchKernelGGL(k_dxdf, dim3(nb), dim3(kBlock), 0, s, dx, p, f, n, part);
+    hipLaunchKernelGGL(k_sum_parts<2>, dim3(1), dim3(kBlock), 0, s, part, nb, out);
+}
+
+// dst = c * src (DevMultiply_noshift; dst may be src)
+__global__ void k_scale3(P3 src, double c, int n, P3 dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst.x[i] = c * src.x[i];
+    dst.y[i] = c * src.y[i];
+    dst.z[i] = c * src.z[i];
+}
+
+void launch_scale3(P3 src, double c, int n, P3 dst, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_scale3, dim3(grid_for(n)), dim3(kBlock), 0, s, src, c, n, dst);
+}
+
+// max |component| into *out as the bit pattern of a non-negative double (*out zero on entry)
+__global__ void k_maxabs3(P3 v, int n, unsigned long long* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.0;
+    if (i < n) d = fmax(fabs(v.x[i]), fmax(fabs(v.y[i]), fabs(v.z[i])));
+    for (int o = 32; o > 0; o >>= 1) d = fmax(d, __shfl_xor(d, o, 64));
+    if ((threadIdx.x & 63) == 0 && d > 0.0)
+        atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(d)));
+}
+
+void launch_maxabs3(P3 v, int n, unsigned long long* out, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_maxabs3, dim3(grid_for(n)), dim3(kBlock), 0, s, v, n, out);
+}
+
+// x += dx with the bounds of DevAdd (AddBD_DevVec_DF_KERNEL0, MSM_MultiGPU_Basic.F90:5444-5477):
+// in a periodic direction a coordinate above BOXUP moves down by the box length, one below
+// BOXLOW up by it; other directions are not bounded.  F' in pos.w is left alone.
+template <int MB>
+__global__ void k_move3(AtomsDev a, P3 dx, BoxDev box0, const BoxDev* __restrict__ boxes) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
+    double4 p = a.pos[i];
+    double x[3] = {p.x + dx.x[i], p.y + dx.y[i], p.z + dx.z[i]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (!b.ifpd[k]) continue;
+        const double lb = b.lo[k], hb = b.lo[k] + b.len[k];
+        if (x[k] > hb)
+            x[k] = x[k] - (hb - lb);
+        else if (x[k] < lb)
+            x[k] = x[k] + (hb - lb);
+    }
+    p.x = x[0];
+    p.y = x[1];
+    p.z = x[2];
+    a.pos[i] = p;
+}
+
+void launch_move3(const AtomsDev& a, P3 dx, const BoxDev& box0, const BoxDev* boxes, bool mb,
+                  hipStream_t s) {
+    if (a.n <= 0) return;
+    if (mb)
+        hipLaunchKernelGGL(k_move3<1>, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a, dx, box0, boxes);
+    else
+        hipLaunchKernelGGL(k_move3<0>, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a, dx, box0, boxes);
 }
 
 // --------------------------------------------------------------------------------------------

@@ -20,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "msm_brick.h"
 #include "msm_lanepair.h"
@@ -34,10 +35,9 @@ constexpr int kGroup = 8;
 constexpr int kGroupD = 6;
 constexpr int kGroupF = 4;
 
-// TPA = 3 stands for two lanes per atom paired as lanes l and l + 32 (lane-pair gathers).
+// TPA = 3 stands for two lanes per atom (lanes 2m, 2m + 1) sharing their position gathers.
 template <int TPA>
 __device__ __forceinline__ double lane_sum(double v) {
-    if constexpr (TPA == 3) return v + __shfl_xor(v, 32, 64);
 #pragma unroll
     for (int d = TPA / 2; d >= 1; d >>= 1) v = v + __shfl_xor(v, d, 64);
     return v;
@@ -49,19 +49,14 @@ constexpr int slot_step() {
     return TPA == 3 ? 2 : TPA;
 }
 
-// Atom and lane-within-atom of this thread in chunk c: TPA consecutive lanes per atom, or (TPA
-// = 3) lanes l and l + 32 of a wave for atom 32 * wave + l.
+// Atom and lane-within-atom of this thread in chunk c: TPA consecutive lanes per atom; TPA = 3
+// is the two-lane layout (lanes 2m, 2m + 1) with the pair's position gathers shared.
 template <int TPA>
 __device__ __forceinline__ void atom_of_lane(int c, int& i, int& sub) {
+    constexpr int L = TPA == 3 ? 2 : TPA;
     const int gt = c * blockDim.x + threadIdx.x;
-    if constexpr (TPA == 3) {
-        const int lane = threadIdx.x & 63;
-        i = (gt >> 6) * 32 + (lane & 31);
-        sub = lane >> 5;
-    } else {
-        i = gt / TPA;
-        sub = gt % TPA;
-    }
+    i = gt / L;
+    sub = gt % L;
 }
 
 // chunk range [c0, c1) of this workgroup
@@ -563,20 +558,30 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
                     den0 = density_group<1, MT, 3>(a, l, t, b, w, s_pb, li, i, ti, kv, base + sub,
                                                    pxy, pz, den0, sub);
             } else {
-                int base = sub;
-                int js[kGroupD];
-                unsigned long long mk = 0;  // this lane's in-cutoff slots (for the force pass)
-                load_slots<kGroupD, TPA>(li, l.stride, base, kv, i, js);
-                for (; base + (kGroupD - 1) * TPA < kv; base += TPA * kGroupD)
-                    den0 = density_group_pf<kGroupD, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv,
-                                                              base, pxy, pz, den0, js, &mk);
-                for (; base + 3 * TPA < kv; base += TPA * 4)
-                    den0 = density_group<4, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy,
-                                                     pz, den0, sub, &mk);
-                for (; base < kv; base += TPA)
-                    den0 = density_group<1, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy,
-                                                     pz, den0, sub, &mk);
-                if (l.cmask) l.cmask[static_cast<size_t>(i) * TPA + sub] = mk;
+                // two copies of the slot loop: the in-cutoff slot mask is built only when the
+                // force pass reads it (a run-time flag would keep its bit arithmetic in the loop)
+                auto slots = [&](auto with_mask) {
+                    constexpr bool WM = decltype(with_mask)::value;
+                    int base = sub;
+                    int js[kGroupD];
+                    unsigned long long mk = 0;  // this lane's in-cutoff slots
+                    unsigned long long* mkp = WM ? &mk : nullptr;
+                    load_slots<kGroupD, TPA>(li, l.stride, base, kv, i, js);
+                    for (; base + (kGroupD - 1) * TPA < kv; base += TPA * kGroupD)
+                        den0 = density_group_pf<kGroupD, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti,
+                                                                  kv, base, pxy, pz, den0, js, mkp);
+                    for (; base + 3 * TPA < kv; base += TPA * 4)
+                        den0 = density_group<4, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base,
+                                                         pxy, pz, den0, sub, mkp);
+                    for (; base < kv; base += TPA)
+                        den0 = density_group<1, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base,
+                                                         pxy, pz, den0, sub, mkp);
+                    if constexpr (WM) l.cmask[static_cast<size_t>(i) * TPA + sub] = mk;
+                };
+                if (l.cmask)
+                    slots(std::true_type{});
+                else
+                    slots(std::false_type{});
             }
         }
         den0 = lane_sum<TPA>(den0);

@@ -23,15 +23,15 @@ namespace {
 constexpr int kGroup = 8;
 
 __device__ __forceinline__ double pair_sum(double v) {
-    // lane l < 32 adds its partner's value (lane l + 32) in the order of the two-lane kernels
-    return v + __shfl_xor(v, 32, 64);
+    // each lane adds its partner's value, in the order of the two-lane kernels
+    return v + __shfl_xor(v, 1, 64);
 }
 
+// lanes 2m and 2m + 1 of the grid share atom m
 __device__ __forceinline__ void pair_atom(int& i, int& sub) {
     const int gt = xcd_block(gridDim.x) * blockDim.x + threadIdx.x;
-    const int lane = gt & 63;
-    i = (gt >> 6) * 32 + (lane & 31);
-    sub = lane >> 5;
+    i = gt >> 1;
+    sub = gt & 1;
 }
 
 }  // namespace

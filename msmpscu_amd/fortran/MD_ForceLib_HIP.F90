@@ -187,6 +187,17 @@ module MSM_HIP_C
       integer(c_int) :: rc
     end function msm_dyn_damp
 
+    function msm_steepest(sim, maxsteps, linesearch, alpha, maxdis, mindis, mindele, steps, &
+        delepot, maxmove) bind(C, name="msm_steepest") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: sim
+      integer(c_int), value :: maxsteps, linesearch
+      real(c_double), value :: alpha, maxdis, mindis, mindele
+      integer(c_int), intent(out) :: steps
+      real(c_double), intent(out) :: delepot, maxmove
+      integer(c_int) :: rc
+    end function msm_steepest
+
     function msm_predictor(sim, h) bind(C, name="msm_predictor") result(rc)
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: sim
@@ -287,6 +298,10 @@ module MD_ForceLib_HIP
 
   integer, parameter, public :: KINDDF = c_double
   real(KINDDF), parameter, public :: CP_KB_EV = 8.617333262d-5   ! Boltzmann constant, eV/K
+  ! quench schemes (MD_Const.F90:23-27)
+  integer, parameter, public :: CP_DAMPSCHEME_LBFGS = 0, CP_DAMPSCHEME_DYN = 1, &
+                                CP_DAMPSCHEME_CG = 2, CP_DAMPSCHEME_ST = 3, &
+                                CP_DAMPSCHEME_LSEARCH = 65536
   integer, parameter :: mp_MXFORCE = 64
 
   ! --- MD_TypeDef_ForceTable.F90:46-78: the registered potential routines
@@ -333,6 +348,7 @@ module MD_ForceLib_HIP
   type, public :: SimMDBox
     integer :: NPRT = 0, NGROUP = 1
     real(KINDDF) :: ZL(3) = 0.0d0, BOXLOW(3) = 0.0d0
+    real(KINDDF) :: RR = 1.0d0   ! lattice unit, in this layer's length unit (Angstrom)
     real(KINDDF), allocatable :: XP(:, :), XP1(:, :), FP(:, :), DIS(:, :), EPOT(:), CM(:)
     integer, allocatable :: ITYP(:), STATU(:), PTYPE(:, :)
   end type SimMDBox
@@ -342,7 +358,9 @@ module MD_ForceLib_HIP
     real(KINDDF), allocatable :: RU(:, :), NB_RM(:, :)
     integer :: NB_UPTAB = 10, NB_MXNBS = 800, NUMFTABR = 10000, NUMFTABE = 10000
     real(KINDDF) :: RHOSCAL = 20.0d0, H = 1.0d0
-    real(KINDDF) :: STEEPEST_MiDelE = 1.0d-3   ! eV (MD_TypeDef_SimCtrlParam.F90:217)
+    ! quench controls (MD_TypeDef_SimCtrlParam.F90:213-217): moves in lattice units RR, energy in eV
+    real(KINDDF) :: STEEPEST_Alpha = 0.1d0, STEEPEST_MiStep = 1.0d-5, STEEPEST_MxStep = 0.1d0
+    real(KINDDF) :: STEEPEST_MiDelE = 1.0d-3
     integer :: IFPD(3) = 1
   end type SimMDCtrl
 
@@ -362,6 +380,7 @@ module MD_ForceLib_HIP
   public :: Initialize_Globle_Variables_DEV, Init_Forcetable_Dev, Cal_NeighBoreList_DEV
   public :: CalForce_ForceClass, CalEpot_ForceClass, CalPTensor_ForceClass, Cal_AtomicStress
   public :: Predictor_DEV, Correction_DEV, Cal_GlobalT_DEV, VelScaling_DEV, Do_DynDamp_Forsteps_DEV
+  public :: Do_Steepest_Forsteps_DEV
   public :: For_One_Step, Run_Steps_DEV, CopyOut_SimBox_DEV, Copyout_NeighboreList_DEV
   public :: Autotune_DEV, Adjoint_ForceTable_DEV, Clear_Globle_Variables_DEV, MSM_Handle
 
@@ -746,6 +765,36 @@ contains
     end if
     write(*, "(A, 1PE13.4)") "                 with max energy uncertainty(ev): ", delepot
   end subroutine Do_DynDamp_Forsteps_DEV
+
+  ! Do_Steepest_Forsteps_DEV (MD_SteepestScheme_GPU.F90:264-290): steepest descent, with the line
+  ! search when METH has the CP_DAMPSCHEME_LSEARCH bit (energy tolerance 1e-3 eV, as the
+  ! reference); moves bounded by STEEPEST_MxStep / STEEPEST_MiStep times SimBox(1)%RR
+  subroutine Do_Steepest_Forsteps_DEV(SimBox, CtrlParam, ForceClass, MXNUMSTEPS, METH, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    type(SimMDCtrl), intent(in) :: CtrlParam
+    type(MDForceClassGPU), intent(in) :: ForceClass
+    integer, intent(in) :: MXNUMSTEPS, METH
+    integer, optional, intent(out) :: IERR
+    integer(c_int) :: rc, iter, ls
+    real(c_double) :: delepot, maxmove, mindele
+    ls = 0
+    mindele = CtrlParam%STEEPEST_MiDelE
+    if (iand(METH, CP_DAMPSCHEME_LSEARCH) > 0) then
+      ls = 1
+      mindele = 1.0d-3
+    end if
+    rc = msm_steepest(gm_SIM, MXNUMSTEPS, ls, CtrlParam%STEEPEST_Alpha, &
+                      CtrlParam%STEEPEST_MxStep * SimBox(1)%RR, &
+                      CtrlParam%STEEPEST_MiStep * SimBox(1)%RR, mindele, iter, delepot, maxmove)
+    call msm_check(rc, "Do_Steepest_Forsteps_DEV", IERR)
+    if (rc /= MSM_OK) return
+    if (iter > MXNUMSTEPS) then
+      write(*, "(A, I8)") " MDPSCU WARNING: steepest finished after max steps: ", iter
+    else
+      write(*, "(A, I8)") " MDPSCU Message: steepest finished after steps: ", iter
+    end if
+    write(*, "(A, 1PE13.4)") "                 with max energy uncertainty(ev): ", delepot
+  end subroutine Do_Steepest_Forsteps_DEV
 
   ! Predictor_DEV / Correction_DEV (MD_DiffScheme_GPU.F90:604-682 / 821-847)
   subroutine Predictor_DEV(ITIME, SimBox, CtrlParam, IERR)

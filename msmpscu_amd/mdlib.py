@@ -35,6 +35,9 @@ from ._lib import MSM_EAM_TYPE, MSM_FS_TYPE, check, dptr, iptr, lib
 PotFn = Union[str, Callable[[float], tuple], None]
 
 CP_KB_EV = 8.617333262e-5  # Boltzmann constant, eV/K
+# quench schemes (MD_Const.F90:23-27); LSEARCH is a bit added to the scheme
+CP_DAMPSCHEME_LBFGS, CP_DAMPSCHEME_DYN, CP_DAMPSCHEME_CG, CP_DAMPSCHEME_ST = 0, 1, 2, 3
+CP_DAMPSCHEME_LSEARCH = 2 ** 16
 
 
 
@@ -272,6 +275,7 @@ class SimMDBox:
     DIS: Optional[np.ndarray] = None
     EPOT: Optional[np.ndarray] = None
     VTENSOR: Optional[np.ndarray] = None
+    RR: float = 1.0  # lattice unit in this layer's length unit (the reference keeps it in cm)
 
     @property
     def BOXUP(self) -> np.ndarray:
@@ -308,7 +312,12 @@ class SimMDCtrl:
     IFPD: Sequence[int] = (1, 1, 1)
     H: float = 1.0
     MULTIBOX: int = 1
-    STEEPEST_MiDelE: float = 1.0e-3  # eV, convergence of the damped dynamics (MD_TypeDef_SimCtrlParam.F90:217)
+    # quench controls (MD_TypeDef_SimCtrlParam.F90:213-217): alpha of the first steepest step,
+    # smallest and largest move (in lattice units RR), energy tolerance (eV)
+    STEEPEST_Alpha: float = 0.1
+    STEEPEST_MiStep: float = 1.0e-5
+    STEEPEST_MxStep: float = 0.1
+    STEEPEST_MiDelE: float = 1.0e-3
 
     def __post_init__(self):
         self.RU = np.asfortranarray(np.atleast_2d(self.RU), dtype=np.float64)
@@ -429,6 +438,18 @@ class DeviceMD:
         check(lib().msm_dyn_damp(self.h, int(maxsteps), float(mindele), float(h), iptr(st),
                                  dptr(de)), "msm_dyn_damp", self.h)
         return int(st[0]), float(de[0])
+
+    def steepest(self, maxsteps: int, linesearch: bool, alpha: float, maxdis: float,
+                 mindis: float, mindele: float):
+        """Steepest-descent quench (msm_steepest): returns (steps, last max |dEPOT|, last largest
+        move)."""
+        st = np.zeros(1, dtype=np.int32)
+        de = np.zeros(1)
+        mm = np.zeros(1)
+        check(lib().msm_steepest(self.h, int(maxsteps), int(bool(linesearch)), float(alpha),
+                                 float(maxdis), float(mindis), float(mindele), iptr(st), dptr(de),
+                                 dptr(mm)), "msm_steepest", self.h)
+        return int(st[0]), float(de[0]), float(mm[0])
 
     def predictor(self, h: float):
         check(lib().msm_predictor(self.h, h), "msm_predictor", self.h)
@@ -629,6 +650,26 @@ def Do_DynDamp_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceCla
         print(f" MDPSCU Message: DnyDamp finished after steps:  {it}")
         print(f"                 with max energy uncertainty(ev): {de:13.4E}")
     return it, de
+
+
+def Do_Steepest_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU,
+                             MXNUMSTEPS: int, METH: int = CP_DAMPSCHEME_ST):
+    """Do_Steepest_Forsteps_DEV (MD_SteepestScheme_GPU.F90:264-290): steepest descent, with the
+    line search when METH has the CP_DAMPSCHEME_LSEARCH bit (whose energy tolerance the reference
+    fixes at 1e-3 eV).  Moves are bounded by STEEPEST_MxStep and STEEPEST_MiStep times the lattice
+    unit of the first box; returns (steps, max |dEPOT|, largest move)."""
+    boxes = SimBox if isinstance(SimBox, (list, tuple)) else [SimBox]
+    rr = boxes[0].RR
+    ls = bool(METH & CP_DAMPSCHEME_LSEARCH)
+    it, de, mm = ForceClass.Device.steepest(
+        MXNUMSTEPS, ls, CtrlParam.STEEPEST_Alpha, CtrlParam.STEEPEST_MxStep * rr,
+        CtrlParam.STEEPEST_MiStep * rr, 1.0e-3 if ls else CtrlParam.STEEPEST_MiDelE)
+    if it > MXNUMSTEPS:
+        print(f" MDPSCU WARNING: steepest finished after max steps: {it - 1 if not ls else it}")
+    else:
+        print(f" MDPSCU Message: steepest finished after steps: {it}")
+    print(f"                 with max energy uncertainty(ev): {de:13.4E}")
+    return it, de, mm
 
 
 def Predictor_DEV(ITIME: int, SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):

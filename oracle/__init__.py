@@ -360,6 +360,104 @@ def dyn_damp(xp, xp1, fp, dis, statu, ityp0, cm, h, lo, ln, ifpd, kvois, indi, r
     return x, v, f, d, st, ep, it, de
 
 
+def _move_bounded(x, dx, lo, ln, ifpd):
+    """x + dx with the bounds of DevAdd (AddBD_DevVec_DF_KERNEL0, MSM_MultiGPU_Basic.F90:
+    5444-5477): in a periodic direction a coordinate above BOXUP moves down by the box length and
+    one below BOXLOW up by it."""
+    r = x + dx
+    for k in range(3):
+        if not ifpd[k]:
+            continue
+        lb, hb = lo[k], lo[k] + ln[k]
+        c = r[:, k]
+        c = np.where(c > hb, c - (hb - lb), np.where(c < lb, c + (hb - lb), c))
+        r[:, k] = c
+    return r
+
+
+def steepest(xp, statu, ityp0, lo, ln, ifpd, kvois, indi, ru2max, table: Table, maxsteps,
+             linesearch, alpha, maxdis, mindis, mindele):
+    """Do_Steepest0_Forsteps_DEV / Do_Steepest1_Forsteps_DEV (MD_SteepestScheme_GPU.F90:20-152,
+    156-260) on the fixed full list (kvois, indi), forces and energies from force_full.  Returns
+    (x, steps, last max |dEPOT|, last largest move) with the step conventions of msm_steepest
+    (where the reference would divide by zero, the same replacements are taken)."""
+    lo, ln = np.asarray(lo, dtype=np.float64), np.asarray(ln, dtype=np.float64)
+    x = np.array(xp, dtype=np.float64)
+
+    def ff(xx):
+        f, _, e = force_full(xx, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table)
+        return f, e
+
+    de = mm = 0.0
+    if not linesearch:
+        f, _ = ff(x)
+        pf = f.copy()
+        dx = alpha * f
+        mm = float(np.max(np.abs(dx)))
+        if mm > maxdis:
+            dx = dx * (maxdis / mm)
+        elif mm <= mindis:
+            return x, 0, 0.0, mm
+        _, e0 = ff(x)
+        x = _move_bounded(x, dx, lo, ln, ifpd)
+        iflag = 0
+        for i in range(1, maxsteps + 1):
+            f, _ = ff(x)
+            g = pf - f
+            dxdf, dfdf = float(np.sum(dx * g)), float(np.sum(g * g))
+            al = dxdf / dfdf if dfdf > 0.0 else -1.0
+            if al < 0.0:
+                al = alpha
+            dx = al * f
+            mm = float(np.max(np.abs(dx)))
+            if mm > maxdis:
+                dx = dx * (maxdis / mm)
+            x = _move_bounded(x, dx, lo, ln, ifpd)
+            if mm <= mindis:
+                iflag = i
+                break
+            _, e = ff(x)
+            de = float(np.max(np.abs(e - e0)))
+            if de <= mindele:
+                iflag = i
+                break
+            e0 = e
+            pf = f
+        return x, (iflag if iflag else maxsteps + 1), de, mm
+    it = 0
+    step = maxdis
+    while it <= maxsteps:
+        f, e0 = ff(x)
+        nrm = float(np.sum(f * f))
+        if not nrm > 0.0:
+            break
+        d = f * (1.0 / np.sqrt(nrm))
+        step = maxdis
+        dx = step * d
+        pf0 = float(np.sum(f * d))
+        while it <= maxsteps:
+            x = _move_bounded(x, dx, lo, ln, ifpd)
+            f, _ = ff(x)
+            pf1 = float(np.sum(f * d))
+            if pf1 != pf0:
+                step = -step * pf1 / (pf1 - pf0)
+            else:
+                step = maxdis if -step * pf1 >= 0.0 else -maxdis
+            if abs(step) > maxdis:
+                step = maxdis if step > 0.0 else -maxdis
+            dx = step * d
+            it += 1
+            if abs(step) <= mindis:
+                break
+            pf0 = pf1
+        _, e = ff(x)
+        de = float(np.max(np.abs(e - e0)))
+        mm = abs(step)
+        if de <= mindele:
+            break
+    return x, it, de, mm
+
+
 def md_run(xp, xp1, statu, ityp0, cm, h, nsteps, nb_uptab, step0, lo, ln, ifpd, nb_rm, ru, mx,
            table: Table, fp=None, init=True):
     """Half-list FS MD with the reference's step order; returns (xp, xp1, fp, dis, statu, epot,

@@ -146,3 +146,38 @@ def test_dyn_damp_matches_oracle(orc, mindele, maxsteps):
     assert np.max(np.abs(f - of)) <= 1e-8
     assert np.array_equal(st & 0xFFFF, ost & 0xFFFF)
     dev.close()
+
+
+@pytest.mark.parametrize("meth", [md.CP_DAMPSCHEME_ST, md.CP_DAMPSCHEME_ST | md.CP_DAMPSCHEME_LSEARCH])
+def test_steepest_matches_oracle(orc, meth):
+    """Steepest-descent quench of a displaced 1024-atom W box (a few inactive atoms) on a fixed
+    list against the oracle's restatement: same number of steps, positions to 1e-9 A."""
+    ctrl = cfg.w_ctrl(mx=64)
+    ctrl.STEEPEST_MiDelE = 1e-6
+    box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.06, 0.0, 91)
+    box.STATU = mixed_statu(box.NPRT, 92)
+    ft = cfg.w_forcetable(box, ctrl)
+    ls = bool(meth & md.CP_DAMPSCHEME_LSEARCH)
+    say(f"steepest linesearch={ls}")
+    fc, dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    x0, _, _, st0, _ = dev.get_atoms()
+    kv, ind = dev.get_neighbors()
+    ind0 = np.where(ind > 0, ind - 1, 0)
+    t = orc.from_mdlib(ft)
+    ru2 = float(np.max(ctrl.RU)) ** 2
+    maxsteps = 300
+    ox, osteps, ode, omm = orc.steepest(
+        x0, st0, box.ITYP - 1, box.BOXLOW, box.ZL, ctrl.IFPD, kv, ind0, ru2, t, maxsteps, ls,
+        ctrl.STEEPEST_Alpha, ctrl.STEEPEST_MxStep * box.RR, ctrl.STEEPEST_MiStep * box.RR,
+        1e-3 if ls else ctrl.STEEPEST_MiDelE)
+    assert 0 < osteps <= maxsteps
+    steps, de, mm = md.Do_Steepest_Forsteps_DEV([box], ctrl, fc, maxsteps, meth)
+    assert steps == osteps
+    assert de == pytest.approx(ode, rel=1e-6, abs=1e-13)
+    x, _, _, st, _ = dev.get_atoms()
+    dx = x - ox
+    dx -= np.round(dx / box.ZL) * box.ZL
+    assert np.max(np.abs(dx)) <= 1e-9
+    assert np.all(x >= box.BOXLOW) and np.all(x <= box.BOXLOW + box.ZL)
+    dev.close()

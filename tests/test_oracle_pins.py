@@ -535,3 +535,34 @@ def test_oracle_integrator_leaves_its_inputs_alone(orc):
         orc.velscale(v, st, it, np.array([2.0]), 300.0, KB)
         for a, b in zip((x, v, f), keep):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("linesearch", [0, 1])
+def test_steepest_relaxes_to_the_perfect_lattice(orc, linesearch):
+    """Steepest descent (plain, and along the normalised force with secant line searches) from a
+    displaced BCC W lattice ends at the closed-form perfect-lattice energy with small forces;
+    a first move already below the smallest step leaves the atoms where they are."""
+    a = 3.165
+    x, lo, ln = bcc(4, a, 0.04, 11)
+    n = len(x)
+    t = w_table(orc, 100000)
+    it = np.zeros(n, np.int32)
+    st = np.ones(n, np.int32)
+    kv, ind = orc.nlist("full", x, it, st, lo, ln, (1, 1, 1), np.full((1, 1), 4.7002), 96)
+    x1, steps, de, mm = orc.steepest(x, st, it, lo, ln, (1, 1, 1), kv, ind, 4.4002 ** 2, t, 500,
+                                     linesearch, 0.1, 0.1 * a, 1e-5 * a, 1e-9)
+    assert 0 < steps <= 500
+    f, _, e = orc.force_full(x1, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t)
+    shells = [(a * math.sqrt(3) / 2, 8), (a, 6), (a * math.sqrt(2), 12)]
+    r = np.array([s[0] for s in shells])
+    nsh = np.array([s[1] for s in shells])
+    v, rho = w_closed(r)
+    e_atom = 0.5 * np.sum(nsh * v) - math.sqrt(np.sum(nsh * rho))
+    assert e.sum() == pytest.approx(n * e_atom, rel=1e-8)
+    assert np.max(np.abs(f)) < 1e-3
+    # positions stay inside the periodic box
+    assert np.all(x1 >= lo) and np.all(x1 <= lo + ln)
+    if not linesearch:
+        x2, steps2, *_ = orc.steepest(x, st, it, lo, ln, (1, 1, 1), kv, ind, 4.4002 ** 2, t, 50,
+                                      0, 0.1, 0.1 * a, 10.0, 1e-9)
+        assert steps2 == 0 and np.array_equal(x2, x)
