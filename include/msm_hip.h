@@ -222,6 +222,16 @@ int msm_dyn_damp(msm_sim* sim, int maxsteps, double mindele, double h, int* step
  * max |dEPOT|, maxmove the last largest displacement component (or line-search step). */
 int msm_steepest(msm_sim* sim, int maxsteps, int linesearch, double alpha, double maxdis,
                  double mindis, double mindele, int* steps, double* delepot, double* maxmove);
+/* Conjugate-gradient quench on the current neighbour list (MD_CGScheme_GPU.F90), directions
+ * updated by Polak-Ribiere, d = F + (F . (F - F0) / |F0|^2) d, from d = F; lengths and energies as
+ * for msm_steepest; stops when |F0|^2 <= 1e-64 (steps = 0 at the start).
+ * linesearch = 0: Do_CG0_Forsteps_DEV (:16-146) -- per direction a trial step of maxdis/|F0| d
+ *   and one secant step; stops when the per-atom energy change is <= mindele; steps receives the
+ *   iteration (maxsteps + 1 if it did not converge).
+ * linesearch = 1: Do_CG1_Forsteps_DEV (:150-291) -- secant steps along each direction until a
+ *   step is <= mindis; steps counts the moves (> maxsteps if it did not converge). */
+int msm_cg(msm_sim* sim, int maxsteps, int linesearch, double maxdis, double mindis,
+           double mindele, int* steps, double* delepot);
 int msm_predictor(msm_sim* sim, double h);
 int msm_correction(msm_sim* sim, double h);
 

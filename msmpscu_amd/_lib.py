@@ -95,6 +95,7 @@ SIGNATURES = {
     "msm_dyn_damp": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, _ip, _dp]),
     "msm_steepest": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                C.c_double, _ip, _dp, _dp]),
+    "msm_cg": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _ip, _dp]),
     "msm_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
 }
 

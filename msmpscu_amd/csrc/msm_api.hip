@@ -1343,15 +1343,25 @@ int msm_dyn_damp(msm_sim* s, int maxsteps, double mindele, double h, int* steps,
     return read_flags(s);
 }
 
+// Secant step to the zero of the force projection on the search direction, from projections pf0
+// (before the last step) and pf1 (after it), capped at maxdis; equal projections (a division by
+// zero in the reference) take the largest step in the downhill direction.
+static double secant_step(double step, double pf0, double pf1, double maxdis) {
+    double st = pf1 != pf0 ? -step * pf1 / (pf1 - pf0) : (-step * pf1 >= 0.0 ? maxdis : -maxdis);
+    if (std::fabs(st) > maxdis) st = st > 0.0 ? maxdis : -maxdis;
+    return st;
+}
+
 namespace {
 
-// Device scratch of one steepest-descent call (freed on every exit) and the small steps it is
+// Device scratch of one quench call (freed on every exit) and the small steps it is
 // built from; every value the host branches on is read back with a stream synchronisation.
 struct Quench {
     msm_sim* s;
     int n = 0;
     double* pf = nullptr;     // previous forces (dPreFP) or the unit direction (dDirect)
     double* dx = nullptr;     // displacement of the next move (dDXP)
+    double* w = nullptr;      // forces of the last direction update (dF0, conjugate gradients)
     double* epot0 = nullptr;  // per-atom energy of the reference configuration (dEPOT0)
     double* part = nullptr;   // partial sums of the reductions
     double* out = nullptr;    // reduced values
@@ -1362,6 +1372,7 @@ struct Quench {
     ~Quench() {
         dfree(&pf);
         dfree(&dx);
+        dfree(&w);
         dfree(&epot0);
         dfree(&part);
         dfree(&out);
@@ -1371,6 +1382,7 @@ struct Quench {
         const size_t m = static_cast<size_t>(std::max(n, 1));
         int rc = dalloc(s, &pf, 3 * m);
         if (rc == MSM_OK) rc = dalloc(s, &dx, 3 * m);
+        if (rc == MSM_OK) rc = dalloc(s, &w, 3 * m);
         if (rc == MSM_OK) rc = dalloc(s, &epot0, m);
         if (rc == MSM_OK) rc = dalloc(s, &part, 2 * static_cast<size_t>(msm::kQuenchBlocks));
         if (rc == MSM_OK) rc = dalloc(s, &out, 2);
@@ -1427,6 +1439,13 @@ struct Quench {
         if (ok()) msm::launch_dot3(u, v, n, part, out, s->stream);
         read_out(&r, 1);
         return r;
+    }
+    // sum of u . (v - w)
+    double dot_diff(msm::P3 u, msm::P3 v, msm::P3 wv) {
+        double r[2] = {0.0, 0.0};
+        if (ok()) msm::launch_dxdf(u, v, wv, n, part, out, s->stream);
+        read_out(r, 2);
+        return r[0];
     }
     void scale(msm::P3 src, double c, msm::P3 dst) {
         if (ok()) msm::launch_scale3(src, c, n, dst, s->stream);
@@ -1515,11 +1534,7 @@ int msm_steepest(msm_sim* s, int maxsteps, int linesearch, double alpha, double 
                 q.move(DX);
                 q.force();
                 const double pf1 = q.dot(q.forces(), D);
-                // secant step to the zero of F . D; equal projections (a division by zero in the
-                // reference) take the largest step in the downhill direction
-                step = pf1 != pf0 ? -step * pf1 / (pf1 - pf0)
-                                  : (-step * pf1 >= 0.0 ? maxdis : -maxdis);
-                if (std::fabs(step) > maxdis) step = step > 0.0 ? maxdis : -maxdis;
+                step = secant_step(step, pf0, pf1, maxdis);
                 q.scale(D, step, DX);
                 ++it;
                 if (std::fabs(step) <= mindis) break;
@@ -1536,6 +1551,97 @@ int msm_steepest(msm_sim* s, int maxsteps, int linesearch, double alpha, double 
     if (steps) *steps = it;
     if (delepot) *delepot = del;
     if (maxmove) *maxmove = mm;
+    return read_flags(s);
+}
+
+int msm_cg(msm_sim* s, int maxsteps, int linesearch, double maxdis, double mindis, double mindele,
+           int* steps, double* delepot) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (maxsteps < 0 || !(maxdis > 0.0) || !(mindis >= 0.0))
+        return set_err(s, MSM_ERR_ARG, "maxsteps >= 0, maxdis > 0 and mindis >= 0");
+    (void)hipSetDevice(s->device);
+    Quench q(s);
+    if ((rc = q.alloc()) != MSM_OK) return rc;
+    clear_flags(s);
+    const double eps = 1.0e-64;  // EPS of Do_CG0/1_Forsteps_DEV
+    const msm::P3 DIR = q.vec(q.pf), F0 = q.vec(q.w), DXP = q.vec(q.dx);
+    int it = 0;
+    double del = 0.0;
+    // set-up of Do_CG0/1_Forsteps_DEV (MD_CGScheme_GPU.F90:16-291): direction = force
+    q.force();
+    q.epot();
+    q.copy_epot0();
+    q.scale(q.forces(), 1.0, DIR);
+    q.scale(q.forces(), 1.0, F0);
+    double f0n = q.dot(F0, F0);
+    double pf0 = q.dot(q.forces(), DIR);
+    // Polak-Ribiere update after a line minimisation: d = F + (F . (F - F0) / |F0|^2) d, F0 = F
+    auto new_direction = [&]() {
+        q.force();
+        const double mf1 = q.dot_diff(q.forces(), q.forces(), F0);
+        if (q.ok()) msm::launch_dir_update(q.forces(), mf1 / f0n, q.n, DIR, s->stream);
+        q.scale(q.forces(), 1.0, F0);
+        f0n = q.dot(F0, F0);
+        pf0 = q.dot(q.forces(), DIR);
+    };
+    if (f0n > eps) {
+        if (!linesearch) {
+            // Do_CG0_Forsteps_DEV: one trial step and one secant step per direction
+            int iflag = 0;
+            for (int i = 1; q.ok() && i <= maxsteps; ++i) {
+                double step = maxdis / std::sqrt(f0n);
+                q.scale(DIR, step, DXP);
+                q.move(DXP);
+                q.force();
+                const double pf1 = q.dot(q.forces(), DIR);
+                step = secant_step(step, pf0, pf1, maxdis);
+                q.scale(DIR, step / std::sqrt(f0n), DXP);
+                q.move(DXP);
+                q.epot();
+                del = q.maxdelta();
+                if (del <= mindele) {
+                    iflag = i;
+                    break;
+                }
+                new_direction();
+                if (f0n <= eps) {
+                    iflag = i;
+                    break;
+                }
+            }
+            it = iflag ? iflag : maxsteps + 1;
+        } else {
+            // Do_CG1_Forsteps_DEV: secant line search to |step| <= mindis along each direction
+            while (q.ok() && it <= maxsteps) {
+                double step = maxdis / std::sqrt(f0n);
+                q.scale(DIR, step, DXP);
+                q.move(DXP);
+                q.force();
+                double pf1 = q.dot(q.forces(), DIR);
+                ++it;
+                while (q.ok() && it <= maxsteps) {
+                    step = secant_step(step, pf0, pf1, maxdis);
+                    q.scale(DIR, step / std::sqrt(f0n), DXP);
+                    q.move(DXP);
+                    ++it;
+                    if (std::fabs(step) <= mindis) break;
+                    pf0 = pf1;
+                    q.force();
+                    pf1 = q.dot(q.forces(), DIR);
+                }
+                q.epot();
+                del = q.maxdelta();
+                if (del <= mindele) break;
+                new_direction();
+                if (f0n <= eps) break;
+            }
+        }
+    }
+    if (q.ok()) q.e = hipStreamSynchronize(s->stream);
+    if (!q.ok()) return set_err(s, MSM_ERR_HIP, hipGetErrorString(q.e));
+    if (steps) *steps = it;
+    if (delepot) *delepot = del;
     return read_flags(s);
 }
 

@@ -240,6 +240,7 @@ int quench_blocks(int n);             // workgroups (and partials per component)
 void launch_dot3(P3 u, P3 v, int n, double* part, double* out, hipStream_t s);
 void launch_dxdf(P3 dx, P3 p, P3 f, int n, double* part, double* out, hipStream_t s);
 void launch_scale3(P3 src, double c, int n, P3 dst, hipStream_t s);
+void launch_dir_update(P3 f, double g, int n, P3 d, hipStream_t s);
 void launch_maxabs3(P3 v, int n, unsigned long long* out, hipStream_t s);
 void launch_move3(const AtomsDev& a, P3 dx, const BoxDev& box0, const BoxDev* boxes, bool mb,
                   hipStream_t s);

@@ -796,6 +796,19 @@ void launch_scale3(P3 src, double c, int n, P3 dst, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_scale3, dim3(grid_for(n)), dim3(kBlock), 0, s, src, c, n, dst);
 }
 
+// d = f + g * d  (the conjugate direction update of Do_CG0_Forsteps_DEV)
+__global__ void k_dir_update(P3 f, double g, int n, P3 d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    d.x[i] = f.x[i] + g * d.x[i];
+    d.y[i] = f.y[i] + g * d.y[i];
+    d.z[i] = f.z[i] + g * d.z[i];
+}
+
+void launch_dir_update(P3 f, double g, int n, P3 d, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_dir_update, dim3(grid_for(n)), dim3(kBlock), 0, s, f, g, n, d);
+}
+
 // max |component| into *out as the bit pattern of a non-negative double (*out zero on entry)
 __global__ void k_maxabs3(P3 v, int n, unsigned long long* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -198,6 +198,17 @@ module MSM_HIP_C
       integer(c_int) :: rc
     end function msm_steepest
 
+    function msm_cg(sim, maxsteps, linesearch, maxdis, mindis, mindele, steps, delepot) &
+        bind(C, name="msm_cg") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: sim
+      integer(c_int), value :: maxsteps, linesearch
+      real(c_double), value :: maxdis, mindis, mindele
+      integer(c_int), intent(out) :: steps
+      real(c_double), intent(out) :: delepot
+      integer(c_int) :: rc
+    end function msm_cg
+
     function msm_predictor(sim, h) bind(C, name="msm_predictor") result(rc)
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: sim
@@ -380,7 +391,7 @@ module MD_ForceLib_HIP
   public :: Initialize_Globle_Variables_DEV, Init_Forcetable_Dev, Cal_NeighBoreList_DEV
   public :: CalForce_ForceClass, CalEpot_ForceClass, CalPTensor_ForceClass, Cal_AtomicStress
   public :: Predictor_DEV, Correction_DEV, Cal_GlobalT_DEV, VelScaling_DEV, Do_DynDamp_Forsteps_DEV
-  public :: Do_Steepest_Forsteps_DEV
+  public :: Do_Steepest_Forsteps_DEV, Do_CG_Forsteps_DEV
   public :: For_One_Step, Run_Steps_DEV, CopyOut_SimBox_DEV, Copyout_NeighboreList_DEV
   public :: Autotune_DEV, Adjoint_ForceTable_DEV, Clear_Globle_Variables_DEV, MSM_Handle
 
@@ -795,6 +806,30 @@ contains
     end if
     write(*, "(A, 1PE13.4)") "                 with max energy uncertainty(ev): ", delepot
   end subroutine Do_Steepest_Forsteps_DEV
+
+  ! Do_CG_Forsteps_DEV (MD_CGScheme_GPU.F90:296-322): conjugate gradients (Polak-Ribiere), with
+  ! secant line searches when METH has the CP_DAMPSCHEME_LSEARCH bit
+  subroutine Do_CG_Forsteps_DEV(SimBox, CtrlParam, ForceClass, MXNUMSTEPS, METH, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    type(SimMDCtrl), intent(in) :: CtrlParam
+    type(MDForceClassGPU), intent(in) :: ForceClass
+    integer, intent(in) :: MXNUMSTEPS, METH
+    integer, optional, intent(out) :: IERR
+    integer(c_int) :: rc, iter, ls
+    real(c_double) :: delepot
+    ls = 0
+    if (iand(METH, CP_DAMPSCHEME_LSEARCH) > 0) ls = 1
+    rc = msm_cg(gm_SIM, MXNUMSTEPS, ls, CtrlParam%STEEPEST_MxStep * SimBox(1)%RR, &
+                CtrlParam%STEEPEST_MiStep * SimBox(1)%RR, CtrlParam%STEEPEST_MiDelE, iter, delepot)
+    call msm_check(rc, "Do_CG_Forsteps_DEV", IERR)
+    if (rc /= MSM_OK) return
+    if (iter > MXNUMSTEPS) then
+      write(*, "(A, I8)") " MDPSCU WARNING: CG finished after max steps: ", iter - 1
+    else
+      write(*, "(A, I8)") " MDPSCU Message: CG finished after steps: ", iter
+    end if
+    write(*, "(A, 1PE13.4)") "                 with max energy uncertainty(ev): ", delepot
+  end subroutine Do_CG_Forsteps_DEV
 
   ! Predictor_DEV / Correction_DEV (MD_DiffScheme_GPU.F90:604-682 / 821-847)
   subroutine Predictor_DEV(ITIME, SimBox, CtrlParam, IERR)

@@ -451,6 +451,14 @@ class DeviceMD:
                                  dptr(mm)), "msm_steepest", self.h)
         return int(st[0]), float(de[0]), float(mm[0])
 
+    def cg(self, maxsteps: int, linesearch: bool, maxdis: float, mindis: float, mindele: float):
+        """Conjugate-gradient quench (msm_cg): returns (steps, last max |dEPOT|)."""
+        st = np.zeros(1, dtype=np.int32)
+        de = np.zeros(1)
+        check(lib().msm_cg(self.h, int(maxsteps), int(bool(linesearch)), float(maxdis),
+                           float(mindis), float(mindele), iptr(st), dptr(de)), "msm_cg", self.h)
+        return int(st[0]), float(de[0])
+
     def predictor(self, h: float):
         check(lib().msm_predictor(self.h, h), "msm_predictor", self.h)
 
@@ -670,6 +678,24 @@ def Do_Steepest_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceCl
         print(f" MDPSCU Message: steepest finished after steps: {it}")
     print(f"                 with max energy uncertainty(ev): {de:13.4E}")
     return it, de, mm
+
+
+def Do_CG_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU,
+                       MXNUMSTEPS: int, METH: int = CP_DAMPSCHEME_CG):
+    """Do_CG_Forsteps_DEV (MD_CGScheme_GPU.F90:296-322): conjugate gradients, with secant line
+    searches when METH has the CP_DAMPSCHEME_LSEARCH bit; returns (steps, max |dEPOT|)."""
+    boxes = SimBox if isinstance(SimBox, (list, tuple)) else [SimBox]
+    rr = boxes[0].RR
+    ls = bool(METH & CP_DAMPSCHEME_LSEARCH)
+    it, de = ForceClass.Device.cg(MXNUMSTEPS, ls, CtrlParam.STEEPEST_MxStep * rr,
+                                  CtrlParam.STEEPEST_MiStep * rr, CtrlParam.STEEPEST_MiDelE)
+    name = "CG-LS" if ls else "CG"
+    if it > MXNUMSTEPS:
+        print(f" MDPSCU WARNING: {name} finished after max steps: {it - 1}")
+    else:
+        print(f" MDPSCU Message: {name} finished after steps: {it}")
+    print(f"                 with max energy uncertainty(ev): {de:13.4E}")
+    return it, de
 
 
 def Predictor_DEV(ITIME: int, SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):

@@ -458,6 +458,91 @@ def steepest(xp, statu, ityp0, lo, ln, ifpd, kvois, indi, ru2max, table: Table, 
     return x, it, de, mm
 
 
+def _secant(step, pf0, pf1, maxdis):
+    if pf1 != pf0:
+        st = -step * pf1 / (pf1 - pf0)
+    else:
+        st = maxdis if -step * pf1 >= 0.0 else -maxdis
+    if abs(st) > maxdis:
+        st = maxdis if st > 0.0 else -maxdis
+    return st
+
+
+def cg(xp, statu, ityp0, lo, ln, ifpd, kvois, indi, ru2max, table: Table, maxsteps, linesearch,
+       maxdis, mindis, mindele):
+    """Do_CG0_Forsteps_DEV / Do_CG1_Forsteps_DEV (MD_CGScheme_GPU.F90:16-291) on the fixed full
+    list: Polak-Ribiere directions from d = F, secant steps along them, moves with the bounds of
+    DevAdd.  Returns (x, steps, last max |dEPOT|) with the conventions of msm_cg."""
+    lo, ln = np.asarray(lo, dtype=np.float64), np.asarray(ln, dtype=np.float64)
+    x = np.array(xp, dtype=np.float64)
+    eps = 1.0e-64
+
+    def ff(xx):
+        f, _, e = force_full(xx, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table)
+        return f, e
+
+    f, e0 = ff(x)
+    d = f.copy()
+    f0 = f.copy()
+    f0n = float(np.sum(f0 * f0))
+    pf0 = float(np.sum(f * d))
+    de = 0.0
+    if not f0n > eps:
+        return x, 0, de
+
+    def new_direction():
+        nonlocal d, f0, f0n, pf0
+        f1, _ = ff(x)
+        g = float(np.sum(f1 * (f1 - f0))) / f0n
+        d = f1 + g * d
+        f0 = f1.copy()
+        f0n = float(np.sum(f0 * f0))
+        pf0 = float(np.sum(f1 * d))
+
+    if not linesearch:
+        for i in range(1, maxsteps + 1):
+            step = maxdis / np.sqrt(f0n)
+            x = _move_bounded(x, step * d, lo, ln, ifpd)
+            f, _ = ff(x)
+            pf1 = float(np.sum(f * d))
+            step = _secant(step, pf0, pf1, maxdis)
+            x = _move_bounded(x, (step / np.sqrt(f0n)) * d, lo, ln, ifpd)
+            _, e = ff(x)
+            de = float(np.max(np.abs(e - e0)))
+            if de <= mindele:
+                return x, i, de
+            e0 = e
+            new_direction()
+            if f0n <= eps:
+                return x, i, de
+        return x, maxsteps + 1, de
+    it = 0
+    while it <= maxsteps:
+        step = maxdis / np.sqrt(f0n)
+        x = _move_bounded(x, step * d, lo, ln, ifpd)
+        f, _ = ff(x)
+        pf1 = float(np.sum(f * d))
+        it += 1
+        while it <= maxsteps:
+            step = _secant(step, pf0, pf1, maxdis)
+            x = _move_bounded(x, (step / np.sqrt(f0n)) * d, lo, ln, ifpd)
+            it += 1
+            if abs(step) <= mindis:
+                break
+            pf0 = pf1
+            f, _ = ff(x)
+            pf1 = float(np.sum(f * d))
+        _, e = ff(x)
+        de = float(np.max(np.abs(e - e0)))
+        if de <= mindele:
+            break
+        e0 = e
+        new_direction()
+        if f0n <= eps:
+            break
+    return x, it, de
+
+
 def md_run(xp, xp1, statu, ityp0, cm, h, nsteps, nb_uptab, step0, lo, ln, ifpd, nb_rm, ru, mx,
            table: Table, fp=None, init=True):
     """Half-list FS MD with the reference's step order; returns (xp, xp1, fp, dis, statu, epot,

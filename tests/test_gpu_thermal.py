@@ -181,3 +181,38 @@ def test_steepest_matches_oracle(orc, meth):
     assert np.max(np.abs(dx)) <= 1e-9
     assert np.all(x >= box.BOXLOW) and np.all(x <= box.BOXLOW + box.ZL)
     dev.close()
+
+
+@pytest.mark.parametrize("meth,maxsteps,mindele", [(md.CP_DAMPSCHEME_CG, 4, 1e-12),
+                                                   (md.CP_DAMPSCHEME_CG | md.CP_DAMPSCHEME_LSEARCH,
+                                                    300, 1e-5)])
+def test_cg_matches_oracle(orc, meth, maxsteps, mindele):
+    """Conjugate-gradient quench against the oracle's restatement on a displaced 1024-atom W box
+    with a few inactive atoms: the plain variant for four iterations (it does not converge on
+    this box: the reference divides its secant step by |F0| twice), the line-search variant to
+    1e-5 eV; same step counts, positions to 1e-9 A."""
+    ctrl = cfg.w_ctrl(mx=64)
+    ctrl.STEEPEST_MiDelE = mindele
+    box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.06, 0.0, 93)
+    box.STATU = mixed_statu(box.NPRT, 94)
+    ft = cfg.w_forcetable(box, ctrl)
+    ls = bool(meth & md.CP_DAMPSCHEME_LSEARCH)
+    say(f"cg linesearch={ls}")
+    fc, dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    x0, _, _, st0, _ = dev.get_atoms()
+    kv, ind = dev.get_neighbors()
+    ind0 = np.where(ind > 0, ind - 1, 0)
+    t = orc.from_mdlib(ft)
+    ru2 = float(np.max(ctrl.RU)) ** 2
+    ox, osteps, ode = orc.cg(x0, st0, box.ITYP - 1, box.BOXLOW, box.ZL, ctrl.IFPD, kv, ind0, ru2,
+                             t, maxsteps, ls, ctrl.STEEPEST_MxStep * box.RR,
+                             ctrl.STEEPEST_MiStep * box.RR, mindele)
+    steps, de = md.Do_CG_Forsteps_DEV([box], ctrl, fc, maxsteps, meth)
+    assert steps == osteps
+    assert de == pytest.approx(ode, rel=1e-6, abs=1e-13)
+    x, _, _, _, _ = dev.get_atoms()
+    dx = x - ox
+    dx -= np.round(dx / box.ZL) * box.ZL
+    assert np.max(np.abs(dx)) <= 1e-9
+    dev.close()

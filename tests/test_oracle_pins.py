@@ -566,3 +566,33 @@ def test_steepest_relaxes_to_the_perfect_lattice(orc, linesearch):
         x2, steps2, *_ = orc.steepest(x, st, it, lo, ln, (1, 1, 1), kv, ind, 4.4002 ** 2, t, 50,
                                       0, 0.1, 0.1 * a, 10.0, 1e-9)
         assert steps2 == 0 and np.array_equal(x2, x)
+
+
+def test_cg_line_search_relaxes_and_starts_along_the_force(orc):
+    """Conjugate gradients with secant line searches relax a displaced BCC W lattice to the
+    closed-form perfect-lattice energy; the first direction of either variant is the force (the
+    first iteration moves every atom parallel to its starting force)."""
+    a = 3.165
+    x, lo, ln = bcc(4, a, 0.04, 11)
+    n = len(x)
+    t = w_table(orc, 100000)
+    it = np.zeros(n, np.int32)
+    st = np.ones(n, np.int32)
+    kv, ind = orc.nlist("full", x, it, st, lo, ln, (1, 1, 1), np.full((1, 1), 4.7002), 96)
+    args = (st, it, lo, ln, (1, 1, 1), kv, ind, 4.4002 ** 2, t)
+    x1, steps, de = orc.cg(x, *args, 800, 1, 0.1 * a, 1e-5 * a, 1e-9)
+    assert 0 < steps <= 800 and de <= 1e-9
+    f, _, e = orc.force_full(x1, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t)
+    shells = [(a * math.sqrt(3) / 2, 8), (a, 6), (a * math.sqrt(2), 12)]
+    r = np.array([s[0] for s in shells])
+    nsh = np.array([s[1] for s in shells])
+    v, rho = w_closed(r)
+    e_atom = 0.5 * np.sum(nsh * v) - math.sqrt(np.sum(nsh * rho))
+    assert e.sum() == pytest.approx(n * e_atom, rel=1e-8)
+    f0, _, _ = orc.force_full(x, it, st, kv, ind, ln, (1, 1, 1), 4.4002 ** 2, t)
+    for ls in (0, 1):
+        x2, steps2, _ = orc.cg(x, *args, 1, ls, 0.1 * a, 1e-5 * a, 1e9)
+        dx = x2 - x
+        dx = (dx - np.round(dx / ln) * ln).ravel()
+        cos = dx @ f0.ravel() / (np.linalg.norm(dx) * np.linalg.norm(f0))
+        assert abs(abs(cos) - 1.0) < 1e-12 and np.linalg.norm(dx) > 0.0
