@@ -276,6 +276,7 @@ class SimMDBox:
     EPOT: Optional[np.ndarray] = None
     VTENSOR: Optional[np.ndarray] = None
     RR: float = 1.0  # lattice unit in this layer's length unit (the reference keeps it in cm)
+    EKIN: Optional[np.ndarray] = None
 
     @property
     def BOXUP(self) -> np.ndarray:

@@ -1,0 +1,109 @@
+"""Configuration files in the reference's text formats (CPU only): the &BOXCFG18 writer's layout
+and round trip, and the loaders against excerpts of configuration files shipped with the
+reference (a &BOXCFG15 file written by its code and a plain initial-configuration table)."""
+import os
+
+import numpy as np
+import pytest
+
+import msmpscu_amd.boxcfg as bc
+import msmpscu_amd.configs as cfg
+import msmpscu_amd.mdlib as md
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _empty_box(ngroup=1):
+    return md.SimMDBox(NPRT=0, NGROUP=ngroup, ZL=np.zeros(3), BOXLOW=np.zeros(3),
+                       XP=np.zeros((0, 3)), XP1=np.zeros((0, 3)), ITYP=np.zeros(0, np.int32),
+                       STATU=np.zeros(0, np.int32), CM=np.array([cfg.W_MASS * cfg.AMU] * ngroup),
+                       PTYPE=np.ones((ngroup, ngroup), np.int32))
+
+
+@pytest.fixture
+def box():
+    b = cfg.make_bcc_box((3, 3, 3), cfg.W_A, 0.05, 600.0, 4)
+    rng = np.random.default_rng(5)
+    b.FP = np.asfortranarray(rng.normal(0, 0.7, (b.NPRT, 3)))
+    b.EPOT = -8.9 + rng.normal(0, 0.01, b.NPRT)
+    b.DIS = np.asfortranarray(rng.normal(0, 0.02, (b.NPRT, 3)))
+    b.STATU[3] |= 16
+    return b
+
+
+def test_boxcfg18_round_trip(box, tmp_path):
+    f = bc.Putout_Instance_Config_SimMDBox(str(tmp_path / "cfg"), box, ITime=120, Time=0.12,
+                                           ICfg=7)
+    assert f.endswith("cfg.0007")
+    b2 = bc.Load_Config_SimMDBox(f, _empty_box())
+    assert b2.NPRT == box.NPRT and b2.RR == pytest.approx(box.RR, rel=1e-5)
+    rel = 1e-8  # 1PE17.8: nine significant digits
+    for got, want in ((b2.XP, box.XP), (b2.XP1, box.XP1), (b2.FP, box.FP), (b2.DIS, box.DIS)):
+        assert np.allclose(got, want, rtol=rel, atol=rel * np.max(np.abs(want)))
+    assert np.allclose(b2.EPOT, box.EPOT, rtol=rel)
+    assert np.array_equal(b2.ITYP, box.ITYP) and np.array_equal(b2.STATU, box.STATU)
+    ek = 0.5 * box.CM[0] * np.sum(np.asarray(box.XP1) ** 2, axis=1)
+    assert np.allclose(b2.EKIN, ek, rtol=rel)
+    assert np.allclose(b2.BOXLOW, box.BOXLOW, rtol=1e-5) and np.allclose(b2.ZL, box.ZL, rtol=1e-5)
+
+
+def test_boxcfg18_layout_follows_the_reference_writer(box, tmp_path):
+    """Header keywords in the writer's order, column tags at the columns the rows use, and rows of
+    I8,3X then 1PE17.8,1X fields (STATU as I6,2X) -- 271 characters."""
+    f = bc.Putout_Instance_Config_SimMDBox(str(tmp_path / "cfg"), box)
+    lines = open(f).read().split("\n")
+    assert lines[0] == "&BOXCFG18"
+    keys = [l.split()[0] for l in lines if l.startswith("&")]
+    order = ["&BOXCFG18", "&APPTYPE", "&DATE", "&TESTID", "&BOXID", "&CFGID", "&RECID",
+             "&TIMESTEPS", "&TIMESECT", "&TIME", "&SCALTIME", "&TEMPSET", "&TYPECOL", "&XYZCOL",
+             "&VELCOL", "&STATCOL", "&FPCOL", "&EPOTCOL", "&EKINCOL", "&DISCOL", "&LATT",
+             "&BOXLOW", "&BOXSIZE", "&NATOM", "&NGROUP", "&TEMPCAL", "&TYPE"]
+    assert keys == order
+    tags = {l.split()[0]: l for l in lines if l.split() and l.split()[0].endswith("COL")}
+    assert tags["&XYZCOL"] == '&XYZCOL         2   3 "D"'
+    assert tags["&DISCOL"] == '&DISCOL        14   3 "D"'
+    i0 = lines.index("!--- Configure:") + 2
+    row = lines[i0]
+    assert len(row) == 271
+    assert int(row[:8]) == box.ITYP[0]
+    x = float(row[11:28])
+    assert x == pytest.approx(box.XP[0, 0] / box.RR, rel=1e-8)
+    assert row[28] == " " and "E" in row[11:28]
+    assert int(row[119:125]) == box.STATU[0]
+
+
+def test_reads_the_reference_boxcfg15_file():
+    """&BOXCFG15 written by the reference (excerpt of examples/NEB_Test/PrevCfg.0000): lattice
+    unit, box, counts, LU positions, STATU, dyn forces, POT = -EPOT, the He atom of group 2."""
+    b = bc.Load_Config_SimMDBox(os.path.join(GOLD, "boxcfg15_w2000_he1_excerpt.cfg"),
+                                _empty_box(2))
+    rr = 3.1652
+    assert b.NPRT == 31 and b.NGROUP == 2 and b.RR == pytest.approx(rr)
+    assert np.allclose(b.BOXLOW, -5.0 * rr) and np.allclose(b.ZL, 10.0 * rr)
+    assert np.allclose(b.XP[0], np.array([4.2502, 4.2498, 4.2500]) * rr)
+    assert np.all(b.STATU == 1) and np.all(b.XP1 == 0.0)
+    assert b.FP[0, 0] == pytest.approx(6.6231e-08 / 1.60219e-12 / 1e8)
+    assert b.EPOT[0] == pytest.approx(-8.8915)
+    assert list(b.ITYP[-2:]) == [1, 2]
+    assert b.EPOT[-1] == pytest.approx(0.47898)
+    assert np.allclose(b.XP[-1], np.array([1.5410e-04, 2.4998e-01, 7.4989e-01]) * rr)
+
+
+def test_reads_the_plain_initial_table():
+    """Plain table (excerpt of examples/use_extern_pottable/W_2000.cfg): type and position in LU,
+    velocity in cm/s; the geometry comes from the box set-up."""
+    b = _empty_box()
+    b.NPRT = 20
+    b.RR = 3.1652
+    b.BOXLOW = np.full(3, -5.0 * b.RR)
+    b.ZL = np.full(3, 10.0 * b.RR)
+    bc.Read_Initial_Config_SimMDBox(os.path.join(GOLD, "plain_w2000_excerpt.cfg"), b,
+                                    with_velocity=True)
+    assert np.allclose(b.XP[0], np.array([4.2425, 4.2517, 4.2564]) * b.RR)
+    assert np.allclose(b.XP1[0], np.array([-5.6296e3, -4.3227e3, 2.0007e4]) * 1e-7)
+    assert np.all(b.ITYP == 1) and np.all(b.STATU == 1)
+    # an atom outside the box is marked OUTOFBOX
+    b.ZL = np.full(3, 9.0 * b.RR)
+    bc.Read_Initial_Config_SimMDBox(os.path.join(GOLD, "plain_w2000_excerpt.cfg"), b)
+    assert np.all(b.STATU[np.any(b.XP > b.BOXLOW + b.ZL, axis=1)] == 65536)
+    assert np.all(b.XP1 == 0.0)
