@@ -22,7 +22,7 @@ namespace msm {
 // list build
 // --------------------------------------------------------------------------------------------
 
-// Small-brick list build (sparse cells): one kBlock-thread workgroup per brick of at most
+// Small-brick list build (sparse cells): one kNlSmallBlock-thread workgroup per brick of at most
 // kNlCapPass halo atoms, one lane per home atom.  The LDS tables stay near 40 KB (a neighbour's
 // sorted index is its cell's base plus its halo slot, not a stored array), so four workgroups
 // share a CU.  The candidates of each stencil cell are tested kNlGroup at a time (their LDS
@@ -30,9 +30,13 @@ namespace msm {
 // neighbour's halo slot (read by the LDS-brick passes).  Measured alternatives: groups of 8,
 // and hit masks with one store round per hit of the busiest lane -- both slower.
 constexpr int kNlGroup = 4;
+// 512 threads: a 4^3-cell brick of a bcc metal at the usual skin holds ~420 home atoms, which a
+// 256-thread workgroup served in two rounds (the second 60% full); with 53 VGPRs and 40 KB of LDS
+// four such workgroups still share a CU.
+constexpr int kNlSmallBlock = 512;
 
 template <int MT, int SIDX>
-__global__ __launch_bounds__(kBlock) void k_nlist_small(
+__global__ __launch_bounds__(kNlSmallBlock) void k_nlist_small(
     const int* __restrict__ brick_box, const double4* __restrict__ pos,
     const int* __restrict__ ityp, const int* __restrict__ start,
     const BoxDev* __restrict__ boxes, int ngroup, const double* __restrict__ rnl2, int mx,
@@ -267,7 +271,7 @@ void launch_nlist_bricks(int wide, int nbricks, int lanes, int cap, int write_si
     if (nbricks <= 0) return;
     if (!wide) {
 #define MSM_NS(MT, S)                                                                         \
-    hipLaunchKernelGGL((k_nlist_small<MT, S>), dim3(nbricks), dim3(kBlock), 0, s, brick_box, pos, \
+    hipLaunchKernelGGL((k_nlist_small<MT, S>), dim3(nbricks), dim3(kNlSmallBlock), 0, s, brick_box, pos, \
                        ityp, start, boxes, ngroup, rnl2, mx, kvois, indi, sidx, stride, err)
         if (ngroup > 1) { if (write_sidx) MSM_NS(1, 1); else MSM_NS(1, 0); }
         else            { if (write_sidx) MSM_NS(0, 1); else MSM_NS(0, 0); }
