@@ -6,9 +6,11 @@
 !
 !  usage: fs_w_driver OUTFILE [NSTEPS [NC]]
 !  OUTFILE: line 1 "N E0 0"; N lines "x y z fx fy fz" of the start; N lines of the end; a line
-!  "# energies E0 E1" (total potential energy at the start and at the end) and a last line
+!  "# energies E0 E1" (total potential energy at the start and at the end), a line
 !  "# stress_sum S11 S12 ... S33" (per-atom stresses of Cal_AtomicStress summed over the atoms at
-!  the end).
+!  the end), a line "# temperature T" (Cal_GlobalT_DEV after VelScaling_DEV to 300 K), a line
+!  "# steepest" and N lines "x y z" after Do_Steepest_Forsteps_DEV (list rebuilt on the end
+!  positions, STEEPEST_MiDelE = 1e-6 eV, moves bounded by RR = a).
 
 module W_FS_Potential
   use MD_ForceLib_HIP, only: KINDDF
@@ -77,6 +79,7 @@ program fs_w_driver
   integer(c_int) :: seed(2), rc
   real(KINDDF) :: e0, e1, z1, z2, x
   real(KINDDF), allocatable :: avp(:, :)
+  real(KINDDF) :: curt
 
   if (command_argument_count() < 1) then
     write(*, "(A)") "usage: fs_w_driver OUTFILE [NSTEPS [NC]]"
@@ -99,6 +102,7 @@ program fs_w_driver
   box(1)%NPRT = n
   box(1)%NGROUP = 1
   box(1)%ZL = nc * AW
+  box(1)%RR = AW
   box(1)%BOXLOW = -0.5d0 * box(1)%ZL
   allocate(box(1)%XP(n, 3), box(1)%XP1(n, 3), box(1)%ITYP(n), box(1)%STATU(n), box(1)%CM(1))
   allocate(box(1)%PTYPE(1, 1))
@@ -178,6 +182,19 @@ program fs_w_driver
   allocate(avp(n, 9))
   call Cal_AtomicStress(box, ctrl, avp)
   write(u, "(A,9(1X,ES25.17E3))") "# stress_sum", (sum(avp(:, m)), m = 1, 9)
+  ! thermalisation and quench through this layer: scale to 300 K and read the temperature back,
+  ! then steepest descent on a list of the current positions, final positions written
+  call VelScaling_DEV(box, ctrl, 300.0d0)
+  call Cal_GlobalT_DEV(box, ctrl, curt)
+  write(u, "(A,1X,ES25.17E3)") "# temperature", curt
+  call Cal_NeighBoreList_DEV(box, ctrl)
+  ctrl%STEEPEST_MiDelE = 1.0d-6
+  call Do_Steepest_Forsteps_DEV(box, ctrl, fc, 200, CP_DAMPSCHEME_ST)
+  call CopyOut_SimBox_DEV(box, WITHEPOT=.false.)
+  write(u, "(A)") "# steepest"
+  do m = 1, n
+    write(u, "(3(ES25.17E3,1X))") box(1)%XP(m, :)
+  end do
   close(u)
   call Clear_Globle_Variables_DEV()
   write(*, "(A,I0,A,ES24.16,A,ES24.16)") "fs_w_driver: n=", n, " E0=", e0, " E1=", e1

@@ -1,7 +1,9 @@
 """The Fortran drop-in layer (msmpscu_amd/fortran: MDLIB-named routines over the C-ABI) driven by
 a Fortran program: forces and energy of the W box at the start and after 100 steps (one
-per-step call, then a fused run) against the oracle on the same positions, and the per-atom
-stresses of Cal_AtomicStress summed over the atoms against twice the oracle's virial."""
+per-step call, then a fused run) against the oracle on the same positions, the per-atom
+stresses of Cal_AtomicStress summed over the atoms against twice the oracle's virial, the
+temperature after VelScaling_DEV, and the positions after Do_Steepest_Forsteps_DEV against the
+oracle's steepest descent."""
 import os
 import subprocess
 import sys
@@ -62,3 +64,25 @@ def test_fortran_driver_against_oracle(orc, tmp_path):
     assert np.all(np.abs(ssum - 2.0 * vt) <= 1e-13 * 2.0 * vabs)
     # the run moved the atoms
     assert np.max(np.abs(end[:, :3] - start[:, :3])) > 1e-3
+    # VelScaling_DEV then Cal_GlobalT_DEV: every atom counts, so the box lands on 300 K
+    temp = float(lines[3 + 2 * n].split()[2])
+    assert temp == pytest.approx(300.0, rel=1e-12)
+    # Do_Steepest_Forsteps_DEV from the end positions against the oracle's restatement
+    assert lines[4 + 2 * n].strip() == "# steepest"
+    quench = np.array([[float(v) for v in l.split()] for l in lines[5 + 2 * n:5 + 3 * n]])
+    box = _box(end[:, :3])
+    box.RR = cfg.W_A
+    ft = cfg.w_forcetable(box, ctrl)
+    t = orc.from_mdlib(ft)
+    it = box.ITYP - 1
+    kv, ind = orc.nlist("full", box.XP, it, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
+                        ctrl.NB_RM, ctrl.NB_MXNBS)
+    ox, osteps, _, _ = orc.steepest(box.XP, box.STATU, it, box.BOXLOW, box.ZL, ctrl.IFPD, kv, ind,
+                                    float(np.max(ctrl.RU)) ** 2, t, 200, 0, ctrl.STEEPEST_Alpha,
+                                    ctrl.STEEPEST_MxStep * box.RR, ctrl.STEEPEST_MiStep * box.RR,
+                                    1e-6)
+    assert 0 < osteps <= 200
+    assert f"steepest finished after steps: {osteps:8d}" in r.stdout
+    dx = quench - ox
+    dx -= np.round(dx / box.ZL) * box.ZL
+    assert np.max(np.abs(dx)) <= 1e-9
