@@ -786,3 +786,66 @@ def test_virial_multibox_equals_single_boxes():
         v1 = d1.calc_ptensor()[0]
         d1.close()
         assert np.max(np.abs(vb[k] - v1)) <= 1e-12 * np.max(np.abs(v1))
+
+
+def test_ragged_and_degenerate_boxes(orc):
+    """One multibox run over edge cases: the smallest box the cell grid allows (3x3x3 BCC cells,
+    where every cell neighbours every other), a box holding a single atom (no neighbours: zero
+    force, F(0) = 0), a box whose atoms are all inactive (listed, but no force or energy), and
+    an ordinary box.  Each box's list,
+    forces and energies equal the oracle's for that box alone, and 20 fused steps keep them
+    equal to the boxes run one at a time."""
+    ctrl = cfg.w_ctrl(mx=64)
+    tiny = cfg.make_bcc_box((3, 3, 3), cfg.W_A, 0.05, 300.0, 61)
+    lone = cfg.make_bcc_box((4, 4, 4), cfg.W_A, 0.0, 0.0, 62)
+    keep = np.zeros(lone.NPRT, dtype=bool)
+    keep[5] = True
+    lone.NPRT = 1
+    lone.XP = np.asfortranarray(lone.XP[keep])
+    lone.XP1 = np.asfortranarray(np.full((1, 3), 0.01))
+    lone.ITYP = lone.ITYP[keep]
+    lone.STATU = lone.STATU[keep]
+    dead = cfg.make_bcc_box((3, 4, 3), cfg.W_A, 0.05, 300.0, 63)
+    dead.STATU = np.zeros(dead.NPRT, np.int32)
+    plain = cfg.make_bcc_box((5, 4, 6), cfg.W_A, 0.05, 300.0, 64, n_vac=2)
+    boxes = [tiny, lone, dead, plain]
+    ft = cfg.w_forcetable(plain, ctrl)
+    say("ragged and degenerate boxes")
+    dev = device_for(boxes, ctrl, ft)
+    dev.build_neighbors()
+    dev.calc_force()
+    dev.calc_epot()
+    _, _, fp, _, _ = dev.get_atoms()
+    ep = dev.get_epot()
+    kv, ind = dev.get_neighbors()
+    off = 0
+    for b in boxes:
+        s = slice(off, off + b.NPRT)
+        it = b.ITYP - 1
+        okv, oind = orc.nlist("full", b.XP, it, b.STATU, b.BOXLOW, b.ZL, ctrl.IFPD, ctrl.NB_RM,
+                              ctrl.NB_MXNBS)
+        assert np.array_equal(kv[s], okv)
+        assert pairset(kv[s], ind[s] - off, True) == pairset(okv, oind, False)
+        fo, _, eo = oracle_full(orc, b, ctrl, ft)
+        assert np.max(np.abs(fp[s] - fo)) <= F_TOL
+        assert np.allclose(ep[s], eo, rtol=0, atol=1e-12 * max(1.0, np.max(np.abs(eo))))
+        off += b.NPRT
+    assert kv[tiny.NPRT] == 0 and np.all(fp[tiny.NPRT] == 0.0) and ep[tiny.NPRT] == 0.0
+    # inactive atoms keep their list entries (the reference's list predicate does not look at
+    # STATU) but get no force and no energy
+    d0 = tiny.NPRT + 1
+    assert np.all(fp[d0:d0 + dead.NPRT] == 0.0) and np.all(ep[d0:d0 + dead.NPRT] == 0.0)
+    dev.run(20, ctrl.NB_UPTAB, 0, ctrl.H)
+    xb, _, fb, _, _ = dev.get_atoms()
+    dev.close()
+    off = 0
+    for b in boxes:
+        d1 = device_for(b, ctrl, ft)
+        d1.build_neighbors()
+        d1.calc_force()
+        d1.run(20, ctrl.NB_UPTAB, 0, ctrl.H)
+        x1, _, f1, _, _ = d1.get_atoms()
+        s = slice(off, off + b.NPRT)
+        assert np.array_equal(xb[s], x1) and np.array_equal(fb[s], f1)
+        off += b.NPRT
+        d1.close()
