@@ -5,9 +5,7 @@ Default workload: BCC W, 80x80x80 conventional cells minus 0.5% vacancies = 1,01
 periodic, sigma = 0.05 A displacements, 600 K Maxwell velocities, Finnis-Sinclair tables with
 5000 points, FP64 velocity Verlet with dt = 1 fs, neighbour list (skin 0.3 A) rebuilt every 10
 steps.  A "step" is one full MD step (predictor, rebuild when due, density, force, corrector)
-over the whole box with all data resident in HBM.  Before the untimed warm-up the library's
-autotuner (msm_autotune) picks the pass kernels on the live state, as in the batch and slab legs
-(--no-autotune, or an explicit kernel option, keeps the library defaults).
+over the whole box with all data resident in HBM.
 
 Prints one JSON line: value = atom-steps/s over the timed steps, a roofline object for the
 dominant kernel (algorithmic bytes per launch / average launch time from HIP events), and a
@@ -301,15 +299,6 @@ def main():
             _lib.check(_lib.lib().msm_set_option(dev.h, opt, v), "msm_set_option")
     dev.build_neighbors()
     dev.calc_force()
-    # the library's autotuner picks the pass kernels (untimed), as in the other legs, unless
-    # kernels were chosen on the command line
-    explicit = any(v is not None for v in (args.force_path, args.compact, args.table_lds,
-                                            args.cell_brick, args.nlist_kernel, args.force_lanes,
-                                            args.brick_lanes)) or bool(args.lt_shape)
-    p1_choice = None
-    if not args.no_autotune and not explicit:
-        p1_choice = dev.autotune(3)
-        log(f"plain-MD autotune: {p1_choice}")
     tot, kmax = dev.list_stats()
     kmean = tot / box.NPRT
     pass_kernels = dev.pass_kernel()
@@ -440,7 +429,6 @@ def main():
                                "rebuild every 10 steps, skin 0.3 A",
                    "atoms": box.NPRT, "list_mean": round(kmean, 3), "list_max": kmax,
                    "dis_tracking": track, "pass_kernels": pass_kernels,
-                   **({"autotune": p1_choice} if p1_choice is not None else {}),
                    "parallelism": "single GPU"},
         "pcie_inclusive_value": round(pcie_value, 1),
         "roofline": roof,
