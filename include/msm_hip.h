@@ -153,6 +153,11 @@ int msm_set_atoms(msm_sim* sim, int n, const double* xp, const double* xp1, cons
 
 /* Copy-out in the caller's atom order; any pointer may be NULL. */
 int msm_get_atoms(msm_sim* sim, double* xp, double* xp1, double* fp, int* statu, double* dis);
+/* CopyXPFrom_Host_to_Devices / CopyXP1From_Host_to_Devices (MD_Globle_Variables_GPU.F90:349-373):
+ * positions and/or velocities (either may be NULL) of all atoms, (N,3) column-major in the
+ * caller's order, written into the device state without a list rebuild (the caller keeps the
+ * moves within what the list allows, as the reference's quench loops do). */
+int msm_put_atoms(msm_sim* sim, const double* xp, const double* xp1);
 /* F'(rho) per atom (the reference's DEN array) and per-atom potential energy EPOT. */
 int msm_get_den(msm_sim* sim, double* den);
 int msm_get_epot(msm_sim* sim, double* epot);

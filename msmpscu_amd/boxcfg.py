@@ -21,10 +21,8 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from .mdlib import SimMDBox
+from .mdlib import CP_CM2A, CP_EVERG, SimMDBox
 
-CP_EVERG = 1.60219e-12   # erg per eV, as the reference
-CP_CM2A = 1.0e8          # Angstrom per cm
 CM_S_TO_A_FS = CP_CM2A * 1.0e-15            # (cm/s) -> (A/fs)
 DYN_TO_EV_A = 1.0 / CP_EVERG / CP_CM2A      # dyn = erg/cm -> eV/A
 

@@ -1056,6 +1056,38 @@ static int pull_sorted_d(msm_sim* s, const double* src, int comp_pos, double* ho
     return MSM_OK;
 }
 
+int msm_put_atoms(msm_sim* s, const double* xp, const double* xp1) {
+    if (!s) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (!s->has_atoms) return set_err(s, MSM_ERR_STATE, "no atoms");
+    const int n = s->n;
+    if (n == 0 || (!xp && !xp1)) return MSM_OK;
+    const size_t na = s->n_alloc;
+    AtomSet& z = s->set[s->scur];
+    double* tmp = nullptr;
+    int rc = dalloc(s, &tmp, 3 * static_cast<size_t>(n));
+    if (rc != MSM_OK) return rc;
+    hipError_t e = hipSuccess;
+    if (xp) {
+        e = hipMemcpyAsync(tmp, xp, 3 * static_cast<size_t>(n) * sizeof(double),
+                           hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess) msm::launch_sort_pos(tmp, s->pos[s->pcur], z.gid, n, s->stream);
+    }
+    if (e == hipSuccess && xp1) {
+        if (xp) e = hipStreamSynchronize(s->stream);  // tmp is reused
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(tmp, xp1, 3 * static_cast<size_t>(n) * sizeof(double),
+                               hipMemcpyHostToDevice, s->stream);
+        for (int k = 0; e == hipSuccess && k < 3; ++k)
+            msm::launch_sort_d(tmp + k * static_cast<size_t>(n), z.v + k * na, z.gid, n,
+                               s->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    dfree(&tmp);
+    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    return MSM_OK;
+}
+
 int msm_get_den(msm_sim* s, double* den) {
     if (!s || !den) return MSM_ERR_ARG;
     (void)hipSetDevice(s->device);

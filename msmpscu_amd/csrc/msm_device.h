@@ -63,12 +63,15 @@ static __device__ __forceinline__ size_t list_base(int p, int mx) {
     return static_cast<size_t>(p / kListRow) * static_cast<size_t>(mx) * kListRow + (p % kListRow);
 }
 
-// Minimum image: the reference writes "if periodic and |s| > L/2 then s - sign(L, s)".  For the
-// separations a neighbour list can hold (|s| < L/2 after at most one box length) this equals
-// s - L*rint(s/L), which needs no branch; ilen = 0 switches it off for free directions.  The
-// subtraction is one rounding either way (fma with an exact product L*k, k in {-1,0,1}).
+// Minimum image: the reference writes "if periodic and |s| > L/2 then s - sign(L, s)" -- one
+// box length at most.  s - L*k with k = rint(s/L) clamped to [-1, 1] is that shift without a
+// branch: for |s| < 3L/2 the clamp is idle, beyond it the pair stays as far apart as the
+// reference leaves it (quench trial points can lie many box lengths out).  ilen = 0 switches
+// it off for free directions.  The subtraction is one rounding either way (fma with an exact
+// product L*k, k in {-1,0,1}).
 static __device__ __forceinline__ double min_image(double s, double len, double ilen) {
-    return fma(-len, rint(s * ilen), s);
+    const double k = fmin(fmax(rint(s * ilen), -1.0), 1.0);
+    return fma(-len, k, s);
 }
 
 // 1/sqrt(x), x > 0: hardware estimate refined by two Newton steps (a few ulp).  Used in place of

@@ -210,6 +210,8 @@ void launch_predictor(const AtomsDev& a, const TabDev& t, const BoxDev& box0,
                       const BoxDev* boxes, int multibox, double h, hipStream_t s);
 void launch_correction(const AtomsDev& a, const TabDev& t, double h, hipStream_t s);
 void launch_unsort_d(const double* src, double* dst, const int* gid, int n, hipStream_t s);
+void launch_sort_pos(const double* src, double4* dst, const int* gid, int n, hipStream_t s);
+void launch_sort_d(const double* src, double* dst, const int* gid, int n, hipStream_t s);
 void launch_unsort_i(const int* src, int* dst, const int* gid, int n, hipStream_t s);
 void launch_unsort_pos(const double4* src, double* dst, const int* gid, int n, int comp,
                        hipStream_t s);

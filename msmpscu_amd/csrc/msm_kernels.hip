@@ -879,6 +879,25 @@ __global__ void k_unsort_pos(const double4* __restrict__ src, double* __restrict
     const double c = comp == 0 ? v.x : comp == 1 ? v.y : comp == 2 ? v.z : v.w;
     dst[gid[p]] = c;
 }
+// Caller-order coordinates (three arrays of n) into the sorted double4 records (w kept) --
+// CopyXPFrom_Host_to_Devices without a list rebuild.
+__global__ void k_sort_pos(const double* __restrict__ src, double4* __restrict__ dst,
+                           const int* __restrict__ gid, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int q = gid[p];
+    double4 v = dst[p];
+    v.x = src[q];
+    v.y = src[q + static_cast<size_t>(n)];
+    v.z = src[q + 2 * static_cast<size_t>(n)];
+    dst[p] = v;
+}
+// One caller-order array into sorted order.
+__global__ void k_sort_d(const double* __restrict__ src, double* __restrict__ dst,
+                         const int* __restrict__ gid, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[gid[p]];
+}
 __global__ void k_unsort_list(const int* __restrict__ kvois, const int* __restrict__ indi,
                               int stride, const int* __restrict__ gid, int n, int mx,
                               int* __restrict__ kv_out, int* __restrict__ indi_out, int ld) {
@@ -923,6 +942,14 @@ void launch_unsort_pos(const double4* src, double* dst, const int* gid, int n, i
                        hipStream_t s) {
     if (n > 0)
         hipLaunchKernelGGL(k_unsort_pos, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, gid, n, comp);
+}
+void launch_sort_pos(const double* src, double4* dst, const int* gid, int n, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_sort_pos, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, gid, n);
+}
+void launch_sort_d(const double* src, double* dst, const int* gid, int n, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_sort_d, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, gid, n);
 }
 void launch_unsort_list(const int* kvois, const int* indi, int stride, const int* gid, int n,
                         int mx, int* kv_out, int* indi_out, int ld, hipStream_t s) {

@@ -94,6 +94,12 @@ module MSM_HIP_C
       integer(c_int) :: rc
     end function msm_get_atoms
 
+    function msm_put_atoms(sim, xp, xp1) bind(C, name="msm_put_atoms") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sim, xp, xp1
+      integer(c_int) :: rc
+    end function msm_put_atoms
+
     function msm_get_epot(sim, epot) bind(C, name="msm_get_epot") result(rc)
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: sim
@@ -393,6 +399,7 @@ module MD_ForceLib_HIP
   public :: Predictor_DEV, Correction_DEV, Cal_GlobalT_DEV, VelScaling_DEV, Do_DynDamp_Forsteps_DEV
   public :: Do_Steepest_Forsteps_DEV, Do_CG_Forsteps_DEV
   public :: For_One_Step, Run_Steps_DEV, CopyOut_SimBox_DEV, Copyout_NeighboreList_DEV
+  public :: CopyXPFrom_Host_to_Devices, CopyXP1From_Host_to_Devices
   public :: Autotune_DEV, Adjoint_ForceTable_DEV, Clear_Globle_Variables_DEV, MSM_Handle
 
 contains
@@ -913,6 +920,53 @@ contains
       off = off + m
     end do
   end subroutine CopyOut_SimBox_DEV
+
+  ! CopyXPFrom_Host_to_Devices / CopyXP1From_Host_to_Devices (MD_Globle_Variables_GPU.F90:349-373):
+  ! SimBox positions or velocities of all boxes onto the device, no list rebuild -- what the
+  ! host-side quench loops (the L-BFGS driver) use between force evaluations
+  subroutine CopyXPFrom_Host_to_Devices(SimBox, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    integer, optional, intent(out) :: IERR
+    call put_xyz(SimBox, .false., "CopyXPFrom_Host_to_Devices", IERR)
+  end subroutine CopyXPFrom_Host_to_Devices
+
+  subroutine CopyXP1From_Host_to_Devices(SimBox, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    integer, optional, intent(out) :: IERR
+    call put_xyz(SimBox, .true., "CopyXP1From_Host_to_Devices", IERR)
+  end subroutine CopyXP1From_Host_to_Devices
+
+  subroutine put_xyz(SimBox, VEL, WHERE, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    logical, intent(in) :: VEL
+    character(*), intent(in) :: WHERE
+    integer, optional, intent(out) :: IERR
+    real(c_double), allocatable, target :: a(:)
+    integer :: n, b, off, k, m
+    integer(c_int) :: rc
+    n = gm_NPRT
+    allocate(a(3 * n))
+    off = 0
+    do b = 1, size(SimBox)
+      m = SimBox(b)%NPRT
+      do k = 1, 3
+        if (VEL) then
+          a((k - 1) * n + off + 1:(k - 1) * n + off + m) = SimBox(b)%XP1(:, k)
+        else
+          a((k - 1) * n + off + 1:(k - 1) * n + off + m) = SimBox(b)%XP(:, k)
+        end if
+      end do
+      off = off + m
+    end do
+    if (off /= n) then
+      rc = MSM_ERR_ARG
+    else if (VEL) then
+      rc = msm_put_atoms(gm_SIM, c_null_ptr, c_loc(a))
+    else
+      rc = msm_put_atoms(gm_SIM, c_loc(a), c_null_ptr)
+    end if
+    call msm_check(rc, WHERE, IERR)
+  end subroutine put_xyz
 
   ! Copyout_NeighboreList_DEV (MD_NeighborsList_GPU.F90:604-693): the full list in the caller's
   ! atom order, 1-based, INDI(N, MXKVOIS)

@@ -10,7 +10,9 @@
 !  "# stress_sum S11 S12 ... S33" (per-atom stresses of Cal_AtomicStress summed over the atoms at
 !  the end), a line "# temperature T" (Cal_GlobalT_DEV after VelScaling_DEV to 300 K), a line
 !  "# steepest" and N lines "x y z" after Do_Steepest_Forsteps_DEV (list rebuilt on the end
-!  positions, STEEPEST_MiDelE = 1e-6 eV, moves bounded by RR = a).
+!  positions, STEEPEST_MiDelE = 1e-6 eV, moves bounded by RR = a), then a line
+!  "# copy-in X T": x of atom 1 moved by 0.01 A and velocities zeroed on the host, copied in with
+!  CopyXPFrom_Host_to_Devices / CopyXP1From_Host_to_Devices and read back (X, and Cal_GlobalT_DEV).
 
 module W_FS_Potential
   use MD_ForceLib_HIP, only: KINDDF
@@ -195,6 +197,13 @@ program fs_w_driver
   do m = 1, n
     write(u, "(3(ES25.17E3,1X))") box(1)%XP(m, :)
   end do
+  box(1)%XP(1, 1) = box(1)%XP(1, 1) + 0.01d0
+  box(1)%XP1 = 0.0d0
+  call CopyXPFrom_Host_to_Devices(box)
+  call CopyXP1From_Host_to_Devices(box)
+  call Cal_GlobalT_DEV(box, ctrl, curt)
+  call CopyOut_SimBox_DEV(box, WITHEPOT=.false.)
+  write(u, "(A,2(1X,ES25.17E3))") "# copy-in", box(1)%XP(1, 1), curt
   close(u)
   call Clear_Globle_Variables_DEV()
   write(*, "(A,I0,A,ES24.16,A,ES24.16)") "fs_w_driver: n=", n, " E0=", e0, " E1=", e1

@@ -35,6 +35,10 @@ from ._lib import MSM_EAM_TYPE, MSM_FS_TYPE, check, dptr, iptr, lib
 PotFn = Union[str, Callable[[float], tuple], None]
 
 CP_KB_EV = 8.617333262e-5  # Boltzmann constant, eV/K
+CP_EVERG = 1.60219e-12     # erg per eV, as the reference
+CP_CM2A = 1.0e8            # Angstrom per cm
+# STATU bits (MD_Const.F90:80-96)
+CP_STATU_ACTIVE, CP_STATU_FIXPOSX, CP_STATU_OUTOFBOX = 1, 2, 2 ** 16
 # quench schemes (MD_Const.F90:23-27); LSEARCH is a bit added to the scheme
 CP_DAMPSCHEME_LBFGS, CP_DAMPSCHEME_DYN, CP_DAMPSCHEME_CG, CP_DAMPSCHEME_ST = 0, 1, 2, 3
 CP_DAMPSCHEME_LSEARCH = 2 ** 16
@@ -319,6 +323,8 @@ class SimMDCtrl:
     STEEPEST_MiStep: float = 1.0e-5
     STEEPEST_MxStep: float = 0.1
     STEEPEST_MiDelE: float = 1.0e-3
+    # corrections kept by the L-BFGS quench (MD_TypeDef_SimCtrlParam.F90:210)
+    LBFGS_Msave: int = 7
 
     def __post_init__(self):
         self.RU = np.asfortranarray(np.atleast_2d(self.RU), dtype=np.float64)
@@ -459,6 +465,20 @@ class DeviceMD:
         check(lib().msm_cg(self.h, int(maxsteps), int(bool(linesearch)), float(maxdis),
                            float(mindis), float(mindele), iptr(st), dptr(de)), "msm_cg", self.h)
         return int(st[0]), float(de[0])
+
+    def put_atoms(self, xp: Optional[np.ndarray] = None, xp1: Optional[np.ndarray] = None):
+        """Positions and/or velocities (N, 3) in the caller's order into the device state, no
+        list rebuild (msm_put_atoms)."""
+        def flat(a):
+            if a is None:
+                return None
+            a = np.asarray(a, dtype=np.float64)
+            if a.shape != (self.n, 3):
+                raise ValueError(f"expected ({self.n}, 3), got {a.shape}")
+            return np.ascontiguousarray(a.T).ravel()
+        x, v = flat(xp), flat(xp1)
+        check(lib().msm_put_atoms(self.h, None if x is None else dptr(x),
+                                  None if v is None else dptr(v)), "msm_put_atoms", self.h)
 
     def predictor(self, h: float):
         check(lib().msm_predictor(self.h, h), "msm_predictor", self.h)
@@ -697,6 +717,85 @@ def Do_CG_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU
         print(f" MDPSCU Message: {name} finished after steps: {it}")
     print(f"                 with max energy uncertainty(ev): {de:13.4E}")
     return it, de
+
+
+def lbfgs_coordinates(statu: np.ndarray) -> List[np.ndarray]:
+    """IDX, IDY, IDZ of DO_LBFGS_FORSTEPS_DEV (MD_LBFGSScheme_GPU.F90:69-97), 0-based: the active
+    atoms inside the box whose position in that direction is not fixed."""
+    st = np.asarray(statu)
+    ok = ((st & CP_STATU_ACTIVE) == CP_STATU_ACTIVE) & ((st & CP_STATU_OUTOFBOX) == 0)
+    return [np.flatnonzero(ok & ((st & (CP_STATU_FIXPOSX << k)) == 0)) for k in range(3)]
+
+
+def lbfgs_quench(xp: np.ndarray, statu: np.ndarray, lo: np.ndarray, zl: np.ndarray,
+                 ifpd: Sequence[int], evaluate: Callable[[], tuple],
+                 put: Callable[[np.ndarray], None], maxsteps: int, msave: int):
+    """The loop of DO_LBFGS_FORSTEPS_DEV (MD_LBFGSScheme_GPU.F90:43-158) over a force provider.
+
+    xp (N, 3) in A are the positions the provider holds; lo, zl (N, 3) the lower corner and edge
+    of each atom's box.  Each pass calls evaluate() -> (FP (N, 3) eV/A, EPOT (N) eV) at the
+    provider's positions, hands the objective to the optimiser in the reference's CGS units
+    (X in cm, G = -FP in dyn, OBJF = sum EPOT in erg; the tolerances EPS = 1e-4 and GTOL = 1e-2
+    are scale-dependent, so the units are kept), wraps the new X into the periodic directions
+    (below the box: +edge, above: -edge) and gives the positions back through put().  Returns
+    (evaluations, iflag, positions); iflag 0 on convergence, 1 when maxsteps ran out, negative
+    for an optimiser error (LBFGS flags)."""
+    from .lbfgs import LBFGS
+    x_a = np.array(xp, dtype=np.float64, order="F")
+    ids = lbfgs_coordinates(statu)
+    cols = [k for k in range(3) for _ in ids[k]]
+    rows = np.concatenate(ids)
+    cols = np.asarray(cols, dtype=np.int64)
+    to_cm = 1.0 / CP_CM2A
+    x = x_a[rows, cols] * to_cm
+    lb = np.asarray(lo, dtype=np.float64)[rows, cols] * to_cm
+    bs = np.asarray(zl, dtype=np.float64)[rows, cols] * to_cm
+    ub = lb + bs
+    per = np.asarray([bool(ifpd[k]) for k in range(3)])[cols]
+    opt = LBFGS(x.size, msave, 1.0e-4, 1.0e-16)
+    g_unit = -CP_EVERG * CP_CM2A  # eV/A -> dyn, with the sign of the gradient
+    iflag, it = 1, 0
+    for it in range(1, maxsteps + 1):
+        fp, ep = evaluate()
+        objf = float(np.sum(np.asarray(ep, dtype=np.float64) * CP_EVERG))
+        g = np.asarray(fp, dtype=np.float64)[rows, cols] * g_unit
+        iflag = opt.step(x, objf, g) if x.size else 0
+        low, high = per & (x < lb), per & (x > ub)
+        x[low] += bs[low]
+        x[high] -= bs[high]
+        x_a[rows, cols] = x * CP_CM2A
+        put(x_a)
+        if iflag == 0:
+            break
+    return it, iflag, x_a
+
+
+def Do_LBFGS_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU,
+                          MXNUMSTEPS: int):
+    """DO_LBFGS_FORSTEPS_DEV (MD_LBFGSScheme_GPU.F90:16-173): L-BFGS quench of all boxes on the
+    current neighbour list, at most MXNUMSTEPS force evaluations; the velocities are zeroed at
+    the end.  Returns (evaluations, iflag) -- iflag 0 on convergence."""
+    boxes = _boxes(SimBox)
+    dev = ForceClass.Device
+    xp, _, _, st, _ = dev.get_atoms()
+    nb = np.asarray([b.NPRT for b in boxes])
+    lo = np.repeat(np.asarray([b.BOXLOW for b in boxes]), nb, axis=0)
+    zl = np.repeat(np.asarray([b.ZL for b in boxes]), nb, axis=0)
+
+    def evaluate():
+        dev.calc_force()
+        dev.calc_epot()
+        return dev.get_atoms()[2], dev.get_epot()
+
+    it, iflag, _ = lbfgs_quench(xp, st, lo, zl, CtrlParam.IFPD, evaluate, dev.put_atoms,
+                                MXNUMSTEPS, CtrlParam.LBFGS_Msave)
+    if iflag == 0:
+        print(f" MDPSCU Message: LBFG succefully performed for steps {it:6d}")
+    dev.put_atoms(None, np.zeros((dev.n, 3)))
+    if iflag != 0:
+        print(f" MDPSCU Warning: LBFG exit with erro code {iflag:2d}")
+        print("                 in O_LBFGS_FORSTEPS_DEV")
+    return it, iflag
 
 
 def Predictor_DEV(ITIME: int, SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):

@@ -2,8 +2,8 @@
 a Fortran program: forces and energy of the W box at the start and after 100 steps (one
 per-step call, then a fused run) against the oracle on the same positions, the per-atom
 stresses of Cal_AtomicStress summed over the atoms against twice the oracle's virial, the
-temperature after VelScaling_DEV, and the positions after Do_Steepest_Forsteps_DEV against the
-oracle's steepest descent."""
+temperature after VelScaling_DEV, the positions after Do_Steepest_Forsteps_DEV against the
+oracle's steepest descent, and the copy-in of positions and velocities."""
 import os
 import subprocess
 import sys
@@ -86,3 +86,8 @@ def test_fortran_driver_against_oracle(orc, tmp_path):
     dx = quench - ox
     dx -= np.round(dx / box.ZL) * box.ZL
     assert np.max(np.abs(dx)) <= 1e-9
+    # copy-in of positions and velocities: the moved coordinate comes back as written, and the
+    # zeroed velocities give 0 K
+    ci = lines[5 + 3 * n].split()
+    assert ci[:2] == ["#", "copy-in"]
+    assert float(ci[2]) == quench[0, 0] + 0.01 and float(ci[3]) == 0.0

@@ -1,5 +1,6 @@
 """GPU parity of the thermalisation and quench entry points (msm_damping, msm_calc_ekin,
-msm_scale_velocities, msm_dyn_damp) against the CPU oracle, through the C-ABI."""
+msm_scale_velocities, msm_dyn_damp, msm_steepest, msm_cg, and the L-BFGS driver over
+msm_put_atoms) against the CPU oracle, through the C-ABI."""
 import sys
 
 import numpy as np
@@ -215,4 +216,98 @@ def test_cg_matches_oracle(orc, meth, maxsteps, mindele):
     dx = x - ox
     dx -= np.round(dx / box.ZL) * box.ZL
     assert np.max(np.abs(dx)) <= 1e-9
+    dev.close()
+
+
+def test_put_atoms_round_trip_and_forces(orc):
+    """msm_put_atoms: positions and velocities written in the caller's order come back bit-exact
+    (either may be left out), and the forces at the new positions match the oracle's on the same
+    list; coordinates several box lengths out (an L-BFGS trial point) are shifted by one box
+    length at most, as the reference's minimum image does."""
+    ctrl = cfg.w_ctrl(mx=64)
+    box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.04, 300.0, 95)
+    box.STATU = mixed_statu(box.NPRT, 96)
+    ft = cfg.w_forcetable(box, ctrl)
+    say("put_atoms")
+    fc, dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    x0, v0, _, st, _ = dev.get_atoms()
+    rng = np.random.default_rng(97)
+    x1 = np.asfortranarray(x0 + rng.normal(0.0, 0.02, x0.shape))
+    v1 = np.asfortranarray(rng.normal(0.0, 1e-3, v0.shape))
+    dev.put_atoms(x1, None)
+    x, v, _, _, _ = dev.get_atoms()
+    assert np.array_equal(x, x1) and np.array_equal(v, v0)
+    dev.put_atoms(None, v1)
+    x, v, _, _, _ = dev.get_atoms()
+    assert np.array_equal(x, x1) and np.array_equal(v, v1)
+    kv, ind = dev.get_neighbors()
+    ind0 = np.where(ind > 0, ind - 1, 0)
+    t = orc.from_mdlib(ft)
+    ru2 = float(np.max(ctrl.RU)) ** 2
+    it = box.ITYP - 1
+    # 40 atoms, none in another's list, to be sent far out
+    far, taken = [], set()
+    for i in rng.permutation(box.NPRT):
+        nb = set(int(j) for j in ind0[i, :kv[i]])
+        if i not in taken and not (nb & set(far)):
+            far.append(int(i))
+            taken |= nb
+        if len(far) == 40:
+            break
+    far = np.asarray(far)
+    for shift in (0.0, 3.0e5):
+        xs = np.array(x1, order="F")
+        xs[far] += shift * rng.choice([-1.0, 1.0], (40, 3))
+        dev.put_atoms(xs)
+        dev.calc_force()
+        dev.calc_epot()
+        _, _, f, _, _ = dev.get_atoms()
+        ep = dev.get_epot()
+        of, _, oe = orc.force_full(xs, it, st, kv, ind0, box.ZL, ctrl.IFPD, ru2, t)
+        assert np.max(np.abs(f - of)) <= 1e-10
+        assert abs(ep.sum() - oe.sum()) <= 1e-12 * abs(oe.sum())
+        if shift:
+            assert np.all(f[far] == 0.0) and np.all(ep[far] == 0.0)
+    dev.close()
+
+
+def test_lbfgs_matches_oracle(orc):
+    """L-BFGS quench (the host optimiser over device forces, msm_put_atoms between evaluations)
+    of a displaced 1024-atom W box with FIXPOS, inactive and OUTOFBOX atoms, against the same
+    loop over the oracle's forces on the device's list: same evaluation count and flag, positions
+    to 1e-8 A, velocities zero at the end."""
+    ctrl = cfg.w_ctrl(mx=64)
+    box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.05, 300.0, 98)
+    box.STATU = mixed_statu(box.NPRT, 99)
+    box.STATU[7] |= 65536
+    ft = cfg.w_forcetable(box, ctrl)
+    say("lbfgs")
+    fc, dev = device_for(box, ctrl, ft)
+    dev.build_neighbors()
+    x0, _, _, st0, _ = dev.get_atoms()
+    kv, ind = dev.get_neighbors()
+    ind0 = np.where(ind > 0, ind - 1, 0)
+    t = orc.from_mdlib(ft)
+    ru2 = float(np.max(ctrl.RU)) ** 2
+    it = box.ITYP - 1
+    state = {"x": np.array(x0)}
+
+    def evaluate():
+        f, _, e = orc.force_full(state["x"], it, st0, kv, ind0, box.ZL, ctrl.IFPD, ru2, t)
+        return f, e
+
+    def put(xa):
+        state["x"] = np.array(xa)
+
+    n = box.NPRT
+    lo3, zl3 = np.tile(box.BOXLOW, (n, 1)), np.tile(box.ZL, (n, 1))
+    osteps, oflag, ox = md.lbfgs_quench(x0, st0, lo3, zl3, ctrl.IFPD, evaluate, put, 300,
+                                        ctrl.LBFGS_Msave)
+    assert oflag == 0 and 0 < osteps < 300
+    steps, iflag = md.Do_LBFGS_Forsteps_DEV([box], ctrl, fc, 300)
+    assert (steps, iflag) == (osteps, oflag)
+    x, v, _, st, _ = dev.get_atoms()
+    assert np.max(np.abs(x - ox)) <= 1e-8
+    assert np.all(v == 0.0) and np.array_equal(st, st0)
     dev.close()
