@@ -241,12 +241,45 @@ def Load_Config_SimMDBox(fname: str, SimBox: SimMDBox) -> SimMDBox:
     return SimBox
 
 
+def _first_keyword(fname: str) -> str:
+    with open(fname) as fh:
+        for line in fh:
+            t = line.split("!")[0].strip()
+            if t:
+                return t.split()[0].upper() if t.startswith("&") else ""
+    return ""
+
+
+def _check_consistent(SimBox: SimMDBox) -> SimMDBox:
+    """Atoms outside the box are marked OUTOFBOX; FIXPOS components of the velocity are zero."""
+    xp = np.asarray(SimBox.XP)
+    st = np.asarray(SimBox.STATU, dtype=np.int32).copy()
+    lo = np.asarray(SimBox.BOXLOW, dtype=np.float64)
+    hi = lo + np.asarray(SimBox.ZL, dtype=np.float64)
+    out = np.any((xp < lo) | (xp > hi), axis=1)
+    st[out] = 65536  # CP_STATU_OUTOFBOX
+    xp1 = np.array(SimBox.XP1, dtype=np.float64, order="F")
+    for k, bit in enumerate((2, 4, 8)):
+        xp1[(st & bit) != 0, k] = 0.0
+    SimBox.STATU = st
+    SimBox.XP1 = xp1
+    return SimBox
+
+
 def Read_Initial_Config_SimMDBox(fname: str, SimBox: SimMDBox, with_velocity: bool = False,
                                  prop: Optional[Sequence[int]] = None) -> SimMDBox:
-    """Plain table of Read_Initial_Config_SimMDBox0: SimBox.NPRT rows of type and position in
-    lattice units (SimBox.RR), with velocities in cm/s when with_velocity (CP_INPUT_VEL); the
-    geometry (RR, BOXLOW, ZL) comes from SimBox.  STATU is prop[type - 1] (ACTIVE by default);
-    atoms outside the box are marked OUTOFBOX, and FIXPOS components of the velocity are zero."""
+    """Read_Initial_Config_SimMDBox (MD_TypeDef_SimBox.F90:1287-1385) for the plain table and
+    &CFGXYZ files.  A file opening with &CFGXYZ goes to Read_ConfigByCol18_SimMDBox; any other
+    leading keyword is an error.  The plain table (Read_Initial_Config_SimMDBox0): SimBox.NPRT rows
+    of type and position in lattice units (SimBox.RR), with velocities in cm/s when with_velocity
+    (CP_INPUT_VEL); the geometry (RR, BOXLOW, ZL) comes from SimBox.  STATU is prop[type - 1]
+    (ACTIVE by default); atoms outside the box are marked OUTOFBOX, and FIXPOS components of the
+    velocity are zero."""
+    kw = _first_keyword(fname)
+    if kw == "&CFGXYZ":
+        return Read_ConfigByCol18_SimMDBox(fname, SimBox, prop)
+    if kw:
+        raise ValueError(f"MDPSCU Error: unknown file format: {kw} in {fname}")
     n = SimBox.NPRT
     rows = []
     with open(fname) as fh:
@@ -267,15 +300,114 @@ def Read_Initial_Config_SimMDBox(fname: str, SimBox: SimMDBox, with_velocity: bo
         xp1 = np.array([[float(v) for v in r[4:7]] for r in rows]) * CM_S_TO_A_FS
     ng = int(ityp.max())
     pr = np.ones(ng, np.int32) if prop is None else np.asarray(prop, np.int32)
-    st = pr[ityp - 1].copy()
-    lo = np.asarray(SimBox.BOXLOW, dtype=np.float64)
-    hi = lo + np.asarray(SimBox.ZL, dtype=np.float64)
-    out = np.any((xp < lo) | (xp > hi), axis=1)
-    st[out] = 65536  # CP_STATU_OUTOFBOX
-    for k, bit in enumerate((2, 4, 8)):
-        xp1[(st & bit) != 0, k] = 0.0
     SimBox.ITYP = ityp
     SimBox.XP = np.asfortranarray(xp)
     SimBox.XP1 = np.asfortranarray(xp1)
-    SimBox.STATU = st
-    return SimBox
+    SimBox.STATU = pr[ityp - 1].copy()
+    return _check_consistent(SimBox)
+
+
+# column keywords of &CFGXYZ files (Read_ConfigByCol18A_SimMDBox, MD_TypeDef_SimBox.F90:2739-3025)
+_COLS = {"&XYZCOL": "xp", "&DISCOL": "dis", "&TYPECOL": "ityp", "&VELCOL": "xp1",
+         "&FPCOL": "fp", "&EPOTCOL": "epot", "&EKINCOL": "ekin", "&STATUCOL": "statu",
+         "&STATCOL": "statu"}
+_WIDTH = {"xp": 3, "dis": 3, "xp1": 3, "fp": 3, "ityp": 1, "epot": 1, "ekin": 1, "statu": 1}
+
+
+def Read_ConfigByCol18_SimMDBox(fname: str, SimBox: SimMDBox,
+                                prop: Optional[Sequence[int]] = None) -> SimMDBox:
+    """Read_ConfigByCol18_SimMDBox (MD_TypeDef_SimBox.F90:2634-2735, 2739-3133): a header of
+    &KEYWORD lines naming the 1-based column of each quantity (&XYZCOL required; &TYPECOL,
+    &VELCOL, &FPCOL, &DISCOL, &EPOTCOL, &EKINCOL, &STATCOL/&STATUCOL optional) and the length
+    unit (&LUNIT or &LENTHUNIT "A" | "CM" | "NM" | "LU", times an optional number; required
+    unless the header has &BOXCFG18), ended by the first line without a keyword; then SimBox.NPRT
+    rows.  Positions and displacements are scaled by the unit; the other columns hold the
+    reference's internal units (velocity cm/s, force dyn, energies erg) and are converted to
+    eV/A/fs; the potential column holds -EPOT.  Without &TYPECOL every atom is of type 1; without
+    a STATU column STATU is prop[type - 1] (ACTIVE by default).  The geometry and NPRT come from
+    SimBox; out-of-box atoms are marked OUTOFBOX."""
+    with open(fname) as fh:
+        lines = fh.read().splitlines()
+    cols = {}
+    unit = None
+    needunit = True
+    i = 0
+    while i < len(lines):
+        t = lines[i].split("!")[0].strip()
+        if not t:
+            i += 1
+            continue
+        if not t.startswith("&"):
+            break
+        kw = t.split()[0].upper()
+        rest = t[len(kw):]
+        if kw == "&BOXCFG18":
+            needunit = False
+        elif kw in ("&LUNIT", "&LENTHUNIT"):
+            words = rest.replace('"', " ").replace("'", " ").replace(",", " ").split()
+            if not words:
+                raise ValueError(f"MDPSCU Error: length unit missing after {kw} in {fname}")
+            u = words[0].upper()
+            if u == "A":
+                unit = 1.0
+            elif u == "CM":
+                unit = CP_CM2A
+            elif u == "NM":
+                unit = 10.0
+            elif u == "LU":
+                unit = float(SimBox.RR) if SimBox.RR > 0.0 else None
+            else:
+                raise ValueError(f"MDPSCU Error: unknown length unit {words[0]} in {fname}")
+            num = _numbers(" ".join(words[1:]))
+            if num and unit is not None:
+                unit *= num[0]
+        elif kw in _COLS:
+            num = _numbers(rest.replace(",", " "))
+            c = int(num[0]) if num else 0
+            if c <= 0:
+                raise ValueError(f"MDPSCU Error: the column of {kw} is missed in {fname}")
+            cols[_COLS[kw]] = c
+        i += 1
+    if "xp" not in cols:
+        raise ValueError(f"MDPSCU Error: keyword &XYZCOL not found in {fname}")
+    if needunit and (unit is None or unit <= 0.0):
+        raise ValueError(f"MDPSCU Error: the length unit (&LUNIT) is missed in {fname}")
+    lunit = unit if needunit else 1.0
+    ncol = max(c + _WIDTH[k] - 1 for k, c in cols.items())
+    n = SimBox.NPRT
+    rows = []
+    while i < len(lines) and len(rows) < n:
+        t = lines[i].split("!")[0].split()
+        i += 1
+        if not t:
+            continue
+        if len(t) < ncol:
+            raise ValueError(f"MDPSCU Error: {fname}: a row has {len(t)} of {ncol} columns")
+        rows.append(t[:ncol])
+    if len(rows) < n:
+        raise ValueError(f"MDPSCU Error: {fname}: only {len(rows)} atoms for NPRT {n}")
+
+    def block(key, width):
+        c = cols[key] - 1
+        return np.array([[float(r[c + k].replace("D", "E")) for k in range(width)] for r in rows])
+
+    SimBox.XP = np.asfortranarray(block("xp", 3) * lunit)
+    if "dis" in cols:
+        SimBox.DIS = np.asfortranarray(block("dis", 3) * lunit)
+    SimBox.XP1 = np.asfortranarray(block("xp1", 3) * CM_S_TO_A_FS) if "xp1" in cols \
+        else np.zeros((n, 3), order="F")
+    if "fp" in cols:
+        SimBox.FP = np.asfortranarray(block("fp", 3) * DYN_TO_EV_A)
+    if "epot" in cols:
+        SimBox.EPOT = -block("epot", 1)[:, 0] / CP_EVERG
+    if "ekin" in cols:
+        SimBox.EKIN = block("ekin", 1)[:, 0] / CP_EVERG
+    SimBox.ITYP = block("ityp", 1)[:, 0].astype(np.int32) if "ityp" in cols \
+        else np.ones(n, np.int32)
+    if "statu" in cols:
+        SimBox.STATU = block("statu", 1)[:, 0].astype(np.int32)
+    else:
+        ng = int(SimBox.ITYP.max())
+        pr = np.ones(ng, np.int32) if prop is None else np.asarray(prop, np.int32)
+        SimBox.STATU = pr[SimBox.ITYP - 1].copy()
+    return _check_consistent(SimBox)

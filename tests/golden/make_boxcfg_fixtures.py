@@ -5,7 +5,9 @@ boxcfg15_w2000_he1_excerpt.cfg -- examples/NEB_Test/PrevCfg.0000 (&BOXCFG15, wri
     reference): its 10 header lines with the atom counts set to the excerpt's, the first 30 W rows
     and the He row;
 plain_w2000_excerpt.cfg -- the first 20 rows of examples/use_extern_pottable/W_2000.cfg (plain
-    table: type, position in LU, velocity in cm/s, then columns the loader does not read).
+    table: type, position in LU, velocity in cm/s, then columns the loader does not read);
+cfgxyz_w2000_excerpt.cfg -- examples/Cluster_Growth_Test/W_2000.cfg (&CFGXYZ): its 5 header
+    lines with &NATOM set to 20, and the first 20 rows.
 """
 import os
 
@@ -27,3 +29,9 @@ with open(os.path.join(REF, "use_extern_pottable", "W_2000.cfg")) as fh:
     plain = [next(fh) for _ in range(20)]
 with open(os.path.join(HERE, "plain_w2000_excerpt.cfg"), "w") as fh:
     fh.write("".join(plain))
+
+with open(os.path.join(REF, "Cluster_Growth_Test", "W_2000.cfg")) as fh:
+    xyz = [next(fh) for _ in range(25)]
+xyz[1] = xyz[1].replace("2000", "20")
+with open(os.path.join(HERE, "cfgxyz_w2000_excerpt.cfg"), "w") as fh:
+    fh.write("".join(xyz))

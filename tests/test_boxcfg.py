@@ -107,3 +107,53 @@ def test_reads_the_plain_initial_table():
     bc.Read_Initial_Config_SimMDBox(os.path.join(GOLD, "plain_w2000_excerpt.cfg"), b)
     assert np.all(b.STATU[np.any(b.XP > b.BOXLOW + b.ZL, axis=1)] == 65536)
     assert np.all(b.XP1 == 0.0)
+
+
+def _geom_box(n, rr=3.1652):
+    b = _empty_box()
+    b.NPRT = n
+    b.RR = rr
+    b.BOXLOW = np.full(3, -5.0 * rr)
+    b.ZL = np.full(3, 10.0 * rr)
+    return b
+
+
+def test_reads_the_reference_cfgxyz_file():
+    """&CFGXYZ (excerpt of examples/Cluster_Growth_Test/W_2000.cfg): TYPECOL 1, XYZCOL 2-4 in
+    lattice units; the same atoms as the plain excerpt of W_2000.cfg read by the plain reader;
+    no velocity column, so velocities are zero."""
+    b = bc.Read_Initial_Config_SimMDBox(os.path.join(GOLD, "cfgxyz_w2000_excerpt.cfg"),
+                                        _geom_box(20))
+    p = bc.Read_Initial_Config_SimMDBox(os.path.join(GOLD, "plain_w2000_excerpt.cfg"),
+                                        _geom_box(20))
+    assert np.allclose(b.XP[0], np.array([4.2425, 4.2517, 4.2564]) * 3.1652)
+    assert np.array_equal(b.XP, p.XP) and np.array_equal(b.ITYP, p.ITYP)
+    assert np.all(b.STATU == 1) and np.all(b.XP1 == 0.0)
+
+
+def test_cfgxyz_columns_and_units(tmp_path):
+    """Every column keyword with its unit: positions and displacements in the &LUNIT unit (here
+    2 nm), velocity cm/s, force dyn, energies erg, the potential column holding -EPOT, STATU
+    read; a missing unit, a short row and an unknown leading keyword are errors."""
+    f = tmp_path / "c.cfg"
+    f.write_text('&CFGXYZ\n&LUNIT "NM" 2.0\n&XYZCOL 3\n&TYPECOL 2\n&VELCOL 6\n&FPCOL 9\n'
+                 "&EPOTCOL 12\n&EKINCOL 13\n&STATCOL 1\n&DISCOL 14\n"
+                 "3 1 0.1 0.2 -0.3 1e5 -2e5 3e5 1.60219e-4 0 0 1.60219e-12 3.20438e-12 0.01 0 0\n"
+                 "1 2 0.4 0.0 0.0 0 0 0 0 0 0 0 0 0 0 0\n")
+    b = bc.Read_Initial_Config_SimMDBox(str(f), _geom_box(2))
+    assert np.allclose(b.XP[0], [2.0, 4.0, -6.0]) and np.allclose(b.XP[1], [8.0, 0.0, 0.0])
+    assert np.allclose(b.DIS[0], [0.2, 0.0, 0.0])
+    # cm/s -> A/fs; STATU 3 carries FIXPOSX, so the x component is zeroed
+    assert np.allclose(b.XP1[0], np.array([0.0, -2e5, 3e5]) * 1e-7)
+    assert b.FP[0, 0] == pytest.approx(1.0)                          # 1.60219e-4 dyn = 1 eV/A
+    assert b.EPOT[0] == pytest.approx(-1.0) and b.EKIN[0] == pytest.approx(2.0)
+    assert list(b.ITYP) == [1, 2] and list(b.STATU) == [3, 1]
+    f.write_text("&CFGXYZ\n&XYZCOL 2\n1 0 0 0\n")
+    with pytest.raises(ValueError, match="LUNIT"):
+        bc.Read_Initial_Config_SimMDBox(str(f), _geom_box(1))
+    f.write_text('&CFGXYZ\n&LUNIT "A"\n&XYZCOL 2\n1 0 0\n')
+    with pytest.raises(ValueError, match="columns"):
+        bc.Read_Initial_Config_SimMDBox(str(f), _geom_box(1))
+    f.write_text("&FOO\n1 0 0 0\n")
+    with pytest.raises(ValueError, match="unknown file format"):
+        bc.Read_Initial_Config_SimMDBox(str(f), _geom_box(1))
