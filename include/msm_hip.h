@@ -189,6 +189,12 @@ int msm_calc_ptensor(msm_sim* sim, double* vtensor);
  * stores the forces as msm_calc_force does; the sum over atoms is twice msm_calc_ptensor's. */
 int msm_calc_atomic_stress(msm_sim* sim, double* avp);
 
+/* CheckTimestep_DEV (MD_DiffScheme_GPU.F90:1068-1210): *exceeded = 1 when an active atom's
+ * displacement over a step th, |th v + th^2/2 F/m| (components without a FIXPOS bit), exceeds
+ * sqrt(mxd2); else 0.  The variable-step search of Predictor_DEV (:629-657) halves th from HMX
+ * until this flag is 0. */
+int msm_check_timestep(msm_sim* sim, double th, double mxd2, int* exceeded);
+
 /* Thermalisation and quench around the force path (MD_DiffScheme_GPU.F90).
  * msm_damping: DAMPING_KERNEL (:125-187) -- each velocity component opposite to its force
  * component is set to zero, and components with FIXPOS set (every atom).

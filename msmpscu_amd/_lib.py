@@ -72,6 +72,7 @@ SIGNATURES = {
     "msm_set_atoms": (C.c_int, [_vp, C.c_int, _dp, _dp, _ip, _ip]),
     "msm_get_atoms": (C.c_int, [_vp, _dp, _dp, _dp, _ip, _dp]),
     "msm_put_atoms": (C.c_int, [_vp, _dp, _dp]),
+    "msm_check_timestep": (C.c_int, [_vp, C.c_double, C.c_double, _ip]),
     "msm_get_den": (C.c_int, [_vp, _dp]),
     "msm_get_epot": (C.c_int, [_vp, _dp]),
     "msm_get_box_epot": (C.c_int, [_vp, _dp]),

@@ -1271,6 +1271,25 @@ int msm_damping(msm_sim* s) {
     return MSM_OK;
 }
 
+int msm_check_timestep(msm_sim* s, double th, double mxd2, int* exceeded) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (!exceeded || !(th > 0.0) || !(mxd2 >= 0.0))
+        return set_err(s, MSM_ERR_ARG, "th > 0, mxd2 >= 0 and an output flag are required");
+    (void)hipSetDevice(s->device);
+    int* flag = nullptr;
+    if ((rc = dalloc(s, &flag, 1)) != MSM_OK) return rc;
+    hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s->stream);
+    if (e == hipSuccess) {
+        msm::launch_check_dt(atoms_view(s, s->scur, s->pcur), s->tab, th, mxd2, flag, s->stream);
+        e = hipMemcpyAsync(exceeded, flag, sizeof(int), hipMemcpyDeviceToHost, s->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    dfree(&flag);
+    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    return MSM_OK;
+}
+
 int msm_calc_ekin(msm_sim* s, double kb, double* ekin, double* tbox) {
     int rc = check_ready(s, true);
     if (rc != MSM_OK) return rc;

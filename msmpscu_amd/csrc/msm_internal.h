@@ -224,6 +224,8 @@ void launch_virial(const AtomsDev& a, const ListDev& l, const TabDev& t, const B
 // do not count), per-box sum and count of non-negative EK, velocity scaling by the factor of the
 // atom's box, and max |epot - epot0| (then epot0 = epot; *out zero on entry, a double's bits).
 void launch_damping(const AtomsDev& a, hipStream_t s);
+void launch_check_dt(const AtomsDev& a, const TabDev& t, double th, double mxd2, int* flag,
+                     hipStream_t s);
 void launch_ekin(const AtomsDev& a, const TabDev& t, double* ek, hipStream_t s);
 void launch_box_ekin(const double* ek, const int* start, const BoxDev* boxes, int nbox,
                      double* sum, double* cnt, hipStream_t s);

@@ -612,6 +612,35 @@ void launch_damping(const AtomsDev& a, hipStream_t s) {
     if (a.n > 0) hipLaunchKernelGGL(k_damping, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a);
 }
 
+// CheckTimestep_KERNEL (MD_DiffScheme_GPU.F90:1068-1139): flag when an active atom's step
+// displacement th v + h2s2 F/m (components without a FIXPOS bit) has a square above mxd2; one
+// atomic per wave that sees such an atom.
+__global__ void k_check_dt(AtomsDev a, TabDev t, double th, double h2s2, double mxd2,
+                           int* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool over = false;
+    if (i < a.n) {
+        const int st = a.statu[i];
+        if ((st & STATU_ACTIVE) == STATU_ACTIVE) {
+            const double cm0 = t.cm[a.ityp[i]];
+            double dx = 0.0, dy = 0.0, dz = 0.0;
+            if ((st & STATU_FIXPOSX) == 0) dx = th * a.vx[i] + h2s2 * (a.fx[i] / cm0);
+            if ((st & STATU_FIXPOSY) == 0) dy = th * a.vy[i] + h2s2 * (a.fy[i] / cm0);
+            if ((st & STATU_FIXPOSZ) == 0) dz = th * a.vz[i] + h2s2 * (a.fz[i] / cm0);
+            over = dx * dx + dy * dy + dz * dz > mxd2;
+        }
+    }
+    const unsigned long long b = __ballot(over);
+    if (b != 0ull && static_cast<int>(threadIdx.x & 63) == __ffsll(b) - 1) atomicOr(flag, 1);
+}
+
+void launch_check_dt(const AtomsDev& a, const TabDev& t, double th, double mxd2, int* flag,
+                     hipStream_t s) {
+    if (a.n > 0)
+        hipLaunchKernelGGL(k_check_dt, dim3(grid_for(a.n)), dim3(kBlock), 0, s, a, t, th,
+                           0.5 * th * th, mxd2, flag);
+}
+
 // CALEKIN_KERNEL (:851-904): EK = CM/2 |v|^2 for active atoms with no FIXPOS bit, else -1e32.
 __global__ void k_ekin(AtomsDev a, TabDev t, double* __restrict__ ek) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -94,6 +94,15 @@ module MSM_HIP_C
       integer(c_int) :: rc
     end function msm_get_atoms
 
+    function msm_check_timestep(sim, th, mxd2, exceeded) bind(C, name="msm_check_timestep") &
+        result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: sim
+      real(c_double), value :: th, mxd2
+      integer(c_int), intent(out) :: exceeded
+      integer(c_int) :: rc
+    end function msm_check_timestep
+
     function msm_put_atoms(sim, xp, xp1) bind(C, name="msm_put_atoms") result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: sim, xp, xp1
@@ -399,7 +408,7 @@ module MD_ForceLib_HIP
   public :: Predictor_DEV, Correction_DEV, Cal_GlobalT_DEV, VelScaling_DEV, Do_DynDamp_Forsteps_DEV
   public :: Do_Steepest_Forsteps_DEV, Do_CG_Forsteps_DEV
   public :: For_One_Step, Run_Steps_DEV, CopyOut_SimBox_DEV, Copyout_NeighboreList_DEV
-  public :: CopyXPFrom_Host_to_Devices, CopyXP1From_Host_to_Devices
+  public :: CopyXPFrom_Host_to_Devices, CopyXP1From_Host_to_Devices, CheckTimestep_DEV
   public :: Autotune_DEV, Adjoint_ForceTable_DEV, Clear_Globle_Variables_DEV, MSM_Handle
 
 contains
@@ -837,6 +846,18 @@ contains
     end if
     write(*, "(A, 1PE13.4)") "                 with max energy uncertainty(ev): ", delepot
   end subroutine Do_CG_Forsteps_DEV
+
+  ! CheckTimestep_DEV (MD_DiffScheme_GPU.F90:1068-1210): IFLAG = 1 when some active atom would
+  ! move more than sqrt(MXD2) in a step TH (the variable-step search halves TH until IFLAG = 0)
+  subroutine CheckTimestep_DEV(TH, MXD2, IFLAG, IERR)
+    real(c_double), intent(in) :: TH, MXD2
+    integer, intent(out) :: IFLAG
+    integer, optional, intent(out) :: IERR
+    integer(c_int) :: fl
+    fl = 0
+    call msm_check(msm_check_timestep(gm_SIM, TH, MXD2, fl), "CheckTimestep_DEV", IERR)
+    IFLAG = fl
+  end subroutine CheckTimestep_DEV
 
   ! Predictor_DEV / Correction_DEV (MD_DiffScheme_GPU.F90:604-682 / 821-847)
   subroutine Predictor_DEV(ITIME, SimBox, CtrlParam, IERR)

@@ -325,6 +325,16 @@ class SimMDCtrl:
     STEEPEST_MiDelE: float = 1.0e-3
     # corrections kept by the L-BFGS quench (MD_TypeDef_SimCtrlParam.F90:210)
     LBFGS_Msave: int = 7
+    # time step control (MD_TypeDef_SimCtrlParam.F90:36-47, 78-79): IHDUP < 0 searches the step
+    # from HMX every |IHDUP| steps so that no atom moves more than DMX; damping is applied in
+    # the predictor for steps DAMPTIME0 .. DAMPTIME0 + DAMPTIME1 - 1
+    IT0: int = 1
+    IHDUP: int = 0
+    HMI: float = 1.0
+    HMX: float = 1.0
+    DMX: float = 0.1
+    DAMPTIME0: int = 1
+    DAMPTIME1: int = 0
 
     def __post_init__(self):
         self.RU = np.asfortranarray(np.atleast_2d(self.RU), dtype=np.float64)
@@ -479,6 +489,14 @@ class DeviceMD:
         x, v = flat(xp), flat(xp1)
         check(lib().msm_put_atoms(self.h, None if x is None else dptr(x),
                                   None if v is None else dptr(v)), "msm_put_atoms", self.h)
+
+    def check_timestep(self, th: float, mxd2: float) -> bool:
+        """msm_check_timestep: True when some active atom would move more than sqrt(mxd2) in a
+        step th."""
+        fl = np.zeros(1, dtype=np.int32)
+        check(lib().msm_check_timestep(self.h, float(th), float(mxd2), iptr(fl)),
+              "msm_check_timestep", self.h)
+        return bool(fl[0])
 
     def predictor(self, h: float):
         check(lib().msm_predictor(self.h, h), "msm_predictor", self.h)
@@ -798,8 +816,31 @@ def Do_LBFGS_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClass
     return it, iflag
 
 
+def Check_Timestep_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU) -> float:
+    """The variable-step search of Predictor_DEV (MD_DiffScheme_GPU.F90:633-650, the check in
+    CheckTimestep_DEV/_KERNEL :1068-1210): from HMX, halve the step while some active atom would
+    move more than DMX; the step found is stored in CtrlParam.H and returned."""
+    dev = ForceClass.Device
+    th = float(CtrlParam.HMX)
+    mxd2 = float(CtrlParam.DMX) ** 2
+    while dev.check_timestep(th, mxd2):
+        th *= 0.5
+        if th == 0.0:
+            raise ValueError("MDPSCU Error: no time step keeps the displacements under DMX")
+    CtrlParam.H = th
+    return th
+
+
 def Predictor_DEV(ITIME: int, SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
-    ForceClass.Device.predictor(CtrlParam.H)
+    """Predictor_DEV (MD_DiffScheme_GPU.F90:604-682): damping when ITIME is in
+    [DAMPTIME0, DAMPTIME0 + DAMPTIME1 - 1]; with IHDUP < 0 and mod(ITIME - IT0 + 1, |IHDUP|) = 0
+    the step search of Check_Timestep_DEV; then the Swope predictor with CtrlParam.H."""
+    dev = ForceClass.Device
+    if CtrlParam.DAMPTIME0 <= ITIME <= CtrlParam.DAMPTIME0 + CtrlParam.DAMPTIME1 - 1:
+        dev.damping()
+    if CtrlParam.IHDUP < 0 and (ITIME - CtrlParam.IT0 + 1) % abs(CtrlParam.IHDUP) == 0:
+        Check_Timestep_DEV(SimBox, CtrlParam, ForceClass)
+    dev.predictor(CtrlParam.H)
 
 
 def Correction_DEV(ITIME: int, SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):

@@ -360,6 +360,29 @@ def dyn_damp(xp, xp1, fp, dis, statu, ityp0, cm, h, lo, ln, ifpd, kvois, indi, r
     return x, v, f, d, st, ep, it, de
 
 
+def check_timestep(xp1, fp, statu, ityp0, cm, th, mxd2) -> bool:
+    """CheckTimestep_KERNEL (MD_DiffScheme_GPU.F90:1068-1139): some active atom has
+    |th v + th^2/2 F/m|^2 > mxd2 over the components without a FIXPOS bit."""
+    v = np.asarray(xp1, dtype=np.float64)
+    f = np.asarray(fp, dtype=np.float64)
+    st = np.asarray(statu)
+    m = np.asarray(cm, dtype=np.float64)[np.asarray(ityp0)]
+    h2s2 = th * th * 0.5
+    d = th * v + h2s2 * (f / m[:, None])
+    for k, bit in enumerate((2, 4, 8)):
+        d[(st & bit) != 0, k] = 0.0
+    act = (st & 1) == 1
+    return bool(np.any(np.sum(d * d, axis=1)[act] > mxd2))
+
+
+def timestep_search(xp1, fp, statu, ityp0, cm, hmx, dmx) -> float:
+    """The halving search of Predictor_DEV (MD_DiffScheme_GPU.F90:633-650)."""
+    th = hmx
+    while check_timestep(xp1, fp, statu, ityp0, cm, th, dmx * dmx):
+        th *= 0.5
+    return th
+
+
 def _move_bounded(x, dx, lo, ln, ifpd):
     """x + dx with the bounds of DevAdd (AddBD_DevVec_DF_KERNEL0, MSM_MultiGPU_Basic.F90:
     5444-5477): in a periodic direction a coordinate above BOXUP moves down by the box length and

@@ -311,3 +311,32 @@ def test_lbfgs_matches_oracle(orc):
     assert np.max(np.abs(x - ox)) <= 1e-8
     assert np.all(v == 0.0) and np.array_equal(st, st0)
     dev.close()
+
+
+def test_check_timestep_matches_oracle(orc, two_boxes):
+    """msm_check_timestep against the oracle at steps around the threshold (the largest step
+    displacement of the box at 300/900 K with forces), and the halving search of
+    Check_Timestep_DEV / Predictor_DEV with IHDUP < 0 against the oracle's."""
+    boxes, ctrl, ft = two_boxes
+    say("check timestep")
+    fc, dev = device_for(boxes, ctrl, ft)
+    dev.build_neighbors()
+    dev.calc_force()
+    _, v, f, st, _ = dev.get_atoms()
+    it = np.concatenate([b.ITYP for b in boxes]) - 1
+    cm = boxes[0].CM
+    for th in (0.5, 1.0, 2.0, 4.0, 8.0, 16.0):
+        for mxd2 in (1e-4, 1e-3, 1e-2, 1e-1):
+            assert dev.check_timestep(th, mxd2) == orc.check_timestep(v, f, st, it, cm, th, mxd2)
+    ctrl.HMX, ctrl.DMX, ctrl.IHDUP, ctrl.IT0 = 16.0, 0.05, -5, 1
+    want = orc.timestep_search(v, f, st, it, cm, 16.0, 0.05)
+    assert want < 16.0
+    assert md.Check_Timestep_DEV(boxes, ctrl, fc) == want and ctrl.H == want
+    # through the predictor: mod(ITIME - IT0 + 1, 5) = 0 at ITIME 5 searches, at ITIME 6 not
+    ctrl.H = 1.0
+    md.Predictor_DEV(5, boxes, ctrl, fc)
+    assert ctrl.H == want
+    ctrl.H = 1.0
+    md.Predictor_DEV(6, boxes, ctrl, fc)
+    assert ctrl.H == 1.0
+    dev.close()

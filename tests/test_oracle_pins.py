@@ -596,3 +596,25 @@ def test_cg_line_search_relaxes_and_starts_along_the_force(orc):
         dx = (dx - np.round(dx / ln) * ln).ravel()
         cos = dx @ f0.ravel() / (np.linalg.norm(dx) * np.linalg.norm(f0))
         assert abs(abs(cos) - 1.0) < 1e-12 and np.linalg.norm(dx) > 0.0
+
+
+def test_check_timestep_closed_form(orc):
+    """One moving atom, v = 0.02 A/fs and F/m = 1e-4 A/fs^2 along x: th v + th^2 F/(2m) reaches
+    DMX = 0.1 A at th* = (sqrt(v^2 + 2 a DMX) - v)/a = 4.94 fs; FIXPOSX masks the atom, an
+    inactive atom does not count; the halving search from HMX = 20 fs stops at 2.5 fs."""
+    v = np.zeros((3, 3))
+    f = np.zeros((3, 3))
+    v[1, 0] = 0.02
+    f[1, 0] = 1e-4 * 2.0          # mass 2
+    v[2, 1] = 5.0                  # inactive, would exceed at any step
+    st = np.array([1, 1, 0], np.int32)
+    it = np.zeros(3, np.int32)
+    cm = np.array([2.0])
+    ts = (math.sqrt(0.02 ** 2 + 2 * 1e-4 * 0.1) - 0.02) / 1e-4
+    assert 4.9 < ts < 5.0
+    assert not orc.check_timestep(v, f, st, it, cm, 4.9, 0.01)
+    assert orc.check_timestep(v, f, st, it, cm, 5.0, 0.01)
+    st[1] |= 2
+    assert not orc.check_timestep(v, f, st, it, cm, 5.0, 0.01)
+    st[1] = 1
+    assert orc.timestep_search(v, f, st, it, cm, 20.0, 0.1) == 2.5
