@@ -1,5 +1,6 @@
 """Potential files of the reference's EAM_NIST family as force-table procedures: setfl (the
-DYNAMO / LAMMPS eam/alloy layout) and moldy (Finnis-Sinclair cubic knots).
+DYNAMO / LAMMPS eam/alloy layout), lspt (a control file naming two-column spline files) and moldy
+(Finnis-Sinclair cubic knots).
 
 setfl restates MDLIB/sor/Potentials/EAM_NIST: the reader
 (Filedatas_Func_Setfl.F90:119-253), the spline procedures that turn the file's samples into the
@@ -29,6 +30,7 @@ from the reference:
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass, field
 from typing import Dict, List, Tuple
 
@@ -362,3 +364,173 @@ def Register_ForceTableProc_Moldy(fname: str, FTable: MDForceTable) -> str:
 
     mdlib.Register_ForceTableProc(1, FTable, nn, ne, em, NOTE="Moldy table:" + fname)
     return data.symbol
+
+
+# ----------------------------------------------------------------------------------------------
+# lspt files (Filedatas_Func_Lspt.F90): a control file naming two-column sample files
+# ----------------------------------------------------------------------------------------------
+
+class _SplineXY:
+    """The natural cubic spline of _Spline on arbitrary increasing abscissae (the .spt samples
+    need not be evenly spaced)."""
+
+    def __init__(self, x: np.ndarray, f: np.ndarray):
+        x = np.asarray(x, dtype=np.float64)
+        f = np.asarray(f, dtype=np.float64)
+        n = len(x)
+        if n < 3:
+            raise ValueError("MDPSCU Error: a spline file needs at least 3 samples")
+        h = np.diff(x)
+        sl = np.diff(f) / h
+        k = n - 2
+        sub = h[:-1] / 6.0
+        dia = (h[:-1] + h[1:]) / 3.0
+        sup = h[1:] / 6.0
+        rhs = sl[1:] - sl[:-1]
+        c = np.zeros(k)
+        d = np.zeros(k)
+        c[0], d[0] = sup[0] / dia[0], rhs[0] / dia[0]
+        for i in range(1, k):
+            m = dia[i] - sub[i] * c[i - 1]
+            c[i] = sup[i] / m
+            d[i] = (rhs[i] - sub[i] * d[i - 1]) / m
+        w = np.zeros(n)
+        w[k] = d[k - 1]
+        for i in range(k - 2, -1, -1):
+            w[i + 1] = d[i] - c[i] * w[i + 2]
+        self.x, self.f, self.w = x.tolist(), f.tolist(), w.tolist()
+        self.n = n
+        self.xmin, self.xmax = float(x.min()), float(x.max())
+
+    def __call__(self, y: float) -> Tuple[float, float]:
+        import bisect
+        j = min(max(bisect.bisect_right(self.x, y) - 1, 0), self.n - 2)
+        a, b = self.x[j], self.x[j + 1]
+        h = b - a
+        f0, f1, w0, w1 = self.f[j], self.f[j + 1], self.w[j], self.w[j + 1]
+        v = ((w0 * (b - y) ** 3 + w1 * (y - a) ** 3) / (6.0 * h)
+             + (f1 / h - w1 * h / 6.0) * (y - a) + (f0 / h - w0 * h / 6.0) * (b - y))
+        dv = ((-w0 * (b - y) ** 2 + w1 * (y - a) ** 2) / (2.0 * h)
+              + (f1 - f0) / h - (w1 - w0) * h / 6.0)
+        return v, dv
+
+
+def _quoted(line: str) -> List[str]:
+    import re
+    return re.findall(r'"([^"]*)"', line)
+
+
+def _spt(path: str) -> _SplineXY:
+    xs, fs = [], []
+    with open(path) as fh:
+        for line in fh:
+            t = line.split("#")[0].split()
+            if len(t) >= 2:
+                xs.append(float(t[0].replace("D", "E")))
+                fs.append(float(t[1].replace("D", "E")))
+    return _SplineXY(np.asarray(xs), np.asarray(fs))
+
+
+@dataclass
+class LsptFile:
+    """get_filedatas_lspt (Filedatas_Func_Lspt.F90:118-301): elements; per element F(rho) and
+    rho(r) splines (None for "NA"); pair splines keyed (I, J), I >= J; RHOMX = the largest rho of
+    the F files, RMAX = the largest r of the pair files (A)."""
+    elements: List[str]
+    frho: List[object]
+    rhor: List[object]
+    vr: Dict[Tuple[int, int], _SplineXY]
+    rhomx: float
+    rmax: float
+
+
+def read_lspt(path: str) -> LsptFile:
+    """The control file: a line opening with &LSPT, a line with the number of elements, a line
+    with the quoted symbols, per element a quoted F(rho) file and a quoted rho(r) file ("NA" for
+    none), then for I = 1..NE a line with the I quoted pair files of (I, 1..I).  Kept from the
+    reference: every pair (I, J) of line I reads the line's FIRST file (the reader opens
+    SUBSTR(1) for each J, :266), so (2,2) holds the W-H samples in the shipped W-H-He file."""
+    base = os.path.dirname(os.path.abspath(path))
+    with open(path) as fh:
+        lines = [l.split("!")[0].rstrip() for l in fh]
+    lines = [l for l in lines if l.strip()]
+    if not lines or not lines[0].strip().upper().startswith("&LSPT"):
+        raise ValueError(f"MDPSCU Error: {path} does not start with &LSPT")
+    nums = _numbers(lines[1].replace("=", " ").split())
+    if not nums:
+        raise ValueError("MDPSCU Error: the number of groups should be given")
+    ne = int(nums[0])
+    elements = _quoted(lines[2])
+    if len(elements) != ne:
+        raise ValueError("MDPSCU Error: the number of elements is different from the given number")
+    frho, rhor = [], []
+    k = 3
+    for _ in range(ne):
+        for dst in (frho, rhor):
+            q = _quoted(lines[k])
+            k += 1
+            name = q[0] if q else "NA"
+            dst.append(None if name.strip().upper() == "NA" else _spt(os.path.join(base, name)))
+    vr: Dict[Tuple[int, int], _SplineXY] = {}
+    for i in range(ne):
+        q = _quoted(lines[k])
+        k += 1
+        if len(q) != i + 1:
+            raise ValueError(f"MDPSCU Error: {i + 1} pair files expected on line {k} of {path}")
+        sp = _spt(os.path.join(base, q[0]))
+        for j in range(i + 1):
+            vr[(i, j)] = sp
+    rhomx = max([f.xmax for f in frho if f is not None] + [0.0])
+    rmax = max([v.xmax for v in vr.values()] + [0.0])
+    return LsptFile(elements, frho, rhor, vr, rhomx, rmax)
+
+
+def Register_ForceTableProc_SPT(fname: str, FTable: MDForceTable) -> List[str]:
+    """Register_ForceTableProc_SPT (Filedatas_Func_Lspt.F90:560-602): ids IT = (I-1) NE + J with
+    pair V of the unordered pair given directly (not r V), (V/2, -dV/dr); density rho_J unless
+    element I has no F file or J no rho file; embedding F_I for I = J.  Below a file's first
+    sample the value is held with zero slope; above its last, pair and density are zero and the
+    embedding holds its value.  Sets Rmax and RHOMX from the files."""
+    data = read_lspt(fname)
+    ne = len(data.elements)
+    it = 0
+    for i in range(ne):
+        for j in range(ne):
+            it += 1
+            vs = data.vr[(max(i, j), min(i, j))]
+
+            def nn(r, vs=vs):
+                if r > vs.xmax:
+                    return 0.0, 0.0
+                if r < vs.xmin:
+                    return 0.5 * vs(vs.xmin)[0], 0.0
+                v, dv = vs(r)
+                return 0.5 * v, -dv
+
+            rs = data.rhor[j] if data.frho[i] is not None else None
+
+            def nb(r, rs=rs):
+                if rs is None or r > rs.xmax:
+                    return 0.0, 0.0
+                if r < rs.xmin:
+                    return rs(rs.xmin)[0], 0.0
+                v, dv = rs(r)
+                return v, -dv
+
+            fs = data.frho[i] if i == j else None
+
+            def em(rho, fs=fs):
+                if fs is None:
+                    return 0.0, 0.0
+                if rho < fs.xmin:
+                    return fs(fs.xmin)[0], 0.0
+                if rho > fs.xmax:
+                    return fs(fs.xmax)[0], 0.0
+                return fs(rho)
+
+            mdlib.Register_ForceTableProc(
+                it, FTable, nn, nb, em,
+                NOTE=f"lspt table: {data.elements[i]}<-{data.elements[j]}  ( {fname} )")
+    FTable.Rmax = float(data.rmax)
+    FTable.RHOMX = float(data.rhomx)
+    return list(data.elements)

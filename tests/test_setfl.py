@@ -270,3 +270,36 @@ def test_moldy_file(orc, tmp_path):
     e_atom = 0.5 * np.sum(nsh * pair(shells)) - math.sqrt(np.sum(nsh * dens(shells)))
     assert e.sum() / box.NPRT == pytest.approx(e_atom, rel=2e-6)  # table lerp error
     assert np.max(np.abs(f)) < 1e-9
+
+
+LSPT = "/root/reference/examples/NIST_Potentials/WHHe_EAM1_LSPT/WHHe_EAM1.lspt"
+
+
+@pytest.mark.skipif(not os.path.exists(LSPT), reason="reference example files not present")
+def test_reference_lspt_matches_its_exported_embedding_table():
+    """The W-H-He lspt potential shipped with the reference (read in place) against the
+    embedding table the reference exported from it (WHHe_EAM1.lspt.embd, 10000 points, nine
+    digits): same rho grid (RHOMX = 10 from the F file), F of W to the printed digits, dF as the
+    reference writes it (still in erg, i.e. times 1.60219e-12), zero tables for H and He (no F
+    files); and the reader's quirk that every pair of a control-file line reads the line's first
+    file (H-H holds the W-H samples)."""
+    ft = md.MDForceTable(PotType="EAM_TYPE", NTAB=10000, NEMBD=10000)
+    assert setfl.Register_ForceTableProc_SPT(LSPT, ft) == ["W", "H", "He"]
+    assert ft.RHOMX == 10.0 and ft.Rmax == pytest.approx(5.4604375)
+    setfl.Generate_NIST_ForceTable(ft)
+    rows = []
+    with open(LSPT + ".embd") as fh:
+        for line in fh:
+            t = line.split()
+            if len(t) == 20 and t[0].isdigit():
+                rows.append([float(v) for v in t])
+    rows = np.array(rows)
+    assert rows.shape == (10000, 20)
+    assert np.max(np.abs(rows[:, 1] - np.arange(10000) * ft.RHOD)) < 1e-12
+    f, df = rows[:, 2], rows[:, 3]
+    assert np.all(np.abs(ft.FEMBD[0] - f) <= 5.01e-9 * np.maximum(np.abs(f), 1.0))
+    big = np.abs(df) > 1e-14
+    assert np.all(np.abs(ft.DFEMBD[0, big] * md.CP_EVERG - df[big]) <= 1e-8 * np.abs(df[big]))
+    assert np.all(rows[:, 4:] == 0.0) and np.all(ft.FEMBD[1:] == 0.0)
+    for r in (1.0, 2.0, 3.0):
+        assert ft.REGPROCS[5].NNFORCE(r) == ft.REGPROCS[2].NNFORCE(r)
