@@ -30,6 +30,7 @@ from typing import Callable, Dict, List, Optional, Sequence, Union
 import numpy as np
 
 from . import _lib
+from .lbfgs import LBFGS
 from ._lib import MSM_EAM_TYPE, MSM_FS_TYPE, check, dptr, iptr, lib
 
 PotFn = Union[str, Callable[[float], tuple], None]
@@ -758,7 +759,6 @@ def lbfgs_quench(xp: np.ndarray, statu: np.ndarray, lo: np.ndarray, zl: np.ndarr
     (below the box: +edge, above: -edge) and gives the positions back through put().  Returns
     (evaluations, iflag, positions); iflag 0 on convergence, 1 when maxsteps ran out, negative
     for an optimiser error (LBFGS flags)."""
-    from .lbfgs import LBFGS
     x_a = np.array(xp, dtype=np.float64, order="F")
     ids = lbfgs_coordinates(statu)
     cols = [k for k in range(3) for _ in ids[k]]

@@ -29,8 +29,10 @@ from the reference:
 """
 from __future__ import annotations
 
+import bisect
 import math
 import os
+import re
 from dataclasses import dataclass, field
 from typing import Dict, List, Tuple
 
@@ -403,7 +405,6 @@ class _SplineXY:
         self.xmin, self.xmax = float(x.min()), float(x.max())
 
     def __call__(self, y: float) -> Tuple[float, float]:
-        import bisect
         j = min(max(bisect.bisect_right(self.x, y) - 1, 0), self.n - 2)
         a, b = self.x[j], self.x[j + 1]
         h = b - a
@@ -416,7 +417,6 @@ class _SplineXY:
 
 
 def _quoted(line: str) -> List[str]:
-    import re
     return re.findall(r'"([^"]*)"', line)
 
 
