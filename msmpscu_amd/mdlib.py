@@ -557,6 +557,14 @@ class DeviceMD:
         f = lambda a: a.reshape(3, n).T.copy(order="F")  # noqa: E731
         return f(xp), f(xp1), f(fp), st, f(dis)
 
+    def get_forces(self) -> np.ndarray:
+        """FP (N, 3) in the caller's order alone (msm_get_atoms with the other outputs NULL)."""
+        n = self.n
+        fp = np.zeros(3 * n)
+        check(lib().msm_get_atoms(self.h, None, None, dptr(fp), None, None), "msm_get_atoms",
+              self.h)
+        return fp.reshape(3, n).T.copy(order="F")
+
     def get_den(self) -> np.ndarray:
         d = np.zeros(self.n)
         check(lib().msm_get_den(self.h, dptr(d)), "msm_get_den", self.h)
@@ -803,7 +811,7 @@ def Do_LBFGS_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClass
     def evaluate():
         dev.calc_force()
         dev.calc_epot()
-        return dev.get_atoms()[2], dev.get_epot()
+        return dev.get_forces(), dev.get_epot()
 
     it, iflag, _ = lbfgs_quench(xp, st, lo, zl, CtrlParam.IFPD, evaluate, dev.put_atoms,
                                 MXNUMSTEPS, CtrlParam.LBFGS_Msave)

@@ -263,6 +263,7 @@ def test_put_atoms_round_trip_and_forces(orc):
         dev.calc_force()
         dev.calc_epot()
         _, _, f, _, _ = dev.get_atoms()
+        assert np.array_equal(dev.get_forces(), f)
         ep = dev.get_epot()
         of, _, oe = orc.force_full(xs, it, st, kv, ind0, box.ZL, ctrl.IFPD, ru2, t)
         assert np.max(np.abs(f - of)) <= 1e-10
