@@ -35,12 +35,28 @@ def _run(cmd, cwd=None):
     return out
 
 
-def build_library() -> str:
-    srcs = [os.path.join(CSRC, f) for f in ("msm_api.hip", "msm_kernels.hip", "msm_bricks.hip", "msm_tpa.hip", "msm_pair.hip", "msm_ldstab.hip", "msm_adjoint.hip",
-                                             "msm_host.cpp")]
+SOURCES = ("msm_api.hip", "msm_kernels.hip", "msm_bricks.hip", "msm_tpa.hip", "msm_pair.hip",
+           "msm_ldstab.hip", "msm_adjoint.hip", "msm_host.cpp")
+
+
+def build_library(jobs: int = 8) -> str:
+    """Each translation unit compiles to its own object (in parallel; every kernel is launched
+    from the unit that defines it, so no relocatable device code is needed), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    odir = os.path.join(HERE, "_build")
+    os.makedirs(odir, exist_ok=True)
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+
+    def compile_one(f):
+        obj = os.path.join(odir, os.path.splitext(f)[0] + ".o")
+        _run([HIPCC, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, f), "-o", obj])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=max(1, min(jobs, len(SOURCES)))) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
-          "-o", tmp, *srcs])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

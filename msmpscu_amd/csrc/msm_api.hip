@@ -35,11 +35,14 @@ struct AtomSet {  // one set of per-atom arrays (cell-sorted order)
     int* cell = nullptr;
 };
 
+// A captured segment bakes in the step parameters (h enters the force kernels by value), so the
+// time step's bit pattern is part of the key: a run with another h captures its own graphs.
 struct GraphKey {
     int set, pos, rebuild, nsteps, last, track;
+    uint64_t hbits;
     bool operator<(const GraphKey& o) const {
-        return std::tie(set, pos, rebuild, nsteps, last, track) <
-               std::tie(o.set, o.pos, o.rebuild, o.nsteps, o.last, o.track);
+        return std::tie(set, pos, rebuild, nsteps, last, track, hbits) <
+               std::tie(o.set, o.pos, o.rebuild, o.nsteps, o.last, o.track, o.hbits);
     }
 };
 
@@ -608,7 +611,9 @@ int run_segment(msm_sim* s, int rebuild, int nsteps, int last, double h) {
         enqueue_segment(s, rebuild, nsteps, last, h);
         return MSM_OK;
     }
-    GraphKey key{s->scur, s->pcur, rebuild, nsteps, last, s->track_dis};
+    uint64_t hbits = 0;
+    std::memcpy(&hbits, &h, sizeof(hbits));
+    GraphKey key{s->scur, s->pcur, rebuild, nsteps, last, s->track_dis, hbits};
     auto it = s->graphs.find(key);
     if (it == s->graphs.end()) {
         const int scur0 = s->scur, pcur0 = s->pcur;
@@ -1277,16 +1282,14 @@ int msm_check_timestep(msm_sim* s, double th, double mxd2, int* exceeded) {
     if (!exceeded || !(th > 0.0) || !(mxd2 >= 0.0))
         return set_err(s, MSM_ERR_ARG, "th > 0, mxd2 >= 0 and an output flag are required");
     (void)hipSetDevice(s->device);
-    int* flag = nullptr;
-    if ((rc = dalloc(s, &flag, 1)) != MSM_OK) return rc;
-    hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s->stream);
-    if (e == hipSuccess) {
-        msm::launch_check_dt(atoms_view(s, s->scur, s->pcur), s->tab, th, mxd2, flag, s->stream);
-        e = hipMemcpyAsync(exceeded, flag, sizeof(int), hipMemcpyDeviceToHost, s->stream);
-    }
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    dfree(&flag);
-    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    // the context's flag slot (no allocation per call: the step search calls this repeatedly)
+    int* flag = s->d_err + msm::kErrDt;
+    msm::launch_fill_int(flag, 0, 1, s->stream);
+    msm::launch_check_dt(atoms_view(s, s->scur, s->pcur), s->tab, th, mxd2, flag, s->stream);
+    HIPCHK(s, hipMemcpyAsync(s->h_err + msm::kErrDt, flag, sizeof(int), hipMemcpyDeviceToHost,
+                             s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    *exceeded = s->h_err[msm::kErrDt] ? 1 : 0;
     return MSM_OK;
 }
 

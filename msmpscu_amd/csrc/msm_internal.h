@@ -21,7 +21,8 @@ namespace msm {
 constexpr int kMaxGroup = 16;
 constexpr int kBlock = 256;
 
-enum ErrSlot { kErrOverflow = 0, kErrOutOfBox = 1, kErrEmbd = 2, kErrSlots = 4 };
+// kErrDt: flag of the time-step check (not an error; read by msm_check_timestep alone)
+enum ErrSlot { kErrOverflow = 0, kErrOutOfBox = 1, kErrEmbd = 2, kErrDt = 3, kErrSlots = 4 };
 
 struct BoxDev {
     double lo[3], len[3], hlen[3];

@@ -96,22 +96,44 @@ class MDForceTable:
     HASREG1: Dict[int, bool] = field(default_factory=dict)
 
 
+class _CallbackErrors(list):
+    """Exceptions raised inside Python potential callbacks during one table evaluation.  ctypes
+    cannot carry an exception through the C caller, so the trampoline records it here, writes NaN
+    to the outputs and the caller re-raises the first one once the C routine has returned."""
+
+
 def _fn_address(fn: PotFn, keep: list):
     """C callable for a potential: a built-in name ("msm_pot_wfs_nn") or a Python callable
-    r -> (value, derivative-term)."""
+    r -> (value, derivative-term).  ``keep`` holds the ctypes objects alive for the C call; its
+    first element, a _CallbackErrors, collects the callbacks' exceptions."""
     if fn is None:
         return None
     if isinstance(fn, str):
         return _lib.builtin_fn(fn)
+    if not keep or not isinstance(keep[0], _CallbackErrors):
+        keep.insert(0, _CallbackErrors())
+    errors = keep[0]
 
     def tramp(rp, vp, dvp, _f=fn):
-        v, dv = _f(rp[0])
-        vp[0] = v
-        dvp[0] = dv
+        try:
+            v, dv = _f(rp[0])
+            vp[0] = v
+            dvp[0] = dv
+        except BaseException as exc:  # noqa: BLE001 -- re-raised by _raise_callback_errors
+            vp[0] = float("nan")
+            dvp[0] = float("nan")
+            errors.append(exc)
 
     cb = _lib.PAIRFN(tramp)
     keep.append(cb)
     return C.cast(cb, C.c_void_p).value
+
+
+def _raise_callback_errors(keep: list, what: str):
+    if keep and isinstance(keep[0], _CallbackErrors) and keep[0]:
+        exc = keep[0][0]
+        raise RuntimeError(f"{what}: a potential callback raised {type(exc).__name__}: {exc}") \
+            from exc
 
 
 def Register_ForceTableProc(IFORCE, FTable: MDForceTable, NNFORCE: PotFn, NEFORCE: PotFn = None,
@@ -202,6 +224,7 @@ def Create_Pairwise_ForceTable(IFORCE: int, FTable: MDForceTable):
     rc = lib().msm_table_pairwise(FTable.NTAB, FTable.Rmax, nn, ne, FTable.RHOMULT,
                                   C.byref(rhomx), _col(FTable.POTR, k), _col(FTable.FPOTR, k),
                                   _col(FTable.POTB, k), _col(FTable.FPOTB, k), FTable.NKIND)
+    _raise_callback_errors(keep, f"Create_Pairwise_ForceTable (potential type #{IFORCE})")
     check(rc, "msm_table_pairwise")
     FTable.RHOMX = rhomx.value
     FTable.RHOD = FTable.RHOMX / float(FTable.NEMBD)
@@ -223,6 +246,7 @@ def Create_EMBDFUNTable(IFORCE: int, FTable: MDForceTable):
     em = _fn_address(proc.EMBDF, keep)
     rc = lib().msm_table_embd(FTable.NEMBD, FTable.RHOD, em, _col(FTable.FEMBD, k),
                               _col(FTable.DFEMBD, k), FTable.NKIND1)
+    _raise_callback_errors(keep, f"Create_EMBDFUNTable (potential type #{IFORCE})")
     check(rc, "msm_table_embd")
     FTable.HASREG1[k] = True
 

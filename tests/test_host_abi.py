@@ -145,3 +145,34 @@ def test_fortran_bindings_name_header_symbols():
     assert len(names) >= 20
     declared = set(re.findall(r"\b(msm_\w+)\s*\(", hdr))
     assert names <= declared, sorted(names - declared)
+
+
+def test_raising_potential_callback_is_reported():
+    """An exception inside a Python potential callback cannot pass through the C table routine;
+    it comes back as an error of Create_Pairwise_ForceTable / Create_EMBDFUNTable instead of a
+    table silently holding zeros or stale entries."""
+    box = cfg.make_bcc_box((3, 3, 3), 3.165, 0.0, 0.0, 1)
+    ctrl = md.SimMDCtrl(RU=[[4.4002]], NB_RM=[[4.7002]], NUMFTABR=200)
+
+    def bad_nn(r):
+        if r > 2.0:
+            raise ZeroDivisionError("planted")
+        return 0.0, 0.0
+
+    ft = md.MDForceTable()
+    md.Register_ForceTableProc(1, ft, bad_nn, "msm_pot_wfs_ne")
+    with pytest.raises(RuntimeError, match="ZeroDivisionError: planted"):
+        md.Create_Interaction_ForceTable(box, ctrl, ft)
+
+    def bad_embd(rho):
+        raise ValueError("planted embedding")
+
+    ft = md.MDForceTable(PotType="EAM_TYPE")
+    md.Register_ForceTableProc(1, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne", EMBDF=bad_embd)
+    with pytest.raises(RuntimeError, match="ValueError: planted embedding"):
+        md.Create_Interaction_ForceTable(box, ctrl, ft)
+    # a well-behaved callback still fills the table
+    ft = md.MDForceTable()
+    md.Register_ForceTableProc(1, ft, lambda r: (0.5 * r, -0.5), "msm_pot_wfs_ne")
+    md.Create_Interaction_ForceTable(box, ctrl, ft)
+    assert np.all(np.isfinite(ft.POTR)) and ft.POTR[0, -1] != 0.0
