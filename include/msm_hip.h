@@ -158,6 +158,19 @@ int msm_get_atoms(msm_sim* sim, double* xp, double* xp1, double* fp, int* statu,
  * caller's order, written into the device state without a list rebuild (the caller keeps the
  * moves within what the list allows, as the reference's quench loops do). */
 int msm_put_atoms(msm_sim* sim, const double* xp, const double* xp1);
+/* External force and energy terms (AddExtForce_ForceClass / Cumulate_ExtForce,
+ * MD_ForceClass_Register_GPU.F90:584-680, 982-1180): the reference calls the registered
+ * ForceArray/EpotArray routines after the force (energy) pass and they add their terms to the
+ * device arrays.  msm_add_forces adds FEXT(N,3) (column-major, caller's atom order) to the current
+ * forces, msm_add_epot adds EEXT(N) to the per-atom energies (and renews the box totals of
+ * msm_get_box_epot); on_device != 0 means the array is a device pointer of this context's device.
+ * The fused steps of msm_run compute their forces inside the graph and do not see these terms:
+ * step with msm_predictor, msm_calc_force, msm_add_forces, msm_correction instead. */
+int msm_add_forces(msm_sim* sim, const double* fext, int on_device);
+int msm_add_epot(msm_sim* sim, const double* eext, int on_device);
+/* DIS(N,3) in the caller's order written into the device state (the restart of a run:
+ * msm_set_atoms resets DIS to zero). */
+int msm_put_dis(msm_sim* sim, const double* dis);
 /* F'(rho) per atom (the reference's DEN array) and per-atom potential energy EPOT. */
 int msm_get_den(msm_sim* sim, double* den);
 int msm_get_epot(msm_sim* sim, double* epot);
@@ -172,6 +185,10 @@ int msm_get_neighbors(msm_sim* sim, int* kvois, int* indi, int ld);
 
 /* CALFORCE_*_Force_Table2A_DEV: density pass then force pass. */
 int msm_calc_force(msm_sim* sim);
+/* CALDEN_FS/EAM_Force_Table2A_DEV (MD_FS_ForceTable_GPU.F90:875-930, bound as pCalEDen at
+ * MD_ForceClass_Register_GPU.F90:405,416): the density pass alone -- F'(rho) of every atom is left
+ * on the device (msm_get_den copies it out); forces are not touched. */
+int msm_calc_density(msm_sim* sim);
 /* CALEPOT_*_Force_Table2A_DEV: per-atom potential energy (and box totals). */
 int msm_calc_epot(msm_sim* sim);
 /* Predictor_DEV / Correction_DEV (Predictor_KERNEL0 / Correction_KERNEL semantics). */
@@ -212,6 +229,15 @@ int msm_check_timestep(msm_sim* sim, double th, double mxd2, int* exceeded);
  * msm_calc_ekin, msm_scale_velocities and msm_dyn_damp need a built neighbour list (the atoms sit
  * in cell order on the device from the first msm_build_neighbors on). */
 int msm_damping(msm_sim* sim);
+/* Thermalizing_MC_DEV (MD_DiffScheme_GPU.F90:1451-1805): every active atom's velocity components
+ * whose FIXVEL and FIXPOS bits are clear become V0 sqrt(-ln z1) cos(2 pi z2), V0 = sqrt(2 temp kb
+ * / CM) (the other components keep their value), inactive atoms get zero velocity, then each
+ * box's mass-weighted mean velocity over all its atoms is subtracted from all its atoms.  The
+ * uniforms z1, z2 in (0, 1] come from Philox4x32-10 (Salmon et al., SC'11) keyed by `seed`, with
+ * counter (caller's atom index, component, draw), draw = the number of earlier msm_thermalize calls
+ * on the context -- the reference's cuRAND XORWOW stream is not reproduced, the distribution is.
+ * Needs a built neighbour list (cell order). */
+int msm_thermalize(msm_sim* sim, double temp, double kb, unsigned long long seed);
 int msm_calc_ekin(msm_sim* sim, double kb, double* ekin, double* tbox);
 int msm_scale_velocities(msm_sim* sim, double temp, double kb);
 int msm_dyn_damp(msm_sim* sim, int maxsteps, double mindele, double h, int* steps,
@@ -268,7 +294,9 @@ int msm_list_stats(msm_sim* sim, long long* total, int* maxk);
                                     per atom); 3: wide bricks (several lanes per atom) */
 #define MSM_OPT_FORCE_PATH    4  /* density/force passes: 0 LDS bricks; 1, 2, 4, 8: global list
                                     with that many lanes per atom (default 2); 3: two lanes per
-                                    atom, each 32-byte record gathered by a lane pair */
+                                    atom, each 32-byte record gathered by a lane pair; 5: staged
+                                    bricks (a brick's halo staged in LDS once, pairs read as halo
+                                    slots; needs a list from the brick list kernels, else as 2) */
 #define MSM_OPT_COMPACT_LIST  5  /* 0 (default); 1: on the global-list path with 1, 2 or 4 lanes
                                     per atom,
                                     the density pass records the pairs inside the force cutoff and
@@ -303,6 +331,7 @@ int msm_set_option(msm_sim* sim, int option, int value);
 #define MSM_PASS_LANE_PAIR   3  /* two lanes per atom, lane-pair gathers */
 #define MSM_PASS_LDS_TABLE   4  /* global list, table window in LDS */
 #define MSM_PASS_BRICK_TABLE 5  /* LDS-staged positions and table window in LDS */
+#define MSM_PASS_STAGED      6  /* staged bricks: halo images resolved once, slots per pair */
 /* Current value of an option (every MSM_OPT_* above). */
 int msm_get_option(msm_sim* sim, int option, int* value);
 

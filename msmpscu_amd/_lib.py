@@ -79,6 +79,11 @@ SIGNATURES = {
     "msm_build_neighbors": (C.c_int, [_vp]),
     "msm_get_neighbors": (C.c_int, [_vp, _ip, _ip, C.c_int]),
     "msm_calc_force": (C.c_int, [_vp]),
+    "msm_calc_density": (C.c_int, [_vp]),
+    "msm_add_forces": (C.c_int, [_vp, _vp, C.c_int]),
+    "msm_add_epot": (C.c_int, [_vp, _vp, C.c_int]),
+    "msm_put_dis": (C.c_int, [_vp, _vp]),
+    "msm_thermalize": (C.c_int, [_vp, C.c_double, C.c_double, C.c_ulonglong]),
     "msm_calc_epot": (C.c_int, [_vp]),
     "msm_predictor": (C.c_int, [_vp, C.c_double]),
     "msm_correction": (C.c_int, [_vp, C.c_double]),
@@ -115,7 +120,7 @@ MSM_OPT_CELL_BRICK = 11
 MSM_OPT_BRICK_LANES = 12
 MSM_OPT_PASS_KERNEL = 13
 MSM_OPT_FORCE_LANES = 14
-PASS_KERNELS = ["bricks", "global", "lanes", "lane_pair", "lds_table", "brick_table"]
+PASS_KERNELS = ["bricks", "global", "lanes", "lane_pair", "lds_table", "brick_table", "staged"]
 KCLASS = ["density", "force", "rebuild", "predictor", "nlist"]
 
 _lib = None

@@ -36,7 +36,7 @@ def _run(cmd, cwd=None):
 
 
 SOURCES = ("msm_api.hip", "msm_kernels.hip", "msm_bricks.hip", "msm_tpa.hip", "msm_pair.hip",
-           "msm_ldstab.hip", "msm_adjoint.hip", "msm_host.cpp")
+           "msm_ldstab.hip", "msm_stage.hip", "msm_thermal.hip", "msm_adjoint.hip", "msm_host.cpp")
 
 
 def build_library(jobs: int = 8) -> str:
@@ -71,12 +71,13 @@ def build_fortran() -> list:
     os.makedirs(bdir, exist_ok=True)
     _run([FC, "-O2", "-fPIC", "-c", mod_src, "-module-dir", bdir, "-o",
           os.path.join(bdir, "MD_ForceLib_HIP.o")])
-    drv = os.path.join(FSRC, "fs_w_driver.F90")
-    exe = os.path.join(FSRC, "fs_w_driver")
-    if os.path.exists(drv):
-        _run([FC, "-O2", drv, os.path.join(bdir, "MD_ForceLib_HIP.o"), "-I" + bdir,
-              "-module-dir", bdir, "-L" + HERE, "-lmsm_hip", "-Wl,-rpath," + HERE, "-o", exe])
-        outs.append(exe)
+    for name in ("fs_w_driver", "cgs_driver"):
+        drv = os.path.join(FSRC, name + ".F90")
+        exe = os.path.join(FSRC, name)
+        if os.path.exists(drv):
+            _run([FC, "-O2", drv, os.path.join(bdir, "MD_ForceLib_HIP.o"), "-I" + bdir,
+                  "-module-dir", bdir, "-L" + HERE, "-lmsm_hip", "-Wl,-rpath," + HERE, "-o", exe])
+            outs.append(exe)
     return outs
 
 

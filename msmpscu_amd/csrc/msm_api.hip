@@ -98,6 +98,9 @@ struct msm_sim {
     int* h_err = nullptr;
     double* d_ebox = nullptr;
     unsigned long long* d_stats = nullptr;
+    double* d_boxwork = nullptr;  // per-box scratch (4 doubles a box)
+    size_t boxwork_cap = 0;
+    unsigned long long draws = 0;  // msm_thermalize calls so far (the generator's draw number)
 
     // options
     int track_dis = 1;
@@ -334,7 +337,7 @@ void enqueue_rebuild(msm_sim* s, bool keep_f = true) {
     } else {
         // halo slots only for the brick passes (a list without them sends those passes to
         // the global list until the next rebuild)
-        s->list_sidx = s->force_path == 0 || s->tuning;
+        s->list_sidx = s->force_path == 0 || s->force_path == 5 || s->tuning;
         msm::launch_nlist_bricks(s->nl_wide, s->nbricks, s->nl_lanes, s->nl_cap, s->list_sidx,
                                  s->d_brick_box, s->n, nc,
                                  s->pos[p1],
@@ -498,17 +501,28 @@ int plan_bricks(msm_sim* s, const double* xp, int n) {
 // table kind in use.
 // Lanes per atom of the LDS-table force pass: MSM_OPT_FORCE_LANES when set (not with lane pairs),
 // else those of the density pass.
+// Staged-brick passes (path 5): over a list from the brick list kernels that carries halo
+// slots, with knot windows of at least a quarter of the table beside the staged halo.
+bool use_staged(const msm_sim* s) {
+    if (s->force_path != 5 || s->nlist_kind == 1 || !s->list_sidx) return false;
+    const int scap = s->nl_wide ? s->nl_cap : msm::kNlCapPass;
+    return msm::sb_plan(s->tab, s->lt, scap).ok;
+}
+
+// Path 5 without a usable staged plan runs the two-lane LDS-table passes.
+int eff_path(const msm_sim* s) { return s->force_path == 5 ? 2 : s->force_path; }
+
 int lt_force_lanes(const msm_sim* s) {
-    return (s->force_lanes > 0 && s->force_path != 3) ? s->force_lanes : s->force_path;
+    return (s->force_lanes > 0 && eff_path(s) != 3) ? s->force_lanes : eff_path(s);
 }
 
 bool use_lds_table(const msm_sim* s) {
     if (!s->table_lds) return false;
-    if (s->force_path < 1 || s->force_path > 4) return false;  // 3: lane-pair gathers
+    const int fp = eff_path(s);
+    if (fp < 1 || fp > 4) return false;  // 3: lane-pair gathers
     // compaction: per-lane 64-bit masks of in-cutoff slots, written by the density pass for the
     // same lanes of the force pass (not with lane pairs, whose two lanes walk their slots together)
-    if (s->compact && (s->force_path == 3 || lt_force_lanes(s) != s->force_path ||
-                       (s->mx + s->force_path - 1) / s->force_path > 64))
+    if (s->compact && (fp == 3 || lt_force_lanes(s) != fp || (s->mx + fp - 1) / fp > 64))
         return false;
     const msm::LtWin w = msm::lt_window(s->tab, s->lt.wgs_d, s->lt.wgs_f);
     return 2 * w.wf >= s->tab.ntab2 && 2 * w.wd >= s->tab.ntab2;
@@ -528,18 +542,21 @@ bool use_brick_table(const msm_sim* s) {
 void enqueue_density(msm_sim* s) {
     TimedScope ts(s, MSM_KCLASS_DENSITY);
     AtomsDev a = atoms_view(s, s->scur, s->pcur);
-    if (use_brick_table(s))
+    if (use_staged(s))
+        msm::launch_density_sb(s->lt, s->nbricks, s->d_brick_box, a, list_view(s), s->tab,
+                               s->d_boxes, s->sb.start, s->ncell_total, s->d_err, s->stream);
+    else if (use_brick_table(s))
         msm::launch_density_bt(s->bt_lanes, s->lt, s->nbricks, s->d_brick_box, a, list_view(s),
                                s->tab, s->d_boxes, s->sb.start, s->ncell_total, s->d_err,
                                s->stream);
     else if (use_lds_table(s))
-        msm::launch_density_lt(s->force_path, s->lt, a, list_view(s), s->tab, s->boxes[0],
+        msm::launch_density_lt(eff_path(s), s->lt, a, list_view(s), s->tab, s->boxes[0],
                                s->d_boxes, multibox(s), s->d_err, s->stream);
-    else if (s->force_path == 3)
+    else if (eff_path(s) == 3)
         msm::launch_density_pair(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
                                  s->d_err, s->stream);
-    else if (s->force_path > 0)
-        msm::launch_density_tpa(s->force_path, a, list_view(s), s->tab, s->boxes[0], s->d_boxes,
+    else if (eff_path(s) > 0)
+        msm::launch_density_tpa(eff_path(s), a, list_view(s), s->tab, s->boxes[0], s->d_boxes,
                                 multibox(s), s->d_err, s->stream);
     else if (s->nlist_kind == 1)
         msm::launch_density(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
@@ -560,17 +577,20 @@ void enqueue_force(msm_sim* s, int mode, int write_f, double h) {
     sp.mode = mode;
     sp.write_f = write_f;
     sp.track_dis = s->track_dis;
-    if (use_brick_table(s))
+    if (use_staged(s))
+        msm::launch_force_sb(s->lt, s->nbricks, s->d_brick_box, a, list_view(s), s->tab,
+                             s->d_boxes, s->sb.start, s->ncell_total, sp, s->stream);
+    else if (use_brick_table(s))
         msm::launch_force_bt(s->bt_lanes, s->lt, s->nbricks, s->d_brick_box, a, list_view(s),
                              s->tab, s->d_boxes, s->sb.start, s->ncell_total, sp, s->stream);
     else if (use_lds_table(s))
         msm::launch_force_lt(lt_force_lanes(s), s->lt, a, list_view(s), s->tab, s->boxes[0],
                              s->d_boxes, multibox(s), sp, s->stream);
-    else if (s->force_path == 3)
+    else if (eff_path(s) == 3)
         msm::launch_force_pair(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), sp,
                                s->stream);
-    else if (s->force_path > 0)
-        msm::launch_force_tpa(s->force_path, a, list_view(s), s->tab, s->boxes[0], s->d_boxes,
+    else if (eff_path(s) > 0)
+        msm::launch_force_tpa(eff_path(s), a, list_view(s), s->tab, s->boxes[0], s->d_boxes,
                               multibox(s), sp, s->stream);
     else if (s->nlist_kind == 1)
         msm::launch_force(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), sp,
@@ -720,6 +740,7 @@ int msm_destroy(msm_sim* s) {
     dfree(&s->d_err);
     dfree(&s->d_ebox);
     dfree(&s->d_stats);
+    dfree(&s->d_boxwork);
     if (s->h_err) (void)hipHostFree(s->h_err);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -951,7 +972,8 @@ int msm_set_atoms(msm_sim* s, int n, const double* xp, const double* xp1, const 
     if (rc == MSM_OK) rc = dalloc(s, &s->kvois, na);
     const size_t icap = static_cast<size_t>(na) * s->mx;
     if (rc == MSM_OK) rc = dalloc(s, &s->indi, icap);
-    if (rc == MSM_OK) rc = dalloc(s, &s->sidx, icap);
+    // the staged passes issue a full chunk of slot loads past an atom's last slot
+    if (rc == MSM_OK) rc = dalloc(s, &s->sidx, icap + msm::kSbSlack);
     if (rc == MSM_OK) rc = dalloc(s, &s->cidx, icap);
     if (rc == MSM_OK) rc = dalloc(s, &s->ckv, static_cast<size_t>(na) * 8);
     if (rc == MSM_OK) rc = dalloc(s, &s->cmask, static_cast<size_t>(na) * 4);
@@ -1093,6 +1115,81 @@ int msm_put_atoms(msm_sim* s, const double* xp, const double* xp1) {
     return MSM_OK;
 }
 
+// Caller-order arrays of ncomp components (stride n) added into sorted per-atom device arrays
+// dst[k] (k < ncomp); src is a host or device pointer.
+static int add_caller_order(msm_sim* s, const double* src, int on_device, double* const* dst,
+                            int ncomp, const char* what) {
+    if (!s) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (!s->has_atoms) return set_err(s, MSM_ERR_STATE, "no atoms");
+    if (!src) return set_err(s, MSM_ERR_ARG, std::string(what) + ": null array");
+    const int n = s->n;
+    if (n == 0) return MSM_OK;
+    const size_t cnt = static_cast<size_t>(ncomp) * n;
+    double* tmp = nullptr;
+    const double* d = src;
+    if (!on_device) {
+        int rc = dalloc(s, &tmp, cnt);
+        if (rc != MSM_OK) return rc;
+        hipError_t e = hipMemcpyAsync(tmp, src, cnt * sizeof(double), hipMemcpyHostToDevice,
+                                      s->stream);
+        if (e != hipSuccess) {
+            dfree(&tmp);
+            return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+        }
+        d = tmp;
+    }
+    const int* gid = s->set[s->scur].gid;
+    for (int k = 0; k < ncomp; ++k)
+        msm::launch_add_sorted(d + static_cast<size_t>(k) * n, dst[k], gid, n, s->stream);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    dfree(&tmp);
+    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    return MSM_OK;
+}
+
+int msm_add_forces(msm_sim* s, const double* fext, int on_device) {
+    if (!s) return MSM_ERR_ARG;
+    const size_t na = s->n_alloc;
+    AtomSet& z = s->set[s->scur];
+    double* const dst[3] = {z.f, z.f + na, z.f + 2 * na};
+    return add_caller_order(s, fext, on_device, dst, 3, "msm_add_forces");
+}
+
+int msm_add_epot(msm_sim* s, const double* eext, int on_device) {
+    if (!s) return MSM_ERR_ARG;
+    double* const dst[1] = {s->epot};
+    int rc = add_caller_order(s, eext, on_device, dst, 1, "msm_add_epot");
+    if (rc != MSM_OK || s->n == 0 || !s->has_list) return rc;
+    // keep the box totals of msm_get_box_epot in step with the per-atom energies
+    msm::launch_box_sum(s->epot, s->sb.start, s->d_boxes, static_cast<int>(s->boxes.size()),
+                        s->d_ebox, s->stream);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MSM_OK;
+}
+
+int msm_put_dis(msm_sim* s, const double* dis) {
+    if (!s) return MSM_ERR_ARG;
+    (void)hipSetDevice(s->device);
+    if (!s->has_atoms) return set_err(s, MSM_ERR_STATE, "no atoms");
+    if (!dis) return set_err(s, MSM_ERR_ARG, "msm_put_dis: null array");
+    const int n = s->n;
+    if (n == 0) return MSM_OK;
+    const size_t na = s->n_alloc;
+    AtomSet& z = s->set[s->scur];
+    double* tmp = nullptr;
+    int rc = dalloc(s, &tmp, 3 * static_cast<size_t>(n));
+    if (rc != MSM_OK) return rc;
+    hipError_t e = hipMemcpyAsync(tmp, dis, 3 * static_cast<size_t>(n) * sizeof(double),
+                                  hipMemcpyHostToDevice, s->stream);
+    for (int k = 0; e == hipSuccess && k < 3; ++k)
+        msm::launch_sort_d(tmp + k * static_cast<size_t>(n), z.dis + k * na, z.gid, n, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    dfree(&tmp);
+    if (e != hipSuccess) return set_err(s, MSM_ERR_HIP, hipGetErrorString(e));
+    return MSM_OK;
+}
+
 int msm_get_den(msm_sim* s, double* den) {
     if (!s || !den) return MSM_ERR_ARG;
     (void)hipSetDevice(s->device);
@@ -1163,6 +1260,15 @@ int msm_calc_force(msm_sim* s) {
     clear_flags(s);
     enqueue_density(s);
     enqueue_force(s, 0, 1, 0.0);
+    return read_flags(s);
+}
+
+int msm_calc_density(msm_sim* s) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    (void)hipSetDevice(s->device);
+    clear_flags(s);
+    enqueue_density(s);
     return read_flags(s);
 }
 
@@ -1290,6 +1396,27 @@ int msm_check_timestep(msm_sim* s, double th, double mxd2, int* exceeded) {
                              s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
     *exceeded = s->h_err[msm::kErrDt] ? 1 : 0;
+    return MSM_OK;
+}
+
+int msm_thermalize(msm_sim* s, double temp, double kb, unsigned long long seed) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (!(temp >= 0.0) || !(kb > 0.0))
+        return set_err(s, MSM_ERR_ARG, "temperature >= 0 and kb > 0 are required");
+    (void)hipSetDevice(s->device);
+    const size_t need = 4 * s->boxes.size();
+    if (s->boxwork_cap < need) {
+        dfree(&s->d_boxwork);
+        s->boxwork_cap = 0;
+        if ((rc = dalloc(s, &s->d_boxwork, need)) != MSM_OK) return rc;
+        s->boxwork_cap = need;
+    }
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_thermalize(a, s->tab, s->set[s->scur].gid, temp * kb, seed, s->draws++,
+                           s->sb.start, s->d_boxes, static_cast<int>(s->boxes.size()),
+                           s->ncell_total, s->d_boxwork, s->stream);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
     return MSM_OK;
 }
 
@@ -1778,8 +1905,8 @@ int msm_set_option(msm_sim* s, int option, int value) {
             return MSM_OK;
         }
         case MSM_OPT_FORCE_PATH:
-            if (value < 0 || value > 8 || (value > 4 && value != 8))
-                return set_err(s, MSM_ERR_ARG, "force path must be 0, 1, 2, 3, 4 or 8");
+            if (value < 0 || value > 8 || (value > 5 && value != 8))
+                return set_err(s, MSM_ERR_ARG, "force path must be 0, 1, 2, 3, 4, 5 or 8");
             s->force_path = value;
             clear_graphs(s);
             return MSM_OK;
@@ -1835,10 +1962,11 @@ int msm_set_option(msm_sim* s, int option, int value) {
 
 // The density/force kernels enqueue_density / enqueue_force launch.
 static int pass_kernel(const msm_sim* s) {
+    if (use_staged(s)) return MSM_PASS_STAGED;
     if (use_brick_table(s)) return MSM_PASS_BRICK_TABLE;
     if (use_lds_table(s)) return MSM_PASS_LDS_TABLE;
-    if (s->force_path == 3) return MSM_PASS_LANE_PAIR;
-    if (s->force_path > 0) return MSM_PASS_LANES;
+    if (eff_path(s) == 3) return MSM_PASS_LANE_PAIR;
+    if (eff_path(s) > 0) return MSM_PASS_LANES;
     if (s->nlist_kind == 1) return MSM_PASS_GLOBAL;
     return MSM_PASS_BRICKS;
 }
@@ -1939,6 +2067,16 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     s->nlist_kind = nl_choice;
     if (rc == MSM_OK) rc = replan_bricks(s);
     if (rc == MSM_OK) enqueue_rebuild(s);
+    // staged-brick passes (slot list of the brick list kernels)
+    if (brick_list && s->nlist_kind != 1)
+        for (const Cand& c : {Cand{5, 1, 512, 3, 512, 2, 1}, Cand{5, 1, 512, 2, 512, 2, 1},
+                              Cand{5, 1, 256, 4, 256, 3, 1}, Cand{5, 1, 1024, 1, 1024, 1, 1},
+                              Cand{5, 1, 256, 6, 256, 4, 1}}) {
+            s->lt = msm::LtShape{c.bd, c.wd, c.bf, c.wf};
+            s->force_path = 5;
+            s->table_lds = 1;
+            if (use_staged(s)) cands.push_back(c);
+        }
     // brick passes with the table in LDS (they read the slot list of the brick list kernels)
     if (brick_list && s->nlist_kind != 1)
         for (const Cand& c : {Cand{0, 1, 512, 2, 512, 2, 1}, Cand{0, 1, 1024, 1, 1024, 1, 2},

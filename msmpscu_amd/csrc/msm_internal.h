@@ -180,6 +180,24 @@ void launch_density_bt(int tpa, const LtShape& sh, int nbricks, const int* brick
 void launch_force_bt(int tpa, const LtShape& sh, int nbricks, const int* brick_box,
                      const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev* boxes,
                      const int* start, int ncell_total, const StepParams& sp, hipStream_t s);
+// Staged-brick passes (msm_stage.hip): one brick's halo staged in LDS per workgroup step, the
+// pairs read as halo slots; halo capacity cap (the list build's) and knot windows in the rest of
+// the LDS share.  ok: both windows hold at least a quarter of the table.
+struct SbPlan {
+    LtWin w;
+    int cap;
+    bool ok;
+};
+// Slot loads the staged passes may issue past an atom's last list slot (allocation slack of
+// the slot list, in entries).
+constexpr int kSbSlack = 32 * kListRow;
+SbPlan sb_plan(const TabDev& t, const LtShape& sh, int scap);
+void launch_density_sb(const LtShape& sh, int nbricks, const int* brick_box, const AtomsDev& a,
+                       const ListDev& l, const TabDev& t, const BoxDev* boxes, const int* start,
+                       int ncell_total, int* err, hipStream_t s);
+void launch_force_sb(const LtShape& sh, int nbricks, const int* brick_box, const AtomsDev& a,
+                     const ListDev& l, const TabDev& t, const BoxDev* boxes, const int* start,
+                     int ncell_total, const StepParams& sp, hipStream_t s);
 // Reverse-mode table gradients (msm_adjoint.hip): scratch of the adjoint passes.
 struct AdjDev {
     double* g;           // dL/dF in the sorted order, 3 x n (zero for inactive atoms)
@@ -213,6 +231,8 @@ void launch_correction(const AtomsDev& a, const TabDev& t, double h, hipStream_t
 void launch_unsort_d(const double* src, double* dst, const int* gid, int n, hipStream_t s);
 void launch_sort_pos(const double* src, double4* dst, const int* gid, int n, hipStream_t s);
 void launch_sort_d(const double* src, double* dst, const int* gid, int n, hipStream_t s);
+// dst[p] += src[gid[p]] (a caller-order array added into the sorted order)
+void launch_add_sorted(const double* src, double* dst, const int* gid, int n, hipStream_t s);
 void launch_unsort_i(const int* src, int* dst, const int* gid, int n, hipStream_t s);
 void launch_unsort_pos(const double4* src, double* dst, const int* gid, int n, int comp,
                        hipStream_t s);
@@ -233,6 +253,14 @@ void launch_box_ekin(const double* ek, const int* start, const BoxDev* boxes, in
 void launch_velscale(const AtomsDev& a, const double* scal, hipStream_t s);
 void launch_epot_maxdelta(const double* epot, double* epot0, int n, unsigned long long* out,
                           hipStream_t s);
+
+// Thermalizing_MC_DEV (msm_thermal.hip): Maxwell velocities from Philox4x32-10 keyed by seed,
+// counter (caller index gid, component, draw), then per-box removal of the mass-weighted mean
+// velocity; mom holds 4 doubles per box (sum m v, M).
+void launch_thermalize(const AtomsDev& a, const TabDev& t, const int* gid, double tkb,
+                       unsigned long long seed, unsigned long long draw, const int* start,
+                       const BoxDev* boxes, int nbox, int ncell_total, double* mom,
+                       hipStream_t s);
 
 // steepest-descent quench helpers (3-component vectors as three arrays of n doubles)
 struct P3 {

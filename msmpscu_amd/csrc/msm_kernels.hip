@@ -976,6 +976,15 @@ void launch_sort_pos(const double* src, double4* dst, const int* gid, int n, hip
     if (n > 0)
         hipLaunchKernelGGL(k_sort_pos, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, gid, n);
 }
+__global__ void k_add_sorted(const double* __restrict__ src, double* __restrict__ dst,
+                             const int* __restrict__ gid, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = dst[p] + src[gid[p]];
+}
+void launch_add_sorted(const double* src, double* dst, const int* gid, int n, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_add_sorted, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, gid, n);
+}
 void launch_sort_d(const double* src, double* dst, const int* gid, int n, hipStream_t s) {
     if (n > 0)
         hipLaunchKernelGGL(k_sort_d, dim3(grid_for(n)), dim3(kBlock), 0, s, src, dst, gid, n);

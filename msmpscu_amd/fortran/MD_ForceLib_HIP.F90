@@ -150,6 +150,12 @@ module MSM_HIP_C
       integer(c_int) :: rc
     end function msm_calc_force
 
+    function msm_calc_density(sim) bind(C, name="msm_calc_density") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sim
+      integer(c_int) :: rc
+    end function msm_calc_density
+
     function msm_calc_epot(sim) bind(C, name="msm_calc_epot") result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: sim
@@ -169,6 +175,34 @@ module MSM_HIP_C
       real(c_double), intent(out) :: avp(*)
       integer(c_int) :: rc
     end function msm_calc_atomic_stress
+
+    function msm_thermalize(sim, temp, kb, seed) bind(C, name="msm_thermalize") result(rc)
+      import :: c_int, c_double, c_ptr, c_long_long
+      type(c_ptr), value :: sim
+      real(c_double), value :: temp, kb
+      integer(c_long_long), value :: seed
+      integer(c_int) :: rc
+    end function msm_thermalize
+
+    function msm_add_forces(sim, fext, on_device) bind(C, name="msm_add_forces") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sim, fext
+      integer(c_int), value :: on_device
+      integer(c_int) :: rc
+    end function msm_add_forces
+
+    function msm_add_epot(sim, eext, on_device) bind(C, name="msm_add_epot") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sim, eext
+      integer(c_int), value :: on_device
+      integer(c_int) :: rc
+    end function msm_add_epot
+
+    function msm_put_dis(sim, dis) bind(C, name="msm_put_dis") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: sim, dis
+      integer(c_int) :: rc
+    end function msm_put_dis
 
     function msm_damping(sim) bind(C, name="msm_damping") result(rc)
       import :: c_int, c_ptr
@@ -324,6 +358,11 @@ module MD_ForceLib_HIP
 
   integer, parameter, public :: KINDDF = c_double
   real(KINDDF), parameter, public :: CP_KB_EV = 8.617333262d-5   ! Boltzmann constant, eV/K
+  ! the reference's CGS constants (MSMLIB/sor/Common/MSM_Const.F90:82-85)
+  real(KINDDF), parameter, public :: CP_KB_CGS = 1.38054d-16       ! Boltzmann constant, erg/K
+  real(KINDDF), parameter, public :: CP_EVERG = 1.60219d-12        ! erg per eV
+  integer, parameter, public :: mp_MXGROUP = 16                    ! MD_Const.F90:78
+  integer, parameter, public :: mp_MXEXTFORCE = 16
   ! quench schemes (MD_Const.F90:23-27)
   integer, parameter, public :: CP_DAMPSCHEME_LBFGS = 0, CP_DAMPSCHEME_DYN = 1, &
                                 CP_DAMPSCHEME_CG = 2, CP_DAMPSCHEME_ST = 3, &
@@ -356,6 +395,7 @@ module MD_ForceLib_HIP
     procedure(NEFORCE_PROC), pointer, nopass :: pNE => null()
     procedure(EMBDF_PROC), pointer, nopass :: pEMB => null()
     character(len=256) :: NOTE = ""
+    character(len=8) :: SYMB = ""   ! "TGT<-SOR" of the symbol form, upper case
   end type ForceProcEntry
 
   ! --- MDForceTable (MD_TypeDef_ForceTable.F90:117-155); arrays (NKIND, NTAB) / (NKIND1, NEMBD)
@@ -370,14 +410,37 @@ module MD_ForceLib_HIP
     type(ForceProcEntry) :: PROC(mp_MXFORCE)
   end type MDForceTable
 
-  ! --- SimMDBox (subset of MD_TypeDef_SimBox.F90:43-132 used by the force path)
+  ! --- SimMDBox (the fields of MD_TypeDef_SimBox.F90:43-132 that the force path, the box setup
+  !     and the binary restart records read; CM and PTYPE hold NGROUP entries here and are padded
+  !     to mp_MXGROUP in the records)
   type, public :: SimMDBox
     integer :: NPRT = 0, NGROUP = 1
     real(KINDDF) :: ZL(3) = 0.0d0, BOXLOW(3) = 0.0d0
-    real(KINDDF) :: RR = 1.0d0   ! lattice unit, in this layer's length unit (Angstrom)
+    real(KINDDF) :: RR = 1.0d0   ! lattice unit, in the caller's length unit
     real(KINDDF), allocatable :: XP(:, :), XP1(:, :), FP(:, :), DIS(:, :), EPOT(:), CM(:)
+    real(KINDDF), allocatable :: EKIN(:)
     integer, allocatable :: ITYP(:), STATU(:), PTYPE(:, :)
+    character(len=8) :: SYMB(mp_MXGROUP) = ""
+    real(KINDDF) :: LATT(3) = 0.0d0
+    real(KINDDF) :: BOXSHAPE(3, 3) = reshape([1.0d0, 0.0d0, 0.0d0, 0.0d0, 1.0d0, 0.0d0, &
+                                              0.0d0, 0.0d0, 1.0d0], [3, 3])
+    character(len=12) :: POTTYPE = ""
+    character(len=128) :: POTLIBNAME = ""
+    character(len=12) :: POTSUBLIBNAME = ""
+    integer :: NA(mp_MXGROUP) = 0, IPA(mp_MXGROUP + 1) = 0, PROP(mp_MXGROUP) = 1
+    real(KINDDF) :: CZ(mp_MXGROUP) = 0.0d0
+    real(KINDDF) :: TEMPERATURE = 0.0d0, VOLUME = 0.0d0, SPRESS0 = 0.0d0, SPRESS1 = 0.0d0
+    real(KINDDF) :: SPRESS = 0.0d0, AVEPOT = 0.0d0, HARMIL = 0.0d0
+    real(KINDDF) :: VTENSOR(3, 3) = 0.0d0, KTENSOR(3, 3) = 0.0d0, PTENSOR(3, 3) = 0.0d0
   end type SimMDBox
+
+  ! --- MDRecordStamp (MD_TypeDef_RecordStamp.F90:14-31)
+  type, public :: MDRecordStamp
+    character(len=8) :: AppType = ""
+    integer :: IProcess = 0, ITest = -1, NBox = 0, IBox(2) = -1, ITime = -1, ISect = -1
+    real(KINDDF) :: Time = -1.0d0, ScalTime = -1.0d0, InstantTemp = -1.0d0
+    integer :: ICfg(2) = -1, IRec(2) = -1
+  end type MDRecordStamp
 
   ! --- SimMDCtrl (fields of MD_TypeDef_SimCtrlParam.F90:56-142 used by the force path)
   type, public :: SimMDCtrl
@@ -388,13 +451,45 @@ module MD_ForceLib_HIP
     real(KINDDF) :: STEEPEST_Alpha = 0.1d0, STEEPEST_MiStep = 1.0d-5, STEEPEST_MxStep = 0.1d0
     real(KINDDF) :: STEEPEST_MiDelE = 1.0d-3
     integer :: IFPD(3) = 1
+    ! Boltzmann constant in the caller's energy unit per kelvin: CP_KB_EV for eV callers,
+    ! CP_KB_CGS for the reference's CGS arrays (its CP_KB); every temperature routine reads it
+    real(KINDDF) :: KB = CP_KB_EV
+    ! random seeds (MD_TypeDef_SimCtrlParam.F90:51); SEED(1:2) key Thermalizing_MC_DEV
+    integer :: SEED(4) = [1, 2, 3, 4]
+    ! time step control (MD_TypeDef_SimCtrlParam.F90:36-47, 78-79): IHDUP < 0 searches the step
+    ! from HMX every |IHDUP| steps so that no active atom moves more than DMX; the predictor damps
+    ! in steps DAMPTIME0 .. DAMPTIME0 + DAMPTIME1 - 1
+    integer :: IT0 = 1, IHDUP = 0, DAMPTIME0 = 1, DAMPTIME1 = 0
+    real(KINDDF) :: HMI = 1.0d0, HMX = 1.0d0, DMX = 0.1d0
+    ! L-BFGS(-B) quench controls (MD_TypeDef_SimCtrlParam.F90:208-210)
+    real(KINDDF) :: LBFGS_PGtol = 0.0d0, LBFGS_Factr = 0.0d0
+    integer :: LBFGS_Msave = 7
   end type SimMDCtrl
 
   ! --- MDForceClassGPU (MD_ForceClass_Register_GPU.F90:248-259): the class name selects the
   !     FS or EAM device passes; the table travels with it
+  abstract interface
+    ! ForceArray / EpotArray routines of AddExtForce_ForceClass (MD_ForceClass_Register_GPU.F90:
+    ! 982-1180); they add their terms with AddForceFrom_Host_to_Devices / AddEpotFrom_Host_to_Devices
+    subroutine EXTFORCE_PROC(SimBox, CtrlParam)
+      import :: SimMDBox, SimMDCtrl
+      type(SimMDBox), intent(inout) :: SimBox(:)
+      type(SimMDCtrl), intent(in) :: CtrlParam
+    end subroutine EXTFORCE_PROC
+  end interface
+  public :: EXTFORCE_PROC
+
+  type :: ExtForceEntry
+    character(len=32) :: Tag = ""
+    procedure(EXTFORCE_PROC), pointer, nopass :: pForceArray => null()
+    procedure(EXTFORCE_PROC), pointer, nopass :: pEpotArray => null()
+  end type ExtForceEntry
+
   type, public :: MDForceClassGPU
     character(len=16) :: CLASSNAME = ""
     type(MDForceTable) :: ForceTable
+    integer :: NEXT = 0                                  ! the ExtForceList of the reference
+    type(ExtForceEntry) :: EXT(mp_MXEXTFORCE)
   end type MDForceClassGPU
 
   ! device context of this process (the reference keeps its device state in module variables too)
@@ -402,9 +497,19 @@ module MD_ForceLib_HIP
   integer, save :: gm_NBOX = 0, gm_NPRT = 0
   integer, allocatable, save :: gm_NAT(:)
 
+  interface Register_ForceTableProc
+    module procedure RegisterByTypeID_ForceTableProc
+    module procedure RegisterBySymbID_ForceTableProc
+  end interface Register_ForceTableProc
+
   public :: Register_ForceTableProc, Create_Interaction_ForceTable, Register_ForceClass
+  public :: SymbID2TypeID_ForceTableProc
+  public :: AddExtForce_ForceClass, Clear_ExtForce, AddForceFrom_Host_to_Devices
+  public :: AddEpotFrom_Host_to_Devices, CopyDISFrom_Host_to_Devices, Thermalizing_MC_DEV
+  public :: Archive_Config_SimBoxArray, Restore_Config_SimBoxArray
   public :: Initialize_Globle_Variables_DEV, Init_Forcetable_Dev, Cal_NeighBoreList_DEV
   public :: CalForce_ForceClass, CalEpot_ForceClass, CalPTensor_ForceClass, Cal_AtomicStress
+  public :: CalEDen_ForceClass, CopyDENFrom_Devices_to_Host
   public :: Predictor_DEV, Correction_DEV, Cal_GlobalT_DEV, VelScaling_DEV, Do_DynDamp_Forsteps_DEV
   public :: Do_Steepest_Forsteps_DEV, Do_CG_Forsteps_DEV
   public :: For_One_Step, Run_Steps_DEV, CopyOut_SimBox_DEV, Copyout_NeighboreList_DEV
@@ -438,8 +543,9 @@ contains
   ! force tables
   ! ---------------------------------------------------------------------------------------------
 
-  ! Register_ForceTableProc (type-ID form, MD_TypeDef_ForceTable.F90:626-730)
-  subroutine Register_ForceTableProc(IFORCE, FTable, NNFORCE, NEFORCE, EMBDF, NOTE)
+  ! Register_ForceTableProc, type-ID form (RegisterByTypeID_ForceTableProc,
+  ! MD_TypeDef_ForceTable.F90:626-730)
+  subroutine RegisterByTypeID_ForceTableProc(IFORCE, FTable, NNFORCE, NEFORCE, EMBDF, NOTE)
     integer, intent(in) :: IFORCE
     type(MDForceTable), intent(inout) :: FTable
     procedure(NNFORCE_PROC) :: NNFORCE
@@ -455,7 +561,64 @@ contains
     if (present(NEFORCE)) FTable%PROC(IFORCE)%pNE => NEFORCE
     if (present(EMBDF)) FTable%PROC(IFORCE)%pEMB => EMBDF
     if (present(NOTE)) FTable%PROC(IFORCE)%NOTE = NOTE
-  end subroutine Register_ForceTableProc
+  end subroutine RegisterByTypeID_ForceTableProc
+
+  function upcase8(str) result(u)
+    character(len=*), intent(in) :: str
+    character(len=8) :: u
+    integer :: i, c
+    u = str
+    do i = 1, len_trim(u)
+      c = iachar(u(i:i))
+      if (c >= iachar("a") .and. c <= iachar("z")) u(i:i) = achar(c - 32)
+    end do
+  end function upcase8
+
+  ! Register_ForceTableProc, symbol form (RegisterBySymbID_ForceTableProc,
+  ! MD_TypeDef_ForceTable.F90:734-852): the routines of the interaction of target atoms Tgt with
+  ! source atoms Sor, symbol "TGT<-SOR" (8 characters, upper case); the type id is one more than
+  ! the largest registered; a symbol registered twice stops
+  subroutine RegisterBySymbID_ForceTableProc(Sor, Tgt, FTable, NNFORCE, NEFORCE, EMBDF, NOTE)
+    character(len=*), intent(in) :: Sor, Tgt
+    type(MDForceTable), intent(inout) :: FTable
+    procedure(NNFORCE_PROC) :: NNFORCE
+    procedure(NEFORCE_PROC), optional :: NEFORCE
+    procedure(EMBDF_PROC), optional :: EMBDF
+    character(len=*), optional, intent(in) :: NOTE
+    character(len=8) :: symb
+    integer :: i, id
+    symb = upcase8(trim(Tgt) // "<-" // trim(Sor))
+    id = 0
+    do i = 1, mp_MXFORCE
+      if (.not. FTable%PROC(i)%REG) cycle
+      if (FTable%PROC(i)%SYMB == symb) then
+        write(*, "(A)") "MDPSCU Error: the potential type by symb" // trim(symb) // &
+                        " has been registed"
+        stop 1
+      end if
+      id = i
+    end do
+    id = id + 1
+    call RegisterByTypeID_ForceTableProc(id, FTable, NNFORCE, NEFORCE, EMBDF, NOTE)
+    FTable%PROC(id)%SYMB = symb
+  end subroutine RegisterBySymbID_ForceTableProc
+
+  ! SymbID2TypeID_ForceTableProc (MD_TypeDef_ForceTable.F90:856-886): type id registered for
+  ! "TGT<-SOR", 0 if none
+  integer function SymbID2TypeID_ForceTableProc(Sor, Tgt, FTable) result(IFORCE)
+    character(len=*), intent(in) :: Sor, Tgt
+    type(MDForceTable), intent(in) :: FTable
+    character(len=8) :: symb
+    integer :: i
+    symb = upcase8(trim(Tgt) // "<-" // trim(Sor))
+    IFORCE = 0
+    do i = 1, mp_MXFORCE
+      if (FTable%PROC(i)%REG .and. FTable%PROC(i)%SYMB == symb) then
+        IFORCE = i
+        return
+      end if
+    end do
+  end function SymbID2TypeID_ForceTableProc
 
   integer function find_or_add(list, n, v) result(k)
     integer, intent(inout) :: list(:)
@@ -475,7 +638,7 @@ contains
   ! CSI = NTAB/sqrt(Rmax), Rmax = maxval(RU); POTR = NN*r, FPOTR = dNN*r, POTB/FPOTB = NE;
   ! RHOMX = RHOSCAL * max POTB, RHOD = RHOMX/NEMBD, F tabulated at rho = (i-1)*RHOD.
   subroutine Create_Interaction_ForceTable(SimBox, CtrlParam, FTable)
-    type(SimMDBox), intent(in) :: SimBox
+    type(SimMDBox), intent(inout) :: SimBox
     type(SimMDCtrl), intent(in) :: CtrlParam
     type(MDForceTable), intent(inout) :: FTable
     integer :: ng, i, j, k, n1, n2, it
@@ -486,6 +649,13 @@ contains
     n2 = 0
     if (allocated(FTable%KPAIR)) deallocate(FTable%KPAIR)
     allocate(FTable%KPAIR(ng, ng))
+    ! pairs without a type id take the one registered for "SYMB(i)<-SYMB(j)" (:1170-1185)
+    do j = 1, ng
+      do i = 1, ng
+        if (SimBox%PTYPE(i, j) == 0 .and. len_trim(SimBox%SYMB(i)) > 0) &
+          SimBox%PTYPE(i, j) = SymbID2TypeID_ForceTableProc(SimBox%SYMB(j), SimBox%SYMB(i), FTable)
+      end do
+    end do
     do j = 1, ng
       do i = 1, ng
         it = SimBox%PTYPE(i, j)
@@ -679,21 +849,62 @@ contains
   end subroutine Cal_NeighBoreList_DEV
 
   ! CalForce_ForceClass (MD_ForceClass_Register_GPU.F90:636-680): density then force pass
+  ! then Cumulate_ExtForce (:584-632): the registered ForceArray routines in registration order
   subroutine CalForce_ForceClass(SimBox, CtrlParam, ForceClass, IERR)
+    type(SimMDBox), intent(inout) :: SimBox(:)
+    type(SimMDCtrl), intent(in) :: CtrlParam
+    type(MDForceClassGPU), intent(in) :: ForceClass
+    integer, optional, intent(out) :: IERR
+    integer(c_int) :: rc
+    integer :: k
+    rc = msm_calc_force(gm_SIM)
+    call msm_check(rc, "CalForce_ForceClass", IERR)
+    if (rc /= MSM_OK) return
+    do k = 1, ForceClass%NEXT
+      if (associated(ForceClass%EXT(k)%pForceArray)) &
+        call ForceClass%EXT(k)%pForceArray(SimBox, CtrlParam)
+    end do
+  end subroutine CalForce_ForceClass
+
+  ! the pCalEDen slot of MDForceClassGPU (MD_ForceClass_Register_GPU.F90:254, 405, 416):
+  ! CALDEN_FS/EAM_Force_Table2A_DEV (MD_FS_ForceTable_GPU.F90:875-930), the density pass alone;
+  ! F'(rho) stays on the device (CopyDENFrom_Devices_to_Host copies it out)
+  subroutine CalEDen_ForceClass(SimBox, CtrlParam, ForceClass, IERR)
     type(SimMDBox), intent(in) :: SimBox(:)
     type(SimMDCtrl), intent(in) :: CtrlParam
     type(MDForceClassGPU), intent(in) :: ForceClass
     integer, optional, intent(out) :: IERR
-    call msm_check(msm_calc_force(gm_SIM), "CalForce_ForceClass", IERR)
-  end subroutine CalForce_ForceClass
+    call msm_check(msm_calc_density(gm_SIM), "CalEDen_ForceClass", IERR)
+  end subroutine CalEDen_ForceClass
+
+  ! the device array DEN (F'(rho) per atom) in the caller's atom order of all boxes
+  subroutine CopyDENFrom_Devices_to_Host(DEN, IERR)
+    real(KINDDF), intent(out) :: DEN(:)
+    integer, optional, intent(out) :: IERR
+    integer(c_int) :: rc
+    if (size(DEN) < gm_NPRT) then
+      rc = MSM_ERR_ARG
+    else
+      rc = msm_get_den(gm_SIM, DEN)
+    end if
+    call msm_check(rc, "CopyDENFrom_Devices_to_Host", IERR)
+  end subroutine CopyDENFrom_Devices_to_Host
 
   ! CalEpot_ForceClass (:778-822)
   subroutine CalEpot_ForceClass(SimBox, CtrlParam, ForceClass, IERR)
-    type(SimMDBox), intent(in) :: SimBox(:)
+    type(SimMDBox), intent(inout) :: SimBox(:)
     type(SimMDCtrl), intent(in) :: CtrlParam
     type(MDForceClassGPU), intent(in) :: ForceClass
     integer, optional, intent(out) :: IERR
-    call msm_check(msm_calc_epot(gm_SIM), "CalEpot_ForceClass", IERR)
+    integer(c_int) :: rc
+    integer :: k
+    rc = msm_calc_epot(gm_SIM)
+    call msm_check(rc, "CalEpot_ForceClass", IERR)
+    if (rc /= MSM_OK) return
+    do k = 1, ForceClass%NEXT
+      if (associated(ForceClass%EXT(k)%pEpotArray)) &
+        call ForceClass%EXT(k)%pEpotArray(SimBox, CtrlParam)
+    end do
   end subroutine CalEpot_ForceClass
 
   ! CalPTensor_ForceClass (MD_ForceClass_Register_GPU.F90:684-728): the virial tensor as
@@ -755,12 +966,12 @@ contains
     integer(c_int) :: rc
     CURT = 0.0d0
     allocate(ek(max(gm_NPRT, 1)), tb(max(gm_NBOX, 1)))
-    rc = msm_calc_ekin(gm_SIM, CP_KB_EV, ek, tb)
+    rc = msm_calc_ekin(gm_SIM, CtrlParam%KB, ek, tb)
     call msm_check(rc, "Cal_GlobalT_DEV", IERR)
     if (rc /= MSM_OK) return
     if (count(ek(1:gm_NPRT) >= 0.0d0) > 0) &
       CURT = 2.0d0 * sum(ek(1:gm_NPRT), mask=(ek(1:gm_NPRT) >= 0.0d0)) / &
-             dble(count(ek(1:gm_NPRT) >= 0.0d0)) / (3.0d0 * CP_KB_EV)
+             dble(count(ek(1:gm_NPRT) >= 0.0d0)) / (3.0d0 * CtrlParam%KB)
   end subroutine Cal_GlobalT_DEV
 
   ! VelScaling_DEV (MD_DiffScheme_GPU.F90:1390-1446): scale the velocities of each box to DT (K)
@@ -769,7 +980,7 @@ contains
     type(SimMDCtrl), intent(in) :: CtrlParam
     real(KINDDF), intent(in) :: DT
     integer, optional, intent(out) :: IERR
-    call msm_check(msm_scale_velocities(gm_SIM, DT, CP_KB_EV), "VelScaling_DEV", IERR)
+    call msm_check(msm_scale_velocities(gm_SIM, DT, CtrlParam%KB), "VelScaling_DEV", IERR)
   end subroutine VelScaling_DEV
 
   ! Do_DynDamp_Forsteps_DEV (MD_DiffScheme_GPU.F90:1809-1883): damped dynamics on the current
@@ -859,13 +1070,38 @@ contains
     IFLAG = fl
   end subroutine CheckTimestep_DEV
 
-  ! Predictor_DEV / Correction_DEV (MD_DiffScheme_GPU.F90:604-682 / 821-847)
+  ! Predictor_DEV (MD_DiffScheme_GPU.F90:604-682): damping (DAMPING_template) when ITIME lies in
+  ! DAMPTIME0 .. DAMPTIME0 + DAMPTIME1 - 1; with IHDUP < 0 and mod(ITIME - IT0 + 1, |IHDUP|) = 0
+  ! the step search of :629-657 -- from HMX, halve the step while CheckTimestep_DEV finds an
+  ! active atom moving more than DMX -- which rewrites CtrlParam%H; then the Swope predictor
   subroutine Predictor_DEV(ITIME, SimBox, CtrlParam, IERR)
     integer, intent(in) :: ITIME
     type(SimMDBox), intent(in) :: SimBox(:)
-    type(SimMDCtrl), intent(in) :: CtrlParam
+    type(SimMDCtrl), intent(inout) :: CtrlParam
     integer, optional, intent(out) :: IERR
-    call msm_check(msm_predictor(gm_SIM, CtrlParam%H), "Predictor_DEV", IERR)
+    integer(c_int) :: rc, fl
+    real(c_double) :: th
+    rc = MSM_OK
+    if (ITIME >= CtrlParam%DAMPTIME0 .and. &
+        ITIME <= CtrlParam%DAMPTIME0 + CtrlParam%DAMPTIME1 - 1) rc = msm_damping(gm_SIM)
+    if (rc == MSM_OK .and. CtrlParam%IHDUP < 0) then
+      if (mod(ITIME - CtrlParam%IT0 + 1, abs(CtrlParam%IHDUP)) == 0) then
+        th = CtrlParam%HMX
+        do
+          fl = 0
+          rc = msm_check_timestep(gm_SIM, th, CtrlParam%DMX * CtrlParam%DMX, fl)
+          if (rc /= MSM_OK .or. fl == 0) exit
+          th = 0.5d0 * th
+          if (th == 0.0d0) then
+            rc = MSM_ERR_ARG
+            exit
+          end if
+        end do
+        if (rc == MSM_OK) CtrlParam%H = th
+      end if
+    end if
+    if (rc == MSM_OK) rc = msm_predictor(gm_SIM, CtrlParam%H)
+    call msm_check(rc, "Predictor_DEV", IERR)
   end subroutine Predictor_DEV
 
   subroutine Correction_DEV(ITIME, SimBox, CtrlParam, IERR)
@@ -878,18 +1114,25 @@ contains
 
   ! For_One_Step (MD_Method_GenericMD_GPU.F90:496-659): predictor, list rebuild when
   ! mod(ITIME, NB_UPTAB) = 0, force, corrector -- one launch sequence per call
+  ! (damping window, step search and external forces included, as Predictor_DEV and
+  ! CalForce_ForceClass do them)
   subroutine For_One_Step(ITIME, SimBox, CtrlParam, ForceClass, IERR)
     integer, intent(in) :: ITIME
-    type(SimMDBox), intent(in) :: SimBox(:)
-    type(SimMDCtrl), intent(in) :: CtrlParam
+    type(SimMDBox), intent(inout) :: SimBox(:)
+    type(SimMDCtrl), intent(inout) :: CtrlParam
     type(MDForceClassGPU), intent(in) :: ForceClass
     integer, optional, intent(out) :: IERR
+    integer :: ie
     integer(c_int) :: rc
-    rc = msm_predictor(gm_SIM, CtrlParam%H)
+    call Predictor_DEV(ITIME, SimBox, CtrlParam, ie)
+    rc = ie
     if (rc == MSM_OK .and. CtrlParam%NB_UPTAB > 0) then
       if (mod(ITIME, CtrlParam%NB_UPTAB) == 0) rc = msm_build_neighbors(gm_SIM)
     end if
-    if (rc == MSM_OK) rc = msm_calc_force(gm_SIM)
+    if (rc == MSM_OK) then
+      call CalForce_ForceClass(SimBox, CtrlParam, ForceClass, ie)
+      rc = ie
+    end if
     if (rc == MSM_OK) rc = msm_correction(gm_SIM, CtrlParam%H)
     call msm_check(rc, "For_One_Step", IERR)
   end subroutine For_One_Step
@@ -1044,5 +1287,311 @@ contains
     gm_NBOX = 0
     gm_NPRT = 0
   end subroutine Clear_Globle_Variables_DEV
+
+  ! ---------------------------------------------------------------------------------------------
+  ! external forces (AddExtForce_ForceClass / Cumulate_ExtForce, MD_ForceClass_Register_GPU.F90:
+  ! 584-680, 982-1180)
+  ! ---------------------------------------------------------------------------------------------
+
+  ! Register ForceArray / EpotArray under Tag (a tag already present is replaced); they run after
+  ! the force pass of CalForce_ForceClass / the energy pass of CalEpot_ForceClass (For_One_Step
+  ! included; the fused steps of Run_Steps_DEV do not call them)
+  subroutine AddExtForce_ForceClass(ForceClass, Tag, ForceArray, EpotArray)
+    type(MDForceClassGPU), intent(inout) :: ForceClass
+    character(len=*), intent(in) :: Tag
+    procedure(EXTFORCE_PROC), optional :: ForceArray, EpotArray
+    integer :: k, slot
+    slot = 0
+    do k = 1, ForceClass%NEXT
+      if (ForceClass%EXT(k)%Tag == Tag) slot = k
+    end do
+    if (slot == 0) then
+      if (ForceClass%NEXT >= mp_MXEXTFORCE) then
+        write(*, "(A)") "MDPSCU Error: too many external forces registered"
+        stop 1
+      end if
+      ForceClass%NEXT = ForceClass%NEXT + 1
+      slot = ForceClass%NEXT
+    end if
+    ForceClass%EXT(slot)%Tag = Tag
+    ForceClass%EXT(slot)%pForceArray => null()
+    ForceClass%EXT(slot)%pEpotArray => null()
+    if (present(ForceArray)) ForceClass%EXT(slot)%pForceArray => ForceArray
+    if (present(EpotArray)) ForceClass%EXT(slot)%pEpotArray => EpotArray
+  end subroutine AddExtForce_ForceClass
+
+  ! Clear_ExtForce: drop the entry of Tag, or every entry when Tag is absent
+  subroutine Clear_ExtForce(ForceClass, Tag)
+    type(MDForceClassGPU), intent(inout) :: ForceClass
+    character(len=*), optional, intent(in) :: Tag
+    integer :: k, m
+    if (.not. present(Tag)) then
+      ForceClass%NEXT = 0
+      return
+    end if
+    m = 0
+    do k = 1, ForceClass%NEXT
+      if (ForceClass%EXT(k)%Tag == Tag) cycle
+      m = m + 1
+      if (m /= k) then
+        ForceClass%EXT(m)%Tag = ForceClass%EXT(k)%Tag
+        ForceClass%EXT(m)%pForceArray => ForceClass%EXT(k)%pForceArray
+        ForceClass%EXT(m)%pEpotArray => ForceClass%EXT(k)%pEpotArray
+      end if
+    end do
+    ForceClass%NEXT = m
+  end subroutine Clear_ExtForce
+
+  ! FEXT(NPRT,3) of all boxes, in the caller's atom order, added to the device forces
+  subroutine AddForceFrom_Host_to_Devices(FEXT, IERR)
+    real(KINDDF), intent(in), target, contiguous :: FEXT(:, :)
+    integer, optional, intent(out) :: IERR
+    integer(c_int) :: rc
+    if (size(FEXT, 1) /= gm_NPRT .or. size(FEXT, 2) /= 3) then
+      rc = MSM_ERR_ARG
+    else
+      rc = msm_add_forces(gm_SIM, c_loc(FEXT), 0_c_int)
+    end if
+    call msm_check(rc, "AddForceFrom_Host_to_Devices", IERR)
+  end subroutine AddForceFrom_Host_to_Devices
+
+  ! EEXT(NPRT) added to the device's per-atom energies
+  subroutine AddEpotFrom_Host_to_Devices(EEXT, IERR)
+    real(KINDDF), intent(in), target, contiguous :: EEXT(:)
+    integer, optional, intent(out) :: IERR
+    integer(c_int) :: rc
+    if (size(EEXT) /= gm_NPRT) then
+      rc = MSM_ERR_ARG
+    else
+      rc = msm_add_epot(gm_SIM, c_loc(EEXT), 0_c_int)
+    end if
+    call msm_check(rc, "AddEpotFrom_Host_to_Devices", IERR)
+  end subroutine AddEpotFrom_Host_to_Devices
+
+  ! DIS of all boxes onto the device (a restart: Initialize_Globle_Variables_DEV starts DIS at 0)
+  subroutine CopyDISFrom_Host_to_Devices(SimBox, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    integer, optional, intent(out) :: IERR
+    real(c_double), allocatable, target :: a(:)
+    integer :: n, b, off, k, m
+    integer(c_int) :: rc
+    n = gm_NPRT
+    allocate(a(3 * max(n, 1)))
+    a = 0.0d0
+    off = 0
+    do b = 1, size(SimBox)
+      m = SimBox(b)%NPRT
+      if (allocated(SimBox(b)%DIS)) then
+        do k = 1, 3
+          a((k - 1) * n + off + 1:(k - 1) * n + off + m) = SimBox(b)%DIS(:, k)
+        end do
+      end if
+      off = off + m
+    end do
+    if (off /= n) then
+      rc = MSM_ERR_ARG
+    else
+      rc = msm_put_dis(gm_SIM, c_loc(a))
+    end if
+    call msm_check(rc, "CopyDISFrom_Host_to_Devices", IERR)
+  end subroutine CopyDISFrom_Host_to_Devices
+
+  ! Thermalizing_MC_DEV (MD_DiffScheme_GPU.F90:1451-1805): Maxwell velocities at TI (K) on the
+  ! device, components with FIXVEL/FIXPOS kept, inactive atoms at rest, each box's mass-weighted
+  ! mean velocity removed; the device generator is keyed by CtrlParam%SEED(1:2) and advances with
+  ! every call (msm_thermalize)
+  subroutine Thermalizing_MC_DEV(SimBox, CtrlParam, TI, IERR)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    type(SimMDCtrl), intent(in) :: CtrlParam
+    real(KINDDF), intent(in) :: TI
+    integer, optional, intent(out) :: IERR
+    integer(c_long_long) :: seed
+    seed = ior(ishft(int(CtrlParam%SEED(2), c_long_long), 32), &
+               iand(int(CtrlParam%SEED(1), c_long_long), int(z'FFFFFFFF', c_long_long)))
+    call msm_check(msm_thermalize(gm_SIM, TI, CtrlParam%KB, seed), "Thermalizing_MC_DEV", IERR)
+  end subroutine Thermalizing_MC_DEV
+
+  ! ---------------------------------------------------------------------------------------------
+  ! binary restart records (Archive/Restore_Config_SimBoxArray, MD_SimBoxArray.F90:1964-2047):
+  ! the stamp (Archive_RecordStamp, MD_TypeDef_RecordStamp.F90:195-226), then per box the records
+  ! of Archive_Config_SimMDBox0 (MD_TypeDef_SimBox.F90:3268-3304) and an empty data pad
+  ! (Archive_DataPad, MSMLIB/sor/Common/MSM_TypeDef_DataPad.F90:2925-2973), then the caller's
+  ! own records (ExtArchive / ExtRestore, optional); Fortran unformatted sequential, unit chosen
+  ! free from 10 up
+  ! ---------------------------------------------------------------------------------------------
+
+  integer function free_unit() result(u)
+    logical :: op
+    do u = 10, 999
+      inquire(unit=u, opened=op)
+      if (.not. op) return
+    end do
+    u = -1
+  end function free_unit
+
+  subroutine Archive_Config_SimBoxArray(SimBox, Stamp, fname, ExtArchive)
+    type(SimMDBox), intent(in) :: SimBox(:)
+    type(MDRecordStamp), intent(in) :: Stamp
+    character(len=*), intent(in) :: fname
+    interface
+      subroutine ExtArchive(hFile)
+        integer, intent(in) :: hFile
+      end subroutine ExtArchive
+    end interface
+    optional :: ExtArchive
+    integer :: hf, nb, i, np, nxp, ng
+    real(KINDDF) :: cm(mp_MXGROUP)
+    integer :: ptype(mp_MXGROUP, mp_MXGROUP)
+    real(KINDDF), allocatable :: z3(:, :), z1(:)
+    character(len=64) :: tag
+    hf = free_unit()
+    nb = size(SimBox)
+    if (Stamp%NBox > 0) nb = Stamp%NBox
+    open(hf, file=fname, form="unformatted", status="unknown")
+    write(hf) Stamp%AppType(1:len(Stamp%AppType))
+    write(hf) Stamp%ITest, Stamp%NBox, Stamp%IBox, Stamp%ICfg, Stamp%IRec, Stamp%ITime, &
+              Stamp%ISect, Stamp%Time, Stamp%ScalTime, Stamp%InstantTemp
+    do i = 1, nb
+      associate(B => SimBox(i))
+        ng = B%NGROUP
+        cm = 0.0d0
+        ptype = 0
+        if (allocated(B%CM)) cm(1:size(B%CM)) = B%CM
+        if (allocated(B%PTYPE)) ptype(1:size(B%PTYPE, 1), 1:size(B%PTYPE, 2)) = B%PTYPE
+        write(hf) B%RR, B%LATT, B%BOXSHAPE, B%BOXLOW, B%BOXLOW + B%ZL, B%ZL
+        write(hf) B%NPRT, B%NGROUP
+        write(hf) B%NA, B%IPA, B%SYMB, B%CZ, cm, B%PROP, ptype
+        write(hf) B%POTTYPE, B%POTLIBNAME, B%POTSUBLIBNAME
+        write(hf) B%TEMPERATURE, B%VOLUME, B%SPRESS0, B%SPRESS1, B%SPRESS, B%AVEPOT, B%HARMIL
+        write(hf) B%VTENSOR, B%KTENSOR, B%PTENSOR
+        np = B%NPRT
+        nxp = 1
+        allocate(z3(np, 3), z1(np))
+        z3 = 0.0d0
+        z1 = 0.0d0
+        write(hf) np, nxp
+        write(hf) B%ITYP(1:np)
+        write(hf) B%XP(1:np, 1:3)
+        write(hf) B%XP1(1:np, 1:3)
+        if (allocated(B%FP)) then
+          write(hf) B%FP(1:np, 1:3)
+        else
+          write(hf) z3
+        end if
+        if (allocated(B%EKIN)) then
+          write(hf) B%EKIN(1:np)
+        else
+          write(hf) z1
+        end if
+        if (allocated(B%EPOT)) then
+          write(hf) B%EPOT(1:np)
+        else
+          write(hf) z1
+        end if
+        write(hf) B%STATU(1:np)
+        if (allocated(B%DIS)) then
+          write(hf) B%DIS(1:np, 1:3)
+        else
+          write(hf) z3
+        end if
+        deallocate(z3, z1)
+      end associate
+      tag = "PADDATA"
+      write(hf) tag
+      write(hf) 0
+    end do
+    if (present(ExtArchive)) call ExtArchive(hf)
+    close(hf)
+  end subroutine Archive_Config_SimBoxArray
+
+  subroutine Restore_Config_SimBoxArray(SimBox, Stamp, fname, ExtRestore)
+    type(SimMDBox), intent(inout) :: SimBox(:)
+    type(MDRecordStamp), intent(inout) :: Stamp
+    character(len=*), intent(in) :: fname
+    interface
+      subroutine ExtRestore(hFile)
+        integer, intent(in) :: hFile
+      end subroutine ExtRestore
+    end interface
+    optional :: ExtRestore
+    integer :: hf, nb, i, np, nxp, ng, npad, ios
+    real(KINDDF) :: cm(mp_MXGROUP), boxup(3)
+    integer :: ptype(mp_MXGROUP, mp_MXGROUP)
+    real(KINDDF), allocatable :: skip3(:, :)
+    character(len=64) :: tag
+    hf = free_unit()
+    nb = size(SimBox)
+    if (Stamp%NBox > 0) nb = Stamp%NBox
+    open(hf, file=fname, form="unformatted", status="old", iostat=ios)
+    if (ios /= 0) then
+      write(*, "(A)") "MDPSUC error: fail to restore the box: cannot open " // trim(fname)
+      stop 1
+    end if
+    read(hf, iostat=ios) Stamp%AppType(1:len(Stamp%AppType))
+    if (ios == 0) read(hf, iostat=ios) Stamp%ITest, Stamp%NBox, Stamp%IBox, Stamp%ICfg, &
+                                       Stamp%IRec, Stamp%ITime, Stamp%ISect, Stamp%Time, &
+                                       Stamp%ScalTime, Stamp%InstantTemp
+    do i = 1, nb
+      if (ios /= 0) exit
+      associate(B => SimBox(i))
+        read(hf, iostat=ios) B%RR, B%LATT, B%BOXSHAPE, B%BOXLOW, boxup, B%ZL
+        if (ios == 0) read(hf, iostat=ios) B%NPRT, B%NGROUP
+        if (ios == 0) read(hf, iostat=ios) B%NA, B%IPA, B%SYMB, B%CZ, cm, B%PROP, ptype
+        if (ios == 0) read(hf, iostat=ios) B%POTTYPE, B%POTLIBNAME, B%POTSUBLIBNAME
+        if (ios == 0) read(hf, iostat=ios) B%TEMPERATURE, B%VOLUME, B%SPRESS0, B%SPRESS1, &
+                                           B%SPRESS, B%AVEPOT, B%HARMIL
+        if (ios == 0) read(hf, iostat=ios) B%VTENSOR, B%KTENSOR, B%PTENSOR
+        if (ios == 0) read(hf, iostat=ios) np, nxp
+        if (ios /= 0) exit
+        ng = max(B%NGROUP, 1)
+        if (allocated(B%CM)) deallocate(B%CM)
+        if (allocated(B%PTYPE)) deallocate(B%PTYPE)
+        allocate(B%CM(ng), B%PTYPE(ng, ng))
+        B%CM = cm(1:ng)
+        B%PTYPE = ptype(1:ng, 1:ng)
+        if (allocated(B%ITYP)) deallocate(B%ITYP)
+        if (allocated(B%STATU)) deallocate(B%STATU)
+        if (allocated(B%XP)) deallocate(B%XP)
+        if (allocated(B%XP1)) deallocate(B%XP1)
+        if (allocated(B%FP)) deallocate(B%FP)
+        if (allocated(B%DIS)) deallocate(B%DIS)
+        if (allocated(B%EKIN)) deallocate(B%EKIN)
+        if (allocated(B%EPOT)) deallocate(B%EPOT)
+        allocate(B%ITYP(np), B%STATU(np), B%XP(np, 3), B%XP1(np, 3), B%FP(np, 3), B%DIS(np, 3), &
+                 B%EKIN(np), B%EPOT(np))
+        read(hf, iostat=ios) B%ITYP(1:np)
+        if (ios == 0) read(hf, iostat=ios) B%XP(1:np, 1:3)
+        if (ios == 0) read(hf, iostat=ios) B%XP1(1:np, 1:3)
+        if (ios == 0) read(hf, iostat=ios) B%FP(1:np, 1:3)
+        if (ios == 0) read(hf, iostat=ios) B%EKIN(1:np)
+        if (ios == 0) read(hf, iostat=ios) B%EPOT(1:np)
+        if (ios == 0) read(hf, iostat=ios) B%STATU(1:np)
+        if (ios == 0) read(hf, iostat=ios) B%DIS(1:np, 1:3)
+        if (ios == 0 .and. nxp >= 5) then
+          ! the higher-order Gear predictor arrays XP2..XP5 (not used by the Swope scheme)
+          allocate(skip3(np, 3))
+          read(hf, iostat=ios) skip3
+          if (ios == 0) read(hf, iostat=ios) skip3
+          if (ios == 0) read(hf, iostat=ios) skip3
+          if (ios == 0) read(hf, iostat=ios) skip3
+          deallocate(skip3)
+        end if
+      end associate
+      ! Restore_DataPad: a data pad this layer does not keep is read past
+      if (ios == 0) read(hf, iostat=ios) tag
+      if (ios == 0) read(hf, iostat=ios) npad
+      if (ios == 0 .and. npad /= 0) then
+        write(*, "(A)") "MDPSUC error: data pads in the restart file are not supported"
+        ios = 1
+      end if
+    end do
+    if (ios /= 0) then
+      write(*, "(A)") "MDPSUC error: fail to restore the box"
+      close(hf)
+      stop 1
+    end if
+    if (present(ExtRestore)) call ExtRestore(hf)
+    close(hf)
+  end subroutine Restore_Config_SimBoxArray
 
 end module MD_ForceLib_HIP

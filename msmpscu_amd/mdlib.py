@@ -36,6 +36,9 @@ from ._lib import MSM_EAM_TYPE, MSM_FS_TYPE, check, dptr, iptr, lib
 PotFn = Union[str, Callable[[float], tuple], None]
 
 CP_KB_EV = 8.617333262e-5  # Boltzmann constant, eV/K
+# the reference's CGS constants (MSMLIB/sor/Common/MSM_Const.F90:82-85)
+CP_KB_CGS = 1.38054e-16    # Boltzmann constant, erg/K
+CP_EVERG = 1.60219e-12     # erg per eV
 CP_EVERG = 1.60219e-12     # erg per eV, as the reference
 CP_CM2A = 1.0e8            # Angstrom per cm
 # STATU bits (MD_Const.F90:80-96)
@@ -360,6 +363,13 @@ class SimMDCtrl:
     DMX: float = 0.1
     DAMPTIME0: int = 1
     DAMPTIME1: int = 0
+    # Boltzmann constant in the caller's energy unit per kelvin: eV/K for eV-Angstrom callers,
+    # CP_KB_CGS for the reference's CGS arrays (its CP_KB, MSM_Const.F90:82); every temperature
+    # routine of this layer reads it from here
+    KB: float = CP_KB_EV
+    # random seeds (MD_TypeDef_SimCtrlParam.F90:51); SEED[0], SEED[1] key the device generator of
+    # Thermalizing_MC_DEV
+    SEED: Sequence[int] = (1, 2, 3, 4)
 
     def __post_init__(self):
         self.RU = np.asfortranarray(np.atleast_2d(self.RU), dtype=np.float64)
@@ -443,6 +453,9 @@ class DeviceMD:
     def calc_force(self):
         check(lib().msm_calc_force(self.h), "msm_calc_force", self.h)
 
+    def calc_density(self):
+        check(lib().msm_calc_density(self.h), "msm_calc_density", self.h)
+
     def calc_epot(self):
         check(lib().msm_calc_epot(self.h), "msm_calc_epot", self.h)
 
@@ -469,6 +482,10 @@ class DeviceMD:
         tb = np.zeros(self.nbox)
         check(lib().msm_calc_ekin(self.h, kb, dptr(ek), dptr(tb)), "msm_calc_ekin", self.h)
         return ek, tb
+
+    def thermalize(self, temp: float, kb: float, seed: int):
+        check(lib().msm_thermalize(self.h, float(temp), float(kb), int(seed) & ((1 << 64) - 1)),
+              "msm_thermalize", self.h)
 
     def scale_velocities(self, temp: float, kb: float = CP_KB_EV):
         check(lib().msm_scale_velocities(self.h, temp, kb), "msm_scale_velocities", self.h)
@@ -514,6 +531,37 @@ class DeviceMD:
         x, v = flat(xp), flat(xp1)
         check(lib().msm_put_atoms(self.h, None if x is None else dptr(x),
                                   None if v is None else dptr(v)), "msm_put_atoms", self.h)
+
+    def add_forces(self, fext, on_device: bool = False):
+        """Add FEXT (N, 3) in the caller's order to the device forces (msm_add_forces); with
+        on_device, fext is a device address (an int, e.g. a torch tensor's data_ptr()) of an
+        (N, 3) column-major array."""
+        if on_device:
+            check(lib().msm_add_forces(self.h, C.c_void_p(int(fext)), 1), "msm_add_forces", self.h)
+            return
+        f = np.asarray(fext, dtype=np.float64)
+        if f.shape != (self.n, 3):
+            raise ValueError(f"expected ({self.n}, 3), got {f.shape}")
+        f = np.ascontiguousarray(f.T).ravel()
+        check(lib().msm_add_forces(self.h, dptr(f), 0), "msm_add_forces", self.h)
+
+    def add_epot(self, eext, on_device: bool = False):
+        """Add EEXT (N) in the caller's order to the per-atom energies (msm_add_epot)."""
+        if on_device:
+            check(lib().msm_add_epot(self.h, C.c_void_p(int(eext)), 1), "msm_add_epot", self.h)
+            return
+        e = np.ascontiguousarray(eext, dtype=np.float64)
+        if e.shape != (self.n,):
+            raise ValueError(f"expected ({self.n},), got {e.shape}")
+        check(lib().msm_add_epot(self.h, dptr(e), 0), "msm_add_epot", self.h)
+
+    def put_dis(self, dis):
+        """DIS (N, 3) in the caller's order into the device state (msm_put_dis)."""
+        d = np.asarray(dis, dtype=np.float64)
+        if d.shape != (self.n, 3):
+            raise ValueError(f"expected ({self.n}, 3), got {d.shape}")
+        d = np.ascontiguousarray(d.T).ravel()
+        check(lib().msm_put_dis(self.h, dptr(d)), "msm_put_dis", self.h)
 
     def check_timestep(self, th: float, mxd2: float) -> bool:
         """msm_check_timestep: True when some active atom would move more than sqrt(mxd2) in a
@@ -632,10 +680,35 @@ class DeviceMD:
 
 @dataclass
 class MDForceClassGPU:
-    """MDForceClassGPU (MD_ForceClass_Register_GPU.F90:248-259): the force table and the device
-    context it was initialised on."""
+    """MDForceClassGPU (MD_ForceClass_Register_GPU.F90:248-259): the force table, the device
+    context it was initialised on and the external-force list (ExtForceList, :240-258): tagged
+    (ForceArray, EpotArray) routines called after the force and energy passes."""
     ForceTable: MDForceTable = field(default_factory=MDForceTable)
     Device: Optional[DeviceMD] = None
+    ExtForceList: List[tuple] = field(default_factory=list)
+
+
+def AddExtForce_ForceClass(ForceClass: MDForceClassGPU, Tag: str, ForceArray=None,
+                           EpotArray=None):
+    """AddExtForce_ForceClass (MD_ForceClass_Register_GPU.F90:982-1180): register routines
+    ForceArray(SimBox, CtrlParam, ForceClass) / EpotArray(SimBox, CtrlParam, ForceClass) that add
+    their terms through ForceClass.Device.add_forces / add_epot (the reference's routines write
+    the device arrays themselves).  A tag already present is replaced."""
+    ForceClass.ExtForceList[:] = [e for e in ForceClass.ExtForceList if e[0] != Tag]
+    ForceClass.ExtForceList.append((Tag, ForceArray, EpotArray))
+
+
+def Clear_ExtForce(ForceClass: MDForceClassGPU, Tag: Optional[str] = None):
+    """Clear_ExtForce: drop one tagged entry, or all of them."""
+    ForceClass.ExtForceList[:] = ([] if Tag is None else
+                                  [e for e in ForceClass.ExtForceList if e[0] != Tag])
+
+
+def Cumulate_ExtForce(SimBox, CtrlParam, ForceClass: MDForceClassGPU):
+    """Cumulate_ExtForce (:584-632): every registered ForceArray in registration order."""
+    for _, fa, _ in ForceClass.ExtForceList:
+        if fa is not None:
+            fa(SimBox, CtrlParam, ForceClass)
 
 
 def Register_ForceClass(classname: str, ForceClass: MDForceClassGPU):
@@ -684,11 +757,26 @@ def Copyout_NeighboreList_DEV(ForceClass: MDForceClassGPU):
 
 
 def CalForce_ForceClass(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
+    """CalForce_ForceClass (MD_ForceClass_Register_GPU.F90:636-680): the device passes, then the
+    external forces (Cumulate_ExtForce)."""
     ForceClass.Device.calc_force()
+    Cumulate_ExtForce(SimBox, CtrlParam, ForceClass)
+
+
+def CalEDen_ForceClass(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU) -> np.ndarray:
+    """The pCalEDen slot (MD_ForceClass_Register_GPU.F90:254, 405, 416):
+    CALDEN_FS/EAM_Force_Table2A_DEV (MD_FS_ForceTable_GPU.F90:875-930), the density pass alone.
+    Returns F'(rho) per atom (the device's DEN) in the caller's order."""
+    ForceClass.Device.calc_density()
+    return ForceClass.Device.get_den()
 
 
 def CalEpot_ForceClass(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
+    """CalEpot_ForceClass (:778-822): the energy pass, then the external EpotArray routines."""
     ForceClass.Device.calc_epot()
+    for _, _, ea in ForceClass.ExtForceList:
+        if ea is not None:
+            ea(SimBox, CtrlParam, ForceClass)
 
 
 def CalPTensor_ForceClass(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU):
@@ -704,18 +792,35 @@ def Cal_AtomicStress(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU) 
 
 
 def Cal_GlobalT_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU,
-                    kb: float = CP_KB_EV) -> float:
+                    kb: Optional[float] = None) -> float:
     """Cal_GlobalT_DEV (MD_DiffScheme_GPU.F90:1042-1055): 2 <EK> / (3 kB) over all atoms that
-    count (active, no FIXPOS bit)."""
+    count (active, no FIXPOS bit); kB is CtrlParam.KB unless given."""
+    kb = CtrlParam.KB if kb is None else kb
     ek, _ = ForceClass.Device.calc_ekin(kb)
     m = ek >= 0.0
     return 2.0 * float(ek[m].sum()) / float(m.sum()) / (3.0 * kb)
 
 
 def VelScaling_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU, DT: float,
-                   kb: float = CP_KB_EV):
-    """VelScaling_DEV (MD_DiffScheme_GPU.F90:1390-1448): scale each box to temperature DT."""
-    ForceClass.Device.scale_velocities(DT, kb)
+                   kb: Optional[float] = None):
+    """VelScaling_DEV (MD_DiffScheme_GPU.F90:1390-1448): scale each box to temperature DT
+    (kB is CtrlParam.KB unless given)."""
+    ForceClass.Device.scale_velocities(DT, CtrlParam.KB if kb is None else kb)
+
+
+def thermal_seed(SEED: Sequence[int]) -> int:
+    """64-bit key of the device generator from the first two control seeds."""
+    return ((int(SEED[1]) & 0xFFFFFFFF) << 32) | (int(SEED[0]) & 0xFFFFFFFF)
+
+
+def Thermalizing_MC_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU, TI: float,
+                        kb: Optional[float] = None):
+    """Thermalizing_MC_DEV (MD_DiffScheme_GPU.F90:1451-1805): Maxwell velocities at TI on the
+    device (components with FIXVEL/FIXPOS kept, inactive atoms at rest), then each box's
+    mass-weighted mean velocity removed; the generator is keyed by CtrlParam.SEED[0:2] and
+    advances with every call (msm_thermalize)."""
+    ForceClass.Device.thermalize(TI, CtrlParam.KB if kb is None else kb,
+                                 thermal_seed(CtrlParam.SEED))
 
 
 def Do_DynDamp_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU,

@@ -609,6 +609,29 @@ def thermalize(ityp0, cm, temp, kb, seed):
     return v.reshape(3, n).T.copy()
 
 
+def philox4x32_10(ctr, key):
+    c = (C.c_uint * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    k = (C.c_uint * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+    o = (C.c_uint * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return [int(v) for v in o]
+
+
+def thermalize_philox(natoms, ityp, statu, cm, tkb, seed, draw, xp1):
+    """Thermalizing_MC_DEV with Philox4x32-10 uniforms over boxes of natoms consecutive atoms;
+    xp1 (N, 3) is the velocity before the call (kept on FIX components)."""
+    n = int(np.asarray(ityp).shape[0])
+    v = np.ascontiguousarray(np.asarray(xp1, dtype=np.float64).T).ravel().copy()
+    na = np.ascontiguousarray(natoms, dtype=np.int32)
+    it = np.ascontiguousarray(ityp, dtype=np.int32)
+    st = np.ascontiguousarray(statu, dtype=np.int32)
+    c = np.ascontiguousarray(cm, dtype=np.float64)
+    lib().orc_thermalize_philox(n, len(na), _i(na), _i(it), _i(st), _d(c), C.c_double(tkb),
+                                C.c_ulonglong(int(seed) & ((1 << 64) - 1)),
+                                C.c_ulonglong(int(draw)), _d(v))
+    return v.reshape(3, n).T.copy()
+
+
 def tree_sum(a) -> float:
     a = np.ascontiguousarray(a, dtype=np.float64)
     return lib().orc_sum(_d(a), int(a.size))

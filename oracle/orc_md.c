@@ -8,6 +8,7 @@
 #include "orc_md.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -893,5 +894,75 @@ void orc_thermalize_mc(int n, const int* ityp, const double* cm, int ngroup, dou
             vt = vt + (cm[ityp[j]] * v) / wt;
         }
         for (int j = 0; j < n; ++j) xp1[j + (size_t)k * n] = xp1[j + (size_t)k * n] - vt;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* device thermalisation: Philox4x32-10 uniforms, reference transformation                    */
+/* ------------------------------------------------------------------------------------------ */
+
+/* One block of Philox4x32 with 10 rounds: c' = (hi(M1 c2) ^ c1 ^ k0, lo(M1 c2),
+ * hi(M0 c0) ^ c3 ^ k1, lo(M0 c0)), the key bumped by the Weyl constants between rounds. */
+void orc_philox4x32_10(const unsigned* ctr, const unsigned* key, unsigned* out) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static double unit_open0(uint32_t lo, uint32_t hi) {
+    uint64_t m = ((uint64_t)(hi & 0x1FFFFFu) << 32) | (uint64_t)lo;
+    return (double)(m + 1u) * 0x1.0p-53;
+}
+
+/* Thermalizing_MC_KERNEL (MD_DiffScheme_GPU.F90:1605-1651) and the per-box momentum removal of
+ * Thermalizing_MC_DEV (:1781-1800): active atoms' free components V0 sqrt(-ln z1) cos(2 pi z2),
+ * V0 = sqrt(2 T kB / CM), FIXVEL/FIXPOS components kept, inactive atoms zero; then
+ * v -= sum(m v) / sum(m) over each box (all its atoms). */
+void orc_thermalize_philox(int n, int nbox, const int* natoms, const int* ityp, const int* statu,
+                           const double* cm, double tkb, unsigned long long seed,
+                           unsigned long long draw, double* xp1) {
+    const unsigned key[2] = {(unsigned)seed, (unsigned)(seed >> 32)};
+    const int fix[3] = {16 | 2, 32 | 4, 64 | 8};
+    for (int i = 0; i < n; ++i) {
+        int st = statu[i];
+        if ((st & 1) != 1) {
+            xp1[i] = xp1[n + i] = xp1[2 * n + i] = 0.0;
+            continue;
+        }
+        double v0 = sqrt(2.0 * tkb / cm[ityp[i] - 1]);
+        for (int k = 0; k < 3; ++k) {
+            if (st & fix[k]) continue;
+            unsigned c[4] = {(unsigned)i, (unsigned)k, (unsigned)draw, (unsigned)(draw >> 32)};
+            unsigned r[4];
+            orc_philox4x32_10(c, key, r);
+            double z1 = unit_open0(r[0], r[1]), z2 = unit_open0(r[2], r[3]);
+            xp1[k * n + i] = v0 * sqrt(-log(z1)) * cos(6.283185307179586 * z2);
+        }
+    }
+    int off = 0;
+    for (int b = 0; b < nbox; ++b) {
+        double wt = 0.0, vt[3] = {0.0, 0.0, 0.0};
+        for (int i = off; i < off + natoms[b]; ++i) {
+            double m = cm[ityp[i] - 1];
+            wt += m;
+            for (int k = 0; k < 3; ++k) vt[k] += m * xp1[k * n + i];
+        }
+        if (wt > 0.0)
+            for (int i = off; i < off + natoms[b]; ++i)
+                for (int k = 0; k < 3; ++k) xp1[k * n + i] -= vt[k] / wt;
+        off += natoms[b];
     }
 }

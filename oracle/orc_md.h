@@ -163,6 +163,13 @@ int orc_md_run(int n, double* xp, double* xp1, double* fp, double* dis, int* sta
 
 /* ---------------- RNG and Maxwell velocities ---------------- */
 double orc_drand32(int* seed2);
+/* Philox4x32-10 (Salmon et al., SC'11): out = the block of counter ctr[4] under key[2]. */
+void orc_philox4x32_10(const unsigned* ctr, const unsigned* key, unsigned* out);
+/* Thermalizing_MC_DEV (MD_DiffScheme_GPU.F90:1451-1805) fed by Philox4x32-10 uniforms: atoms in
+ * the caller's order, box b holding natoms[b] consecutive atoms; xp1 (N,3) column-major in/out. */
+void orc_thermalize_philox(int n, int nbox, const int* natoms, const int* ityp, const int* statu,
+                           const double* cm, double tkb, unsigned long long seed,
+                           unsigned long long draw, double* xp1);
 /* Thermalize_MC_SimMDBox over all atoms: three passes (x, y, z) over atoms in index order,
  * two DRAND32 draws per component; subtracts the mass-weighted mean velocity. kb in
  * energy/K; cm per group. */

@@ -341,3 +341,85 @@ def test_check_timestep_matches_oracle(orc, two_boxes):
     md.Predictor_DEV(6, boxes, ctrl, fc)
     assert ctrl.H == 1.0
     dev.close()
+
+
+def test_thermalizing_mc_matches_oracle(orc, two_boxes):
+    """Thermalizing_MC_DEV on the device (msm_thermalize) against the oracle's restatement fed by
+    the same Philox4x32-10 stream: two uneven boxes with FIXPOS/FIXVEL and inactive atoms, two
+    draws in a row; then the velocities keep zero box momentum and the caller-order counter makes
+    them independent of the cell order (the same draw on a context whose atoms were sorted from
+    other positions gives the same velocities)."""
+    boxes, ctrl, ft = two_boxes
+    say("thermalizing_mc")
+    ctrl.SEED = (12345, 678, 0, 0)
+    fc, dev = device_for(boxes, ctrl, ft)
+    dev.build_neighbors()
+    x0, v0, _, st0, _ = dev.get_atoms()
+    natoms = [b.NPRT for b in boxes]
+    ityp = np.concatenate([b.ITYP for b in boxes])
+    seed = md.thermal_seed(ctrl.SEED)
+    vo = v0
+    for draw in range(2):
+        md.Thermalizing_MC_DEV(boxes, ctrl, fc, 800.0)
+        _, v, _, st, _ = dev.get_atoms()
+        vo = orc.thermalize_philox(natoms, ityp, st0, boxes[0].CM, 800.0 * KB, seed, draw, vo)
+        scale = float(np.max(np.abs(vo)))
+        assert np.max(np.abs(v - vo)) <= 1e-13 * scale, draw
+        off = 0
+        for b in boxes:
+            s = slice(off, off + b.NPRT)
+            m = b.CM[b.ITYP - 1]
+            p = (m[:, None] * v[s]).sum(axis=0)
+            assert np.max(np.abs(p)) <= 1e-12 * (m[:, None] * np.abs(v[s])).sum()
+            off += b.NPRT
+    # a second context with every box shifted by 1.3 A along x (another cell order): draw 0
+    import copy
+    boxes2 = copy.deepcopy(boxes)
+    for b in boxes2:
+        b.XP = cfg.wrap_into(b.XP + np.array([1.3, 0.0, 0.0]), b.BOXLOW, b.ZL, ctrl.IFPD)
+    fc2, dev2 = device_for(boxes2, ctrl, ft)
+    dev2.build_neighbors()
+    md.Thermalizing_MC_DEV(boxes2, ctrl, fc2, 800.0)
+    _, v2, _, _, _ = dev2.get_atoms()
+    v_first = orc.thermalize_philox(natoms, ityp, st0, boxes[0].CM, 800.0 * KB, seed, 0, v0)
+    assert np.max(np.abs(v2 - v_first)) <= 1e-13 * float(np.max(np.abs(v_first)))
+    fc.Device.close()
+    dev2.close()
+
+
+def test_predictor_damping_window_and_step_search(orc, two_boxes):
+    """Predictor_DEV inside the damping window at a search step (damping, then the halving search
+    from HMX on the damped velocities, then the Swope predictor at the searched step) and outside
+    both (plain predictor at H): positions, velocities and DIS against the oracle's damping,
+    timestep_search and predictor on the same state."""
+    boxes, ctrl, ft = two_boxes
+    say("predictor damping window + step search")
+    it_all = np.concatenate([b.ITYP for b in boxes]) - 1
+    cm = boxes[0].CM
+    for itime, h0 in ((5, 1.0), (8, 0.75)):
+        fc, dev = device_for(boxes, ctrl, ft)
+        dev.build_neighbors()
+        dev.calc_force()
+        x0, v0, f0, st0, d0 = dev.get_atoms()
+        c = md.SimMDCtrl(RU=ctrl.RU, NB_RM=ctrl.NB_RM, NB_MXNBS=ctrl.NB_MXNBS, H=h0, HMX=16.0,
+                         DMX=0.05, IHDUP=-5, IT0=1, DAMPTIME0=3, DAMPTIME1=4)
+        md.Predictor_DEV(itime, boxes, c, fc)
+        x1, v1, _, st1, d1 = dev.get_atoms()
+        damp = 3 <= itime <= 6
+        search = (itime - 1 + 1) % 5 == 0
+        v = orc.damping(v0, f0, st0) if damp else v0
+        h = orc.timestep_search(v, f0, st0, it_all, cm, 16.0, 0.05) if search else h0
+        assert c.H == h
+        if search:
+            assert h < 16.0
+        off = 0
+        for b in boxes:
+            s = slice(off, off + b.NPRT)
+            ox, ov, od, ost = orc.predictor(x0[s], v[s], f0[s], d0[s], st0[s], b.ITYP - 1, cm, h,
+                                            b.BOXLOW, b.ZL, ctrl.IFPD)
+            assert np.max(np.abs(x1[s] - ox)) <= 1e-12
+            assert np.max(np.abs(v1[s] - ov)) <= 1e-13
+            assert np.max(np.abs(d1[s] - od)) <= 1e-12
+            assert np.array_equal(st1[s], ost)
+            off += b.NPRT
+        dev.close()
