@@ -215,6 +215,8 @@ def main():
                     help="list build: 0 automatic, 1 global scan, 2 small bricks, 3 wide bricks")
     ap.add_argument("--force-lanes", type=int, default=None,
                     help="lanes per atom of the LDS-table force pass (0: as --force-path)")
+    ap.add_argument("--chunk-sweep", type=int, default=None,
+                    help="1: the workgroups of an XCD take its atom chunks round-robin")
     ap.add_argument("--brick-lanes", type=int, default=None,
                     help="lanes per home atom of the LDS-brick passes with the table in LDS")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch measurement")
@@ -288,6 +290,9 @@ def main():
                    "msm_set_option")
     if args.force_lanes is not None:
         _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_FORCE_LANES, args.force_lanes),
+                   "msm_set_option")
+    if args.chunk_sweep is not None:
+        _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_CHUNK_SWEEP, args.chunk_sweep),
                    "msm_set_option")
     if args.brick_lanes is not None:
         _lib.check(_lib.lib().msm_set_option(dev.h, _lib.MSM_OPT_BRICK_LANES, args.brick_lanes),

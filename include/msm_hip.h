@@ -323,6 +323,10 @@ int msm_set_option(msm_sim* sim, int option, int value);
 /* Lanes per atom (1, 2, 4) of the LDS-table force pass when it should differ from the density
  * pass's (MSM_OPT_FORCE_PATH); 0 (default) follows MSM_OPT_FORCE_PATH. */
 #define MSM_OPT_FORCE_LANES      14
+/* Chunk order of the persistent LDS-table passes: 0 (default) each workgroup walks a contiguous
+ * range of its XCD's atom chunks; 1 the workgroups of an XCD take its chunks round-robin, so the
+ * chunks in flight on one XCD are adjacent and share that XCD's L2. */
+#define MSM_OPT_CHUNK_SWEEP      15
 /* Read-only: the density/force kernels the current options and list select (MSM_PASS_*). */
 #define MSM_OPT_PASS_KERNEL      13
 #define MSM_PASS_BRICKS      0  /* LDS-staged positions, global table */

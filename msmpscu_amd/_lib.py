@@ -111,6 +111,7 @@ MSM_OPT_KERNEL_TIMING = 2
 MSM_OPT_NLIST_KERNEL = 3
 MSM_OPT_FORCE_PATH = 4
 MSM_OPT_COMPACT_LIST = 5
+MSM_OPT_CHUNK_SWEEP = 15
 MSM_OPT_TABLE_LDS = 6
 MSM_OPT_LT_DENSITY_BLOCK = 7
 MSM_OPT_LT_DENSITY_WGS = 8

