@@ -51,12 +51,21 @@ def test_density_pass_alone(orc, pot_type):
     dev.close()
 
 
+def _hip_runtime():
+    """HIP runtime calls as libmsm_hip.so resolves them: a symbol looked up through the library's
+    handle is searched in the library and then its own dependencies, so this is the runtime the
+    library uses, whatever else the process has mapped (torch carries a second copy of
+    libamdhip64 of its own; memory from one copy is not usable through the other)."""
+    from msmpscu_amd._lib import lib
+    return lib()
+
+
 def test_external_forces_and_energies():
     """A registered ForceArray/EpotArray pair adds its terms after the passes (host arrays), the
     same terms from a device array (allocated with the HIP runtime the library uses) give the
     same result, box energies follow, and clearing the list restores the plain forces."""
     import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
+    hip = _hip_runtime()
     boxes = [cfg.make_bcc_box((6, 6, 6), cfg.W_A, 0.05, 300.0, 5),
              cfg.make_bcc_box((5, 7, 6), cfg.W_A, 0.05, 300.0, 6)]
     ctrl = cfg.w_ctrl(mx=64)

@@ -431,40 +431,49 @@ def test_lbfgsb_matches_oracle(orc):
     """L-BFGS-B quench (Do_LBFGSB_Forsteps_DEV: the unbounded SETULB over device forces, the
     positions written to the device after every call) of a displaced 1024-atom W box with
     FIXPOS, inactive and OUTOFBOX atoms, against the same loop over the oracle's forces on the
-    device's list: same steps, flag and task, positions to 1e-8 A, velocities zero at the end."""
+    device's list.  A 40-step prefix: same steps, flag and task, positions to 1e-8 A.  To
+    convergence: both stop on the same test, at positions within 1e-8 A; the step count is not
+    compared, because the FACTR test near the minimum turns on energy differences of 1e-11 eV
+    (forces perturbed by 1e-13 relative move the oracle's own count between 87 and 107 here)."""
     ctrl = cfg.w_ctrl(mx=64)
     ctrl.LBFGS_PGtol = 1e-8
     box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.05, 300.0, 98)
     box.STATU = mixed_statu(box.NPRT, 99)
     box.STATU[7] |= 65536
     ft = cfg.w_forcetable(box, ctrl)
-    say("lbfgsb")
-    fc, dev = device_for(box, ctrl, ft)
-    dev.build_neighbors()
-    x0, _, _, st0, _ = dev.get_atoms()
-    kv, ind = dev.get_neighbors()
-    ind0 = np.where(ind > 0, ind - 1, 0)
     t = orc.from_mdlib(ft)
     ru2 = float(np.max(ctrl.RU)) ** 2
     it = box.ITYP - 1
-    state = {"x": np.array(x0)}
-
-    def evaluate():
-        f, _, e = orc.force_full(state["x"], it, st0, kv, ind0, box.ZL, ctrl.IFPD, ru2, t)
-        return f, e
-
-    def put(xa):
-        state["x"] = np.array(xa)
-
     n = box.NPRT
     lo3, zl3 = np.tile(box.BOXLOW, (n, 1)), np.tile(box.ZL, (n, 1))
-    osteps, oflag, ox, otask = md.lbfgsb_quench(x0, st0, lo3, zl3, ctrl.IFPD, evaluate, put, 400,
-                                                ctrl.LBFGS_Msave, ctrl.LBFGS_Factr,
-                                                ctrl.LBFGS_PGtol)
-    assert oflag == 0 and otask.startswith("CONVERGENCE"), otask
-    steps, iflag, task = md.Do_LBFGSB_Forsteps_DEV([box], ctrl, fc, 400)
-    assert (steps, iflag, task) == (osteps, oflag, otask)
-    x, v, _, st, _ = dev.get_atoms()
-    assert np.max(np.abs(x - ox)) <= 1e-8
-    assert np.all(v == 0.0) and np.array_equal(st, st0)
-    dev.close()
+    for maxsteps in (40, 400):
+        say(f"lbfgsb maxsteps={maxsteps}")
+        fc, dev = device_for(box, ctrl, ft)
+        dev.build_neighbors()
+        x0, _, _, st0, _ = dev.get_atoms()
+        kv, ind = dev.get_neighbors()
+        ind0 = np.where(ind > 0, ind - 1, 0)
+        state = {"x": np.array(x0)}
+
+        def evaluate():
+            f, _, e = orc.force_full(state["x"], it, st0, kv, ind0, box.ZL, ctrl.IFPD, ru2, t)
+            return f, e
+
+        def put(xa):
+            state["x"] = np.array(xa)
+
+        osteps, oflag, ox, otask = md.lbfgsb_quench(x0, st0, lo3, zl3, ctrl.IFPD, evaluate, put,
+                                                    maxsteps, ctrl.LBFGS_Msave, ctrl.LBFGS_Factr,
+                                                    ctrl.LBFGS_PGtol)
+        steps, iflag, task = md.Do_LBFGSB_Forsteps_DEV([box], ctrl, fc, maxsteps)
+        if maxsteps == 40:
+            assert (osteps, oflag) == (41, 1)
+            assert (steps, iflag, task) == (osteps, oflag, otask)
+        else:
+            assert oflag == 0 and otask.startswith("CONVERGENCE"), otask
+            assert (iflag, task) == (oflag, otask)
+            assert steps <= maxsteps
+        x, v, _, st, _ = dev.get_atoms()
+        assert np.max(np.abs(x - ox)) <= 1e-8
+        assert np.all(v == 0.0) and np.array_equal(st, st0)
+        dev.close()
