@@ -197,6 +197,15 @@ int msm_calc_epot(msm_sim* sim);
  * b = 0..nbox-1; recomputes the density pass and stores the forces as msm_calc_force does.  The
  * reference's COPYOUT_VIRIALTENSOR returns the mean over boxes; callers average these. */
 int msm_calc_ptensor(msm_sim* sim, double* vtensor);
+/* msm_calc_ptensor without any device allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (serving loops, harnesses that forbid hipMalloc).  Same
+ * inputs, outputs, forces and flags as msm_calc_ptensor (the per-box sums are added in another
+ * fixed order, so the tensor agrees to rounding, the forces bit for bit).  Its partial sums live
+ * in a fixed 4 MiB device array per GPU, held from the first msm_create on that GPU until the
+ * process ends: MSM_ERR_ARG when nbox * ceil(largest box's atoms / 256) * 6 + 9 * nbox exceeds
+ * 524288 doubles (one box of up to about 22 million atoms).  Calls on one GPU share the array:
+ * calls that overlap in time on the same GPU (other contexts, other threads) race. */
+int msm_calc_ptensor_noalloc(msm_sim* sim, double* vtensor);
 
 /* Per-atom stress (virial part), the reference's CAL_FS/EAM_AtomicStress_KERNEL through
  * Cal_AtomicStress (MD_FS_ForceTable_GPU.F90:1746-2076, Analysis/AtomicVirialStress_GPU.F90:734-773):

@@ -95,6 +95,7 @@ SIGNATURES = {
     "msm_kernel_times": (C.c_int, [_vp, _dp, C.POINTER(C.c_longlong), C.c_int, C.c_int]),
     "msm_autotune": (C.c_int, [_vp, C.c_int, _ip, _dp]),
     "msm_calc_ptensor": (C.c_int, [_vp, _dp]),
+    "msm_calc_ptensor_noalloc": (C.c_int, [_vp, _dp]),
     "msm_calc_atomic_stress": (C.c_int, [_vp, _dp]),
     "msm_damping": (C.c_int, [_vp]),
     "msm_calc_ekin": (C.c_int, [_vp, C.c_double, _dp, _dp]),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -656,6 +657,33 @@ int run_segment(msm_sim* s, int rebuild, int nsteps, int last, double h) {
     return MSM_OK;
 }
 
+// Workspace of msm_calc_ptensor_noalloc: per-box partial sums of the virial, in a fixed device
+// array (each GPU that loads the library holds its own copy until the process ends) so that the
+// call allocates nothing.  Host code must never pass `scratch` itself to a kernel (on the host it
+// names a stand-in, not device memory): it uses the per-device address taken below.
+constexpr size_t kScratchDoubles = size_t(1) << 19;  // 4 MiB
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchMaxDevices] = {};
+
+// Once per device, with that device current: the device address of `scratch` on it.
+int scratch_setup(int device) {
+    if (device < 0 || device >= kScratchMaxDevices) return MSM_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MSM_OK;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return MSM_ERR_HIP;
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+    return MSM_OK;
+}
+
+double* scratch_for(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < kScratchMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+
 }  // namespace
 
 // ============================================================================================
@@ -685,6 +713,8 @@ int msm_create(msm_sim** out, int device) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MSM_ERR_HIP;
     if (device < 0 || device >= ndev) return MSM_ERR_ARG;
     if (hipSetDevice(device) != hipSuccess) return MSM_ERR_HIP;
+    const int src = scratch_setup(device);
+    if (src != MSM_OK) return src;
     msm_sim* s = new msm_sim();
     s->device = device;
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1319,6 +1349,36 @@ int msm_calc_ptensor(msm_sim* s, double* vtensor) {
             vtensor[9 * b + row[c] + 3 * col[c]] = v;
             vtensor[9 * b + col[c] + 3 * row[c]] = v;
         }
+    return read_flags(s);
+}
+
+int msm_calc_ptensor_noalloc(msm_sim* s, double* vtensor) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (!vtensor) return set_err(s, MSM_ERR_ARG, "null argument");
+    (void)hipSetDevice(s->device);
+    double* ws = scratch_for(s->device);
+    if (!ws) return set_err(s, MSM_ERR_STATE, "no workspace address for this device");
+    const int nbox = static_cast<int>(s->boxes.size());
+    int most = 1;
+    for (const BoxDev& b : s->boxes) most = std::max(most, b.natoms);
+    const size_t nchunk = (static_cast<size_t>(most) + msm::kBlock - 1) / msm::kBlock;
+    const size_t need = static_cast<size_t>(nbox) * nchunk * 6 + 9 * static_cast<size_t>(nbox);
+    if (need > kScratchDoubles)
+        return set_err(s, MSM_ERR_ARG,
+                       "msm_calc_ptensor_noalloc: nbox * ceil(largest box's atoms / 256) * 6 + "
+                       "9 * nbox = " + std::to_string(need) + " doubles exceeds its fixed " +
+                       std::to_string(kScratchDoubles) + "-double workspace");
+    double* part = ws;
+    double* out = ws + static_cast<size_t>(nbox) * nchunk * 6;
+    clear_flags(s);
+    enqueue_density(s);
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_virial_boxes(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
+                             s->sb.start, nbox, s->ncell_total, static_cast<int>(nchunk), part,
+                             out, s->stream);
+    HIPCHK(s, hipMemcpyAsync(vtensor, out, 9 * static_cast<size_t>(nbox) * sizeof(double),
+                             hipMemcpyDeviceToHost, s->stream));
     return read_flags(s);
 }
 

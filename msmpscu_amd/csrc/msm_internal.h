@@ -245,6 +245,14 @@ void launch_unsort_list(const int* kvois, const int* indi, int stride, const int
 // Virial tensor per atom (6 components, component-major, stride a.n) and forces.
 void launch_virial(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                    const BoxDev* boxes, int multibox, double* vir, hipStream_t s);
+// Per-box virial tensors without a per-atom buffer: nchunk >= ceil(atoms in any box's cells /
+// kBlock); part holds nbox * nchunk * 6 partial sums, out the 9 * nbox tensor entries
+// (column-major per box, as msm_calc_ptensor returns them).  Forces are written as by
+// launch_virial.
+void launch_virial_boxes(const AtomsDev& a, const ListDev& l, const TabDev& t,
+                         const BoxDev& box0, const BoxDev* boxes, int multibox, const int* start,
+                         int nbox, int ncell_total, int nchunk, double* part, double* out,
+                         hipStream_t s);
 // Thermalisation and quench (MD_DiffScheme_GPU.F90): DAMPING, per-atom EK (-1e32 for atoms that
 // do not count), per-box sum and count of non-negative EK, velocity scaling by the factor of the
 // atom's box, and max |epot - epot0| (then epot0 = epot; *out zero on entry, a double's bits).

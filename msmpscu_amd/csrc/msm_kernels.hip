@@ -998,8 +998,53 @@ void launch_unsort_list(const int* kvois, const int* indi, int stride, const int
 
 // Virial tensor (CALPTENSOR_KERNEL, MD_FS_ForceTable_GPU.F90:998-1272): per active atom i,
 // sum over in-cutoff neighbours of FORTOT/2 * s (x) s (the full list sees each pair twice), in
-// the six distinct components {xx, yy, zz, xy, xz, yz}, stored component-major (stride n); the
-// forces are written as by the force pass.  F' of the density pass is read from pos.w.
+// the six distinct components {xx, yy, zz, xy, xz, yz}; the atom's force is returned beside them,
+// as the force pass computes it.  F' of the density pass is read from pos.w.
+template <int MB, int MT>
+__device__ inline void virial_atom(const AtomsDev& a, const ListDev& l, const TabDev& t,
+                                   const BoxDev& box0, const BoxDev* __restrict__ boxes, int i,
+                                   double (&p)[6], double& fpx, double& fpy, double& fpz) {
+    if ((a.statu[i] & STATU_ACTIVE) != STATU_ACTIVE) return;
+    const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
+    const double4 pi = a.pos[i];
+    const int ti = MT ? a.ityp[i] : 0;
+    const int kv = l.kvois[i];
+    for (int iw = 0; iw < kv; ++iw) {
+        const int j = l.indi[list_base(i, l.mx) + static_cast<size_t>(iw) * l.stride];
+        const double4 pj = a.pos[j];
+        const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
+        const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
+        const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
+        const double r2 = sx * sx + sy * sy + sz * sz;
+        if (!(r2 <= t.ru2max)) continue;
+        int kind = t.kpair[0], kind1 = t.kpair[0];
+        if constexpr (MT) {
+            const int tj = a.ityp[j];
+            kind = t.kpair[ti * t.ngroup + tj];
+            kind1 = t.kpair[tj * t.ngroup + ti];
+        }
+        double inv_r;
+        const double sk = table_coord(r2, t.csi, inv_r);
+        const int kk = min(static_cast<int>(sk), t.ntab2 - 1);
+        const double dk = sk - kk;
+        const double4 f = t.f4[kind * t.ntab2 + kk];
+        const double4 f1 = t.f4[kind1 * t.ntab2 + kk];
+        const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
+                              ((f.z + dk * f.w) * pi.w + (f1.z + dk * f1.w) * pj.w) * inv_r;
+        fpx = fpx + fortot * sx;
+        fpy = fpy + fortot * sy;
+        fpz = fpz + fortot * sz;
+        const double h = fortot * 0.5;
+        p[0] = p[0] + sx * sx * h;
+        p[1] = p[1] + sy * sy * h;
+        p[2] = p[2] + sz * sz * h;
+        p[3] = p[3] + sx * sy * h;
+        p[4] = p[4] + sx * sz * h;
+        p[5] = p[5] + sy * sz * h;
+    }
+}
+
+// Per-atom virial, stored component-major (stride n), and the forces.
 template <int MB, int MT>
 __global__ __launch_bounds__(kBlock) void k_virial(AtomsDev a, ListDev l, TabDev t, BoxDev box0,
                                                    const BoxDev* __restrict__ boxes,
@@ -1008,49 +1053,99 @@ __global__ __launch_bounds__(kBlock) void k_virial(AtomsDev a, ListDev l, TabDev
     if (i >= a.n) return;
     double p[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double fpx = 0.0, fpy = 0.0, fpz = 0.0;
-    if ((a.statu[i] & STATU_ACTIVE) == STATU_ACTIVE) {
-        const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
-        const double4 pi = a.pos[i];
-        const int ti = MT ? a.ityp[i] : 0;
-        const int kv = l.kvois[i];
-        for (int iw = 0; iw < kv; ++iw) {
-            const int j = l.indi[list_base(i, l.mx) + static_cast<size_t>(iw) * l.stride];
-            const double4 pj = a.pos[j];
-            const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
-            const double sy = min_image(pi.y - pj.y, b.len[1], b.ilen[1]);
-            const double sz = min_image(pi.z - pj.z, b.len[2], b.ilen[2]);
-            const double r2 = sx * sx + sy * sy + sz * sz;
-            if (!(r2 <= t.ru2max)) continue;
-            int kind = t.kpair[0], kind1 = t.kpair[0];
-            if constexpr (MT) {
-                const int tj = a.ityp[j];
-                kind = t.kpair[ti * t.ngroup + tj];
-                kind1 = t.kpair[tj * t.ngroup + ti];
-            }
-            double inv_r;
-            const double sk = table_coord(r2, t.csi, inv_r);
-            const int kk = min(static_cast<int>(sk), t.ntab2 - 1);
-            const double dk = sk - kk;
-            const double4 f = t.f4[kind * t.ntab2 + kk];
-            const double4 f1 = t.f4[kind1 * t.ntab2 + kk];
-            const double fortot = (f.x + dk * f.y) * (inv_r * inv_r) +
-                                  ((f.z + dk * f.w) * pi.w + (f1.z + dk * f1.w) * pj.w) * inv_r;
-            fpx = fpx + fortot * sx;
-            fpy = fpy + fortot * sy;
-            fpz = fpz + fortot * sz;
-            const double h = fortot * 0.5;
-            p[0] = p[0] + sx * sx * h;
-            p[1] = p[1] + sy * sy * h;
-            p[2] = p[2] + sz * sz * h;
-            p[3] = p[3] + sx * sy * h;
-            p[4] = p[4] + sx * sz * h;
-            p[5] = p[5] + sy * sz * h;
-        }
-    }
+    virial_atom<MB, MT>(a, l, t, box0, boxes, i, p, fpx, fpy, fpz);
     a.fx[i] = fpx;
     a.fy[i] = fpy;
     a.fz[i] = fpz;
     for (int c = 0; c < 6; ++c) vir[static_cast<size_t>(c) * a.n + i] = p[c];
+}
+
+// The same sums reduced without a per-atom buffer.  Block (x, y < nbox) takes atoms
+// [a0 + 256 x, a0 + 256 x + 256) of box y's cell range [a0, a1) and writes its six partial sums
+// (fixed-shape tree) to part[(y nchunk + x) 6 + c]; blocks with y = nbox write the forces of the
+// atoms after the last cell (the out-of-box and inactive buckets), grid-strided, and no partials.
+// Every atom's force is written exactly as k_virial writes it.
+template <int MB, int MT>
+__global__ __launch_bounds__(kBlock) void k_virial_part(AtomsDev a, ListDev l, TabDev t,
+                                                        BoxDev box0,
+                                                        const BoxDev* __restrict__ boxes,
+                                                        const int* __restrict__ start, int nbox,
+                                                        int ncell_total, int nchunk,
+                                                        double* __restrict__ part) {
+    __shared__ double lds[6][kBlock];
+    const int by = blockIdx.y, tid = threadIdx.x;
+    if (by == nbox) {
+        for (int i = start[ncell_total] + blockIdx.x * kBlock + tid; i < a.n;
+             i += nchunk * kBlock) {
+            double p[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            double fpx = 0.0, fpy = 0.0, fpz = 0.0;
+            virial_atom<MB, MT>(a, l, t, box0, boxes, i, p, fpx, fpy, fpz);
+            a.fx[i] = fpx;
+            a.fy[i] = fpy;
+            a.fz[i] = fpz;
+        }
+        return;
+    }
+    const BoxDev& bx = boxes[by];
+    const int c0 = bx.cell_off, c1 = bx.cell_off + bx.ncell[0] * bx.ncell[1] * bx.ncell[2];
+    const int i = start[c0] + blockIdx.x * kBlock + tid;
+    double p[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < start[c1]) {
+        double fpx = 0.0, fpy = 0.0, fpz = 0.0;
+        virial_atom<MB, MT>(a, l, t, box0, boxes, i, p, fpx, fpy, fpz);
+        a.fx[i] = fpx;
+        a.fy[i] = fpy;
+        a.fz[i] = fpz;
+    }
+    for (int c = 0; c < 6; ++c) lds[c][tid] = p[c];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w)
+            for (int c = 0; c < 6; ++c) lds[c][tid] += lds[c][tid + w];
+        __syncthreads();
+    }
+    if (tid < 6) part[(static_cast<size_t>(by) * nchunk + blockIdx.x) * 6 + tid] = lds[tid][0];
+}
+
+// One workgroup per box: the box's nchunk partials in fixed order (strided sums, then a tree),
+// written as the symmetric 3x3 tensor, column-major, at out[9 b].
+__global__ __launch_bounds__(kBlock) void k_virial_reduce(const double* __restrict__ part,
+                                                          int nchunk, double* __restrict__ out) {
+    __shared__ double lds[6][kBlock];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    double p[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = tid; k < nchunk; k += kBlock)
+        for (int c = 0; c < 6; ++c) p[c] += part[(static_cast<size_t>(b) * nchunk + k) * 6 + c];
+    for (int c = 0; c < 6; ++c) lds[c][tid] = p[c];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w)
+            for (int c = 0; c < 6; ++c) lds[c][tid] += lds[c][tid + w];
+        __syncthreads();
+    }
+    if (tid < 6) {
+        const int row[6] = {0, 1, 2, 0, 0, 1}, col[6] = {0, 1, 2, 1, 2, 2};
+        const double v = lds[tid][0];
+        out[9 * b + row[tid] + 3 * col[tid]] = v;
+        out[9 * b + col[tid] + 3 * row[tid]] = v;
+    }
+}
+
+void launch_virial_boxes(const AtomsDev& a, const ListDev& l, const TabDev& t,
+                         const BoxDev& box0, const BoxDev* boxes, int multibox, const int* start,
+                         int nbox, int ncell_total, int nchunk, double* part, double* out,
+                         hipStream_t s) {
+    if (a.n <= 0 || nbox <= 0 || nchunk <= 0) return;
+    const dim3 g(nchunk, nbox + 1), bl(kBlock);
+    const bool mt = t.ngroup > 1;
+    if (multibox) {
+        if (mt) hipLaunchKernelGGL((k_virial_part<1, 1>), g, bl, 0, s, a, l, t, box0, boxes, start, nbox, ncell_total, nchunk, part);
+        else    hipLaunchKernelGGL((k_virial_part<1, 0>), g, bl, 0, s, a, l, t, box0, boxes, start, nbox, ncell_total, nchunk, part);
+    } else {
+        if (mt) hipLaunchKernelGGL((k_virial_part<0, 1>), g, bl, 0, s, a, l, t, box0, boxes, start, nbox, ncell_total, nchunk, part);
+        else    hipLaunchKernelGGL((k_virial_part<0, 0>), g, bl, 0, s, a, l, t, box0, boxes, start, nbox, ncell_total, nchunk, part);
+    }
+    hipLaunchKernelGGL(k_virial_reduce, dim3(nbox), bl, 0, s, part, nchunk, out);
 }
 
 void launch_virial(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
