@@ -2132,11 +2132,10 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     s->nlist_kind = nl_choice;
     if (rc == MSM_OK) rc = replan_bricks(s);
     if (rc == MSM_OK) enqueue_rebuild(s);
-    // staged-brick passes (slot list of the brick list kernels)
+    // staged-brick passes (slot list of the brick list kernels; their workgroup shape is fixed,
+    // the LDS-table shape fields are not used)
     if (brick_list && s->nlist_kind != 1)
-        for (const Cand& c : {Cand{5, 1, 512, 3, 512, 2, 1}, Cand{5, 1, 512, 2, 512, 2, 1},
-                              Cand{5, 1, 256, 4, 256, 3, 1}, Cand{5, 1, 1024, 1, 1024, 1, 1},
-                              Cand{5, 1, 256, 6, 256, 4, 1}}) {
+        for (const Cand& c : {Cand{5, 1, 512, 2, 512, 2, 1}}) {
             s->lt = msm::LtShape{c.bd, c.wd, c.bf, c.wf};
             s->force_path = 5;
             s->table_lds = 1;

@@ -1,26 +1,31 @@
-// msm_stage.hip -- density and force passes over LDS-staged bricks of cells, with the neighbour
-// records of a brick staged once, coalesced, and every pair read from LDS.
+// msm_stage.hip -- density and force passes over LDS-staged bricks of cells: the neighbour
+// records of a brick are staged once, with row-contiguous loads, and every pair reads LDS.
 //
-// The global-list passes gather a 24/32-byte neighbour record from the L1/L2 for every pair; a
+// The global-list passes gather a 32-byte neighbour record from the L1/L2 for every pair; a
 // wave's gather touches up to 64 distinct cache lines, so those passes are bound by the vector
-// memory pipeline's line rate, not by HBM.  Here a workgroup takes one brick of at most 4^3 cells
-// at a time (the bricks of the list build, whose per-pair halo slots `sidx` it reads), copies the
-// brick's halo -- the brick plus one cell layer -- into LDS with row-contiguous loads, and then
-// every home atom walks its list as 2-byte halo slots: one coalesced slot load per pair and LDS
-// reads for the neighbour, so the per-pair global traffic is the slot stream alone.
+// memory pipeline's line rate, not by HBM.  Here one workgroup takes one brick of at most 4^3
+// cells (the bricks of the list build, whose per-pair halo slots `sidx` it reads), copies the
+// brick's halo -- the brick plus one cell layer -- into LDS, and then the home atoms walk their
+// lists as 2-byte halo slots: one coalesced slot load per pair, the neighbour from LDS.
 //
 // Periodic images are resolved once per staged atom instead of once per pair: a staged record
 // holds the image of the atom nearest the centre of its halo cell, x - L*rint((x - c)/L) (one
-// fma, exact for the image 0), which for every listed pair is the image the list build chose, so
-// the pair separation is a plain difference.  For pairs whose image is 0 on both sides this is
-// bit for bit the minimum-image difference of the other passes; across a periodic face the two
-// roundings of (xi + kL) - (xj + k'L) replace the one of (xi - xj) - kL, a difference of an ulp
-// of L in the separation.
+// fma, exact for the image 0), which for every listed pair is the image the list build chose.
+// (A fixed per-cell shift would not do: the predictor wraps an atom that crosses a periodic face
+// between rebuilds, and the atom keeps its cell slot until the next rebuild.)  The pair
+// separation is then a plain difference.  For pairs whose image is 0 on both sides this is bit
+// for bit the minimum-image difference of the other passes; across a periodic face the two
+// roundings of (xi + kL) - (xj + k'L) replace the one of (xi - xj) - kL, an ulp of L apart.
 //
-// The persistent grid sweeps the bricks XCD by XCD: the workgroups of one XCD take interleaved
-// bricks of that XCD's contiguous share, so the bricks in flight on an XCD are neighbours whose
-// halos overlap in that XCD's L2.  Brick halos larger than the staging capacity (or bricks of a
-// list without halo slots) take the global-list path with the minimum image per pair.
+// Lanes: a 512-thread workgroup serves 256 home atoms per round with two lanes each (lane s of an
+// atom takes list slots s, s+2, ...).  The home atoms left after the full rounds take one more
+// round with 2, 4, 8 or 16 lanes each, so that a brick a little larger than 256 atoms does not pay
+// a second full-length round.  The lanes of an atom are adjacent; their sums meet by xor shuffles.
+//
+// The tables: the knots a solid's pairs use (the upper part of each kind's table) sit in LDS as a
+// window; the rare closer pair reads the global table.  Density knots are the {POTB, dPOTB} pairs
+// of the global table; force knots are {FPOTR, FPOTB}, with the slope taken from the next knot
+// (the same subtraction the host makes for the global table).
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -32,20 +37,435 @@ namespace msm {
 
 namespace {
 
-constexpr int kSbMaxBlock = 1024;
-// slots loaded ahead per chunk of an atom's list (a lane issues them all before its first pair);
-// the force pass holds more per pair, so its chunk is shorter (128 VGPRs at 1024 threads)
-constexpr int kSbChunk = 32;
-constexpr int kSbChunkF = 16;
-// pairs per arithmetic group
-constexpr int kSbGroupD = 4;
-constexpr int kSbGroupF = 4;
+constexpr int kSbBlock = 512;        // threads per workgroup, one brick per workgroup
+constexpr int kSbLanes = 2;          // lanes per home atom in the full rounds
+constexpr int kSbGroup = 4;          // list slots per lane per arithmetic group
 
-// Brick order: XCD x (workgroup ids x, x+8, ...) owns the contiguous brick range
-// [x*nb/8, (x+1)*nb/8) and its workgroups take that range's bricks round-robin, so the bricks in
-// flight on one XCD are adjacent.  With fewer than 8 workgroups every brick is taken by plain
-// striding.  Each brick is visited by exactly one workgroup.
-__device__ __forceinline__ void sweep_range(int nb, int& first, int& step, int& end) {
+struct SbShared {
+    BrickLds L;
+    int hbase[kNlMaxHalo];           // sorted index of halo slot s of cell h: hbase[h] + s
+    double hc[3][kNlMaxHalo];        // centre of halo cell h in the brick's unwrapped frame
+    unsigned char homeh[64];         // halo cell of home cell c
+    int kp[kMaxGroup * kMaxGroup];   // KPAIR (0-based kinds) for several atom types
+};
+
+// Brick of this workgroup, its halo table, the per-halo-cell atom bases and centres, and (when
+// the halo fits cap) the halo cell of every slot.  Returns the number of halo atoms; all tables
+// are valid on return.
+__device__ __forceinline__ int sb_setup(const BoxDev& B, int bi, const int* __restrict__ start,
+                                        const TabDev& t, int cap, BrickInfo& g, SbShared& S,
+                                        unsigned char* smap) {
+    brick_info(B, bi, g);
+    const int total = brick_halo(B, g, start, S.L.hcell, nullptr, S.L.hoff, S.L.homeoff, S.L.scan);
+    for (int k = threadIdx.x; k < g.nh; k += blockDim.x) {
+        S.hbase[k] = S.L.hcell[k] >= 0 ? start[S.L.hcell[k]] - S.L.hoff[k] : 0;
+        const int h3[3] = {k % g.H[0], (k / g.H[0]) % g.H[1], k / (g.H[0] * g.H[1])};
+        for (int d = 0; d < 3; ++d)
+            S.hc[d][k] = B.lo[d] + (static_cast<double>(g.c0[d] - 1 + h3[d]) + 0.5) *
+                                       (B.len[d] / B.ncell[d]);
+        if (total <= cap)
+            for (int sl = S.L.hoff[k]; sl < S.L.hoff[k + 1]; ++sl)
+                smap[sl] = static_cast<unsigned char>(k);
+    }
+    if (threadIdx.x < g.nhc) {
+        const int c = threadIdx.x;
+        const int ix = c % g.e[0], iy = (c / g.e[0]) % g.e[1], iz = c / (g.e[0] * g.e[1]);
+        S.homeh[c] = static_cast<unsigned char>((ix + 1) + g.H[0] * ((iy + 1) + g.H[1] * (iz + 1)));
+    }
+    for (int k = threadIdx.x; k < t.ngroup * t.ngroup; k += blockDim.x) S.kp[k] = t.kpair[k];
+    __syncthreads();  // the tables above are read by every wave
+    return total;
+}
+
+// Image of coordinate v nearest the centre c of its halo cell (periodic directions; ilen = 0
+// leaves free directions unchanged).
+__device__ __forceinline__ double near_image(double v, double c, double len, double ilen) {
+    return fma(-len, rint((v - c) * ilen), v);
+}
+
+// SK = sqrt(sqrt(r2)) * CSI (MD_FS_ForceTable.F90:248-252) and 1/r from hardware reciprocal
+// square roots, each refined by one Newton step (relative error ~1e-16 after it):
+// 1/r = rsq(r2), sqrt(r) = rsq(1/r).
+__device__ __forceinline__ double sb_coord(double r2, double csi, double& inv_r) {
+    double y = __builtin_amdgcn_rsq(r2);
+    y = y * fma(-0.5 * r2 * y, y, 1.5);
+    inv_r = y;
+    double z = __builtin_amdgcn_rsq(y);
+    z = z * fma(-0.5 * y * z, z, 1.5);
+    return z * csi;
+}
+
+// Stage the halo: record s is sorted atom hbase[h] + s of halo cell h, at its image nearest the
+// cell's centre.  WITH_W: also the record's w (F'); without it only x, y, z are read, since the
+// density pass writes w of other atoms meanwhile.  Thread tid takes slots tid, tid + blockDim,
+// ... (consecutive slots are consecutive atoms of a cell row, so the loads coalesce), reads each
+// slot's cell from the slot map, and issues all its loads before its stores.
+template <int WITH_W, int MT>
+__device__ __forceinline__ void sb_stage(const SbShared& S, const unsigned char* smap,
+                                         const AtomsDev& a, const BoxDev& B, int total,
+                                         double4* lp, int* lty) {
+    constexpr int U = 3;
+    for (int s0 = 0; s0 < total; s0 += U * static_cast<int>(blockDim.x)) {
+        double4 r[U];
+        int hh[U], ty[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int sl = s0 + u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+            hh[u] = -1;
+            if (sl >= total) continue;
+            const int h = smap[sl];
+            hh[u] = h;
+            const int q = S.hbase[h] + sl;
+            if constexpr (WITH_W) {
+                r[u] = a.pos[q];
+            } else {
+                const double2 xy = *reinterpret_cast<const double2*>(&a.pos[q]);
+                r[u] = make_double4(xy.x, xy.y, a.pos[q].z, 0.0);
+            }
+            if constexpr (MT) ty[u] = a.ityp[q];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (hh[u] < 0) continue;
+            const int sl = s0 + u * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+            const int h = hh[u];
+            lp[sl] = make_double4(near_image(r[u].x, S.hc[0][h], B.len[0], B.ilen[0]),
+                                  near_image(r[u].y, S.hc[1][h], B.len[1], B.ilen[1]),
+                                  near_image(r[u].z, S.hc[2][h], B.len[2], B.ilen[2]), r[u].w);
+            if constexpr (MT) lty[sl] = ty[u];
+        }
+    }
+}
+
+// Home atom number u of the brick -> halo cell h and offset within that cell.
+__device__ __forceinline__ void sb_home(int u, const BrickInfo& g, const SbShared& S, int& h,
+                                        int& off) {
+    int lo = 0, hi = g.nhc;  // homeoff[lo] <= u < homeoff[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (S.L.homeoff[mid] <= u) lo = mid; else hi = mid;
+    }
+    h = S.homeh[lo];
+    off = u - S.L.homeoff[lo];
+}
+
+template <int L>
+__device__ __forceinline__ double sb_lane_sum(double v) {
+#pragma unroll
+    for (int m = 1; m < L; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// rho_i over the list slots sub, sub + L, ... of one home atom (staged halo).  Per group of
+// kSbGroup slots: the slot loads of the next group are issued first, then every neighbour record,
+// then the arithmetic without branches; pairs below the LDS window are added from the global
+// table after the group (rare in a solid).
+template <int MT, int L>
+__device__ __forceinline__ double sb_density_sum(const double4* lp, const int* lty,
+                                                 const double2* win, const SbShared& S,
+                                                 const TabDev& t, const LtWin& w,
+                                                 const unsigned short* __restrict__ li,
+                                                 int stride, int kv, int sub, int si,
+                                                 const double4& pi, int ti, int kmax) {
+    double den = 0.0;
+    unsigned short nxt[kSbGroup];
+#pragma unroll
+    for (int u = 0; u < kSbGroup; ++u) {
+        const int k = sub + u * L;
+        nxt[u] = k < kv ? li[static_cast<size_t>(k) * stride] : 0;
+    }
+    for (int k0 = sub; k0 < kv; k0 += kSbGroup * L) {
+        int q[kSbGroup];
+#pragma unroll
+        for (int u = 0; u < kSbGroup; ++u) q[u] = k0 + u * L < kv ? nxt[u] : si;
+#pragma unroll
+        for (int u = 0; u < kSbGroup; ++u) {
+            const int k = k0 + (kSbGroup + u) * L;
+            nxt[u] = k < kv ? li[static_cast<size_t>(k) * stride] : 0;
+        }
+        double4 pj[kSbGroup];
+        int row[kSbGroup];
+#pragma unroll
+        for (int u = 0; u < kSbGroup; ++u) {
+            pj[u] = lp[q[u]];
+            row[u] = MT ? S.kp[ti * t.ngroup + lty[q[u]]] : 0;
+        }
+        double sk[kSbGroup];
+        int kk[kSbGroup];
+        unsigned fix = 0;
+#pragma unroll
+        for (int u = 0; u < kSbGroup; ++u) {
+            const double sx = pi.x - pj[u].x, sy = pi.y - pj[u].y, sz = pi.z - pj[u].z;
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            const bool in = k0 + u * L < kv && r2 <= t.ru2max;
+            double inv_r;
+            sk[u] = sb_coord(in ? r2 : 1.0, t.csi, inv_r);
+            kk[u] = min(static_cast<int>(sk[u]), kmax);
+            const int kl = kk[u] - w.k0d;
+            const double2 e = win[row[u] * w.wd + max(kl, 0)];
+            const double v = e.x + (sk[u] - kk[u]) * e.y;
+            den = den + ((in && kl >= 0) ? v : 0.0);
+            if (in && kl < 0) fix |= 1u << u;
+        }
+        if (fix) {
+#pragma unroll
+            for (int u = 0; u < kSbGroup; ++u) {
+                if (!(fix & (1u << u))) continue;
+                const int gk = MT ? row[u] : t.kpair[0];
+                const double2 eg = t.b2[static_cast<size_t>(gk) * t.ntab2 + kk[u]];
+                den = den + (eg.x + (sk[u] - kk[u]) * eg.y);
+            }
+        }
+    }
+    return den;
+}
+
+// One round of home atoms [base, base + nat) with L lanes each (density).
+template <int MT, int EAM, int L>
+__device__ __forceinline__ void sb_density_round(int base, int nat, bool staged,
+                                                 const BrickInfo& g, const SbShared& S,
+                                                 const double4* lp, const int* lty,
+                                                 const double2* win, AtomsDev& a,
+                                                 const ListDev& l, const TabDev& t,
+                                                 const LtWin& w, const BoxDev& B, int kmax,
+                                                 int* err) {
+    const int idx = threadIdx.x / L, sub = threadIdx.x % L;
+    if (idx >= nat) return;
+    int h, off;
+    sb_home(base + idx, g, S, h, off);
+    const int si = S.L.hoff[h] + off;
+    const int p = S.hbase[h] + si;
+    const int ti = MT ? a.ityp[p] : 0;
+    const bool active = (a.statu[p] & STATU_ACTIVE) == STATU_ACTIVE;
+    double den = 0.0;
+    if (active) {
+        const int kv = l.kvois[p];
+        if (staged) {
+            den = sb_density_sum<MT, L>(lp, lty, win, S, t, w, l.sidx + list_base(p, l.mx),
+                                        l.stride, kv, sub, si, lp[si], ti, kmax);
+        } else {
+            // halo too large to stage: neighbour records from global memory
+            const double4 pi = a.pos[p];
+            const int* __restrict__ li = l.indi + list_base(p, l.mx);
+            for (int k = sub; k < kv; k += L) {
+                const int j = li[static_cast<size_t>(k) * l.stride];
+                const double4 pj = a.pos[j];
+                const double sx = min_image(pi.x - pj.x, B.len[0], B.ilen[0]);
+                const double sy = min_image(pi.y - pj.y, B.len[1], B.ilen[1]);
+                const double sz = min_image(pi.z - pj.z, B.len[2], B.ilen[2]);
+                const double r2 = sx * sx + sy * sy + sz * sz;
+                if (!(r2 <= t.ru2max)) continue;
+                const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
+                double inv_r;
+                const double sk = table_coord(r2, t.csi, inv_r);
+                const int kk = min(static_cast<int>(sk), kmax);
+                const double2 e = t.b2[static_cast<size_t>(kind) * t.ntab2 + kk];
+                den = den + (e.x + (sk - kk) * e.y);
+            }
+        }
+    }
+    den = sb_lane_sum<L>(den);
+    if (sub != 0) return;
+    if (active) {
+        if constexpr (!EAM) {
+            if (den > 0.0) den = -0.5 / sqrt(den);
+        } else {
+            if (den > 0.0) {
+                const double sk = den / t.rhod + 1.0;
+                int kk = static_cast<int>(sk + 0.000001);
+                if (kk > t.nembd) {
+                    atomicOr(&err[kErrEmbd], 1);
+                    kk = t.nembd;
+                }
+                const double4 m = t.m4[t.kembd[ti] * t.nembd2 + kk];
+                den = m.z + (sk - kk) * m.w;
+            }
+        }
+    }
+    a.pos[p].w = den;
+}
+
+// F_i over the list slots sub, sub + L, ... of one home atom (staged halo, w = F'); grouped as
+// the density sum.
+template <int MT, int L>
+__device__ __forceinline__ void sb_force_sum(const double4* lp, const int* lty,
+                                             const double2* win, const SbShared& S,
+                                             const TabDev& t, const LtWin& w,
+                                             const unsigned short* __restrict__ li, int stride,
+                                             int kv, int sub, int si, const double4& pi, int ti,
+                                             int kmax, double& fpx, double& fpy, double& fpz) {
+    unsigned short nxt[kSbGroup];
+#pragma unroll
+    for (int u = 0; u < kSbGroup; ++u) {
+        const int k = sub + u * L;
+        nxt[u] = k < kv ? li[static_cast<size_t>(k) * stride] : 0;
+    }
+    for (int k0 = sub; k0 < kv; k0 += kSbGroup * L) {
+        int q[kSbGroup];
+#pragma unroll
+        for (int u = 0; u < kSbGroup; ++u) q[u] = k0 + u * L < kv ? nxt[u] : si;
+#pragma unroll
+        for (int u = 0; u < kSbGroup; ++u) {
+            const int k = k0 + (kSbGroup + u) * L;
+            nxt[u] = k < kv ? li[static_cast<size_t>(k) * stride] : 0;
+        }
+        double4 pj[kSbGroup];
+        int kind[kSbGroup], kind1[kSbGroup];
+#pragma unroll
+        for (int u = 0; u < kSbGroup; ++u) {
+            pj[u] = lp[q[u]];
+            kind[u] = 0;
+            kind1[u] = 0;
+            if constexpr (MT) {
+                const int tj = lty[q[u]];
+                kind[u] = S.kp[ti * t.ngroup + tj];
+                kind1[u] = S.kp[tj * t.ngroup + ti];
+            }
+        }
+        unsigned fix = 0;
+#pragma unroll
+        for (int u = 0; u < kSbGroup; ++u) {
+            const double sx = pi.x - pj[u].x, sy = pi.y - pj[u].y, sz = pi.z - pj[u].z;
+            const double r2 = sx * sx + sy * sy + sz * sz;
+            const bool in = k0 + u * L < kv && r2 <= t.ru2max;
+            double inv_r;
+            const double sk = sb_coord(in ? r2 : 1.0, t.csi, inv_r);
+            const int kk = min(static_cast<int>(sk), kmax);
+            const double dk = sk - kk;
+            const int kl = kk - w.k0f;
+            const int qq = max(kl, 0);
+            const double2 t0 = win[kind[u] * w.wf + qq], t1 = win[kind[u] * w.wf + qq + 1];
+            const double fb = t0.y + dk * (t1.y - t0.y);
+            double fb1 = fb;
+            if constexpr (MT) {
+                const double b0 = win[kind1[u] * w.wf + qq].y;
+                const double b1 = win[kind1[u] * w.wf + qq + 1].y;
+                fb1 = b0 + dk * (b1 - b0);
+            }
+            // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r
+            const double ft = (t0.x + dk * (t1.x - t0.x)) * (inv_r * inv_r) +
+                              (fb * pi.w + fb1 * pj[u].w) * inv_r;
+            const double fortot = (in && kl >= 0) ? ft : 0.0;
+            fpx = fpx + fortot * sx;
+            fpy = fpy + fortot * sy;
+            fpz = fpz + fortot * sz;
+            if (in && kl < 0) fix |= 1u << u;
+        }
+        if (fix) {
+            // below the LDS window: the global table
+#pragma unroll
+            for (int u = 0; u < kSbGroup; ++u) {
+                if (!(fix & (1u << u))) continue;
+                const double sx = pi.x - pj[u].x, sy = pi.y - pj[u].y, sz = pi.z - pj[u].z;
+                const double r2 = sx * sx + sy * sy + sz * sz;
+                double inv_r;
+                const double sk = sb_coord(r2, t.csi, inv_r);
+                const int kk = min(static_cast<int>(sk), kmax);
+                const double dk = sk - kk;
+                const int gk = MT ? kind[u] : t.kpair[0], gk1 = MT ? kind1[u] : t.kpair[0];
+                const double4 f = t.f4[static_cast<size_t>(gk) * t.ntab2 + kk];
+                const double4 f1 = t.f4[static_cast<size_t>(gk1) * t.ntab2 + kk];
+                const double fortot =
+                    (f.x + dk * f.y) * (inv_r * inv_r) +
+                    ((f.z + dk * f.w) * pi.w + (f1.z + dk * f1.w) * pj[u].w) * inv_r;
+                fpx = fpx + fortot * sx;
+                fpy = fpy + fortot * sy;
+                fpz = fpz + fortot * sz;
+            }
+        }
+    }
+}
+
+// One round of home atoms [base, base + nat) with L lanes each (force, then the fused
+// integrator of MODE on the atom's first lane).
+template <int MT, int MODE, int L>
+__device__ __forceinline__ void sb_force_round(int base, int nat, bool staged, const BrickInfo& g,
+                                               const SbShared& S, const double4* lp,
+                                               const int* lty, const double2* win, AtomsDev& a,
+                                               const ListDev& l, const TabDev& t, const LtWin& w,
+                                               const BoxDev& B, const StepParams& sp, int kmax) {
+    const int idx = threadIdx.x / L, sub = threadIdx.x % L;
+    if (idx >= nat) return;
+    int h, off;
+    sb_home(base + idx, g, S, h, off);
+    const int si = S.L.hoff[h] + off;
+    const int p = S.hbase[h] + si;
+    const int st = a.statu[p];
+    const bool active = (st & STATU_ACTIVE) == STATU_ACTIVE;
+    const int ti = MT ? a.ityp[p] : 0;
+    double fpx = 0.0, fpy = 0.0, fpz = 0.0;
+    if (active) {
+        const int kv = l.kvois[p];
+        if (staged) {
+            sb_force_sum<MT, L>(lp, lty, win, S, t, w, l.sidx + list_base(p, l.mx), l.stride,
+                                kv, sub, si, lp[si], ti, kmax, fpx, fpy, fpz);
+        } else {
+            const double4 pi = a.pos[p];
+            const int* __restrict__ li = l.indi + list_base(p, l.mx);
+            for (int k = sub; k < kv; k += L) {
+                const int j = li[static_cast<size_t>(k) * l.stride];
+                const double4 pj = a.pos[j];
+                const double sx = min_image(pi.x - pj.x, B.len[0], B.ilen[0]);
+                const double sy = min_image(pi.y - pj.y, B.len[1], B.ilen[1]);
+                const double sz = min_image(pi.z - pj.z, B.len[2], B.ilen[2]);
+                const double r2 = sx * sx + sy * sy + sz * sz;
+                if (!(r2 <= t.ru2max)) continue;
+                int kind = t.kpair[0], kind1 = t.kpair[0];
+                if constexpr (MT) {
+                    const int tj = a.ityp[j];
+                    kind = t.kpair[ti * t.ngroup + tj];
+                    kind1 = t.kpair[tj * t.ngroup + ti];
+                }
+                double inv_r;
+                const double sk = table_coord(r2, t.csi, inv_r);
+                const int kk = min(static_cast<int>(sk), kmax);
+                const double dk = sk - kk;
+                const double4 f = t.f4[static_cast<size_t>(kind) * t.ntab2 + kk];
+                const double4 f1 = t.f4[static_cast<size_t>(kind1) * t.ntab2 + kk];
+                const double fortot =
+                    (f.x + dk * f.y) * (inv_r * inv_r) +
+                    ((f.z + dk * f.w) * pi.w + (f1.z + dk * f1.w) * pj.w) * inv_r;
+                fpx = fpx + fortot * sx;
+                fpy = fpy + fortot * sy;
+                fpz = fpz + fortot * sz;
+            }
+        }
+    }
+    fpx = sb_lane_sum<L>(fpx);
+    fpy = sb_lane_sum<L>(fpy);
+    fpz = sb_lane_sum<L>(fpz);
+    if (sub != 0) return;
+    if (sp.write_f) {
+        a.fx[p] = fpx;
+        a.fy[p] = fpy;
+        a.fz[p] = fpz;
+    }
+    if constexpr (MODE >= 1) {
+        const double4 pg = a.pos[p];  // unshifted: the integrator's position
+        integrate_atom<MODE>(a, t, B, sp, p, st, active, pg, ti, fpx, fpy, fpz);
+    }
+}
+
+// Home atoms of a brick in rounds: full rounds of kSbBlock / kSbLanes atoms, then the rest with
+// the most lanes per atom (up to 16) that one round can give them.
+template <typename F2, typename F4, typename F8, typename F16>
+__device__ __forceinline__ void sb_rounds(int nhome, F2&& r2, F4&& r4, F8&& r8, F16&& r16) {
+    constexpr int full = kSbBlock / kSbLanes;
+    int base = 0;
+    for (; nhome - base > full / 2; base += full) r2(base, min(full, nhome - base));
+    const int m = nhome - base;
+    if (m <= 0) return;
+    if (m <= kSbBlock / 16) r16(base, m);
+    else if (m <= kSbBlock / 8) r8(base, m);
+    else r4(base, m);
+}
+
+// Brick order of the persistent grid: XCD x (workgroup ids x, x+8, ...) owns the contiguous
+// brick range [x*nb/8, (x+1)*nb/8) and its workgroups take that range's bricks round-robin, so
+// the bricks in flight on one XCD are neighbours whose halos overlap in that XCD's L2.  With
+// fewer than 8 workgroups the bricks are taken by plain striding.  Each brick is visited by
+// exactly one workgroup.
+__device__ __forceinline__ void sb_sweep(int nb, int& first, int& step, int& end) {
     const int nblk = gridDim.x;
     if (nblk < 8) {
         first = blockIdx.x;
@@ -60,343 +480,91 @@ __device__ __forceinline__ void sweep_range(int nb, int& first, int& step, int& 
     step = nx;
 }
 
-// Image of coordinate v nearest the centre c of its halo cell (periodic directions; ilen = 0
-// leaves free directions unchanged).
-__device__ __forceinline__ double near_image(double v, double c, double len, double ilen) {
-    return fma(-len, rint((v - c) * ilen), v);
-}
-
-// Centre of halo cell h (brick frame, unwrapped) in direction k.
-__device__ __forceinline__ double halo_centre(const BoxDev& B, const BrickInfo& g, int hk, int k) {
-    return B.lo[k] + (static_cast<double>(g.c0[k] - 1 + hk) + 0.5) * (B.len[k] / B.ncell[k]);
-}
-
-// 1/r and SK = sqrt(r) * CSI from r^2: hardware reciprocal square roots, each refined by one
-// Newton step (relative error ~1e-16 after the step), sqrt(r) = rsqrt(1/r).
-__device__ __forceinline__ double coord_fast(double r2, double csi, double& inv_r) {
-    double y = __builtin_amdgcn_rsq(r2);
-    y = y * fma(-0.5 * r2 * y, y, 1.5);
-    inv_r = y;
-    double z = __builtin_amdgcn_rsq(y);
-    z = z * fma(-0.5 * y * z, z, 1.5);
-    return z * csi;
-}
-
 }  // namespace
 
-// Dynamic LDS: the POTB knot window (nk x wd doubles), then the staged halo as double4 records
-// {x, y, z, -} (one LDS address per pair), then (MT) the types.
+// Dynamic LDS: the density window (nk x wd {POTB, dPOTB} knots), the staged halo as double4
+// records {x, y, z, -} (cap of them), then (MT) the types, then the slot -> halo cell map.
 template <int MT, int EAM>
-__global__ __launch_bounds__(kSbMaxBlock) void k_density_sb(const int* __restrict__ brick_box,
-                                                            int nbricks, AtomsDev a, ListDev l,
-                                                            TabDev t,
-                                                            const BoxDev* __restrict__ boxes,
-                                                            const int* __restrict__ start, LtWin w,
-                                                            int cap, int* err) {
-    extern __shared__ double s_dyn[];
-    __shared__ BrickLds L;
+__global__ __launch_bounds__(kSbBlock, 4) void k_density_sb(const int* __restrict__ brick_box,
+                                                         int nbricks, AtomsDev a, ListDev l,
+                                                         TabDev t,
+                                                         const BoxDev* __restrict__ boxes,
+                                                         const int* __restrict__ start, LtWin w,
+                                                         int cap, int* err) {
+    extern __shared__ double4 s_dyn4[];
+    __shared__ SbShared S;
     const int nk = MT ? t.nkind : 1;
-    double* s_pb = s_dyn;
-    const int nwin = (nk * w.wd + 3) & ~3;  // keep the records 32-byte aligned
-    double4* lp = reinterpret_cast<double4*>(s_dyn + nwin);
+    double2* win = reinterpret_cast<double2*>(s_dyn4);
+    double4* lp = s_dyn4 + (nk * w.wd + 1) / 2;
     int* lty = reinterpret_cast<int*>(lp + cap);
+    unsigned char* smap = reinterpret_cast<unsigned char*>(lty + (MT ? cap : 0));
     for (int k = threadIdx.x; k < nk * w.wd; k += blockDim.x) {
         const int kd = MT ? k / w.wd : t.kpair[0], q = MT ? k % w.wd : k;
-        s_pb[k] = t.b2[static_cast<size_t>(kd) * t.ntab2 + w.k0d + q].x;
+        win[k] = t.b2[static_cast<size_t>(kd) * t.ntab2 + w.k0d + q];
     }
     const int kmax = t.ntab2 - 2;
-    int bi, bstep, bend;
-    sweep_range(nbricks, bi, bstep, bend);
-    for (; bi < bend; bi += bstep) {
-        __syncthreads();  // the previous brick's halo is no longer read
+    const int bi = xcd_block(gridDim.x);
+    if (bi >= nbricks) return;
+    {
         const BoxDev& B = boxes[brick_box[bi]];
         BrickInfo g;
-        brick_info(B, bi, g);
-        const int total = brick_halo(B, g, start, L.hcell, nullptr, L.hoff, L.homeoff, L.scan);
+        const int total = sb_setup(B, bi, start, t, cap, g, S, smap);
         const bool staged = total <= cap && l.scap > 0 && total <= l.scap;
-        if (staged) {
-            for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int h) {
-                // x, y, z only: w (F') of other atoms is being written by this pass
-                const int h0 = h % g.H[0], h1 = (h / g.H[0]) % g.H[1], h2 = h / (g.H[0] * g.H[1]);
-                const double2 xy = *reinterpret_cast<const double2*>(&a.pos[q]);
-                lp[s] = make_double4(
-                    near_image(xy.x, halo_centre(B, g, h0, 0), B.len[0], B.ilen[0]),
-                    near_image(xy.y, halo_centre(B, g, h1, 1), B.len[1], B.ilen[1]),
-                    near_image(a.pos[q].z, halo_centre(B, g, h2, 2), B.len[2], B.ilen[2]), 0.0);
-                if constexpr (MT) lty[s] = a.ityp[q];
-            });
-        }
+        if (staged) sb_stage<0, MT>(S, smap, a, B, total, lp, lty);
         __syncthreads();
-        const int nhome = L.homeoff[g.nhc];
-        for (int ha = threadIdx.x; ha - static_cast<int>(threadIdx.x) < nhome;
-             ha += blockDim.x) {
-            if (ha >= nhome) continue;
-            int h, off;
-            home_atom(ha, g, L.homeoff, h, off);
-            const int p = start[L.hcell[h]] + off;
-            const bool active = (a.statu[p] & STATU_ACTIVE) == STATU_ACTIVE;
-            double den0 = 0.0;
-            const int ti = MT ? a.ityp[p] : 0;
-            if (active) {
-                const int kv = l.kvois[p];
-                if (staged) {
-                    const int si = L.hoff[h] + off;
-                    const double4 pi = lp[si];
-                    const unsigned short* __restrict__ li = l.sidx + list_base(p, l.mx);
-                    for (int c0 = 0; c0 < kv; c0 += kSbChunk) {
-                        // every slot of the chunk is issued before the first pair (the list is
-                        // allocated with a full chunk of slack past MXKVOIS; slots past kv are
-                        // masked below)
-                        int qs[kSbChunk];
-#pragma unroll
-                        for (int u = 0; u < kSbChunk; ++u)
-                            qs[u] = li[static_cast<size_t>(c0 + u) * l.stride];
-                        const int nc = min(kv - c0, kSbChunk);
-#pragma unroll
-                        for (int u0 = 0; u0 < kSbChunk; u0 += kSbGroupD) {
-                            if (u0 >= nc) break;
-                            unsigned fix = 0;
-                            double sk_[kSbGroupD];
-#pragma unroll
-                            for (int u = u0; u < u0 + kSbGroupD; ++u) {
-                                const bool use = u < nc;
-                                const int q = use ? qs[u] : si;
-                                const double4 pj = lp[q];
-                                const double sx = pi.x - pj.x, sy = pi.y - pj.y, sz = pi.z - pj.z;
-                                const double r2 = sx * sx + sy * sy + sz * sz;
-                                const int row = MT ? t.kpair[ti * t.ngroup + lty[q]] * w.wd : 0;
-                                double inv_r;
-                                const double sk = coord_fast(use ? r2 : 1.0, t.csi, inv_r);
-                                sk_[u - u0] = sk;
-                                const int kk = min(static_cast<int>(sk), kmax);
-                                const bool in = use && r2 <= t.ru2max;
-                                const int kl = kk - w.k0d;
-                                if (kl < 0 && in) fix |= 1u << (u - u0);
-                                const int qq = row + max(kl, 0);
-                                const double p0 = s_pb[qq], p1 = s_pb[qq + 1];
-                                const double v = p0 + (sk - kk) * (p1 - p0);
-                                den0 = den0 + ((in && kl >= 0) ? v : 0.0);
-                            }
-                            if (fix) {
-                                // pairs below the LDS window: knots of the global table
-#pragma unroll
-                                for (int u = u0; u < u0 + kSbGroupD; ++u) {
-                                    if (!(fix & (1u << (u - u0)))) continue;
-                                    const int q = qs[u];
-                                    const int kind =
-                                        MT ? t.kpair[ti * t.ngroup + lty[q]] : t.kpair[0];
-                                    const double sk = sk_[u - u0];
-                                    const int kk = min(static_cast<int>(sk), kmax);
-                                    const double2 e = t.b2[static_cast<size_t>(kind) * t.ntab2 + kk];
-                                    den0 = den0 + (e.x + (sk - kk) * e.y);
-                                }
-                            }
-                        }
-                    }
-                } else {
-                    // halo too large to stage: neighbour records from global memory
-                    const double4 pi = a.pos[p];
-                    const int* __restrict__ li = l.indi + list_base(p, l.mx);
-                    for (int iw = 0; iw < kv; ++iw) {
-                        const int j = li[static_cast<size_t>(iw) * l.stride];
-                        const double4 pj = a.pos[j];
-                        const double sx = min_image(pi.x - pj.x, B.len[0], B.ilen[0]);
-                        const double sy = min_image(pi.y - pj.y, B.len[1], B.ilen[1]);
-                        const double sz = min_image(pi.z - pj.z, B.len[2], B.ilen[2]);
-                        const double r2 = sx * sx + sy * sy + sz * sz;
-                        if (!(r2 <= t.ru2max)) continue;
-                        const int kind = MT ? t.kpair[ti * t.ngroup + a.ityp[j]] : t.kpair[0];
-                        double inv_r;
-                        const double sk = coord_fast(r2, t.csi, inv_r);
-                        const int kk = min(static_cast<int>(sk), kmax);
-                        const double2 e = t.b2[static_cast<size_t>(kind) * t.ntab2 + kk];
-                        den0 = den0 + (e.x + (sk - kk) * e.y);
-                    }
-                }
-                if constexpr (!EAM) {
-                    if (den0 > 0.0) den0 = -0.5 / sqrt(den0);
-                } else {
-                    if (den0 > 0.0) {
-                        const double sk = den0 / t.rhod + 1.0;
-                        int kk = static_cast<int>(sk + 0.000001);
-                        if (kk > t.nembd) {
-                            atomicOr(&err[kErrEmbd], 1);
-                            kk = t.nembd;
-                        }
-                        const double4 m = t.m4[t.kembd[ti] * t.nembd2 + kk];
-                        den0 = m.z + (sk - kk) * m.w;
-                    }
-                }
-            }
-            a.pos[p].w = den0;
-        }
+        auto round = [&](auto lanes, int base, int nat) {
+            sb_density_round<MT, EAM, decltype(lanes)::value>(base, nat, staged, g, S, lp, lty, win,
+                                                              a, l, t, w, B, kmax, err);
+        };
+        sb_rounds(
+            S.L.homeoff[g.nhc],
+            [&](int b, int m) { round(std::integral_constant<int, 2>{}, b, m); },
+            [&](int b, int m) { round(std::integral_constant<int, 4>{}, b, m); },
+            [&](int b, int m) { round(std::integral_constant<int, 8>{}, b, m); },
+            [&](int b, int m) { round(std::integral_constant<int, 16>{}, b, m); });
     }
 }
 
-// Dynamic LDS: the {FPOTR, FPOTB} knot window (nk x wf), then the staged halo as double4 records
-// {x, y, z, F'} (cap of them), then (MT) the types.
+// Dynamic LDS: the force window (nk x wf {FPOTR, FPOTB} knots), the staged halo as double4
+// records {x, y, z, F'} (cap of them), then (MT) the types, then the slot -> halo cell map.
 template <int MT, int MODE>
-__global__ __launch_bounds__(kSbMaxBlock) void k_force_sb(const int* __restrict__ brick_box,
-                                                          int nbricks, AtomsDev a, ListDev l,
-                                                          TabDev t,
-                                                          const BoxDev* __restrict__ boxes,
-                                                          const int* __restrict__ start,
-                                                          StepParams sp, LtWin w, int cap) {
-    extern __shared__ double2 s_dyn2[];
-    __shared__ BrickLds L;
+__global__ __launch_bounds__(kSbBlock, 4) void k_force_sb(const int* __restrict__ brick_box,
+                                                       int nbricks, AtomsDev a, ListDev l,
+                                                       TabDev t, const BoxDev* __restrict__ boxes,
+                                                       const int* __restrict__ start,
+                                                       StepParams sp, LtWin w, int cap) {
+    extern __shared__ double4 s_dyn4[];
+    __shared__ SbShared S;
     const int nk = MT ? t.nkind : 1;
-    double2* s_ft = s_dyn2;
-    double4* lp = reinterpret_cast<double4*>(s_dyn2 + ((nk * w.wf + 1) & ~1));
+    double2* win = reinterpret_cast<double2*>(s_dyn4);
+    double4* lp = s_dyn4 + (nk * w.wf + 1) / 2;
     int* lty = reinterpret_cast<int*>(lp + cap);
+    unsigned char* smap = reinterpret_cast<unsigned char*>(lty + (MT ? cap : 0));
     for (int k = threadIdx.x; k < nk * w.wf; k += blockDim.x) {
         const int kd = MT ? k / w.wf : t.kpair[0], q = MT ? k % w.wf : k;
         const double4 f = t.f4[static_cast<size_t>(kd) * t.ntab2 + w.k0f + q];
-        s_ft[k] = make_double2(f.x, f.z);
+        win[k] = make_double2(f.x, f.z);
     }
     const int kmax = t.ntab2 - 2;
-    int bi, bstep, bend;
-    sweep_range(nbricks, bi, bstep, bend);
-    for (; bi < bend; bi += bstep) {
-        __syncthreads();
+    const int bi = xcd_block(gridDim.x);
+    if (bi >= nbricks) return;
+    {
         const BoxDev& B = boxes[brick_box[bi]];
         BrickInfo g;
-        brick_info(B, bi, g);
-        const int total = brick_halo(B, g, start, L.hcell, nullptr, L.hoff, L.homeoff, L.scan);
+        const int total = sb_setup(B, bi, start, t, cap, g, S, smap);
         const bool staged = total <= cap && l.scap > 0 && total <= l.scap;
-        if (staged) {
-            for_halo_slots(g, start, L.hcell, L.hoff, [&](int s, int q, int h) {
-                const int h0 = h % g.H[0], h1 = (h / g.H[0]) % g.H[1], h2 = h / (g.H[0] * g.H[1]);
-                const double4 r = a.pos[q];
-                lp[s] = make_double4(
-                    near_image(r.x, halo_centre(B, g, h0, 0), B.len[0], B.ilen[0]),
-                    near_image(r.y, halo_centre(B, g, h1, 1), B.len[1], B.ilen[1]),
-                    near_image(r.z, halo_centre(B, g, h2, 2), B.len[2], B.ilen[2]), r.w);
-                if constexpr (MT) lty[s] = a.ityp[q];
-            });
-        }
+        if (staged) sb_stage<1, MT>(S, smap, a, B, total, lp, lty);
         __syncthreads();
-        const int nhome = L.homeoff[g.nhc];
-        for (int ha = threadIdx.x; ha - static_cast<int>(threadIdx.x) < nhome;
-             ha += blockDim.x) {
-            if (ha >= nhome) continue;
-            int h, off;
-            home_atom(ha, g, L.homeoff, h, off);
-            const int p = start[L.hcell[h]] + off;
-            const int st = a.statu[p];
-            const bool active = (st & STATU_ACTIVE) == STATU_ACTIVE;
-            const int ti = MT ? a.ityp[p] : 0;
-            double fpx = 0.0, fpy = 0.0, fpz = 0.0;
-            if (active) {
-                const int kv = l.kvois[p];
-                if (staged) {
-                    const int si = L.hoff[h] + off;
-                    const double4 pi = lp[si];
-                    const double denki = pi.w;
-                    const unsigned short* __restrict__ li = l.sidx + list_base(p, l.mx);
-                    for (int c0 = 0; c0 < kv; c0 += kSbChunkF) {
-                        int qs[kSbChunkF];
-#pragma unroll
-                        for (int u = 0; u < kSbChunkF; ++u)
-                            qs[u] = li[static_cast<size_t>(c0 + u) * l.stride];
-                        const int nc = min(kv - c0, kSbChunkF);
-#pragma unroll
-                        for (int u0 = 0; u0 < kSbChunkF; u0 += kSbGroupF) {
-                            if (u0 >= nc) break;
-#pragma unroll
-                            for (int u = u0; u < u0 + kSbGroupF; ++u) {
-                                const bool use = u < nc;
-                                const int q = use ? qs[u] : si;
-                                const double4 pj = lp[q];
-                                const double sx = pi.x - pj.x, sy = pi.y - pj.y, sz = pi.z - pj.z;
-                                const double r2 = sx * sx + sy * sy + sz * sz;
-                                int row = 0, row1 = 0;
-                                if constexpr (MT) {
-                                    const int tj = lty[q];
-                                    row = t.kpair[ti * t.ngroup + tj] * w.wf;
-                                    row1 = t.kpair[tj * t.ngroup + ti] * w.wf;
-                                }
-                                double inv_r;
-                                const double sk = coord_fast(use ? r2 : 1.0, t.csi, inv_r);
-                                const int kk = min(static_cast<int>(sk), kmax);
-                                const double dk = sk - kk;
-                                const bool in = use && r2 <= t.ru2max;
-                                const int kl = kk - w.k0f;
-                                const int qq = max(kl, 0);
-                                const double2 t0 = s_ft[row + qq], t1 = s_ft[row + qq + 1];
-                                const double fb = t0.y + dk * (t1.y - t0.y);
-                                double fb1 = fb;
-                                if constexpr (MT) {
-                                    const double b0 = s_ft[row1 + qq].y, b1 = s_ft[row1 + qq + 1].y;
-                                    fb1 = b0 + dk * (b1 - b0);
-                                }
-                                // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r
-                                double ft = (t0.x + dk * (t1.x - t0.x)) * (inv_r * inv_r) +
-                                            (fb * denki + fb1 * pj.w) * inv_r;
-                                if (kl < 0 && in) {
-                                    // below the LDS window: knots of the global table
-                                    int kind = t.kpair[0], kind1 = t.kpair[0];
-                                    if constexpr (MT) {
-                                        const int tj = lty[q];
-                                        kind = t.kpair[ti * t.ngroup + tj];
-                                        kind1 = t.kpair[tj * t.ngroup + ti];
-                                    }
-                                    const double4 f = t.f4[static_cast<size_t>(kind) * t.ntab2 + kk];
-                                    const double4 f1 =
-                                        t.f4[static_cast<size_t>(kind1) * t.ntab2 + kk];
-                                    ft = (f.x + dk * f.y) * (inv_r * inv_r) +
-                                         ((f.z + dk * f.w) * denki + (f1.z + dk * f1.w) * pj.w) *
-                                             inv_r;
-                                }
-                                const double fortot = in ? ft : 0.0;
-                                fpx = fpx + fortot * sx;
-                                fpy = fpy + fortot * sy;
-                                fpz = fpz + fortot * sz;
-                            }
-                        }
-                    }
-                } else {
-                    const double4 pg = a.pos[p];
-                    const double denki = pg.w;
-                    const int* __restrict__ li = l.indi + list_base(p, l.mx);
-                    for (int iw = 0; iw < kv; ++iw) {
-                        const int j = li[static_cast<size_t>(iw) * l.stride];
-                        const double4 pj = a.pos[j];
-                        const double sx = min_image(pg.x - pj.x, B.len[0], B.ilen[0]);
-                        const double sy = min_image(pg.y - pj.y, B.len[1], B.ilen[1]);
-                        const double sz = min_image(pg.z - pj.z, B.len[2], B.ilen[2]);
-                        const double r2 = sx * sx + sy * sy + sz * sz;
-                        if (!(r2 <= t.ru2max)) continue;
-                        int kind = t.kpair[0], kind1 = t.kpair[0];
-                        if constexpr (MT) {
-                            const int tj = a.ityp[j];
-                            kind = t.kpair[ti * t.ngroup + tj];
-                            kind1 = t.kpair[tj * t.ngroup + ti];
-                        }
-                        double inv_r;
-                        const double sk = coord_fast(r2, t.csi, inv_r);
-                        const int kk = min(static_cast<int>(sk), kmax);
-                        const double dk = sk - kk;
-                        const double4 f = t.f4[static_cast<size_t>(kind) * t.ntab2 + kk];
-                        const double4 f1 = t.f4[static_cast<size_t>(kind1) * t.ntab2 + kk];
-                        const double fortot =
-                            (f.x + dk * f.y) * (inv_r * inv_r) +
-                            ((f.z + dk * f.w) * denki + (f1.z + dk * f1.w) * pj.w) * inv_r;
-                        fpx = fpx + fortot * sx;
-                        fpy = fpy + fortot * sy;
-                        fpz = fpz + fortot * sz;
-                    }
-                }
-            }
-            if (sp.write_f) {
-                a.fx[p] = fpx;
-                a.fy[p] = fpy;
-                a.fz[p] = fpz;
-            }
-            if constexpr (MODE >= 1) {
-                const double4 pg = a.pos[p];  // unshifted: the integrator's position
-                integrate_atom<MODE>(a, t, B, sp, p, st, active, pg, ti, fpx, fpy, fpz);
-            }
-        }
+        auto round = [&](auto lanes, int base, int nat) {
+            sb_force_round<MT, MODE, decltype(lanes)::value>(base, nat, staged, g, S, lp, lty, win, a,
+                                                             l, t, w, B, sp, kmax);
+        };
+        sb_rounds(
+            S.L.homeoff[g.nhc],
+            [&](int b, int m) { round(std::integral_constant<int, 2>{}, b, m); },
+            [&](int b, int m) { round(std::integral_constant<int, 4>{}, b, m); },
+            [&](int b, int m) { round(std::integral_constant<int, 8>{}, b, m); },
+            [&](int b, int m) { round(std::integral_constant<int, 16>{}, b, m); });
     }
 }
 
@@ -428,54 +596,62 @@ const SbLimits& sb_limits() {
     return d;
 }
 
-// Workgroups of kernel k that fit a CU at this block and LDS size (cached), capped by wgs.
-unsigned sb_grid(const void* k, int block, size_t lds, int wgs, int nbricks) {
+// Workgroups of kernel k that fit a CU at this LDS size (cached), times the CUs, at most nb.
+unsigned sb_grid(const void* k, size_t lds, int nb) {
     static std::mutex mu;
-    static std::map<std::tuple<const void*, int, size_t>, int> cache;
+    static std::map<std::pair<const void*, size_t>, int> cache;
     int per_cu = 0;
     {
         std::lock_guard<std::mutex> g(mu);
-        auto it = cache.find({k, block, lds});
+        auto it = cache.find({k, lds});
         if (it != cache.end()) per_cu = it->second;
     }
     if (per_cu == 0) {
         int v = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k, block, lds) != hipSuccess || v <= 0)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k, kSbBlock, lds) != hipSuccess ||
+            v <= 0)
             v = 1;
         per_cu = v;
         std::lock_guard<std::mutex> g(mu);
-        cache[{k, block, lds}] = v;
+        cache[{k, lds}] = v;
     }
-    if (wgs > 0 && wgs < per_cu) per_cu = wgs;
     const long long g = static_cast<long long>(per_cu) * sb_limits().cus;
-    return static_cast<unsigned>(g < nbricks ? g : nbricks);
+    return static_cast<unsigned>(g < nb ? g : nb);
 }
 
+// LDS bytes of one workgroup when wgs of them share a CU, less the static tables.
 long sb_budget(int wgs) {
     const SbLimits& d = sb_limits();
     long b = d.lds_cu / (wgs > 0 ? wgs : 1);
     if (b > d.lds_block) b = d.lds_block;
-    return (b & ~255L) - static_cast<long>(sizeof(BrickLds)) - 512;
+    return (b & ~255L) - static_cast<long>(sizeof(SbShared)) - 512;
+}
+
+// Knot window of `per` bytes per knot and kind in `bytes`: the top of each kind's table.
+int sb_window(long bytes, long per, long nk, int ntab2) {
+    const long w = bytes / (per * nk) - 2;
+    return static_cast<int>(std::max(0L, std::min<long>(ntab2, w)));
 }
 
 }  // namespace
 
 SbPlan sb_plan(const TabDev& t, const LtShape& sh, int scap) {
+    (void)sh;
     SbPlan p{};
     const bool mt = t.ngroup > 1;
     const long nk = mt ? t.nkind : 1;
-    const long ad = sizeof(double4) + (mt ? sizeof(int) : 0);
-    const long af = sizeof(double4) + (mt ? sizeof(int) : 0);
+    const long rec = static_cast<long>(scap) * (sizeof(double4) + (mt ? sizeof(int) : 0) + 1);
     p.cap = scap;
-    const long wd = (sb_budget(sh.wgs_d) - static_cast<long>(scap) * ad) / (8 * nk) - 4;
-    const long wf = (sb_budget(sh.wgs_f) - static_cast<long>(scap) * af) / (16 * nk) - 2;
-    p.w.wd = static_cast<int>(std::max(0L, std::min<long>(t.ntab2, wd)));
-    p.w.wf = static_cast<int>(std::max(0L, std::min<long>(t.ntab2, wf)));
-    p.w.k0d = t.ntab2 - p.w.wd;
-    p.w.k0f = t.ntab2 - p.w.wf;
-    // usable when both windows hold at least a quarter of the table (the pairs of a solid lie
-    // in its top part; the rare closer pair reads the global table)
-    p.ok = scap > 0 && 4 * p.w.wd >= t.ntab2 && 4 * p.w.wf >= t.ntab2;
+    // two workgroups per CU when both windows keep a quarter of the table beside the halo, else one
+    for (int wgs : {2, 1}) {
+        const long room = sb_budget(wgs) - rec;
+        p.w.wd = sb_window(room, sizeof(double2), nk, t.ntab2);
+        p.w.wf = sb_window(room, sizeof(double2), nk, t.ntab2);
+        p.w.k0d = t.ntab2 - p.w.wd;
+        p.w.k0f = t.ntab2 - p.w.wf;
+        p.ok = scap > 0 && 4 * p.w.wd >= t.ntab2 && 4 * p.w.wf >= t.ntab2;
+        if (p.ok) break;
+    }
     return p;
 }
 
@@ -488,14 +664,13 @@ void launch_density_sb(const LtShape& sh, int nbricks, const int* brick_box, con
     const SbPlan pl = sb_plan(t, sh, l.scap);
     const bool mt = t.ngroup > 1, eam = t.pot_type != 0;
     const int nk = mt ? t.nkind : 1;
-    const size_t lds = static_cast<size_t>((nk * pl.w.wd + 3) & ~3) * sizeof(double) +
-                       static_cast<size_t>(pl.cap) * (sizeof(double4) + (mt ? sizeof(int) : 0));
+    const size_t lds = static_cast<size_t>((nk * pl.w.wd + 1) / 2) * sizeof(double4) +
+                       static_cast<size_t>(pl.cap) * (sizeof(double4) + (mt ? sizeof(int) : 0) + 1);
 #define MSM_DSB(MT, E)                                                                         \
     do {                                                                                       \
         auto k = k_density_sb<MT, E>;                                                          \
-        const unsigned g = sb_grid(reinterpret_cast<const void*>(k), sh.block_d, lds,          \
-                                   sh.wgs_d, nbricks);                                         \
-        hipLaunchKernelGGL(k, dim3(g), dim3(sh.block_d), lds, s, brick_box, nbricks, a, l, t,  \
+        const unsigned g = static_cast<unsigned>(nbricks);            \
+        hipLaunchKernelGGL(k, dim3(g), dim3(kSbBlock), lds, s, brick_box, nbricks, a, l, t,    \
                            boxes, start, pl.w, pl.cap, err);                                   \
     } while (0)
     if (mt) { if (eam) MSM_DSB(1, 1); else MSM_DSB(1, 0); }
@@ -512,14 +687,13 @@ void launch_force_sb(const LtShape& sh, int nbricks, const int* brick_box, const
     const SbPlan pl = sb_plan(t, sh, l.scap);
     const bool mt = t.ngroup > 1;
     const int nk = mt ? t.nkind : 1;
-    const size_t lds = static_cast<size_t>((nk * pl.w.wf + 1) & ~1) * sizeof(double2) +
-                       static_cast<size_t>(pl.cap) * (sizeof(double4) + (mt ? sizeof(int) : 0));
+    const size_t lds = static_cast<size_t>((nk * pl.w.wf + 1) / 2) * sizeof(double4) +
+                       static_cast<size_t>(pl.cap) * (sizeof(double4) + (mt ? sizeof(int) : 0) + 1);
 #define MSM_FSB(MT, M)                                                                         \
     do {                                                                                       \
         auto k = k_force_sb<MT, M>;                                                            \
-        const unsigned g = sb_grid(reinterpret_cast<const void*>(k), sh.block_f, lds,          \
-                                   sh.wgs_f, nbricks);                                         \
-        hipLaunchKernelGGL(k, dim3(g), dim3(sh.block_f), lds, s, brick_box, nbricks, a, l, t,  \
+        const unsigned g = static_cast<unsigned>(nbricks);            \
+        hipLaunchKernelGGL(k, dim3(g), dim3(kSbBlock), lds, s, brick_box, nbricks, a, l, t,    \
                            boxes, start, sp, pl.w, pl.cap);                                    \
     } while (0)
 #define MSM_FSBM(MT)                            \
