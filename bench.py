@@ -179,11 +179,15 @@ def run_adjoint(reps: int):
     c0 = fm.model.fit(ft)
     fm.set_targets(adj.perturbed(c0, 0.01, seed=5))
     tm = fm.timing(c0, reps)
+    dt = fm.device_timing(c0, reps)
     fm.close()
-    log(f"adjoint: {tm}")
+    log(f"adjoint: {tm} device: {dt}")
     return {"value": round(tm["grad_per_s"], 2), "unit": "grad/s",
             "cost_vs_forward": round(tm["cost_ratio"], 3),
             "forward_ms": round(1e3 * tm["forward_s"], 3), "grad_ms": round(1e3 * tm["grad_s"], 3),
+            "device_adjoint_vs_forward": round(dt["adjoint_vs_forward"], 3),
+            "device_forward_passes_ms": round(dt["forward_passes_ms"], 4),
+            "device_adjoint_passes_ms": round(dt["adjoint_passes_ms"], 4),
             "workload": "64 W boxes x 1024 atoms (8^3 bcc, sigma 0.1 A, diagonal strain up to "
                         "2%), EAM 5000-pt tables, gradient of mean|dF|^2 + 0.1 mean(dE/atom)^2 "
                         "wrt 3 x 5000 B-spline coefficients, FP64"}
@@ -339,9 +343,10 @@ def main():
         dev.run(args.steps, ctrl.NB_UPTAB, step0, ctrl.H)
         dev.sync()
         te1 = time.perf_counter()
-        ms = (C.c_double * 5)()
-        cnt = (C.c_longlong * 5)()
-        _lib.check(L.msm_kernel_times(dev.h, ms, cnt, 5, 1), "msm_kernel_times")
+        nk = len(_lib.KCLASS)
+        ms = (C.c_double * nk)()
+        cnt = (C.c_longlong * nk)()
+        _lib.check(L.msm_kernel_times(dev.h, ms, cnt, nk, 1), "msm_kernel_times")
         _lib.check(L.msm_set_option(dev.h, _lib.MSM_OPT_KERNEL_TIMING, 0), "msm_set_option")
         for k, name in enumerate(_lib.KCLASS):
             if cnt[k]:

@@ -349,11 +349,14 @@ int msm_set_option(msm_sim* sim, int option, int value);
 int msm_get_option(msm_sim* sim, int option, int* value);
 
 /* Adjoint (reverse mode) of the force and energy evaluation with respect to the table knots,
- * for one atom type (one table kind).  Given dL/dF (dldf, 3*N doubles in the caller's atom order
- * and FP(N,3) layout) and dL/dE per box (dlde, one double per box, E_b = the box sum of
- * msm_get_epot), writes dL/d(knot) for every knot i = 1..NTAB of POTR, FPOTR, POTB, FPOTB and,
- * for EAM tables, i = 1..NEMBD of FEMBD and DFEMBD (same layouts as msm_set_force_table; the
- * embedding arrays may be NULL for FS tables).  Uses the current positions and neighbour list;
+ * for every table kind.  Given dL/dF (dldf, 3*N doubles in the caller's atom order and FP(N,3)
+ * layout) and dL/dE per box (dlde, one double per box, E_b = the box sum of msm_get_epot), writes
+ * dL/d(knot) for every kind k and knot i = 1..NTAB of POTR, FPOTR, POTB, FPOTB, at
+ * [k + (i-1)*NKIND] as msm_set_force_table reads them, and, for EAM tables, for every embedding
+ * kind and knot i = 1..NEMBD of FEMBD and DFEMBD at [k + (i-1)*NKIND1] (may be NULL for FS
+ * tables).  With several atom types a pair (i, j) reaches the kinds KPAIR(ti, tj) (density,
+ * pair force, pair energy, and the FPOTB term of F'_i) and KPAIR(tj, ti) (the FPOTB term of
+ * F'_j); kinds no pair reaches get zeros.  Uses the current positions and neighbour list;
  * recomputes rho and F' itself.  The per-pair terms are summed with LDS atomics, so repeated
  * calls agree to rounding.  Corresponds to the C-ABI item "msm_adjoint(h, dLdF, dLdE,
  * out_grad_tables)" of the hot-path boundary; the reference has no derivative code. */
@@ -379,7 +382,8 @@ int msm_autotune(msm_sim* sim, int reps, int* choice, double* best_ms);
 #define MSM_KCLASS_REBUILD   2  /* whole neighbour-list rebuild (binning, sort, gather, list) */
 #define MSM_KCLASS_PREDICTOR 3
 #define MSM_KCLASS_NLIST     4  /* the list kernel alone (inside the rebuild) */
-#define MSM_KCLASS_COUNT     5
+#define MSM_KCLASS_ADJOINT   5  /* the device passes of msm_adjoint (rho, pair, density, sums) */
+#define MSM_KCLASS_COUNT     6
 /* Accumulated device milliseconds and launch counts per kernel class since the last reset
  * (collected while MSM_OPT_KERNEL_TIMING is on); reset != 0 clears the counters after reading. */
 int msm_kernel_times(msm_sim* sim, double* ms, long long* count, int nclass, int reset);

@@ -123,7 +123,7 @@ MSM_OPT_BRICK_LANES = 12
 MSM_OPT_PASS_KERNEL = 13
 MSM_OPT_FORCE_LANES = 14
 PASS_KERNELS = ["bricks", "global", "lanes", "lane_pair", "lds_table", "brick_table", "staged"]
-KCLASS = ["density", "force", "rebuild", "predictor", "nlist"]
+KCLASS = ["density", "force", "rebuild", "predictor", "nlist", "adjoint"]
 
 _lib = None
 

@@ -126,18 +126,25 @@ class SplineModel:
 
 
 def table_grads(dev, dldf: np.ndarray, dlde: np.ndarray, ntab: int, nembd: int,
-                eam: bool = True) -> Dict[str, np.ndarray]:
-    """msm_adjoint on a DeviceMD: dL/d(knot) of the six tables (embedding ones zero for FS)."""
-    g = {k: np.zeros(ntab) for k in ("POTR", "FPOTR", "POTB", "FPOTB")}
-    g["FEMBD"] = np.zeros(nembd)
-    g["DFEMBD"] = np.zeros(nembd)
+                eam: bool = True, nkind: int = 1, nkind1: int = 1) -> Dict[str, np.ndarray]:
+    """msm_adjoint on a DeviceMD: dL/d(knot) of the six tables (embedding ones zero for FS).
+    With one kind each array has shape (NTAB,) / (NEMBD,); with several, (NKIND, NTAB) and
+    (NKIND1, NEMBD), the shapes of MDForceTable's arrays."""
+    g = {k: np.zeros(nkind * ntab) for k in ("POTR", "FPOTR", "POTB", "FPOTB")}
+    g["FEMBD"] = np.zeros(nkind1 * nembd)
+    g["DFEMBD"] = np.zeros(nkind1 * nembd)
     f = np.ascontiguousarray(np.asarray(dldf, dtype=np.float64).ravel(order="F"))
     e = np.ascontiguousarray(np.asarray(dlde, dtype=np.float64))
     check(lib().msm_adjoint(dev.h, dptr(f), dptr(e), dptr(g["POTR"]), dptr(g["FPOTR"]),
                             dptr(g["POTB"]), dptr(g["FPOTB"]),
                             dptr(g["FEMBD"]) if eam else None,
                             dptr(g["DFEMBD"]) if eam else None), "msm_adjoint", dev.h)
-    return g
+    if nkind == 1 and nkind1 == 1:
+        return g
+    # the C layout is [k + (i-1) * NKIND]
+    out = {k: g[k].reshape(ntab, nkind).T.copy() for k in ("POTR", "FPOTR", "POTB", "FPOTB")}
+    out.update({k: g[k].reshape(nembd, nkind1).T.copy() for k in ("FEMBD", "DFEMBD")})
+    return out
 
 
 class ForceMatch:
@@ -209,6 +216,23 @@ class ForceMatch:
         tg = (time.perf_counter() - t0) / reps
         return {"forward_s": tf, "grad_s": tg, "grad_per_s": 1.0 / tg,
                 "cost_ratio": tg / tf}
+
+    def device_timing(self, c: np.ndarray, reps: int = 20) -> Dict[str, float]:
+        """Device time of the passes alone, from HIP events on the library's stream: per
+        gradient, the adjoint passes (msm_adjoint's kernels) against the forward density and
+        force passes of the same evaluation (the energy pass and all copies excluded)."""
+        from ._lib import MSM_OPT_KERNEL_TIMING
+        self.loss_and_grad(c)
+        self.dev.set_option(MSM_OPT_KERNEL_TIMING, 1)
+        self.dev.kernel_times(reset=True)
+        for _ in range(reps):
+            self.loss_and_grad(c)
+        kt = self.dev.kernel_times(reset=True)
+        self.dev.set_option(MSM_OPT_KERNEL_TIMING, 0)
+        fwd = (kt["density"][0] + kt["force"][0]) / reps
+        adjm = kt["adjoint"][0] / reps
+        return {"forward_passes_ms": fwd, "adjoint_passes_ms": adjm,
+                "adjoint_vs_forward": adjm / fwd if fwd > 0 else float("nan")}
 
     def close(self):
         self.dev.close()

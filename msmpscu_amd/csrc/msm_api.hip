@@ -135,8 +135,8 @@ struct msm_sim {
     // per-class kernel timing (MSM_OPT_KERNEL_TIMING)
     std::vector<hipEvent_t> evpool;
     std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> pending;
-    double kms[MSM_KCLASS_COUNT] = {0, 0, 0, 0, 0};
-    long long kcnt[MSM_KCLASS_COUNT] = {0, 0, 0, 0, 0};
+    double kms[MSM_KCLASS_COUNT] = {};
+    long long kcnt[MSM_KCLASS_COUNT] = {};
 
     std::map<GraphKey, hipGraphExec_t> graphs;
 };
@@ -856,11 +856,13 @@ int msm_set_force_table(msm_sim* s, int pot_type, int ngroup, int nkind, int nta
         HIPCHK(s, hipMemcpy(s->d_m4, hm4.data(), hm4.size() * sizeof(double4), hipMemcpyHostToDevice));
         t.nembd = nembd;
         t.nembd2 = nembd2;
+        t.nkind1 = nkind1;
         t.rhod = rhod;
     } else {
         dfree(&s->d_m4);
         t.nembd = 0;
         t.nembd2 = 0;
+        t.nkind1 = 1;
         t.rhod = 1.0;
     }
     t.b2 = s->d_b2;
@@ -2206,12 +2208,12 @@ int msm_adjoint(msm_sim* s, const double* dldf, const double* dlde, double* g_po
     const bool eam = s->tab.pot_type != 0;
     if (eam && (!g_fembd || !g_dfembd))
         return set_err(s, MSM_ERR_ARG, "EAM tables need the embedding gradient arrays");
-    if (s->tab.ngroup != 1)
-        return set_err(s, MSM_ERR_ARG, "the adjoint supports one atom type (one table kind)");
     (void)hipSetDevice(s->device);
     const int n = s->n, nbox = static_cast<int>(s->boxes.size());
     const int nt = s->tab.ntab2, ne = eam ? s->tab.nembd2 : 0;
-    const size_t nb1 = 3 * static_cast<size_t>(nt), nb2 = nt + 2 * static_cast<size_t>(ne);
+    int nbin1 = 0, nbin2 = 0;
+    msm::adjoint_bins(s->tab, nbin1, nbin2);
+    const size_t nb1 = static_cast<size_t>(nbin1), nb2 = static_cast<size_t>(nbin2);
     const size_t stride = std::max(nb1, nb2);
     const size_t npart = stride * msm::adjoint_workgroups(n);
     if (s->adj_cap_n < static_cast<size_t>(s->n_alloc)) {
@@ -2257,26 +2259,56 @@ int msm_adjoint(msm_sim* s, const double* dldf, const double* dlde, double* g_po
     d.part = s->adj_part;
     d.part_stride = stride;
     AtomsDev a = atoms_view(s, s->scur, s->pcur);
-    msm::launch_adjoint(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
-                        s->adj_dldf, s->set[s->scur].gid, d, s->adj_out, s->adj_out + nb1,
-                        s->stream);
+    {
+        TimedScope ts(s, MSM_KCLASS_ADJOINT);
+        msm::launch_adjoint(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
+                            s->adj_dldf, s->set[s->scur].gid, d, s->adj_out, s->adj_out + nb1,
+                            s->stream);
+    }
     std::vector<double> h(nb1 + nb2);
     HIPCHK(s, hipMemcpyAsync(h.data(), s->adj_out, h.size() * sizeof(double),
                              hipMemcpyDeviceToHost, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    // knot i of a table (row i = 1..NTAB of the caller's layout) is bin i
+    resolve_timing(s);
+    // bin (row r, knot i) -> the caller's [kind + (i-1)*NKIND]; with one atom type the kernels
+    // keep one row, that of KPAIR(1,1) (KEMBD(1) for the embedding tables)
+    const bool mt = s->tab.ngroup > 1;
+    const int nk = s->tab.nkind, nk1 = eam ? s->tab.nkind1 : 1;
+    const int rows = mt ? nk : 1, erows = mt ? nk1 : 1;
     const int ntab = nt - 2;
-    for (int i = 1; i <= ntab; ++i) {
-        g_fpotr[i - 1] = h[i];
-        g_fpotb[i - 1] = h[nt + i];
-        g_potr[i - 1] = h[2 * nt + i];
-        g_potb[i - 1] = h[nb1 + i];
+    for (int k = 0; k < nk; ++k)
+        for (int i = 1; i <= ntab; ++i) {
+            const size_t o = static_cast<size_t>(k) + static_cast<size_t>(i - 1) * nk;
+            g_fpotr[o] = g_fpotb[o] = g_potr[o] = g_potb[o] = 0.0;
+        }
+    const size_t nkt = static_cast<size_t>(rows) * nt;
+    for (int r = 0; r < rows; ++r) {
+        const int k = mt ? r : s->tab.kpair[0];
+        for (int i = 1; i <= ntab; ++i) {
+            const size_t o = static_cast<size_t>(k) + static_cast<size_t>(i - 1) * nk;
+            const size_t b = static_cast<size_t>(r) * nt + i;
+            g_fpotr[o] = h[b];
+            g_fpotb[o] = h[nkt + b];
+            g_potr[o] = h[2 * nkt + b];
+            g_potb[o] = h[nb1 + b];
+        }
     }
     if (eam) {
         const int nembd = ne - 2;
-        for (int i = 1; i <= nembd; ++i) {
-            g_fembd[i - 1] = h[nb1 + nt + i];
-            g_dfembd[i - 1] = h[nb1 + nt + ne + i];
+        const size_t nke = static_cast<size_t>(erows) * ne;
+        for (int k = 0; k < nk1; ++k)
+            for (int i = 1; i <= nembd; ++i) {
+                const size_t o = static_cast<size_t>(k) + static_cast<size_t>(i - 1) * nk1;
+                g_fembd[o] = g_dfembd[o] = 0.0;
+            }
+        for (int r = 0; r < erows; ++r) {
+            const int k = mt ? r : s->tab.kembd[0];
+            for (int i = 1; i <= nembd; ++i) {
+                const size_t o = static_cast<size_t>(k) + static_cast<size_t>(i - 1) * nk1;
+                const size_t b = static_cast<size_t>(r) * ne + i;
+                g_fembd[o] = h[nb1 + nkt + b];
+                g_dfembd[o] = h[nb1 + nkt + nke + b];
+            }
         }
     }
     return MSM_OK;

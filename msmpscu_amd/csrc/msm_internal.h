@@ -51,6 +51,7 @@ struct TabDev {
     const double4* m4;  // embedding {FEMBD, dFEMBD, DFEMBD, dDFEMBD}, nkind1 x (nembd+2)
     int ntab2, nembd2, nembd, ngroup;
     int nkind;          // table kinds (rows of b2/f4/e4)
+    int nkind1;         // embedding kinds (rows of m4; 1 for FS tables)
     double csi, rhod, ru2max;
     int pot_type;
     int kpair[kMaxGroup * kMaxGroup];  // 0-based kinds, [ti*ngroup + tj]
@@ -212,8 +213,10 @@ struct AdjDev {
     size_t part_stride;
 };
 int adjoint_workgroups(int n);
-// out_pairs: 3 x ntab2 knot gradients [FPOTR | FPOTB | POTR]; out_dens: ntab2 + 2 x nembd2
-// [POTB | FEMBD | DFEMBD] (embedding parts for EAM only).
+// Bin counts of the two adjoint passes (doubles): pairs nb1, density nb2.
+void adjoint_bins(const TabDev& t, int& nb1, int& nb2);
+// out_pairs: 3 x nkind x ntab2 knot gradients [FPOTR | FPOTB | POTR], each kind-major; out_dens:
+// nkind x ntab2 + 2 x nkind1 x nembd2 [POTB | FEMBD | DFEMBD] (embedding parts for EAM only).
 void launch_adjoint(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                     const BoxDev* boxes, int multibox, const double* dldf, const int* gid,
                     AdjDev d, double* out_pairs, double* out_dens, hipStream_t s);

@@ -256,13 +256,13 @@ def atomic_stress_full(xp, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table: T
 
 def adjoint_full(xp, ityp0, statu, kvois, indi, ln, ifpd, ru2max, table: Table, dldf, dlde,
                  grads=None):
-    """Reverse mode of force_full for one table kind: adds dL/d(knot) for G = dL/dFP (N,3) and
-    g = dL/dE (E = sum of EPOT) into grads (dict of padded rows; created when None)."""
+    """Reverse mode of force_full: adds dL/d(knot) for G = dL/dFP (N,3) and g = dL/dE (E = sum
+    of EPOT) into grads (dict of padded kind-major tables, the shapes of the table's arrays;
+    created when None)."""
     n = int(np.asarray(ityp0).shape[0])
-    nt, ne = table.ntab + 2, table.nembd + 2
     if grads is None:
-        grads = {k: np.zeros(nt) for k in ("potr", "fpotr", "potb", "fpotb")}
-        grads.update({k: np.zeros(ne) for k in ("fembd", "dfembd")})
+        grads = {k: np.zeros_like(getattr(table, k)) for k in ("potr", "fpotr", "potb", "fpotb")}
+        grads.update({k: np.zeros_like(getattr(table, k)) for k in ("fembd", "dfembd")})
     x = _xp(xp)
     it = np.ascontiguousarray(ityp0, dtype=np.int32)
     st = np.ascontiguousarray(statu, dtype=np.int32)

@@ -574,13 +574,13 @@ int orc_force_full(int n, const double* xp, const int* ityp, const int* statu, c
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* adjoint of the full-list force and energy (one table kind)                                   */
+/* adjoint of the full-list force and energy (every table kind)                                 */
 /* ------------------------------------------------------------------------------------------ */
 
 /* F'(rho) and dF'/drho of the density pass; E_emb(rho) and dE_emb/drho of the energy pass, with
  * the knot (kk) and weight (w = sk - kk) of each EAM lookup. */
-static void emb_terms(const orc_ftable* t, double rho, double* fp, double* dfp, double* de,
-                      int* kf, double* wf, int* ke, double* we) {
+static void emb_terms(const orc_ftable* t, int k1, double rho, double* fp, double* dfp,
+                      double* de, int* kf, double* wf, int* ke, double* we) {
     *kf = -1;
     *ke = -1;
     if (t->pot_type == 0) {
@@ -600,7 +600,6 @@ static void emb_terms(const orc_ftable* t, double rho, double* fp, double* dfp, 
     int kk = (int)(sk + 0.000001);
     if (kk < 0) kk = 0;
     if (kk > t->nembd) kk = t->nembd;
-    int k1 = t->kembd[0];
     *de = (ETAB(t->fembd, k1, kk + 1) - ETAB(t->fembd, k1, kk)) / t->rhod;
     *ke = kk;
     *we = sk - (double)kk;
@@ -620,9 +619,8 @@ int orc_adjoint_full(int n, const double* xp, const int* ityp, const int* statu,
                      double ru2max, const orc_ftable* t, const double* dldf, double dlde,
                      double* g_potr, double* g_fpotr, double* g_potb, double* g_fpotb,
                      double* g_fembd, double* g_dfembd) {
-    (void)ityp;
-    if (t->nkind != 1 || t->ngroup != 1) return ORC_ERR_ARG;
-    if (t->pot_type != 0 && (!g_fembd || !g_dfembd || t->nkind1 != 1)) return ORC_ERR_ARG;
+    int ng = t->ngroup, nt = t->ntab + 2, ne = t->nembd + 2;
+    if (t->pot_type != 0 && (!g_fembd || !g_dfembd)) return ORC_ERR_ARG;
     double hbox[3] = {0.5 * len[0], 0.5 * len[1], 0.5 * len[2]};
     double* rho = (double*)calloc((size_t)n, sizeof(double));
     double* fpr = (double*)calloc((size_t)n, sizeof(double));
@@ -638,10 +636,15 @@ int orc_adjoint_full(int n, const double* xp, const int* ityp, const int* statu,
     for (int k = 0; k < 3; ++k)                                                           \
         s[k] = min_image(xp[i + k * n] - xp[j + k * n], len[k], hbox[k], ifpd[k]);        \
     double r2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+#define BIN(arr, kd, kk, w, dk)                                                           \
+    do {                                                                                  \
+        (arr)[(size_t)(kd) * nt + (kk)] += (w) * (1.0 - (dk));                            \
+        (arr)[(size_t)(kd) * nt + (kk) + 1] += (w) * (dk);                                \
+    } while (0)
     /* G of inactive atoms does not reach the tables (their force is identically zero) */
     for (int i = 0; i < n; ++i)
         for (int k = 0; k < 3; ++k) g[i + k * n] = ACT(i) ? dldf[i + k * n] : 0.0;
-    /* forward density and F' */
+    /* forward density (kind KPAIR(ti, tj)) and F' (embedding kind KEMBD(ti)) */
     for (int i = 0; i < n; ++i) {
         if (!ACT(i)) continue;
         double den0 = 0.0;
@@ -649,39 +652,37 @@ int orc_adjoint_full(int n, const double* xp, const int* ityp, const int* statu,
             int j = indi[i + (size_t)iw * n];
             PAIR_GEOM(i, j)
             if (r2 <= ru2max) {
+                int kd = t->kpair[ityp[i] * ng + ityp[j]];
                 double r = sqrt(r2), sk = sqrt(r) * t->csi;
                 int kk = (int)sk;
-                den0 = den0 + (TAB(t->potb, 0, kk) + (sk - kk) * (TAB(t->potb, 0, kk + 1) - TAB(t->potb, 0, kk)));
+                den0 = den0 + (TAB(t->potb, kd, kk) + (sk - kk) * (TAB(t->potb, kd, kk + 1) - TAB(t->potb, kd, kk)));
             }
         }
         rho[i] = den0;
         double dfp, de, wf, we;
         int kf, ke;
-        emb_terms(t, den0, &fpr[i], &dfp, &de, &kf, &wf, &ke, &we);
+        emb_terms(t, t->kembd[ityp[i]], den0, &fpr[i], &dfp, &de, &kf, &wf, &ke, &we);
     }
-    /* pair terms and lambda */
+    /* pair terms and lambda: F_i's pair term reads FPOTR and FPOTB of KPAIR(ti, tj) (times F'_i)
+     * and FPOTB of KPAIR(tj, ti) (times F'_j); E_i reads POTR of KPAIR(ti, tj) */
     for (int i = 0; i < n; ++i) {
         if (!ACT(i)) continue;
         for (int iw = 0; iw < kvois[i]; ++iw) {
             int j = indi[i + (size_t)iw * n];
             PAIR_GEOM(i, j)
             if (!(r2 <= ru2max)) continue;
+            int kd = t->kpair[ityp[i] * ng + ityp[j]], kd1 = t->kpair[ityp[j] * ng + ityp[i]];
             double r = sqrt(r2), sk = sqrt(r) * t->csi;
             int kk = (int)sk;
             double dk = sk - kk;
-            double bp = TAB(t->fpotb, 0, kk) + dk * (TAB(t->fpotb, 0, kk + 1) - TAB(t->fpotb, 0, kk));
+            double bp = TAB(t->fpotb, kd, kk) + dk * (TAB(t->fpotb, kd, kk + 1) - TAB(t->fpotb, kd, kk));
             double ai = g[i] * s[0] + g[i + n] * s[1] + g[i + 2 * n] * s[2];
             double aj = g[j] * s[0] + g[j + n] * s[1] + g[j + 2 * n] * s[2];
-            double w;
-            w = ai / r2;
-            g_fpotr[kk] += w * (1.0 - dk);
-            g_fpotr[kk + 1] += w * dk;
-            w = ai * (fpr[i] + fpr[j]) / r;
-            g_fpotb[kk] += w * (1.0 - dk);
-            g_fpotb[kk + 1] += w * dk;
-            w = dlde / r;
-            g_potr[kk] += w * (1.0 - dk);
-            g_potr[kk + 1] += w * dk;
+            BIN(g_fpotr, kd, kk, ai / r2, dk);
+            BIN(g_fpotb, kd, kk, ai * fpr[i] / r, dk);
+            BIN(g_fpotb, kd1, kk, ai * fpr[j] / r, dk);
+            BIN(g_potr, kd, kk, dlde / r, dk);
+            /* F'_i enters F_i through FPOTB(KPAIR(ti, tj)) and F_j through the same kind */
             lam[i] += (ai - aj) * bp / r;
         }
     }
@@ -690,13 +691,14 @@ int orc_adjoint_full(int n, const double* xp, const int* ityp, const int* statu,
         if (!ACT(i)) continue;
         double fp, dfp, de, wf = 0.0, we = 0.0;
         int kf, ke;
-        emb_terms(t, rho[i], &fp, &dfp, &de, &kf, &wf, &ke, &we);
+        int k1 = t->kembd[ityp[i]];
+        emb_terms(t, k1, rho[i], &fp, &dfp, &de, &kf, &wf, &ke, &we);
         if (t->pot_type != 0) {
-            g_fembd[ke] += dlde * (1.0 - we);
-            g_fembd[ke + 1] += dlde * we;
+            g_fembd[(size_t)k1 * ne + ke] += dlde * (1.0 - we);
+            g_fembd[(size_t)k1 * ne + ke + 1] += dlde * we;
             if (kf >= 0) {
-                g_dfembd[kf] += lam[i] * (1.0 - wf);
-                g_dfembd[kf + 1] += lam[i] * wf;
+                g_dfembd[(size_t)k1 * ne + kf] += lam[i] * (1.0 - wf);
+                g_dfembd[(size_t)k1 * ne + kf + 1] += lam[i] * wf;
             }
         }
         double mu = lam[i] * dfp + dlde * de;
@@ -704,13 +706,14 @@ int orc_adjoint_full(int n, const double* xp, const int* ityp, const int* statu,
             int j = indi[i + (size_t)iw * n];
             PAIR_GEOM(i, j)
             if (!(r2 <= ru2max)) continue;
+            int kd = t->kpair[ityp[i] * ng + ityp[j]];
             double r = sqrt(r2), sk = sqrt(r) * t->csi;
             int kk = (int)sk;
             double dk = sk - kk;
-            g_potb[kk] += mu * (1.0 - dk);
-            g_potb[kk + 1] += mu * dk;
+            BIN(g_potb, kd, kk, mu, dk);
         }
     }
+#undef BIN
 #undef PAIR_GEOM
 #undef ACT
     free(rho);

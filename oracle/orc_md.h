@@ -129,10 +129,10 @@ int orc_force_full(int n, const double* xp, const int* ityp, const int* statu, c
                    const int* indi, int mxkvois, const double len[3], const int ifpd[3],
                    double ru2max, const orc_ftable* t, double* fp, double* dden, double* epot);
 
-/* Reverse mode of orc_force_full for one table kind: given G = dL/dFP (N,3) and g = dL/dE
- * (E = sum of EPOT over the atoms), accumulates dL/d(knot) into the padded rows
- * g_potr, g_fpotr, g_potb, g_fpotb (ntab+2 each) and, for EAM, g_fembd, g_dfembd (nembd+2
- * each; may be NULL for FS).  Outputs are added to, not overwritten. */
+/* Reverse mode of orc_force_full: given G = dL/dFP (N,3) and g = dL/dE (E = sum of EPOT over the
+ * atoms), accumulates dL/d(knot) into the padded kind-major rows g_potr, g_fpotr, g_potb,
+ * g_fpotb (nkind x (ntab+2)) and, for EAM, g_fembd, g_dfembd (nkind1 x (nembd+2); may be NULL
+ * for FS), every kind that KPAIR and KEMBD select.  Outputs are added to, not overwritten. */
 int orc_adjoint_full(int n, const double* xp, const int* ityp, const int* statu,
                      const int* kvois, const int* indi, const double len[3], const int ifpd[3],
                      double ru2max, const orc_ftable* t, const double* dldf, double dlde,

@@ -39,7 +39,7 @@ def test_oracle_adjoint_matches_central_differences(orc, pot_type):
     grads = orc.adjoint_full(box.XP, it, st, kv, ind, box.ZL, ctrl.IFPD, ru2, t, G, g)
     names = ["potr", "fpotr", "potb", "fpotb"] + (["fembd", "dfembd"] if t.pot_type else [])
     for name in names:
-        gr = grads[name]
+        gr = grads[name][0]
         arr = getattr(t, name)[0]
         hot = np.flatnonzero(np.abs(gr) > 0)
         assert hot.size > 0, name
@@ -59,6 +59,79 @@ def test_oracle_adjoint_matches_central_differences(orc, pot_type):
             arr[k] = v
             fd = (lp - lm) / (2 * h)
             assert abs(fd - gr[k]) <= 1e-6 * max(abs(gr[k]), 1e-3 * scale), (name, k, fd, gr[k])
+
+
+def _alloy(orc, case):
+    """Two atom types: 'asym' -- W lattice with alternating types and four FS kinds with KPAIR
+    [[1, 3], [4, 2]] (density and force channels differ by direction); 'cuni' -- a Cu-Ni slab
+    (free in z) with three TB-SMA kinds and two EAM embedding kinds."""
+    import msmpscu_amd.mdlib as md
+    if case == "asym":
+        box = cfg.make_bcc_box((4, 4, 4), cfg.W_A, 0.12, 300.0, 8)
+        box.NGROUP = 2
+        box.ITYP = (np.arange(box.NPRT) % 2 + 1).astype(np.int32)
+        box.CM = np.array([box.CM[0], box.CM[0] * 0.5])
+        box.PTYPE = np.array([[1, 3], [4, 2]], dtype=np.int32)
+        ctrl = md.SimMDCtrl(RU=np.full((2, 2), 4.4002), NB_RM=np.full((2, 2), 4.7002),
+                            NB_MXNBS=64, NUMFTABR=2000)
+        ft = md.MDForceTable()
+        md.Register_ForceTableProc(1, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne")
+        md.Register_ForceTableProc(2, ft, "msm_pot_ackland_ww_nn", "msm_pot_ackland_ww_ne")
+        md.Register_ForceTableProc(3, ft, "msm_pot_wfs_nn", "msm_pot_ackland_ww_ne")
+        md.Register_ForceTableProc(4, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne")
+        md.Create_Interaction_ForceTable(box, ctrl, ft)
+    else:
+        box = cfg.make_cuni_slab((4, 4, 4), 3.57, 300.0, 5)
+        ctrl = cfg.cuni_ctrl(ntab=2000)
+        ft = cfg.cuni_forcetable(box, ctrl, pot_type="EAM_TYPE")
+    st = box.STATU.copy()
+    st[[3, 50]] = 0
+    t = orc.from_mdlib(ft)
+    it = box.ITYP - 1
+    kv, ind = orc.nlist("full", box.XP, it, st, box.BOXLOW, box.ZL, ctrl.IFPD, ctrl.NB_RM,
+                        ctrl.NB_MXNBS)
+    ru2 = float(np.max(ctrl.RU)) ** 2
+    return box, st, ctrl, t, it, kv, ind, ru2
+
+
+@pytest.mark.parametrize("case", ["asym", "cuni"])
+def test_oracle_adjoint_of_several_kinds_matches_central_differences(orc, case):
+    """Every table kind: knots drawn from all kinds of each table (the asymmetric KPAIR of 'asym'
+    sends F'_i and F'_j through different FPOTB kinds; 'cuni' has two embedding kinds) agree with
+    central differences of the forward model to 1e-6 relative (plus the rounding of the loss over
+    the step), and every kind in use carries gradient."""
+    box, st, ctrl, t, it, kv, ind, ru2 = _alloy(orc, case)
+    rng = np.random.default_rng(17)
+    G = rng.normal(size=(box.NPRT, 3))
+    g = -0.23
+    grads = orc.adjoint_full(box.XP, it, st, kv, ind, box.ZL, ctrl.IFPD, ru2, t, G, g)
+    names = ["potr", "fpotr", "potb", "fpotb"] + (["fembd", "dfembd"] if t.pot_type else [])
+    used = set(int(k) for k in np.unique(t.kpair))
+    for name in names:
+        gr = grads[name]
+        arr = getattr(t, name)
+        assert gr.shape == arr.shape
+        kinds = used if name not in ("fembd", "dfembd") else set(int(k) for k in t.kembd[:2])
+        for kd in kinds:
+            assert np.any(gr[kd] != 0.0), (name, kd)
+        hot = np.argwhere(np.abs(gr) > 0)
+        pick = [tuple(hot[i]) for i in rng.choice(len(hot), size=min(16, len(hot)), replace=False)]
+        scale = np.max(np.abs(gr))
+        l0 = abs(_loss(orc, box, st, ctrl, t, it, kv, ind, ru2, G, g))
+        for kd, k in pick:
+            v = arr[kd, k]
+            # small steps: the piecewise-linear embedding lookup has a kink at every knot of
+            # rho / RHOD, which a larger step in POTB can move an atom's density across; the
+            # bar then allows the rounding of L over the step (8 ulp of |L| / h)
+            h = 1e-6 * max(1.0, abs(v))
+            arr[kd, k] = v + h
+            lp = _loss(orc, box, st, ctrl, t, it, kv, ind, ru2, G, g)
+            arr[kd, k] = v - h
+            lm = _loss(orc, box, st, ctrl, t, it, kv, ind, ru2, G, g)
+            arr[kd, k] = v
+            fd = (lp - lm) / (2 * h)
+            bar = 1e-6 * max(abs(gr[kd, k]), 1e-3 * scale) + 8 * 2.2e-16 * l0 / h
+            assert abs(fd - gr[kd, k]) <= bar, (name, kd, k, fd, gr[kd, k])
 
 
 def test_oracle_adjoint_is_linear_in_the_loss_weights(orc):

@@ -60,10 +60,70 @@ def test_adjoint_matches_oracle(orc, pot_type, nbox):
                                  grads)
         off += b.NPRT
     for name in NAMES[: 6 if eam else 4]:
-        ref = grads[name.lower()][1:-1]
+        ref = grads[name.lower()][0, 1:-1]
         got = gd[name]
         scale = np.max(np.abs(ref))
         assert scale > 0, name
+        assert np.max(np.abs(got - ref)) <= 1e-10 * scale, (name, np.max(np.abs(got - ref)),
+                                                            scale)
+
+
+def _alloy(case):
+    """Two atom types: 'asym' -- W lattice with alternating types, four FS kinds, KPAIR
+    [[1, 3], [4, 2]]; 'cuni' -- a Cu-Ni slab, three TB-SMA kinds and two EAM embedding kinds."""
+    if case == "asym":
+        box = cfg.make_bcc_box((7, 7, 7), cfg.W_A, 0.1, 300.0, 8)
+        box.NGROUP = 2
+        box.ITYP = (np.arange(box.NPRT) % 2 + 1).astype(np.int32)
+        box.CM = np.array([box.CM[0], box.CM[0] * 0.5])
+        box.PTYPE = np.array([[1, 3], [4, 2]], dtype=np.int32)
+        ctrl = md.SimMDCtrl(RU=np.full((2, 2), 4.4002), NB_RM=np.full((2, 2), 4.7002),
+                            NB_MXNBS=64, NUMFTABR=3000)
+        ft = md.MDForceTable()
+        md.Register_ForceTableProc(1, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne")
+        md.Register_ForceTableProc(2, ft, "msm_pot_ackland_ww_nn", "msm_pot_ackland_ww_ne")
+        md.Register_ForceTableProc(3, ft, "msm_pot_wfs_nn", "msm_pot_ackland_ww_ne")
+        md.Register_ForceTableProc(4, ft, "msm_pot_wfs_nn", "msm_pot_wfs_ne")
+        md.Create_Interaction_ForceTable(box, ctrl, ft)
+    else:
+        box = cfg.make_cuni_slab((5, 5, 5), 3.57, 600.0, 5)
+        ctrl = cfg.cuni_ctrl(ntab=3000)
+        ft = cfg.cuni_forcetable(box, ctrl, pot_type="EAM_TYPE")
+    box.STATU[[3, 50]] = 0
+    return box, ctrl, ft
+
+
+@pytest.mark.parametrize("case", ["asym", "cuni"])
+def test_adjoint_of_several_kinds_matches_oracle(orc, case):
+    """Two atom types (asymmetric KPAIR: F'_i and F'_j reach different FPOTB kinds; two EAM
+    embedding kinds): every kind of every table equals the oracle's adjoint to 1e-10 of that
+    table's largest entry, and every kind in use carries gradient."""
+    box, ctrl, ft = _alloy(case)
+    say(f"adjoint of several kinds: {case}")
+    dev = _device([box], ctrl, ft)
+    dev.build_neighbors()
+    rng = np.random.default_rng(12)
+    G = rng.normal(size=(box.NPRT, 3))
+    g = np.array([0.31])
+    eam = ft.PotType.strip() == "EAM_TYPE"
+    t = orc.from_mdlib(ft)
+    gd = adj.table_grads(dev, G, g, ft.NTAB, ft.NEMBD if eam else 1, eam, nkind=t.nkind,
+                         nkind1=max(t.nkind1, 1))
+    dev.close()
+    it = box.ITYP - 1
+    kv, ind = orc.nlist("full", box.XP, it, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
+                        ctrl.NB_RM, ctrl.NB_MXNBS)
+    grads = orc.adjoint_full(box.XP, it, box.STATU, kv, ind, box.ZL, ctrl.IFPD,
+                             float(np.max(ctrl.RU)) ** 2, t, G, g[0])
+    used = set(int(k) for k in np.unique(t.kpair))
+    for name in NAMES[: 6 if eam else 4]:
+        ref = grads[name.lower()][:, 1:-1]
+        got = gd[name]
+        assert got.shape == ref.shape, name
+        kinds = used if name not in ("FEMBD", "DFEMBD") else set(int(k) for k in t.kembd[:2])
+        for kd in kinds:
+            assert np.any(ref[kd] != 0.0), (name, kd)
+        scale = np.max(np.abs(ref))
         assert np.max(np.abs(got - ref)) <= 1e-10 * scale, (name, np.max(np.abs(got - ref)),
                                                             scale)
 
