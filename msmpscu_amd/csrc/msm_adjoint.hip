@@ -279,14 +279,36 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_density(AtomsDev a, ListDev l
     adj_flush<LDSB>(bins, d, nkt + 2 * nke);
 }
 
-// out[k] = sum over workgroups w (in order) of part[w * stride + k], k < nbin.
-__global__ void k_adj_reduce(const double* __restrict__ part, int nwg, size_t stride, int nbin,
-                             double* __restrict__ out) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nbin) return;
+// out[k] = sum over workgroups w of part[w * stride + k], k < nbin, in a fixed order: a
+// workgroup takes kRedBins consecutive bins; its kRedSplit thread rows each sum the partials of
+// every kRedSplit-th workgroup (eight loads in flight per thread), and the rows' sums meet in a
+// fixed-shape tree in LDS.
+constexpr int kRedBins = 64, kRedSplit = 4;
+
+__global__ __launch_bounds__(kRedBins * kRedSplit) void k_adj_reduce(
+    const double* __restrict__ part, int nwg, size_t stride, int nbin, double* __restrict__ out) {
+    __shared__ double acc[kRedSplit][kRedBins];
+    const int kb = threadIdx.x % kRedBins, row = threadIdx.x / kRedBins;
+    const int k = blockIdx.x * kRedBins + kb;
     double s = 0.0;
-    for (int w = 0; w < nwg; ++w) s = s + part[static_cast<size_t>(w) * stride + k];
-    out[k] = s;
+    if (k < nbin) {
+        int w = row;
+        for (; w + 7 * kRedSplit < nwg; w += 8 * kRedSplit) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[static_cast<size_t>(w + u * kRedSplit) * stride + k];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s = s + v[u];
+        }
+        for (; w < nwg; w += kRedSplit) s = s + part[static_cast<size_t>(w) * stride + k];
+    }
+    acc[row][kb] = s;
+    __syncthreads();
+    if (row == 0 && k < nbin) {
+        double t = acc[0][kb];
+        for (int r = 1; r < kRedSplit; ++r) t = t + acc[r][kb];
+        out[k] = t;
+    }
 }
 
 namespace {
@@ -360,9 +382,9 @@ void launch_adjoint(const AtomsDev& a, const ListDev& l, const TabDev& t, const 
     else          { if (eam) MSM_APL(0, 1); else MSM_APL(0, 0); }
 #undef MSM_APL
 #undef MSM_AP
-    const unsigned gr1 = static_cast<unsigned>((nb1 + 255) / 256);
-    hipLaunchKernelGGL(k_adj_reduce, dim3(gr1), dim3(256), 0, s, d.part, nwg, d.part_stride, nb1,
-                       out_pairs);
+    const unsigned gr1 = static_cast<unsigned>((nb1 + kRedBins - 1) / kRedBins);
+    hipLaunchKernelGGL(k_adj_reduce, dim3(gr1), dim3(kRedBins * kRedSplit), 0, s, d.part, nwg,
+                       d.part_stride, nb1, out_pairs);
 #define MSM_AD(MB, E, LB, M)                                                                 \
     hipLaunchKernelGGL((k_adj_density<MB, E, LB, M>), dim3(nwg), dim3(kAdjBlock), s2, s, a, l, \
                        t, box0, boxes, d, nchunks)
@@ -375,9 +397,9 @@ void launch_adjoint(const AtomsDev& a, const ListDev& l, const TabDev& t, const 
     else          { if (eam) MSM_ADL(0, 1); else MSM_ADL(0, 0); }
 #undef MSM_ADL
 #undef MSM_AD
-    const unsigned gr2 = static_cast<unsigned>((nb2 + 255) / 256);
-    hipLaunchKernelGGL(k_adj_reduce, dim3(gr2), dim3(256), 0, s, d.part, nwg, d.part_stride, nb2,
-                       out_dens);
+    const unsigned gr2 = static_cast<unsigned>((nb2 + kRedBins - 1) / kRedBins);
+    hipLaunchKernelGGL(k_adj_reduce, dim3(gr2), dim3(kRedBins * kRedSplit), 0, s, d.part, nwg,
+                       d.part_stride, nb2, out_dens);
 }
 
 }  // namespace msm
