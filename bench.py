@@ -61,6 +61,18 @@ def profiled_traffic(kernel: str):
     return int(vals[kernel]) if kernel in vals else None
 
 
+def oracle_cflags() -> str:
+    """Compiler and flags the oracle library was built with (its recipe, oracle/Makefile)."""
+    try:
+        with open(os.path.join(ROOT, "oracle", "Makefile")) as fh:
+            for line in fh:
+                if line.startswith("CFLAGS"):
+                    return "gcc " + line.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return "gcc (flags unknown)"
+
+
 def cpu_baseline(box, ctrl, ft, nsteps: int):
     import numpy as np
 
@@ -420,8 +432,8 @@ def main():
         cpu = {"value": round(rate, 1), "unit": "atom-steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_steps} MD steps (incl. one list rebuild) of the same "
                          f"{box.NPRT}-atom box, {dt:.1f} s, serial C restatement of the "
-                         "reference host path (half list, Newton's 3rd law), gcc -O2 "
-                         f"-ffp-contract=off, one thread of {os.cpu_count()} on {model}"}
+                         "reference host path (half list, Newton's 3rd law), "
+                         f"{oracle_cflags()}, one thread of {os.cpu_count()} on {model}"}
         log(f"cpu {rate:.4e} atom-steps/s")
 
     out = {
