@@ -179,8 +179,7 @@ class ForceMatch:
         self.dev.set_force_table(self.ft)
         self.dev.calc_force()
         self.dev.calc_epot()
-        _, _, fp, _, _ = self.dev.get_atoms()
-        return fp, self.dev.get_box_epot()
+        return self.dev.get_forces(), self.dev.get_box_epot()
 
     def set_targets(self, c_target: np.ndarray):
         self.f_target, self.e_target = self.forward(c_target)

@@ -59,6 +59,7 @@ struct msm_sim {
     TabDev tab{};
     double2* d_b2 = nullptr;
     double4 *d_f4 = nullptr, *d_e4 = nullptr, *d_m4 = nullptr;
+    size_t cap_b2 = 0, cap_f4 = 0, cap_e4 = 0, cap_m4 = 0;  // elements allocated
     int tab_ngroup = 0;
 
     // control
@@ -174,6 +175,17 @@ template <typename T>
 void dfree(T** p) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
+}
+
+// Device buffer of at least count elements: kept when it already holds that many (so refilling a
+// table of the same size allocates nothing and keeps its address), else freed and reallocated.
+template <typename T>
+int dreserve(msm_sim* s, T** p, size_t count, size_t& cap) {
+    if (*p && cap >= count) return MSM_OK;
+    cap = 0;
+    const int rc = dalloc(s, p, count);
+    if (rc == MSM_OK) cap = count;
+    return rc;
 }
 
 void clear_graphs(msm_sim* s) {
@@ -814,7 +826,8 @@ int msm_set_force_table(msm_sim* s, int pot_type, int ngroup, int nkind, int nta
             const double pr = knot(potr, k, i), pr1 = knot(potr, k, i + 1);
             he4[a] = make_double4(pr, pr1 - pr, pb, pb1 - pb);
         }
-    TabDev t{};
+    TabDev t;
+    std::memset(&t, 0, sizeof(t));  // byte-wise comparable with the current table
     t.pot_type = pot_type;
     t.ngroup = ngroup;
     t.ntab2 = ntab2;
@@ -826,9 +839,13 @@ int msm_set_force_table(msm_sim* s, int pot_type, int ngroup, int nkind, int nta
             if (kd < 1 || kd > nkind) return set_err(s, MSM_ERR_ARG, "KPAIR entry out of range");
             t.kpair[i * ngroup + j] = kd - 1;
         }
-    int rc = dalloc(s, &s->d_b2, hb2.size());
-    if (rc == MSM_OK) rc = dalloc(s, &s->d_f4, hf4.size());
-    if (rc == MSM_OK) rc = dalloc(s, &s->d_e4, he4.size());
+    if (pot_type == MSM_EAM_TYPE)
+        for (int i = 0; i < ngroup; ++i)
+            if (kembd[i] < 1 || kembd[i] > nkind1)
+                return set_err(s, MSM_ERR_ARG, "KEMBD entry out of range");
+    int rc = dreserve(s, &s->d_b2, hb2.size(), s->cap_b2);
+    if (rc == MSM_OK) rc = dreserve(s, &s->d_f4, hf4.size(), s->cap_f4);
+    if (rc == MSM_OK) rc = dreserve(s, &s->d_e4, he4.size(), s->cap_e4);
     if (rc != MSM_OK) return rc;
     HIPCHK(s, hipMemcpy(s->d_b2, hb2.data(), hb2.size() * sizeof(double2), hipMemcpyHostToDevice));
     HIPCHK(s, hipMemcpy(s->d_f4, hf4.data(), hf4.size() * sizeof(double4), hipMemcpyHostToDevice));
@@ -846,12 +863,8 @@ int msm_set_force_table(msm_sim* s, int pot_type, int ngroup, int nkind, int nta
                 const double d0 = eknot(dfembd, k, i), d1 = eknot(dfembd, k, i + 1);
                 hm4[static_cast<size_t>(k) * nembd2 + i] = make_double4(f0, f1 - f0, d0, d1 - d0);
             }
-        for (int i = 0; i < ngroup; ++i) {
-            if (kembd[i] < 1 || kembd[i] > nkind1)
-                return set_err(s, MSM_ERR_ARG, "KEMBD entry out of range");
-            t.kembd[i] = kembd[i] - 1;
-        }
-        rc = dalloc(s, &s->d_m4, hm4.size());
+        for (int i = 0; i < ngroup; ++i) t.kembd[i] = kembd[i] - 1;
+        rc = dreserve(s, &s->d_m4, hm4.size(), s->cap_m4);
         if (rc != MSM_OK) return rc;
         HIPCHK(s, hipMemcpy(s->d_m4, hm4.data(), hm4.size() * sizeof(double4), hipMemcpyHostToDevice));
         t.nembd = nembd;
@@ -872,10 +885,14 @@ int msm_set_force_table(msm_sim* s, int pot_type, int ngroup, int nkind, int nta
     // keep control-dependent fields if the control block came first
     t.ru2max = s->tab.ru2max;
     for (int g = 0; g < msm::kMaxGroup; ++g) t.cm[g] = s->tab.cm[g];
+    // captured segments hold the table by value: keep them only when nothing but knot values
+    // changed (same buffers, sizes, kinds and grid)
+    const bool same = s->has_table && s->tab_ngroup == ngroup &&
+                      std::memcmp(&s->tab, &t, sizeof(t)) == 0;
     s->tab = t;
     s->tab_ngroup = ngroup;
     s->has_table = true;
-    clear_graphs(s);
+    if (!same) clear_graphs(s);
     return MSM_OK;
 }
 
