@@ -39,7 +39,7 @@ def kernel_bytes(kmean: float, track_dis: bool):
     return {"density": dens, "force": force}
 
 
-def survey_step_bytes(kmean: float, n_rb: int) -> float:
+def step_model_bytes(kmean: float, n_rb: int) -> float:
     """Whole-step model of the full-list algorithm: list read twice, own x twice, F' write+read,
     F write, fused VV (read x,v,F; write x,v), amortised rebuild."""
     return 2 * (4 * kmean + 4) + 48 + 16 + 24 + 120 + (100 + 4 * kmean) / n_rb
@@ -264,7 +264,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is a "
+        print(json.dumps({"error": "this benchmark runs on one GPU only; multi-GPU sharding is a "
                                    "design note, not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
 
@@ -398,7 +398,7 @@ def main():
                 "avg_launch_us": round(kt[dom]["avg_us"], 3),
                 "traffic_source": "profiles/p1_hbm_traffic.md" if traffic else None}
 
-    step_bytes = survey_step_bytes(kmean, ctrl.NB_UPTAB)
+    step_bytes = step_model_bytes(kmean, ctrl.NB_UPTAB)
     step_model = {"bytes_per_atom_step": round(step_bytes, 1),
                   "achieved_GBs": round(step_bytes * value / 1e9, 2),
                   "frac": round(step_bytes * value / 1e9 / HBM_PEAK_GBS, 4)}

@@ -1,6 +1,6 @@
 """Extract small golden vectors from the reference's exported force-table files.
 
-Reads (in this build container only) the reference's table export
+Reads (where the reference tree is present; it never travels with the repository) its table export
   examples/use_ForceTableGen/EM_TB_WANGJUN_W-HE_2010.pair / .embd
 (written by Export_ForceTable, MDLIB/sor/Common/MD_TypeDef_ForceTable.F90:1315-1459) and keeps a
 subset of rows as plain-text fixtures:
