@@ -69,14 +69,20 @@ def build_fortran() -> list:
         return outs
     bdir = os.path.join(FSRC, "_fbuild")
     os.makedirs(bdir, exist_ok=True)
-    _run([FC, "-O2", "-fPIC", "-c", mod_src, "-module-dir", bdir, "-o",
-          os.path.join(bdir, "MD_ForceLib_HIP.o")])
-    for name in ("fs_w_driver", "cgs_driver"):
+    # the layer, then (in dependency order) the modules the example callers use: the reference's
+    # type modules as the glue reads them, the CGS potential routines, the force-class glue
+    objs = []
+    for name in ("MD_ForceLib_HIP", "mdlib_ref_types", "w_fs_cgs", "hip_forceclass_wrappers"):
+        src = os.path.join(FSRC, name + ".F90")
+        obj = os.path.join(bdir, name + ".o")
+        _run([FC, "-O2", "-fPIC", "-c", src, "-I" + bdir, "-module-dir", bdir, "-o", obj])
+        objs.append(obj)
+    for name in ("fs_w_driver", "cgs_driver", "ref_glue_driver"):
         drv = os.path.join(FSRC, name + ".F90")
         exe = os.path.join(FSRC, name)
         if os.path.exists(drv):
-            _run([FC, "-O2", drv, os.path.join(bdir, "MD_ForceLib_HIP.o"), "-I" + bdir,
-                  "-module-dir", bdir, "-L" + HERE, "-lmsm_hip", "-Wl,-rpath," + HERE, "-o", exe])
+            _run([FC, "-O2", drv, *objs, "-I" + bdir, "-module-dir", bdir, "-L" + HERE,
+                  "-lmsm_hip", "-Wl,-rpath," + HERE, "-o", exe])
             outs.append(exe)
     return outs
 

@@ -164,3 +164,52 @@ def test_cgs_driver_against_oracle(orc, tmp_path):
     rl = [l for l in rest if l.startswith("# restart")][0].split()
     assert [float(v) for v in rl[2:]] == [0.0] * 5
     assert CP_KB_CGS == KB_CGS
+
+
+GLUE_DRIVER = os.path.join(os.path.dirname(CGS_DRIVER), "ref_glue_driver")
+
+
+def test_reference_type_glue_against_oracle(orc, tmp_path):
+    """A caller written against the reference's own type names (SimMDBox, SimMDCtrl, MDForceTable,
+    the procedure-pointer MDForceClassGPU; CGS values), bound through the committed glue module
+    and driven through the force-class pointers only: forces (dyn), energy (erg) and F' against
+    the oracle in eV/A after unit conversion; the virial tensor (erg) against the oracle's
+    half-list VTENSOR to 1e-14 of the sum of its absolute pair terms; the per-atom stress traces
+    sum to twice the tensor's trace."""
+    if not os.path.exists(GLUE_DRIVER):
+        pytest.fail("ref_glue_driver is not built (msmpscu_amd.build.build_fortran)")
+    out = tmp_path / "glue.txt"
+    print("[gpu-test] reference-type glue driver", flush=True)
+    r = subprocess.run([GLUE_DRIVER, str(out), "8"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = out.read_text().splitlines()
+    n = int(lines[0].split()[0])
+    e0 = float(lines[0].split()[1])
+    blk = _blocks(lines, n, 1)
+    x_cm, f_cgs, den_cgs, avp_tr = blk[:, :3], blk[:, 3:6], blk[:, 6], blk[:, 7]
+    assert lines[1 + n].startswith("# vtensor")
+    vt_cgs = np.array([float(v) for v in lines[2 + n].split()]).reshape(3, 3, order="F")
+    ang = 1e8
+    nc = round((n / 2) ** (1 / 3))
+    zl = np.full(3, nc * 3.165e-8) * ang
+    box = SimMDBox(NPRT=n, NGROUP=1, ZL=zl, BOXLOW=-0.5 * zl, XP=np.asfortranarray(x_cm * ang),
+                   XP1=np.zeros((n, 3), order="F"), ITYP=np.ones(n, dtype=np.int32),
+                   STATU=np.ones(n, dtype=np.int32), CM=np.array([cfg.W_MASS * cfg.AMU]),
+                   PTYPE=np.array([[1]], dtype=np.int32), SYMB=["W"])
+    ctrl = cfg.w_ctrl()
+    ft = cfg.w_forcetable(box, ctrl)
+    t = orc.from_mdlib(ft)
+    it = box.ITYP - 1
+    kv, ind = orc.nlist("full", box.XP, it, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
+                        ctrl.NB_RM, ctrl.NB_MXNBS)
+    fo, do, eo = orc.force_full(box.XP, it, box.STATU, kv, ind, box.ZL, ctrl.IFPD,
+                                float(np.max(ctrl.RU)) ** 2, t)
+    assert np.max(np.abs(f_cgs / EVERG / ang - fo)) <= F_TOL
+    assert abs(e0 / EVERG - eo.sum()) <= E_REL * abs(eo.sum())
+    assert np.max(np.abs(den_cgs * EVERG - do)) <= 1e-13
+    kvh, indh = orc.nlist("half", box.XP, it, box.STATU, box.BOXLOW, box.ZL, ctrl.IFPD,
+                          ctrl.NB_RM, ctrl.NB_MXNBS)
+    _, _, _, _, vt, vabs = orc.force_fs_half(box.XP, it, kvh, indh, box.ZL, ctrl.IFPD, ctrl.RU,
+                                             t, with_vabs=True)
+    assert np.all(np.abs(vt_cgs / EVERG - vt) <= 1e-14 * vabs + 1e-300)
+    assert abs(avp_tr.sum() - 2.0 * np.trace(vt_cgs)) <= 1e-13 * EVERG * np.trace(vabs)
