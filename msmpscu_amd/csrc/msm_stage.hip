@@ -460,26 +460,6 @@ __device__ __forceinline__ void sb_rounds(int nhome, F2&& r2, F4&& r4, F8&& r8, 
     else r4(base, m);
 }
 
-// Brick order of the persistent grid: XCD x (workgroup ids x, x+8, ...) owns the contiguous
-// brick range [x*nb/8, (x+1)*nb/8) and its workgroups take that range's bricks round-robin, so
-// the bricks in flight on one XCD are neighbours whose halos overlap in that XCD's L2.  With
-// fewer than 8 workgroups the bricks are taken by plain striding.  Each brick is visited by
-// exactly one workgroup.
-__device__ __forceinline__ void sb_sweep(int nb, int& first, int& step, int& end) {
-    const int nblk = gridDim.x;
-    if (nblk < 8) {
-        first = blockIdx.x;
-        step = nblk;
-        end = nb;
-        return;
-    }
-    const int x = blockIdx.x & 7, k = blockIdx.x >> 3;
-    const int nx = (nblk - x + 7) >> 3;  // workgroups on XCD x
-    first = static_cast<int>(static_cast<long long>(x) * nb / 8) + k;
-    end = static_cast<int>(static_cast<long long>(x + 1) * nb / 8);
-    step = nx;
-}
-
 }  // namespace
 
 // Dynamic LDS: the density window (nk x wd {POTB, dPOTB} knots), the staged halo as double4
@@ -505,24 +485,22 @@ __global__ __launch_bounds__(kSbBlock, 4) void k_density_sb(const int* __restric
     const int kmax = t.ntab2 - 2;
     const int bi = xcd_block(gridDim.x);
     if (bi >= nbricks) return;
-    {
-        const BoxDev& B = boxes[brick_box[bi]];
-        BrickInfo g;
-        const int total = sb_setup(B, bi, start, t, cap, g, S, smap);
-        const bool staged = total <= cap && l.scap > 0 && total <= l.scap;
-        if (staged) sb_stage<0, MT>(S, smap, a, B, total, lp, lty);
-        __syncthreads();
-        auto round = [&](auto lanes, int base, int nat) {
-            sb_density_round<MT, EAM, decltype(lanes)::value>(base, nat, staged, g, S, lp, lty, win,
-                                                              a, l, t, w, B, kmax, err);
-        };
-        sb_rounds(
-            S.L.homeoff[g.nhc],
-            [&](int b, int m) { round(std::integral_constant<int, 2>{}, b, m); },
-            [&](int b, int m) { round(std::integral_constant<int, 4>{}, b, m); },
-            [&](int b, int m) { round(std::integral_constant<int, 8>{}, b, m); },
-            [&](int b, int m) { round(std::integral_constant<int, 16>{}, b, m); });
-    }
+    const BoxDev& B = boxes[brick_box[bi]];
+    BrickInfo g;
+    const int total = sb_setup(B, bi, start, t, cap, g, S, smap);
+    const bool staged = total <= cap && l.scap > 0 && total <= l.scap;
+    if (staged) sb_stage<0, MT>(S, smap, a, B, total, lp, lty);
+    __syncthreads();
+    auto round = [&](auto lanes, int base, int nat) {
+        sb_density_round<MT, EAM, decltype(lanes)::value>(base, nat, staged, g, S, lp, lty, win,
+                                                          a, l, t, w, B, kmax, err);
+    };
+    sb_rounds(
+        S.L.homeoff[g.nhc],
+        [&](int b, int m) { round(std::integral_constant<int, 2>{}, b, m); },
+        [&](int b, int m) { round(std::integral_constant<int, 4>{}, b, m); },
+        [&](int b, int m) { round(std::integral_constant<int, 8>{}, b, m); },
+        [&](int b, int m) { round(std::integral_constant<int, 16>{}, b, m); });
 }
 
 // Dynamic LDS: the force window (nk x wf {FPOTR, FPOTB} knots), the staged halo as double4
@@ -548,24 +526,22 @@ __global__ __launch_bounds__(kSbBlock, 4) void k_force_sb(const int* __restrict_
     const int kmax = t.ntab2 - 2;
     const int bi = xcd_block(gridDim.x);
     if (bi >= nbricks) return;
-    {
-        const BoxDev& B = boxes[brick_box[bi]];
-        BrickInfo g;
-        const int total = sb_setup(B, bi, start, t, cap, g, S, smap);
-        const bool staged = total <= cap && l.scap > 0 && total <= l.scap;
-        if (staged) sb_stage<1, MT>(S, smap, a, B, total, lp, lty);
-        __syncthreads();
-        auto round = [&](auto lanes, int base, int nat) {
-            sb_force_round<MT, MODE, decltype(lanes)::value>(base, nat, staged, g, S, lp, lty, win, a,
-                                                             l, t, w, B, sp, kmax);
-        };
-        sb_rounds(
-            S.L.homeoff[g.nhc],
-            [&](int b, int m) { round(std::integral_constant<int, 2>{}, b, m); },
-            [&](int b, int m) { round(std::integral_constant<int, 4>{}, b, m); },
-            [&](int b, int m) { round(std::integral_constant<int, 8>{}, b, m); },
-            [&](int b, int m) { round(std::integral_constant<int, 16>{}, b, m); });
-    }
+    const BoxDev& B = boxes[brick_box[bi]];
+    BrickInfo g;
+    const int total = sb_setup(B, bi, start, t, cap, g, S, smap);
+    const bool staged = total <= cap && l.scap > 0 && total <= l.scap;
+    if (staged) sb_stage<1, MT>(S, smap, a, B, total, lp, lty);
+    __syncthreads();
+    auto round = [&](auto lanes, int base, int nat) {
+        sb_force_round<MT, MODE, decltype(lanes)::value>(base, nat, staged, g, S, lp, lty, win, a,
+                                                         l, t, w, B, sp, kmax);
+    };
+    sb_rounds(
+        S.L.homeoff[g.nhc],
+        [&](int b, int m) { round(std::integral_constant<int, 2>{}, b, m); },
+        [&](int b, int m) { round(std::integral_constant<int, 4>{}, b, m); },
+        [&](int b, int m) { round(std::integral_constant<int, 8>{}, b, m); },
+        [&](int b, int m) { round(std::integral_constant<int, 16>{}, b, m); });
 }
 
 namespace {
@@ -594,29 +570,6 @@ const SbLimits& sb_limits() {
         d.cus = sb_device_attr(hipDeviceAttributeMultiprocessorCount, 256);
     }
     return d;
-}
-
-// Workgroups of kernel k that fit a CU at this LDS size (cached), times the CUs, at most nb.
-unsigned sb_grid(const void* k, size_t lds, int nb) {
-    static std::mutex mu;
-    static std::map<std::pair<const void*, size_t>, int> cache;
-    int per_cu = 0;
-    {
-        std::lock_guard<std::mutex> g(mu);
-        auto it = cache.find({k, lds});
-        if (it != cache.end()) per_cu = it->second;
-    }
-    if (per_cu == 0) {
-        int v = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k, kSbBlock, lds) != hipSuccess ||
-            v <= 0)
-            v = 1;
-        per_cu = v;
-        std::lock_guard<std::mutex> g(mu);
-        cache[{k, lds}] = v;
-    }
-    const long long g = static_cast<long long>(per_cu) * sb_limits().cus;
-    return static_cast<unsigned>(g < nb ? g : nb);
 }
 
 // LDS bytes of one workgroup when wgs of them share a CU, less the static tables.
@@ -669,8 +622,7 @@ void launch_density_sb(const LtShape& sh, int nbricks, const int* brick_box, con
 #define MSM_DSB(MT, E)                                                                         \
     do {                                                                                       \
         auto k = k_density_sb<MT, E>;                                                          \
-        const unsigned g = static_cast<unsigned>(nbricks);            \
-        hipLaunchKernelGGL(k, dim3(g), dim3(kSbBlock), lds, s, brick_box, nbricks, a, l, t,    \
+        hipLaunchKernelGGL(k, dim3(nbricks), dim3(kSbBlock), lds, s, brick_box, nbricks, a, l, t, \
                            boxes, start, pl.w, pl.cap, err);                                   \
     } while (0)
     if (mt) { if (eam) MSM_DSB(1, 1); else MSM_DSB(1, 0); }
@@ -692,8 +644,7 @@ void launch_force_sb(const LtShape& sh, int nbricks, const int* brick_box, const
 #define MSM_FSB(MT, M)                                                                         \
     do {                                                                                       \
         auto k = k_force_sb<MT, M>;                                                            \
-        const unsigned g = static_cast<unsigned>(nbricks);            \
-        hipLaunchKernelGGL(k, dim3(g), dim3(kSbBlock), lds, s, brick_box, nbricks, a, l, t,    \
+        hipLaunchKernelGGL(k, dim3(nbricks), dim3(kSbBlock), lds, s, brick_box, nbricks, a, l, t, \
                            boxes, start, sp, pl.w, pl.cap);                                    \
     } while (0)
 #define MSM_FSBM(MT)                            \
