@@ -74,15 +74,17 @@ static __device__ __forceinline__ double min_image(double s, double len, double 
     return fma(-len, k, s);
 }
 
-// 1/sqrt(x), x > 0: hardware estimate refined by two Newton steps (a few ulp).  Used in place of
-// the reference's two correctly rounded square roots and two divisions per pair; the difference
-// is at the 1e-16 relative level, far inside the 1e-10 eV/A force bar.
+// 1/sqrt(x), x > 0: the hardware estimate y (relative error below 6e-8 on gfx950) refined by one
+// second-order step of the binomial series of (1 - e)^(-1/2), e = 1 - x y^2:
+//   y (1 + e/2 + 3e^2/8)
+// whose truncation error 5e^3/16 is below 1e-21; the result is within one ulp over the pair
+// ranges of the passes (two Newton steps, which cost two more operations, reach two).  Used in
+// place of the reference's two correctly rounded square roots and two divisions per pair; the
+// difference is at the 1e-16 relative level, far inside the 1e-10 eV/A force bar.
 static __device__ __forceinline__ double rsqrt_nr(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double hx = 0.5 * x;
-    y = y * fma(-hx * y, y, 1.5);
-    y = y * fma(-hx * y, y, 1.5);
-    return y;
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-(x * y), y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 // Table coordinate of a pair: SK = sqrt(sqrt(r2)) * CSI (MD_FS_ForceTable.F90:248-252), with

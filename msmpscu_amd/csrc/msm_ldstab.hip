@@ -264,15 +264,24 @@ __device__ __forceinline__ void force_group(const AtomsDev& a, const ListDev& l,
 // group (slots base + G*S, ..., into js), and only then does the arithmetic wait for the gathers,
 // so the next group's index latency hides behind this group's gathers and arithmetic instead of
 // adding a second round trip.  Slots at or past kv read the atom itself and add nothing.
+// The slot stride of the list is the constant kListRow, so the G loads sit at fixed offsets from
+// slot base.  Each load is predicated on its slot being below kv: issuing them all and discarding
+// the values measured 10% slower in the density pass, since most slots past an atom's list length
+// lie on lines of the list array that no other load touches.
 template <int G, int TPA>
-__device__ __forceinline__ void load_slots(const int* __restrict__ li, int stride, int base,
-                                           int kv, int self, int (&js)[G]) {
+__device__ __forceinline__ void load_slots(const int* __restrict__ li, int base, int kv, int self,
+                                           int (&js)[G]) {
     constexpr int S = slot_step<TPA>();
 #pragma unroll
     for (int u = 0; u < G; ++u) {
         const int iw = base + u * S;
-        js[u] = iw < kv ? li[static_cast<size_t>(iw) * stride] : self;
+        js[u] = iw < kv ? li[static_cast<size_t>(iw) * kListRow] : self;
     }
+}
+
+// Record j of the position array (j >= 0: zero-extended index).
+__device__ __forceinline__ const double4& rec(const double4* __restrict__ p, int j) {
+    return p[static_cast<unsigned>(j)];
 }
 
 template <int G, int MT, int TPA>
@@ -291,11 +300,11 @@ __device__ __forceinline__ double density_group_pf(const AtomsDev& a, const List
     int tj[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
-        qxy[u] = *reinterpret_cast<const double2*>(&a.pos[js[u]]);
-        qz[u] = a.pos[js[u]].z;
+        qxy[u] = *reinterpret_cast<const double2*>(&rec(a.pos, js[u]));
+        qz[u] = rec(a.pos, js[u]).z;
         tj[u] = MT ? a.ityp[js[u]] : 0;
     }
-    load_slots<G, TPA>(li, l.stride, base + G * S, kv, i, js);
+    load_slots<G, TPA>(li, base + G * S, kv, i, js);
     unsigned fix = 0;
 #pragma unroll
     for (int u = 0; u < G; ++u) {
@@ -352,10 +361,10 @@ __device__ __forceinline__ void force_group_pf(const AtomsDev& a, const ListDev&
     int tj[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
-        pj[u] = a.pos[js[u]];
+        pj[u] = rec(a.pos, js[u]);
         tj[u] = MT ? a.ityp[js[u]] : 0;
     }
-    load_slots<G, TPA>(li, l.stride, base + G * S, kv, i, js);
+    load_slots<G, TPA>(li, base + G * S, kv, i, js);
     unsigned fix = 0;
 #pragma unroll
     for (int u = 0; u < G; ++u) {
@@ -588,7 +597,7 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
                     int js[kGroupD];
                     unsigned long long mk = 0;  // this lane's in-cutoff slots
                     unsigned long long* mkp = WM ? &mk : nullptr;
-                    load_slots<kGroupD, TPA>(li, l.stride, base, kv, i, js);
+                    load_slots<kGroupD, TPA>(li, base, kv, i, js);
                     for (; base + (kGroupD - 1) * TPA < kv; base += TPA * kGroupD)
                         den0 = density_group_pf<kGroupD, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti,
                                                                   kv, base, pxy, pz, den0, js, mkp);
@@ -677,7 +686,7 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
             } else {
                 int base = sub;
                 int js[kGroupF];
-                load_slots<kGroupF, TPA>(li, l.stride, base, kv, i, js);
+                load_slots<kGroupF, TPA>(li, base, kv, i, js);
                 for (; base + (kGroupF - 1) * TPA < kv; base += TPA * kGroupF)
                     force_group_pf<kGroupF, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi,
                                                      denki, fpx, fpy, fpz, js);

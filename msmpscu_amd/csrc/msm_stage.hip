@@ -83,16 +83,11 @@ __device__ __forceinline__ double near_image(double v, double c, double len, dou
     return fma(-len, rint((v - c) * ilen), v);
 }
 
-// SK = sqrt(sqrt(r2)) * CSI (MD_FS_ForceTable.F90:248-252) and 1/r from hardware reciprocal
-// square roots, each refined by one Newton step (relative error ~1e-16 after it):
-// 1/r = rsq(r2), sqrt(r) = rsq(1/r).
+// SK = sqrt(sqrt(r2)) * CSI (MD_FS_ForceTable.F90:248-252) and 1/r from refined reciprocal
+// square roots (rsqrt_nr, within one ulp): 1/r = rsqrt(r2), sqrt(r) = rsqrt(1/r).
 __device__ __forceinline__ double sb_coord(double r2, double csi, double& inv_r) {
-    double y = __builtin_amdgcn_rsq(r2);
-    y = y * fma(-0.5 * r2 * y, y, 1.5);
-    inv_r = y;
-    double z = __builtin_amdgcn_rsq(y);
-    z = z * fma(-0.5 * y * z, z, 1.5);
-    return z * csi;
+    inv_r = rsqrt_nr(r2);
+    return rsqrt_nr(inv_r) * csi;
 }
 
 // Stage the halo: record s is sorted atom hbase[h] + s of halo cell h, at its image nearest the
