@@ -88,11 +88,10 @@ static __device__ __forceinline__ double rsqrt_nr(double x) {
 }
 
 // Table coordinate of a pair: SK = sqrt(sqrt(r2)) * CSI (MD_FS_ForceTable.F90:248-252), with
-// r = r2 * rsqrt(r2) and sqrt(r) = r * rsqrt(r).  Returns 1/r through inv_r.
+// 1/r = rsqrt(r2) and sqrt(r) = rsqrt(1/r).  Returns 1/r through inv_r.
 static __device__ __forceinline__ double table_coord(double r2, double csi, double& inv_r) {
     inv_r = rsqrt_nr(r2);
-    const double r = r2 * inv_r;
-    return (r * rsqrt_nr(r)) * csi;
+    return rsqrt_nr(inv_r) * csi;
 }
 
 template <int MB>
