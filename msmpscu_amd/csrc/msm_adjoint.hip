@@ -242,7 +242,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_density(AtomsDev a, ListDev l
         fprime<EAM>(t, ke, rho, fp, dfp);
         double de;  // dF/drho of the energy term
         if constexpr (!EAM) {
-            de = rho > 0.0 ? -0.5 / sqrt(rho) : 0.0;
+            de = rho > 0.0 ? fs_fprime(rho) : 0.0;
         } else {
             // the energy kernel's lookup: KK = int(rho/RHOD + 1 + 1e-6) clamped to [0, NEMBD]
             const double sk = rho / t.rhod + 1.0;

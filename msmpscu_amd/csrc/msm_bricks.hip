@@ -390,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_density_bricks(const int* __restrict
                 }
             }
             if constexpr (!EAM) {
-                if (den0 > 0.0) den0 = -0.5 / sqrt(den0);
+                if (den0 > 0.0) den0 = fs_fprime(den0);
             } else {
                 if (den0 > 0.0) {
                     const double sk = den0 / t.rhod + 1.0;

@@ -87,6 +87,13 @@ static __device__ __forceinline__ double rsqrt_nr(double x) {
     return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
+// F'(rho) of the Finnis-Sinclair embedding F = -sqrt(rho): -1/2 rho^(-1/2) for rho > 0 (the
+// reference's -0.5/DSQRT(DEN)), from the refined reciprocal square root (within two ulp of the
+// correctly rounded quotient).
+static __device__ __forceinline__ double fs_fprime(double rho) {
+    return -0.5 * rsqrt_nr(rho);
+}
+
 // Table coordinate of a pair: SK = sqrt(sqrt(r2)) * CSI (MD_FS_ForceTable.F90:248-252), with
 // 1/r = rsqrt(r2) and sqrt(r) = rsqrt(1/r).  Returns 1/r through inv_r.
 static __device__ __forceinline__ double table_coord(double r2, double csi, double& inv_r) {

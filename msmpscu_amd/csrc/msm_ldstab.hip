@@ -619,7 +619,7 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
         if (sub != 0 || !have) continue;
         if (active) {
             if constexpr (!EAM) {
-                if (den0 > 0.0) den0 = -0.5 / sqrt(den0);
+                if (den0 > 0.0) den0 = fs_fprime(den0);
             } else {
                 if (den0 > 0.0) {
                     const double sk = den0 / t.rhod + 1.0;
@@ -961,7 +961,7 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_bt(const int* __restric
             if (sub != 0 || !have) continue;
             if (active) {
                 if constexpr (!EAM) {
-                    if (den0 > 0.0) den0 = -0.5 / sqrt(den0);
+                    if (den0 > 0.0) den0 = fs_fprime(den0);
                 } else {
                     if (den0 > 0.0) {
                         const double sk = den0 / t.rhod + 1.0;

@@ -263,7 +263,7 @@ __device__ __forceinline__ void sb_density_round(int base, int nat, bool staged,
     if (sub != 0) return;
     if (active) {
         if constexpr (!EAM) {
-            if (den > 0.0) den = -0.5 / sqrt(den);
+            if (den > 0.0) den = fs_fprime(den);
         } else {
             if (den > 0.0) {
                 const double sk = den / t.rhod + 1.0;
