@@ -142,19 +142,41 @@ static __constant__ int c_niy[27] = {0, 0, -1, 1, 1, 0, 0, 0, -1, -1, -1, 1, 1, 
 static __constant__ int c_niz[27] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1,
                               0, 0, 0, 0, -1, -1, -1, -1, -1, -1, -1, -1, -1};
 
+// Velocity and accumulated displacement of one atom, read by a force pass before the atom's pair
+// loop so that the integrator at its end does not wait on them (only the atom's own lane reads or
+// writes them during the pass).
+struct AtomState {
+    double vx, vy, vz, dx, dy, dz;
+};
+
+__device__ __forceinline__ void load_atom_state(const AtomsDev& a, const StepParams& sp, int i,
+                                                AtomState& s) {
+    s.vx = a.vx[i];
+    s.vy = a.vy[i];
+    s.vz = a.vz[i];
+    s.dx = s.dy = s.dz = 0.0;
+    if (sp.track_dis) {
+        s.dx = a.dx[i];
+        s.dy = a.dy[i];
+        s.dz = a.dz[i];
+    }
+}
+
 // Corrector of this step (Correction_KERNEL, MD_DiffScheme_GPU.F90:686-758) and, for MODE 2, the
 // predictor of the next step (Predictor_KERNEL0, :242-384) with the same forces, for atom i.
-// Positions of MODE 2 go to a.pos_out (the next step's buffer).
+// Positions of MODE 2 go to a.pos_out (the next step's buffer).  pre: the atom's velocity and
+// displacement as read by load_atom_state, or null to read them here.
 template <int MODE>
 __device__ __forceinline__ void integrate_atom(const AtomsDev& a, const TabDev& t, const BoxDev& b,
                                                const StepParams& sp, int i, int st, bool active,
                                                const double4& pi, int ti, double fpx, double fpy,
-                                               double fpz) {
+                                               double fpz, const AtomState* pre = nullptr) {
         if (active) {
 #pragma clang fp contract(off)
             const double cm0 = t.cm[ti];
             const double ax = fpx / cm0, ay = fpy / cm0, az = fpz / cm0;
-            double vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+            double vx = pre ? pre->vx : a.vx[i], vy = pre ? pre->vy : a.vy[i],
+                   vz = pre ? pre->vz : a.vz[i];
             // Correction_KERNEL
             if (!(st & (STATU_FIXVELX | STATU_FIXPOSX))) vx = vx + sp.hs2 * ax;
             if (!(st & (STATU_FIXVELY | STATU_FIXPOSY))) vy = vy + sp.hs2 * ay;
@@ -188,9 +210,9 @@ __device__ __forceinline__ void integrate_atom(const AtomsDev& a, const TabDev& 
                 if (!(stat & (STATU_FIXVELY | STATU_FIXPOSY))) vy = vy + sp.hs2 * ay;
                 if (!(stat & (STATU_FIXVELZ | STATU_FIXPOSZ))) vz = vz + sp.hs2 * az;
                 if (sp.track_dis) {
-                    a.dx[i] = a.dx[i] + dx;
-                    a.dy[i] = a.dy[i] + dy;
-                    a.dz[i] = a.dz[i] + dz;
+                    a.dx[i] = (pre ? pre->dx : a.dx[i]) + dx;
+                    a.dy[i] = (pre ? pre->dy : a.dy[i]) + dy;
+                    a.dz[i] = (pre ? pre->dz : a.dz[i]) + dz;
                 }
                 if (x > b.lo[0] + b.len[0] || y > b.lo[1] + b.len[1] || z > b.lo[2] + b.len[2])
                     stat = STATU_OUTOFBOX | STATU_REFLECT;

@@ -663,6 +663,10 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
         const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, have ? i : 0);
         const int ti = (MT && have) ? a.ityp[i] : 0;
         double fpx = 0.0, fpy = 0.0, fpz = 0.0;
+        // the integrator's inputs, read ahead of the pair loop (by the atom's first lane)
+        AtomState as;
+        if constexpr (MODE >= 1)
+            if (active && sub == 0) load_atom_state(a, sp, i, as);
         if (active) {
             const double denki = pi.w;
             const int kv = l.kvois[i];
@@ -708,7 +712,7 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
             a.fz[i] = fpz;
         }
         if constexpr (MODE >= 1)
-            integrate_atom<MODE>(a, t, b, sp, i, st, active, pi, ti, fpx, fpy, fpz);
+            integrate_atom<MODE>(a, t, b, sp, i, st, active, pi, ti, fpx, fpy, fpz, &as);
     }
 }
 
