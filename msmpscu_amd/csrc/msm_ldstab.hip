@@ -663,14 +663,28 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
         const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, have ? i : 0);
         const int ti = (MT && have) ? a.ityp[i] : 0;
         double fpx = 0.0, fpy = 0.0, fpz = 0.0;
-        // the integrator's inputs, read ahead of the pair loop (by the atom's first lane)
+        // The atom's loads in one batch, issued before its status is known: the integrator's
+        // inputs (by the atom's first lane), the list length and, on the plain list walk, the
+        // first group's slot indices (slots clamped below mx, masked by the length once it is
+        // known), so that the first gathers wait on one round trip instead of three.
         AtomState as;
-        if constexpr (MODE >= 1)
-            if (active && sub == 0) load_atom_state(a, sp, i, as);
+        int kv = 0;
+        int js[kGroupF];
+        const bool walk = TPA != 3 && !l.cmask;
+        const int* __restrict__ li = l.indi + list_base(have ? i : 0, l.mx);
+        if (have) {
+            if constexpr (MODE >= 1)
+                if (sub == 0) load_atom_state(a, sp, i, as);
+            kv = l.kvois[i];
+            if (walk) {
+                constexpr int S = slot_step<TPA>();
+#pragma unroll
+                for (int u = 0; u < kGroupF; ++u)
+                    js[u] = li[static_cast<size_t>(min(sub + u * S, l.mx - 1)) * kListRow];
+            }
+        }
         if (active) {
             const double denki = pi.w;
-            const int kv = l.kvois[i];
-            const int* __restrict__ li = l.indi + list_base(i, l.mx);
             if constexpr (TPA == 3) {
                 // the two lanes of a pair run the same groups (their gathers are shared)
                 int base = 0;
@@ -689,8 +703,9 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
                                                denki, fpx, fpy, fpz);
             } else {
                 int base = sub;
-                int js[kGroupF];
-                load_slots<kGroupF, TPA>(li, base, kv, i, js);
+#pragma unroll
+                for (int u = 0; u < kGroupF; ++u)
+                    if (base + u * TPA >= kv) js[u] = i;
                 for (; base + (kGroupF - 1) * TPA < kv; base += TPA * kGroupF)
                     force_group_pf<kGroupF, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi,
                                                      denki, fpx, fpy, fpz, js);
