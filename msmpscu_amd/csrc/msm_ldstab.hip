@@ -564,16 +564,33 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
         int i, sub;
         atom_of_lane<TPA>(c, i, sub);
         const bool have = i < a.n;
-        const bool active = have && (a.statu[i] & STATU_ACTIVE) == STATU_ACTIVE;
+        // The atom's loads in one batch, issued before its status is known: status, position,
+        // list length and, on the plain list walk (TPA 1, 2, 4), the first group's slot indices
+        // (slots clamped below mx, masked by the length once it is known), so that the first
+        // gathers wait on one round trip instead of three.
+        int st = 0, kv = 0;
+        double2 pxy = make_double2(0.0, 0.0);
+        double pz = 0.0;
+        int js0[kGroupD];
+        const int* __restrict__ li = l.indi + list_base(have ? i : 0, l.mx);
+        if (have) {
+            st = a.statu[i];
+            pxy = *reinterpret_cast<const double2*>(&a.pos[i]);
+            pz = a.pos[i].z;
+            kv = l.kvois[i];
+            if constexpr (TPA != 3) {
+                constexpr int S = slot_step<TPA>();
+#pragma unroll
+                for (int u = 0; u < kGroupD; ++u)
+                    js0[u] = li[static_cast<size_t>(min(sub + u * S, l.mx - 1)) * kListRow];
+            }
+        }
+        const bool active = have && (st & STATU_ACTIVE) == STATU_ACTIVE;
         double den0 = 0.0;
         int ti = 0;
         if (active) {
             const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
-            const double2 pxy = *reinterpret_cast<const double2*>(&a.pos[i]);
-            const double pz = a.pos[i].z;
             ti = MT ? a.ityp[i] : 0;
-            const int kv = l.kvois[i];
-            const int* __restrict__ li = l.indi + list_base(i, l.mx);
             // full groups of kGroup slots, then groups of 4 and single slots for the tail (the
             // slots of a lane are visited in increasing order either way)
             if constexpr (TPA == 3) {
@@ -597,7 +614,8 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
                     int js[kGroupD];
                     unsigned long long mk = 0;  // this lane's in-cutoff slots
                     unsigned long long* mkp = WM ? &mk : nullptr;
-                    load_slots<kGroupD, TPA>(li, base, kv, i, js);
+#pragma unroll
+                    for (int u = 0; u < kGroupD; ++u) js[u] = base + u * TPA < kv ? js0[u] : i;
                     for (; base + (kGroupD - 1) * TPA < kv; base += TPA * kGroupD)
                         den0 = density_group_pf<kGroupD, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti,
                                                                   kv, base, pxy, pz, den0, js, mkp);
