@@ -100,7 +100,8 @@ __device__ __forceinline__ double density_group(const AtomsDev& a, const ListDev
                                                 const double* s_pb, const int* __restrict__ li,
                                                 int i, int ti, int kv, int base, double2 pxy,
                                                 double pz, double den0, int sub = 0,
-                                                unsigned long long* mk = nullptr) {
+                                                unsigned long long* mk = nullptr,
+                                                const int* jsp = nullptr) {
     constexpr int S = slot_step<TPA>();
     const int kmax = t.ntab2 - 2;
     const int k0 = base / S;  // the lane's slot number of u = 0 (base = sub + k0 * S)
@@ -108,7 +109,7 @@ __device__ __forceinline__ double density_group(const AtomsDev& a, const ListDev
 #pragma unroll
     for (int u = 0; u < G; ++u) {
         const int iw = base + u * S;
-        js[u] = iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
+        js[u] = jsp ? jsp[u] : iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
     }
     unsigned fix = 0;
 #pragma unroll
@@ -172,14 +173,15 @@ __device__ __forceinline__ void force_group(const AtomsDev& a, const ListDev& l,
                                             const BoxDev& b, const LtWin& w, const double2* s_ft,
                                             const int* __restrict__ li, int i, int ti, int kv,
                                             int base, const double4& pi, double denki,
-                                            double& fpx, double& fpy, double& fpz, int sub = 0) {
+                                            double& fpx, double& fpy, double& fpz, int sub = 0,
+                                            const int* jsp = nullptr) {
     constexpr int S = slot_step<TPA>();
     const int kmax = t.ntab2 - 2;
     int js[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) {
         const int iw = base + u * S;
-        js[u] = iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
+        js[u] = jsp ? jsp[u] : iw < kv ? li[static_cast<size_t>(iw) * l.stride] : i;
     }
     unsigned fix = 0;
 #pragma unroll
@@ -619,12 +621,18 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
                     for (; base + (kGroupD - 1) * TPA < kv; base += TPA * kGroupD)
                         den0 = density_group_pf<kGroupD, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti,
                                                                   kv, base, pxy, pz, den0, js, mkp);
-                    for (; base + 3 * TPA < kv; base += TPA * 4)
-                        den0 = density_group<4, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base,
-                                                         pxy, pz, den0, sub, mkp);
-                    for (; base < kv; base += TPA)
-                        den0 = density_group<1, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv, base,
-                                                         pxy, pz, den0, sub, mkp);
+                    // the last kGroupD - 1 slots at most, in pairs and singles, from the indices
+                    // the last group (or the atom's batch) already read into js
+#pragma unroll
+                    for (int u = 0; u < kGroupD - 1; u += 2) {
+                        const int iw = base + u * TPA;
+                        if (iw + TPA < kv)
+                            den0 = density_group<2, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv,
+                                                             iw, pxy, pz, den0, sub, mkp, &js[u]);
+                        else if (iw < kv)
+                            den0 = density_group<1, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti, kv,
+                                                             iw, pxy, pz, den0, sub, mkp, &js[u]);
+                    }
                     if constexpr (WM) l.cmask[static_cast<size_t>(i) * TPA + sub] = mk;
                 };
                 if (l.cmask)
@@ -727,12 +735,18 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
                 for (; base + (kGroupF - 1) * TPA < kv; base += TPA * kGroupF)
                     force_group_pf<kGroupF, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi,
                                                      denki, fpx, fpy, fpz, js);
-                for (; base + 3 * TPA < kv; base += TPA * 4)
-                    force_group<4, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki,
-                                            fpx, fpy, fpz);
-                for (; base < kv; base += TPA)
-                    force_group<1, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi, denki,
-                                            fpx, fpy, fpz);
+                // the last kGroupF - 1 slots at most, in pairs and singles, from the indices the
+                // last group (or the atom's batch) already read into js
+#pragma unroll
+                for (int u = 0; u < kGroupF - 1; u += 2) {
+                    const int iw = base + u * TPA;
+                    if (iw + TPA < kv)
+                        force_group<2, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, iw, pi, denki,
+                                                fpx, fpy, fpz, sub, &js[u]);
+                    else if (iw < kv)
+                        force_group<1, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, iw, pi, denki,
+                                                fpx, fpy, fpz, sub, &js[u]);
+                }
             }
         }
         fpx = lane_sum<TPA>(fpx);
