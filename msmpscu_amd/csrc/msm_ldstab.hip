@@ -30,10 +30,11 @@ namespace msm {
 namespace {
 
 constexpr int kGroup = 8;
-// Slots per software-pipelined group of the density and force passes (measured: 8 for the force
-// pass spills and is 25% slower than 6).
+// Slots per software-pipelined group of the density and force passes, measured on the 1M-atom
+// W box: density 4 / 6 / 8 took 86.7 / 86.2 / 87.6 us, force 3 / 4 took 105.7 / 107.2 us (8 for
+// the force pass spills).
 constexpr int kGroupD = 6;
-constexpr int kGroupF = 4;
+constexpr int kGroupF = 3;
 
 // TPA = 3 stands for two lanes per atom (lanes 2m, 2m + 1) sharing their position gathers.
 template <int TPA>
