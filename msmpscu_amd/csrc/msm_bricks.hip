@@ -28,7 +28,8 @@ namespace msm {
 // share a CU.  The candidates of each stencil cell are tested kNlGroup at a time (their LDS
 // loads issued together) and the hits stored in candidate order.  SIDX: also store each
 // neighbour's halo slot (read by the LDS-brick passes).  Measured alternatives: groups of 8,
-// and hit masks with one store round per hit of the busiest lane -- both slower.
+// and hit masks with one store round per hit of the busiest lane -- both slower.  The list's slot
+// stride is kListRow (the `stride` argument, which only the global fallback reads, equals it).
 constexpr int kNlGroup = 4;
 // 512 threads: a 4^3-cell brick of a bcc metal at the usual skin holds ~420 home atoms, which a
 // 256-thread workgroup served in two rounds (the second 60% full); with 53 VGPRs and 40 KB of LDS
@@ -113,9 +114,10 @@ __global__ __launch_bounds__(kNlSmallBlock) void k_nlist_small(
                 for (int u = 0; u < kNlGroup; ++u) {
                     if (hit[u]) {
                         if (c < mx) {
-                            li[c * stride] = gb + q0 + u;
-                            if constexpr (SIDX)
-                                ls[c * stride] = static_cast<unsigned short>(q0 + u);
+                            // the slot stride is kListRow (a shift, not a multiply)
+                            const unsigned at = static_cast<unsigned>(c) * kListRow;
+                            li[at] = gb + q0 + u;
+                            if constexpr (SIDX) ls[at] = static_cast<unsigned short>(q0 + u);
                         }
                         ++c;
                     }
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
                         const int w = c + __popc(grp & below);
                         if (w < mx) {
                             const int q = q0 + u * T + sub;
-                            const size_t at = lb + static_cast<size_t>(w) * stride;
+                            const size_t at = lb + static_cast<unsigned>(w) * kListRow;
                             indi[at] = gb + q;
                             if constexpr (SIDX) sidx[at] = static_cast<unsigned short>(q);
                         }
