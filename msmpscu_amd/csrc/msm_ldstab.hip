@@ -594,10 +594,10 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
         if (active) {
             const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
             ti = MT ? a.ityp[i] : 0;
-            // full groups of kGroup slots, then groups of 4 and single slots for the tail (the
-            // slots of a lane are visited in increasing order either way)
+            // the slots of a lane are visited in increasing order on every path
             if constexpr (TPA == 3) {
-                // the two lanes of a pair run the same groups (their gathers are shared)
+                // the two lanes of a pair run the same groups (their gathers are shared): full
+                // groups of kGroup slots, then groups of 4 and single slots for the tail
                 int base = 0;
                 for (; base + 2 * kGroup <= kv; base += 2 * kGroup)
                     den0 = density_group<kGroup, MT, 3>(a, l, t, b, w, s_pb, li, i, ti, kv,
