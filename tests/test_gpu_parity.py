@@ -484,6 +484,43 @@ def test_force_paths_agree(orc, p0, path, compact, lds):
     dev.close()
 
 
+
+@pytest.mark.parametrize("path", [1, 2, 4])
+@pytest.mark.parametrize("vacancy,seed", [(0.8, 77), (0.95, 1)])
+def test_short_lists_at_capacity(orc, vacancy, seed, path):
+    """Lists of every length from 0 up to MXKVOIS, with MXKVOIS set to the longest list (11 and
+    4 on these sparse W boxes): the LDS-table passes read each atom's first slot indices before
+    its length is known, clamping the slot below MXKVOIS, and take the last slots of a list in
+    pairs and singles from indices already read.  Forces and F' against the oracle with 1, 2 and
+    4 lanes per atom."""
+    import msmpscu_amd._lib as L
+    box = cfg.make_bcc_box((10, 10, 10), cfg.W_A, 0.05, 300.0, seed, vacancy_frac=vacancy)
+    big = cfg.w_ctrl(mx=64)
+    kv, _ = orc.nlist("full", box.XP, box.ITYP - 1, box.STATU, box.BOXLOW, box.ZL, big.IFPD,
+                      big.NB_RM, big.NB_MXNBS)
+    mx = int(kv.max())
+    assert kv.min() == 0 and np.all(np.bincount(kv, minlength=mx + 1) > 0)
+    ctrl = cfg.w_ctrl(mx=mx)
+    ft = cfg.w_forcetable(box, ctrl)
+    say(f"short lists vacancy {vacancy} mx {mx} lanes {path}")
+    dev = device_for(box, ctrl, ft)
+    L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_FORCE_PATH, path), "msm_set_option")
+    L.check(L.lib().msm_set_option(dev.h, L.MSM_OPT_TABLE_LDS, 1), "msm_set_option")
+    assert dev.pass_kernel() == "lds_table"
+    dev.build_neighbors()
+    dev.calc_force()
+    _, _, fp, _, _ = dev.get_atoms()
+    fo, do, _ = oracle_full(orc, box, ctrl, ft)
+    assert np.max(np.abs(fp - fo)) <= F_TOL
+    # an atom whose only neighbours sit near the cutoff has a tiny rho and a large
+    # F' = -rho^(-1/2)/2: the bar is relative to max(1, |F'|)
+    den = dev.get_den()
+    rel = np.abs(den - do) / np.maximum(1.0, np.abs(do))
+    say(f"  F' max |F'| {np.max(np.abs(do)):.3e} worst relative {np.max(rel):.3e}")
+    assert np.max(rel) <= 1e-13
+    dev.close()
+
+
 def _asym_case(ntab=5000):
     box = cfg.make_bcc_box((8, 8, 8), cfg.W_A, 0.08, 300.0, 8)
     box.NGROUP = 2
