@@ -287,7 +287,9 @@ __device__ __forceinline__ const double4& rec(const double4* __restrict__ p, int
     return p[static_cast<unsigned>(j)];
 }
 
-template <int G, int MT, int TPA>
+// Pipelined groups are full: every slot lies below kv.  FW: the LDS window holds the whole table
+// (k0 = 0), so no pair falls below it.
+template <int G, int MT, int TPA, bool FW>
 __device__ __forceinline__ double density_group_pf(const AtomsDev& a, const ListDev& l,
                                                    const TabDev& t, const BoxDev& b,
                                                    const LtWin& w, const double* s_pb,
@@ -319,16 +321,16 @@ __device__ __forceinline__ double density_group_pf(const AtomsDev& a, const List
         double inv_r;
         const double sk = table_coord(r2, t.csi, inv_r);
         const int kk = min(static_cast<int>(sk), kmax);
-        const bool in = base + u * S < kv && r2 <= t.ru2max;
+        const bool in = r2 <= t.ru2max;
         if (mk && in) *mk |= 1ull << (k0 + u);
-        const int kl = kk - w.k0d;
-        if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
-        const int q = row + max(kl, 0);
+        const int kl = FW ? kk : kk - w.k0d;
+        if (!FW && kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+        const int q = row + (FW ? kl : max(kl, 0));
         const double p0 = s_pb[q], p1 = s_pb[q + 1];
         const double v = p0 + (sk - kk) * (p1 - p0);
-        den0 = den0 + ((in && kl >= 0) ? v : 0.0);
+        den0 = den0 + ((in && (FW || kl >= 0)) ? v : 0.0);
     }
-    if (fix) {
+    if (!FW && fix) {
         // rare close pairs: reload the neighbour (keeping the group's records live would cost
         // registers) and take the knots of the global table
 #pragma unroll
@@ -351,7 +353,8 @@ __device__ __forceinline__ double density_group_pf(const AtomsDev& a, const List
     return den0;
 }
 
-template <int G, int MT, int TPA>
+// As density_group_pf: full groups; FW: the window holds the whole table.
+template <int G, int MT, int TPA, bool FW>
 __device__ __forceinline__ void force_group_pf(const AtomsDev& a, const ListDev& l,
                                                const TabDev& t, const BoxDev& b, const LtWin& w,
                                                const double2* s_ft, const int* __restrict__ li,
@@ -384,10 +387,10 @@ __device__ __forceinline__ void force_group_pf(const AtomsDev& a, const ListDev&
         const double sk = table_coord(r2, t.csi, inv_r);
         const int kk = min(static_cast<int>(sk), kmax);
         const double dk = sk - kk;
-        const bool in = base + u * S < kv && r2 <= t.ru2max;
-        const int kl = kk - w.k0f;
-        if (kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
-        const int q = max(kl, 0);
+        const bool in = r2 <= t.ru2max;
+        const int kl = FW ? kk : kk - w.k0f;
+        if (!FW && kl < 0 && in) fix |= 1u << u;  // below the LDS window: done below
+        const int q = FW ? kl : max(kl, 0);
         const double2 t0 = s_ft[row + q], t1 = s_ft[row + q + 1];
         const double fb = t0.y + dk * (t1.y - t0.y);
         double fb1 = fb;
@@ -398,12 +401,12 @@ __device__ __forceinline__ void force_group_pf(const AtomsDev& a, const ListDev&
         // FPOTR/r^2 + (FPOTB(KTAB) F'_i + FPOTB(KTAB1) F'_j)/r  (CALFORCE_KERNEL)
         const double ft = (t0.x + dk * (t1.x - t0.x)) * (inv_r * inv_r) +
                           (fb * denki + fb1 * pj[u].w) * inv_r;
-        const double fortot = (in && kl >= 0) ? ft : 0.0;
+        const double fortot = (in && (FW || kl >= 0)) ? ft : 0.0;
         fpx = fpx + fortot * sx;
         fpy = fpy + fortot * sy;
         fpz = fpz + fortot * sz;
     }
-    if (fix) {
+    if (!FW && fix) {
         // rare close pairs: reload the neighbour and take the knots of the global table
 #pragma unroll
         for (int u = 0; u < G; ++u) {
@@ -619,9 +622,16 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_density_lt(AtomsDev a, ListDev 
                     unsigned long long* mkp = WM ? &mk : nullptr;
 #pragma unroll
                     for (int u = 0; u < kGroupD; ++u) js[u] = base + u * TPA < kv ? js0[u] : i;
-                    for (; base + (kGroupD - 1) * TPA < kv; base += TPA * kGroupD)
-                        den0 = density_group_pf<kGroupD, MT, TPA>(a, l, t, b, w, s_pb, li, i, ti,
-                                                                  kv, base, pxy, pz, den0, js, mkp);
+                    auto full = [&](auto fw) {
+                        constexpr bool FW = decltype(fw)::value;
+                        for (; base + (kGroupD - 1) * TPA < kv; base += TPA * kGroupD)
+                            den0 = density_group_pf<kGroupD, MT, TPA, FW>(
+                                a, l, t, b, w, s_pb, li, i, ti, kv, base, pxy, pz, den0, js, mkp);
+                    };
+                    if (w.k0d == 0)
+                        full(std::true_type{});
+                    else
+                        full(std::false_type{});
                     // the last kGroupD - 1 slots at most, in pairs and singles, from the indices
                     // the last group (or the atom's batch) already read into js
 #pragma unroll
@@ -733,9 +743,16 @@ __global__ __launch_bounds__(kLtMaxBlock) void k_force_lt(AtomsDev a, ListDev l,
 #pragma unroll
                 for (int u = 0; u < kGroupF; ++u)
                     if (base + u * TPA >= kv) js[u] = i;
-                for (; base + (kGroupF - 1) * TPA < kv; base += TPA * kGroupF)
-                    force_group_pf<kGroupF, MT, TPA>(a, l, t, b, w, s_ft, li, i, ti, kv, base, pi,
-                                                     denki, fpx, fpy, fpz, js);
+                auto full = [&](auto fw) {
+                    constexpr bool FW = decltype(fw)::value;
+                    for (; base + (kGroupF - 1) * TPA < kv; base += TPA * kGroupF)
+                        force_group_pf<kGroupF, MT, TPA, FW>(a, l, t, b, w, s_ft, li, i, ti, kv,
+                                                             base, pi, denki, fpx, fpy, fpz, js);
+                };
+                if (w.k0f == 0)
+                    full(std::true_type{});
+                else
+                    full(std::false_type{});
                 // the last kGroupF - 1 slots at most, in pairs and singles, from the indices the
                 // last group (or the atom's batch) already read into js
 #pragma unroll
