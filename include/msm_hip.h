@@ -364,6 +364,36 @@ int msm_adjoint(msm_sim* sim, const double* dldf, const double* dlde, double* g_
                 double* g_fpotr, double* g_potb, double* g_fpotb, double* g_fembd,
                 double* g_dfembd);
 
+/* Device-resident force matching (no counterpart in the reference, which has no derivative
+ * code; for fitting loops that keep every per-atom array on the device).  The potential is a
+ * cubic B-spline model of one kind: coef holds 3 rows of ld doubles (row-major), the pair row
+ * u(s), the density row rho(s) (s = sqrt(r)*CSI, knots 1..NTAB) and the embedding row F(m)
+ * (m = rho/RHOD + 1, knots 1..NEMBD; ignored for FS tables); coefficients beyond NTAB/NEMBD are
+ * not read.  At knot i a row c has value v = ((c[i-2] + 4 c[i-1]) + c[i]) / 6 and slope
+ * s = (c[i] - c[i-2]) / 2 (c zero outside the row), and the tables are
+ *   POTR = v(u) (i/CSI)^2, FPOTR = -i s(u), POTB = v(rho), FPOTB = ((-s(rho)) CSI^2) / (2i),
+ *   FEMBD = v(F), DFEMBD = s(F) / RHOD.
+ *
+ * msm_set_spline_table builds those tables on the device into the buffers of the table set
+ * by msm_set_force_table (whose NTAB, CSI, NEMBD, RHOD, KPAIR and KEMBD it keeps; it must hold
+ * one pair kind and, for EAM, one embedding kind; ld >= max(NTAB, NEMBD)).  The result is
+ * bit-identical to msm_set_force_table on the same tables evaluated on the host with these
+ * formulas.  coef is a device pointer when coef_on_device is nonzero.
+ *
+ * msm_fm_set_targets keeps the target forces (3*N doubles, caller order, FP(N,3) layout) and box
+ * energies (one per box) on the device with the energy weight w_e >= 0.
+ *
+ * msm_fm_loss_grad evaluates the forces and box energies with the current tables and returns
+ *   L = sum_i |F_i - F*_i|^2 / N + w_e sum_b ((E_b - E*_b) / N_b)^2 / NBOX
+ * in *loss and, when coef_grad is not NULL, dL/dcoef (3 rows of ld, zero beyond NTAB/NEMBD)
+ * through the adjoint passes of msm_adjoint and the transpose of the table map above
+ * (coef_grad is a device pointer when grad_on_device is nonzero).  Only the loss (and the
+ * gradient, when on the host) cross PCIe.  Sums run in a fixed order except the adjoint's LDS
+ * atomics, so the gradient of repeated calls agrees to rounding. */
+int msm_set_spline_table(msm_sim* sim, const double* coef, int ld, int coef_on_device);
+int msm_fm_set_targets(msm_sim* sim, const double* f_target, const double* e_target, double w_e);
+int msm_fm_loss_grad(msm_sim* sim, double* loss, double* coef_grad, int ld, int grad_on_device);
+
 /* Kernel-variant autotuner (no counterpart in the reference, whose kernel shapes are fixed at
  * compile time).  On the current atoms and list it times `reps` rebuilds with each list-build
  * kernel and `reps` density + force passes (no integration) of every applicable variant of the

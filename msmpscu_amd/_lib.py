@@ -105,6 +105,9 @@ SIGNATURES = {
                                C.c_double, _ip, _dp, _dp]),
     "msm_cg": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _ip, _dp]),
     "msm_adjoint": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "msm_set_spline_table": (C.c_int, [_vp, _vp, C.c_int, C.c_int]),
+    "msm_fm_set_targets": (C.c_int, [_vp, _dp, _dp, C.c_double]),
+    "msm_fm_loss_grad": (C.c_int, [_vp, _dp, _vp, C.c_int, C.c_int]),
 }
 
 MSM_OPT_TRACK_DIS = 1

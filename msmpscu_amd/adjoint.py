@@ -4,8 +4,8 @@ The reference has no derivative code; this module adds reverse mode on top of th
 path (the device force and energy passes of MD_FS/EAM_ForceTable_GPU.F90) through the C-ABI entry
 point msm_adjoint, which returns dL/d(knot) of every table.
 
-Coefficient model (one atom type, EAM tables): three cubic B-splines with unit knot spacing in
-the reference's own table coordinates, 3 x NTAB coefficients in all:
+Coefficient model (one atom type; FS tables use the first two rows): three cubic B-splines with
+unit knot spacing in the reference's own table coordinates, 3 x max(NTAB, NEMBD) coefficients:
   u(r)   = V(r)/2    in s = sqrt(r) * CSI     (pair)       POTR_i  = u(i) r_i
                                                            FPOTR_i = -2 u'(r_i) r_i = -i du/ds(i)
   rho(r)             in s = sqrt(r) * CSI     (density)    POTB_i  = rho(i)
@@ -60,7 +60,8 @@ class SplineModel:
 
     @classmethod
     def from_table(cls, ft: MDForceTable) -> "SplineModel":
-        return cls(int(ft.NTAB), float(ft.CSI), int(ft.NEMBD), float(ft.RHOD))
+        eam = ft.PotType.strip() == "EAM_TYPE"
+        return cls(int(ft.NTAB), float(ft.CSI), int(ft.NEMBD) if eam else 0, float(ft.RHOD))
 
     @property
     def shape(self):
@@ -78,7 +79,7 @@ class SplineModel:
             "POTR": _val(cp) * r,
             "FPOTR": -i * _slope(cp),
             "POTB": _val(cq),
-            "FPOTB": -_slope(cq) * self.csi ** 2 / (2.0 * i),
+            "FPOTB": -_slope(cq) * (self.csi * self.csi) / (2.0 * i),
             "FEMBD": _val(ce),
             "DFEMBD": _slope(ce) / self.rhod,
         }
@@ -88,7 +89,7 @@ class SplineModel:
         i, r = self._geom()
         out = np.zeros(self.shape)
         out[0, : self.ntab] = _val_t(g["POTR"] * r) + _slope_t(-i * g["FPOTR"])
-        out[1, : self.ntab] = _val_t(g["POTB"]) + _slope_t(-g["FPOTB"] * self.csi ** 2 /
+        out[1, : self.ntab] = _val_t(g["POTB"]) + _slope_t(-g["FPOTB"] * (self.csi * self.csi) /
                                                            (2.0 * i))
         out[2, : self.nembd] = _val_t(g["FEMBD"]) + _slope_t(g["DFEMBD"] / self.rhod)
         return out
@@ -110,7 +111,8 @@ class SplineModel:
         c = np.zeros(self.shape)
         c[0, : self.ntab] = solve(np.asarray(ft.POTR[0], dtype=np.float64) / r)
         c[1, : self.ntab] = solve(np.asarray(ft.POTB[0], dtype=np.float64))
-        c[2, : self.nembd] = solve(np.asarray(ft.FEMBD[0], dtype=np.float64))
+        if self.nembd:
+            c[2, : self.nembd] = solve(np.asarray(ft.FEMBD[0], dtype=np.float64))
         return c
 
     def apply(self, c: np.ndarray, ft: MDForceTable) -> MDForceTable:
@@ -120,8 +122,9 @@ class SplineModel:
         ft.FPOTR = t["FPOTR"][None, :].copy(order="F")
         ft.POTB = t["POTB"][None, :].copy(order="F")
         ft.FPOTB = t["FPOTB"][None, :].copy(order="F")
-        ft.FEMBD = t["FEMBD"][None, :].copy(order="F")
-        ft.DFEMBD = t["DFEMBD"][None, :].copy(order="F")
+        if self.nembd:
+            ft.FEMBD = t["FEMBD"][None, :].copy(order="F")
+            ft.DFEMBD = t["DFEMBD"][None, :].copy(order="F")
         return ft
 
 
@@ -235,6 +238,57 @@ class ForceMatch:
 
     def close(self):
         self.dev.close()
+
+
+class DeviceForceMatch(ForceMatch):
+    """ForceMatch with the whole evaluation on the device: msm_set_spline_table builds the
+    tables from the coefficients in one kernel, msm_fm_loss_grad runs the forward passes, the
+    loss against targets kept on the device, the adjoint passes and the coefficient transpose,
+    and only the loss and dL/dc (3 x ld doubles) come back to the host.  Same L and dL/dc as
+    ForceMatch (the tables are bit-identical; the loss sums run in another order)."""
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        self.ld = self.model.shape[1]
+
+    def _coef(self, c: np.ndarray) -> np.ndarray:
+        c = np.ascontiguousarray(np.asarray(c, dtype=np.float64))
+        if c.shape != (3, self.ld):
+            raise ValueError(f"coefficients must have shape (3, {self.ld})")
+        return c
+
+    def set_tables(self, c: np.ndarray):
+        check(lib().msm_set_spline_table(self.dev.h, dptr(self._coef(c)), self.ld, 0),
+              "msm_set_spline_table", self.dev.h)
+
+    def forward(self, c: np.ndarray):
+        self.set_tables(c)
+        self.dev.calc_force()
+        self.dev.calc_epot()
+        return self.dev.get_forces(), self.dev.get_box_epot()
+
+    def set_targets(self, c_target: np.ndarray):
+        f, e = self.forward(c_target)
+        self.f_target, self.e_target = f, e
+        ff = np.ascontiguousarray(f.ravel(order="F"))
+        ee = np.ascontiguousarray(e, dtype=np.float64)
+        check(lib().msm_fm_set_targets(self.dev.h, dptr(ff), dptr(ee), float(self.w_e)),
+              "msm_fm_set_targets", self.dev.h)
+
+    def loss(self, c: np.ndarray) -> float:
+        self.set_tables(c)
+        out = np.zeros(1)
+        check(lib().msm_fm_loss_grad(self.dev.h, dptr(out), None, self.ld, 0),
+              "msm_fm_loss_grad", self.dev.h)
+        return float(out[0])
+
+    def loss_and_grad(self, c: np.ndarray):
+        self.set_tables(c)
+        out = np.zeros(1)
+        g = np.zeros((3, self.ld))
+        check(lib().msm_fm_loss_grad(self.dev.h, dptr(out), dptr(g), self.ld, 0),
+              "msm_fm_loss_grad", self.dev.h)
+        return float(out[0]), g
 
 
 def make_torch_function():

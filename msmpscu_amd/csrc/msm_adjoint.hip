@@ -26,6 +26,8 @@
 // terms into per-workgroup bins in LDS (ds_add_f64), then an ordered sum of the workgroups'
 // bins.  The bins of one workgroup see their terms in an order that can vary from run to run,
 // so the result is reproducible to rounding, not bit for bit.
+#include <algorithm>
+
 #include "msm_device.h"
 #include "msm_internal.h"
 
@@ -400,6 +402,233 @@ void launch_adjoint(const AtomsDev& a, const ListDev& l, const TabDev& t, const 
     const unsigned gr2 = static_cast<unsigned>((nb2 + kRedBins - 1) / kRedBins);
     hipLaunchKernelGGL(k_adj_reduce, dim3(gr2), dim3(kRedBins * kRedSplit), 0, s, d.part, nwg,
                        d.part_stride, nb2, out_dens);
+}
+
+// --------------------------------------------------------------------------------------------
+// Device-resident force matching: the tables of the cubic B-spline coefficient model, the loss
+// with the inputs of the adjoint, and the coefficient gradient (the transpose of the table map).
+// The arithmetic is operation for operation that of the host model (adjoint.py SplineModel), with
+// contraction off, so the tables and gradients equal the host's bit for bit.
+// --------------------------------------------------------------------------------------------
+
+namespace {
+
+// Value and slope at knot i (1..n) of a unit-spacing cubic B-spline with coefficients c[0..n-1]
+// (zero outside): ((c[i-2] + 4 c[i-1]) + c[i]) / 6 and (c[i] - c[i-2]) / 2; zero for knots
+// outside 1..n (the tables' padding).
+__device__ __forceinline__ double bs_val(const double* c, int n, int i) {
+#pragma clang fp contract(off)
+    if (i < 1 || i > n) return 0.0;
+    const double a = i >= 2 ? c[i - 2] : 0.0, d = i < n ? c[i] : 0.0;
+    return ((a + 4.0 * c[i - 1]) + d) / 6.0;
+}
+
+__device__ __forceinline__ double bs_slope(const double* c, int n, int i) {
+#pragma clang fp contract(off)
+    if (i < 1 || i > n) return 0.0;
+    const double a = i >= 2 ? c[i - 2] : 0.0, d = i < n ? c[i] : 0.0;
+    return 0.5 * (d - a);
+}
+
+// The spline tables at knot i of one kind (zero outside 1..ntab / 1..nembd):
+//   POTR = v r, FPOTR = -i s, POTB = v, FPOTB = ((-s) CSI^2) / (2i), r = (i/CSI)^2,
+//   FEMBD = v, DFEMBD = s / RHOD.
+struct SplineTab {
+    const double *cp, *cq, *ce;
+    int ntab, nembd;
+    double csi, csi2, rhod;
+    __device__ double potr(int i) const {
+#pragma clang fp contract(off)
+        if (i < 1 || i > ntab) return 0.0;
+        const double x = static_cast<double>(i) / csi;
+        return bs_val(cp, ntab, i) * (x * x);
+    }
+    __device__ double fpotr(int i) const {
+#pragma clang fp contract(off)
+        if (i < 1 || i > ntab) return 0.0;
+        return -static_cast<double>(i) * bs_slope(cp, ntab, i);
+    }
+    __device__ double potb(int i) const { return bs_val(cq, ntab, i); }
+    __device__ double fpotb(int i) const {
+#pragma clang fp contract(off)
+        if (i < 1 || i > ntab) return 0.0;
+        return (-bs_slope(cq, ntab, i) * csi2) / (2.0 * static_cast<double>(i));
+    }
+    __device__ double fembd(int i) const { return bs_val(ce, nembd, i); }
+    __device__ double dfembd(int i) const {
+#pragma clang fp contract(off)
+        if (i < 1 || i > nembd) return 0.0;
+        return bs_slope(ce, nembd, i) / rhod;
+    }
+};
+
+// One thread per padded knot a: the device table entries msm_set_force_table builds on the host,
+// (T(a), T(a+1) - T(a)) pairs, from the spline tables.
+__global__ void k_spline_tables(SplineTab st, double2* __restrict__ b2, double4* __restrict__ f4,
+                                double4* __restrict__ e4, double4* __restrict__ m4) {
+#pragma clang fp contract(off)
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < st.ntab + 2) {
+        const double pb = st.potb(a), pb1 = st.potb(a + 1);
+        const double fr = st.fpotr(a), fr1 = st.fpotr(a + 1);
+        const double fb = st.fpotb(a), fb1 = st.fpotb(a + 1);
+        const double pr = st.potr(a), pr1 = st.potr(a + 1);
+        b2[a] = make_double2(pb, pb1 - pb);
+        f4[a] = make_double4(fr, fr1 - fr, fb, fb1 - fb);
+        e4[a] = make_double4(pr, pr1 - pr, pb, pb1 - pb);
+    }
+    if (m4 && a < st.nembd + 2) {
+        const double f0 = st.fembd(a), f1 = st.fembd(a + 1);
+        const double d0 = st.dfembd(a), d1 = st.dfembd(a + 1);
+        m4[a] = make_double4(f0, f1 - f0, d0, d1 - d0);
+    }
+}
+
+// Per sorted atom p (caller index gid[p]): dF = F_p - F*, dL/dF = (2 dF) / N into the caller's
+// order and FP(N,3) layout (the adjoint's input), and |dF|^2 summed per workgroup in a fixed
+// order (grid-stride partials, then a fixed-shape tree).
+__global__ __launch_bounds__(kAdjBlock) void k_fm_forces(const double* __restrict__ fx,
+                                                         const double* __restrict__ fy,
+                                                         const double* __restrict__ fz,
+                                                         const int* __restrict__ gid, int n,
+                                                         const double* __restrict__ ft,
+                                                         double* __restrict__ dldf,
+                                                         double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ double lds[kAdjBlock];
+    double acc = 0.0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const size_t c = static_cast<size_t>(gid[p]);
+        const double dx = fx[p] - ft[c], dy = fy[p] - ft[c + n], dz = fz[p] - ft[c + 2 * static_cast<size_t>(n)];
+        const double nd = static_cast<double>(n);
+        dldf[c] = (2.0 * dx) / nd;
+        dldf[c + n] = (2.0 * dy) / nd;
+        dldf[c + 2 * static_cast<size_t>(n)] = (2.0 * dz) / nd;
+        acc = acc + ((dx * dx + dy * dy) + dz * dz);
+    }
+    lds[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kAdjBlock / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = lds[0];
+}
+
+// One workgroup: L = (sum of the force partials) / N + w_e sum_b ((E_b - E*_b)/N_b)^2 / NBOX and
+// dL/dE_b = 2 w_e (E_b - E*_b) / (N_b^2 NBOX), each sum in a fixed order.
+__global__ __launch_bounds__(kAdjBlock) void k_fm_final(const double* __restrict__ part, int np,
+                                                        const double* __restrict__ ebox,
+                                                        const double* __restrict__ et,
+                                                        const BoxDev* __restrict__ boxes,
+                                                        int nbox, int n, double w_e,
+                                                        double* __restrict__ gbox,
+                                                        double* __restrict__ loss) {
+#pragma clang fp contract(off)
+    __shared__ double lf[kAdjBlock], le[kAdjBlock];
+    double sf = 0.0, se = 0.0;
+    for (int k = threadIdx.x; k < np; k += kAdjBlock) sf = sf + part[k];
+    for (int b = threadIdx.x; b < nbox; b += kAdjBlock) {
+        const double nb = static_cast<double>(boxes[b].natoms);
+        const double d = ebox[b] - et[b];
+        const double de = d / nb;
+        gbox[b] = 2.0 * w_e * d / (nb * nb * static_cast<double>(nbox));
+        se = se + de * de;
+    }
+    lf[threadIdx.x] = sf;
+    le[threadIdx.x] = se;
+    __syncthreads();
+    for (int w = kAdjBlock / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            lf[threadIdx.x] += lf[threadIdx.x + w];
+            le[threadIdx.x] += le[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *loss = lf[0] / static_cast<double>(n) + w_e * le[0] / static_cast<double>(nbox);
+}
+
+// Knot gradient y(k), k = 1..n, from a bin row (zero outside 1..n).
+__device__ __forceinline__ double bin_at(const double* row, int n, int k) {
+    return (k < 1 || k > n) ? 0.0 : row[k];
+}
+
+// dL/dc of coefficient j of each row, the transpose of the table map:
+//   row 0: V^T(POTR' r) + S^T(-i FPOTR'), row 1: V^T(POTB') + S^T(((-FPOTB') CSI^2) / (2i)),
+//   row 2: V^T(FEMBD') + S^T(DFEMBD' / RHOD),
+// with V^T(y)_j = ((y(j) + 4 y(j+1)) + y(j+2)) / 6 and S^T(y)_j = (y(j) - y(j+2)) / 2 over the
+// knots k = 1..n (row 2 zero for FS tables).
+__global__ void k_spline_grad(SplineTab st, const double* __restrict__ pairs,
+                              const double* __restrict__ dens, int nt, int ne, int eam, int ld,
+                              double* __restrict__ g) {
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ld) return;
+    const double* g_fpotr = pairs;
+    const double* g_fpotb = pairs + nt;
+    const double* g_potr = pairs + 2 * static_cast<size_t>(nt);
+    const double* g_potb = dens;
+    const int n = st.ntab;
+    auto yr = [&](int k) {
+        if (k < 1 || k > n) return 0.0;
+        const double x = static_cast<double>(k) / st.csi;
+        return bin_at(g_potr, n, k) * (x * x);
+    };
+    auto yf = [&](int k) { return (k < 1 || k > n) ? 0.0 : -static_cast<double>(k) * g_fpotr[k]; };
+    auto yb = [&](int k) { return bin_at(g_potb, n, k); };
+    auto yfb = [&](int k) {
+        return (k < 1 || k > n) ? 0.0 : (-g_fpotb[k] * st.csi2) / (2.0 * static_cast<double>(k));
+    };
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    if (j < n) {
+        r0 = ((yr(j) + 4.0 * yr(j + 1)) + yr(j + 2)) / 6.0 + 0.5 * (yf(j) - yf(j + 2));
+        r1 = ((yb(j) + 4.0 * yb(j + 1)) + yb(j + 2)) / 6.0 + 0.5 * (yfb(j) - yfb(j + 2));
+    }
+    if (eam && j < st.nembd) {
+        const int m = st.nembd;
+        const double* g_fembd = dens + nt;
+        const double* g_dfembd = dens + nt + ne;
+        auto ye = [&](int k) { return bin_at(g_fembd, m, k); };
+        auto yd = [&](int k) { return (k < 1 || k > m) ? 0.0 : g_dfembd[k] / st.rhod; };
+        r2 = ((ye(j) + 4.0 * ye(j + 1)) + ye(j + 2)) / 6.0 + 0.5 * (yd(j) - yd(j + 2));
+    }
+    g[j] = r0;
+    g[ld + j] = r1;
+    g[2 * static_cast<size_t>(ld) + j] = r2;
+}
+
+}  // namespace
+
+void launch_spline_tables(const double* coef, int ld, const TabDev& t, double2* b2, double4* f4,
+                          double4* e4, double4* m4, hipStream_t s) {
+    SplineTab st{coef, coef + ld, coef + 2 * static_cast<size_t>(ld), t.ntab2 - 2,
+                 t.pot_type != 0 ? t.nembd : 0, t.csi, t.csi * t.csi, t.rhod};
+    const int m = std::max(t.ntab2, t.pot_type != 0 ? t.nembd2 : 0);
+    hipLaunchKernelGGL(k_spline_tables, dim3((m + 255) / 256), dim3(256), 0, s, st, b2, f4, e4,
+                       t.pot_type != 0 ? m4 : nullptr);
+}
+
+int fm_parts(int n) { return std::max(1, std::min(512, (n + kAdjBlock - 1) / kAdjBlock)); }
+
+void launch_fm_loss(const double* fx, const double* fy, const double* fz, const int* gid, int n,
+                    const double* ft, const double* ebox, const double* et, const BoxDev* boxes,
+                    int nbox, double w_e, double* dldf, double* gbox, double* part, double* loss,
+                    hipStream_t s) {
+    const int np = fm_parts(n);
+    hipLaunchKernelGGL(k_fm_forces, dim3(np), dim3(kAdjBlock), 0, s, fx, fy, fz, gid, n, ft,
+                       dldf, part);
+    hipLaunchKernelGGL(k_fm_final, dim3(1), dim3(kAdjBlock), 0, s, part, np, ebox, et, boxes, nbox,
+                       n, w_e, gbox, loss);
+}
+
+void launch_spline_grad(const double* pairs, const double* dens, const TabDev& t, int ld,
+                        double* g, hipStream_t s) {
+    const bool eam = t.pot_type != 0;
+    SplineTab st{nullptr, nullptr, nullptr, t.ntab2 - 2, eam ? t.nembd : 0, t.csi, t.csi * t.csi,
+                 t.rhod};
+    hipLaunchKernelGGL(k_spline_grad, dim3((ld + 255) / 256), dim3(256), 0, s, st, pairs, dens,
+                       t.ntab2, eam ? t.nembd2 : 0, eam ? 1 : 0, ld, g);
 }
 
 }  // namespace msm

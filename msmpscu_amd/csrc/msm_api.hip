@@ -123,6 +123,15 @@ struct msm_sim {
            *adj_lam = nullptr, *adj_part = nullptr, *adj_out = nullptr;
     size_t adj_cap_n = 0, adj_cap_part = 0, adj_cap_out = 0, adj_cap_box = 0;
 
+    // device-resident force matching (msm_set_spline_table, msm_fm_*): coefficients, targets
+    // (caller order), scratch of the loss, the loss and the coefficient gradient
+    double *fm_coef = nullptr, *fm_ft = nullptr, *fm_et = nullptr, *fm_part = nullptr,
+           *fm_loss = nullptr, *fm_grad = nullptr;
+    size_t fm_cap_coef = 0, fm_cap_ft = 0, fm_cap_et = 0, fm_cap_part = 0, fm_cap_loss = 0,
+           fm_cap_grad = 0;
+    int fm_n = -1;         // atom count the targets were given for (-1: none)
+    double fm_w_e = 0.0;
+
     // LDS-staged list bricks
     int nbricks = 0;
     int nl_lanes = 1;  // lanes per home atom of the list build
@@ -753,7 +762,8 @@ int msm_destroy(msm_sim* s) {
     clear_graphs(s);
     free_atoms(s);
     for (double** p : {&s->adj_dldf, &s->adj_gbox, &s->adj_g, &s->adj_rho, &s->adj_lam,
-                       &s->adj_part, &s->adj_out})
+                       &s->adj_part, &s->adj_out, &s->fm_coef, &s->fm_ft, &s->fm_et,
+                       &s->fm_part, &s->fm_loss, &s->fm_grad})
         dfree(p);
     for (auto& p : s->pending) {
         (void)hipEventDestroy(std::get<1>(p));
@@ -2215,23 +2225,16 @@ int msm_autotune(msm_sim* s, int reps, int* choice, double* best_ms) {
     return read_flags(s);
 }
 
-int msm_adjoint(msm_sim* s, const double* dldf, const double* dlde, double* g_potr,
-                double* g_fpotr, double* g_potb, double* g_fpotb, double* g_fembd,
-                double* g_dfembd) {
-    int rc = check_ready(s, true);
-    if (rc != MSM_OK) return rc;
-    if (!dldf || !dlde || !g_potr || !g_fpotr || !g_potb || !g_fpotb)
-        return set_err(s, MSM_ERR_ARG, "null argument");
-    const bool eam = s->tab.pot_type != 0;
-    if (eam && (!g_fembd || !g_dfembd))
-        return set_err(s, MSM_ERR_ARG, "EAM tables need the embedding gradient arrays");
-    (void)hipSetDevice(s->device);
+// Scratch of the adjoint passes for the current atoms and table (kept between calls); nb1, nb2:
+// the bin counts of the pair and density passes, stride: the per-workgroup bin stride.
+static int adj_reserve(msm_sim* s, size_t& nb1, size_t& nb2, size_t& stride) {
+    int rc = MSM_OK;
     const int n = s->n, nbox = static_cast<int>(s->boxes.size());
-    const int nt = s->tab.ntab2, ne = eam ? s->tab.nembd2 : 0;
     int nbin1 = 0, nbin2 = 0;
     msm::adjoint_bins(s->tab, nbin1, nbin2);
-    const size_t nb1 = static_cast<size_t>(nbin1), nb2 = static_cast<size_t>(nbin2);
-    const size_t stride = std::max(nb1, nb2);
+    nb1 = static_cast<size_t>(nbin1);
+    nb2 = static_cast<size_t>(nbin2);
+    stride = std::max(nb1, nb2);
     const size_t npart = stride * msm::adjoint_workgroups(n);
     if (s->adj_cap_n < static_cast<size_t>(s->n_alloc)) {
         dfree(&s->adj_dldf);
@@ -2264,10 +2267,12 @@ int msm_adjoint(msm_sim* s, const double* dldf, const double* dlde, double* g_po
         if ((rc = dalloc(s, &s->adj_gbox, nbox)) != MSM_OK) return rc;
         s->adj_cap_box = nbox;
     }
-    HIPCHK(s, hipMemcpyAsync(s->adj_dldf, dldf, 3 * static_cast<size_t>(n) * sizeof(double),
-                             hipMemcpyHostToDevice, s->stream));
-    HIPCHK(s, hipMemcpyAsync(s->adj_gbox, dlde, nbox * sizeof(double), hipMemcpyHostToDevice,
-                             s->stream));
+    return MSM_OK;
+}
+
+// The adjoint passes on dL/dF (s->adj_dldf, caller order) and dL/dE (s->adj_gbox) into the bins
+// s->adj_out (pairs) and s->adj_out + nb1 (density).
+static void adj_launch(msm_sim* s, size_t nb1, size_t stride) {
     msm::AdjDev d{};
     d.g = s->adj_g;
     d.gbox = s->adj_gbox;
@@ -2276,12 +2281,32 @@ int msm_adjoint(msm_sim* s, const double* dldf, const double* dlde, double* g_po
     d.part = s->adj_part;
     d.part_stride = stride;
     AtomsDev a = atoms_view(s, s->scur, s->pcur);
-    {
-        TimedScope ts(s, MSM_KCLASS_ADJOINT);
-        msm::launch_adjoint(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
-                            s->adj_dldf, s->set[s->scur].gid, d, s->adj_out, s->adj_out + nb1,
-                            s->stream);
-    }
+    TimedScope ts(s, MSM_KCLASS_ADJOINT);
+    msm::launch_adjoint(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
+                        s->adj_dldf, s->set[s->scur].gid, d, s->adj_out, s->adj_out + nb1,
+                        s->stream);
+}
+
+int msm_adjoint(msm_sim* s, const double* dldf, const double* dlde, double* g_potr,
+                double* g_fpotr, double* g_potb, double* g_fpotb, double* g_fembd,
+                double* g_dfembd) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (!dldf || !dlde || !g_potr || !g_fpotr || !g_potb || !g_fpotb)
+        return set_err(s, MSM_ERR_ARG, "null argument");
+    const bool eam = s->tab.pot_type != 0;
+    if (eam && (!g_fembd || !g_dfembd))
+        return set_err(s, MSM_ERR_ARG, "EAM tables need the embedding gradient arrays");
+    (void)hipSetDevice(s->device);
+    const int n = s->n, nbox = static_cast<int>(s->boxes.size());
+    const int nt = s->tab.ntab2, ne = eam ? s->tab.nembd2 : 0;
+    size_t nb1 = 0, nb2 = 0, stride = 0;
+    if ((rc = adj_reserve(s, nb1, nb2, stride)) != MSM_OK) return rc;
+    HIPCHK(s, hipMemcpyAsync(s->adj_dldf, dldf, 3 * static_cast<size_t>(n) * sizeof(double),
+                             hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->adj_gbox, dlde, nbox * sizeof(double), hipMemcpyHostToDevice,
+                             s->stream));
+    adj_launch(s, nb1, stride);
     std::vector<double> h(nb1 + nb2);
     HIPCHK(s, hipMemcpyAsync(h.data(), s->adj_out, h.size() * sizeof(double),
                              hipMemcpyDeviceToHost, s->stream));
@@ -2329,6 +2354,107 @@ int msm_adjoint(msm_sim* s, const double* dldf, const double* dlde, double* g_po
         }
     }
     return MSM_OK;
+}
+
+// Reserves *p for at least m doubles (capacity in cap).
+static int fm_reserve(msm_sim* s, double** p, size_t m, size_t& cap) {
+    if (cap >= m && *p) return MSM_OK;
+    dfree(p);
+    cap = 0;
+    int rc = dalloc(s, p, m);
+    if (rc == MSM_OK) cap = m;
+    return rc;
+}
+
+// The spline model's shape checks: one pair kind (one embedding kind for EAM) and ld covering
+// both grids.
+static int fm_check_model(msm_sim* s, int ld) {
+    if (s->tab.nkind != 1 || (s->tab.pot_type != 0 && s->tab.nkind1 != 1))
+        return set_err(s, MSM_ERR_ARG, "the spline model covers tables of one kind");
+    const int m = std::max(s->tab.ntab2 - 2, s->tab.pot_type != 0 ? s->tab.nembd : 0);
+    if (ld < m) return set_err(s, MSM_ERR_ARG, "ld smaller than NTAB or NEMBD");
+    return MSM_OK;
+}
+
+int msm_set_spline_table(msm_sim* s, const double* coef, int ld, int coef_on_device) {
+    if (!s || !coef) return MSM_ERR_ARG;
+    if (!s->has_table) return set_err(s, MSM_ERR_STATE, "no force table (msm_set_force_table)");
+    int rc = fm_check_model(s, ld);
+    if (rc != MSM_OK) return rc;
+    (void)hipSetDevice(s->device);
+    const size_t m = 3 * static_cast<size_t>(ld);
+    const double* c = coef;
+    if (!coef_on_device) {
+        if ((rc = fm_reserve(s, &s->fm_coef, m, s->fm_cap_coef)) != MSM_OK) return rc;
+        HIPCHK(s, hipMemcpyAsync(s->fm_coef, coef, m * sizeof(double), hipMemcpyHostToDevice,
+                                 s->stream));
+        c = s->fm_coef;
+    }
+    // same buffers, sizes and grid: the table descriptor and captured segments stay valid
+    msm::launch_spline_tables(c, ld, s->tab, s->d_b2, s->d_f4, s->d_e4, s->d_m4, s->stream);
+    HIPCHK(s, hipGetLastError());
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return MSM_OK;
+}
+
+int msm_fm_set_targets(msm_sim* s, const double* f_target, const double* e_target, double w_e) {
+    if (!s || !f_target || !e_target) return MSM_ERR_ARG;
+    if (!s->has_atoms || !s->has_boxes) return set_err(s, MSM_ERR_STATE, "no atoms or boxes");
+    if (!(w_e >= 0.0)) return set_err(s, MSM_ERR_ARG, "w_e must be non-negative");
+    (void)hipSetDevice(s->device);
+    const size_t n3 = 3 * static_cast<size_t>(std::max(s->n, 1));
+    const size_t nbox = s->boxes.size();
+    int rc = fm_reserve(s, &s->fm_ft, n3, s->fm_cap_ft);
+    if (rc == MSM_OK) rc = fm_reserve(s, &s->fm_et, nbox, s->fm_cap_et);
+    if (rc != MSM_OK) return rc;
+    HIPCHK(s, hipMemcpyAsync(s->fm_ft, f_target, 3 * static_cast<size_t>(s->n) * sizeof(double),
+                             hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->fm_et, e_target, nbox * sizeof(double), hipMemcpyHostToDevice,
+                             s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->fm_n = s->n;
+    s->fm_w_e = w_e;
+    return MSM_OK;
+}
+
+int msm_fm_loss_grad(msm_sim* s, double* loss, double* coef_grad, int ld, int grad_on_device) {
+    int rc = check_ready(s, true);
+    if (rc != MSM_OK) return rc;
+    if (!loss) return set_err(s, MSM_ERR_ARG, "null argument");
+    if (s->fm_n != s->n || s->n < 1)
+        return set_err(s, MSM_ERR_STATE, "no targets for these atoms (msm_fm_set_targets)");
+    if (coef_grad && (rc = fm_check_model(s, ld)) != MSM_OK) return rc;
+    (void)hipSetDevice(s->device);
+    const int n = s->n, nbox = static_cast<int>(s->boxes.size());
+    size_t nb1 = 0, nb2 = 0, stride = 0;
+    if ((rc = adj_reserve(s, nb1, nb2, stride)) != MSM_OK) return rc;
+    if ((rc = fm_reserve(s, &s->fm_part, msm::fm_parts(n), s->fm_cap_part)) != MSM_OK) return rc;
+    if ((rc = fm_reserve(s, &s->fm_loss, 1, s->fm_cap_loss)) != MSM_OK) return rc;
+    if (coef_grad && !grad_on_device &&
+        (rc = fm_reserve(s, &s->fm_grad, 3 * static_cast<size_t>(ld), s->fm_cap_grad)) != MSM_OK)
+        return rc;
+    clear_flags(s);
+    enqueue_density(s);
+    enqueue_force(s, 0, 1, 0.0);
+    AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    msm::launch_epot(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), s->d_err,
+                     s->stream);
+    msm::launch_box_sum(s->epot, s->sb.start, s->d_boxes, nbox, s->d_ebox, s->stream);
+    const AtomSet& z = s->set[s->scur];
+    const size_t na = s->n_alloc;
+    msm::launch_fm_loss(z.f, z.f + na, z.f + 2 * na, z.gid, n, s->fm_ft, s->d_ebox, s->fm_et,
+                        s->d_boxes, nbox, s->fm_w_e, s->adj_dldf, s->adj_gbox, s->fm_part,
+                        s->fm_loss, s->stream);
+    if (coef_grad) {
+        adj_launch(s, nb1, stride);
+        double* g = grad_on_device ? coef_grad : s->fm_grad;
+        msm::launch_spline_grad(s->adj_out, s->adj_out + nb1, s->tab, ld, g, s->stream);
+        if (!grad_on_device)
+            HIPCHK(s, hipMemcpyAsync(coef_grad, g, 3 * static_cast<size_t>(ld) * sizeof(double),
+                                     hipMemcpyDeviceToHost, s->stream));
+    }
+    HIPCHK(s, hipMemcpyAsync(loss, s->fm_loss, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    return read_flags(s);
 }
 
 int msm_list_stats(msm_sim* s, long long* total, int* maxk) {

@@ -220,6 +220,21 @@ void adjoint_bins(const TabDev& t, int& nb1, int& nb2);
 void launch_adjoint(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                     const BoxDev* boxes, int multibox, const double* dldf, const int* gid,
                     AdjDev d, double* out_pairs, double* out_dens, hipStream_t s);
+// Device-resident force matching (msm_adjoint.hip).  launch_spline_tables: the knot tables b2,
+// f4, e4 (and m4 for EAM) of one kind from the cubic B-spline coefficients coef (rows pair,
+// density, embedding of ld doubles) on t's grid.  launch_fm_loss: from the sorted forces and the
+// box energies, dL/dF (caller order, FP(N,3)) and dL/dE for the adjoint and the loss into *loss
+// (part: fm_parts(n) doubles of scratch).  launch_spline_grad: dL/dc (3 rows of ld) from the
+// adjoint's bins of one kind.
+void launch_spline_tables(const double* coef, int ld, const TabDev& t, double2* b2, double4* f4,
+                          double4* e4, double4* m4, hipStream_t s);
+int fm_parts(int n);
+void launch_fm_loss(const double* fx, const double* fy, const double* fz, const int* gid, int n,
+                    const double* ft, const double* ebox, const double* et, const BoxDev* boxes,
+                    int nbox, double w_e, double* dldf, double* gbox, double* part, double* loss,
+                    hipStream_t s);
+void launch_spline_grad(const double* pairs, const double* dens, const TabDev& t, int ld,
+                        double* g, hipStream_t s);
 // Two lanes per atom with 32-byte records gathered by lane pairs (msm_pair.hip); the compacted
 // list is not used there.
 void launch_density_pair(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,

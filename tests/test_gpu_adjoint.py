@@ -193,3 +193,116 @@ def test_torch_autograd_function_matches_loss_and_grad():
     fm.close()
     assert abs(lt.item() - L) <= 1e-12 * abs(L)
     assert np.max(np.abs(c.grad.numpy() - grad)) <= 1e-12 * np.max(np.abs(grad))
+
+
+@pytest.mark.parametrize("pot_type", ["FS_TYPE", "EAM_TYPE"])
+def test_device_spline_tables_equal_host_tables(pot_type):
+    """msm_set_spline_table builds the table bits that the host spline map plus
+    msm_set_force_table upload: forces and box energies of 4 strained boxes are bitwise equal
+    for two coefficient sets taken in alternation (each build overwrites every knot of the
+    other), and the two sets give different forces."""
+    say(f"device spline tables vs host tables {pot_type}")
+    boxes, ctrl, ft = adj.make_d_config(nbox=4, pot_type=pot_type)
+    fm = adj.DeviceForceMatch(boxes, ctrl, ft)
+    c0 = fm.model.fit(ft)
+    c1 = adj.perturbed(c0, 0.02, seed=9)
+    fh0, eh0 = adj.ForceMatch.forward(fm, c0)
+    fd1, ed1 = fm.forward(c1)
+    fh1, eh1 = adj.ForceMatch.forward(fm, c1)
+    fd0, ed0 = fm.forward(c0)
+    fm.close()
+    assert np.array_equal(fd0, fh0) and np.array_equal(ed0, eh0)
+    assert np.array_equal(fd1, fh1) and np.array_equal(ed1, eh1)
+    assert not np.array_equal(fh0, fh1) and not np.array_equal(eh0, eh1)
+
+
+@pytest.mark.parametrize("pot_type", ["FS_TYPE", "EAM_TYPE"])
+def test_device_force_match_equals_host_path(pot_type):
+    """DeviceForceMatch (tables, loss, dL/dF, dL/dE, adjoint passes and coefficient transpose on
+    the device) against ForceMatch (host spline map, forces and energies copied out, loss and
+    transpose in numpy) on 8 strained boxes: targets bitwise equal, the loss to 1e-13 relative
+    (the sums run in another order) and dL/dc to 1e-11 of max|dL/dc|, with the same zero
+    pattern, at two coefficient points; loss() alone equals loss_and_grad()'s loss bit for
+    bit."""
+    say(f"device force matching vs host path {pot_type}")
+    hb, hctrl, hft = adj.make_d_config(nbox=8, pot_type=pot_type)
+    db, dctrl, dft = adj.make_d_config(nbox=8, pot_type=pot_type)
+    c0 = adj.SplineModel.from_table(hft).fit(hft)
+    host = adj.ForceMatch(hb, hctrl, hft)
+    dev = adj.DeviceForceMatch(db, dctrl, dft)
+    ct = adj.perturbed(c0, 0.01, seed=5)
+    host.set_targets(ct)
+    dev.set_targets(ct)
+    try:
+        assert np.array_equal(host.f_target, dev.f_target)
+        assert np.array_equal(host.e_target, dev.e_target)
+        for c in (c0, adj.perturbed(c0, 0.005, seed=8)):
+            lh, gh = host.loss_and_grad(c)
+            ld, gd = dev.loss_and_grad(c)
+            scale = np.max(np.abs(gh))
+            say(f"L host {lh:.15e} device {ld:.15e}; max|dg| {np.max(np.abs(gd - gh)):.3e} "
+                f"of {scale:.3e}")
+            assert gd.shape == gh.shape
+            assert lh > 0 and abs(ld - lh) <= 1e-13 * lh
+            assert np.max(np.abs(gd - gh)) <= 1e-11 * scale
+            assert np.array_equal(gd == 0.0, gh == 0.0)
+            assert dev.loss(c) == ld
+        if pot_type == "FS_TYPE":
+            assert np.all(gd[2] == 0.0)
+    finally:
+        host.close()
+        dev.close()
+
+
+def _hip_runtime():
+    """The HIP runtime libmsm_hip.so is linked against (already loaded in this process, so
+    dlopen returns that instance; torch ships a second runtime of its own, whose device pointers
+    the library's runtime does not know)."""
+    import ctypes as C
+    hip = C.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    return hip
+
+
+def test_device_force_match_with_device_pointers():
+    """Coefficients and gradient as device pointers (hipMalloc of the library's own runtime):
+    the same loss bit for bit and dL/dc to 1e-12 of max|dL/dc| (the adjoint's LDS atomics may
+    round differently between calls), every gradient entry written over a NaN fill; calls out
+    of order or with a short row are refused."""
+    import ctypes as C
+    from msmpscu_amd._lib import check, dptr, lib
+    say("device force matching with device pointers")
+    boxes, ctrl, ft = adj.make_d_config(nbox=4)
+    fm = adj.DeviceForceMatch(boxes, ctrl, ft)
+    c0 = fm.model.fit(ft)
+    out = np.zeros(1)
+    rc = lib().msm_fm_loss_grad(fm.dev.h, dptr(out), None, fm.ld, 0)
+    assert rc == 7  # MSM_ERR_STATE: no targets yet
+    short = np.zeros((3, fm.ld - 1))
+    assert lib().msm_set_spline_table(fm.dev.h, dptr(short), fm.ld - 1, 0) == 1  # MSM_ERR_ARG
+    fm.set_targets(adj.perturbed(c0, 0.01, seed=6))
+    L, g = fm.loss_and_grad(c0)
+    hip = _hip_runtime()
+    nb = c0.nbytes
+    cd, gd = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(cd), nb) == 0 and hip.hipMalloc(C.byref(gd), nb) == 0
+    try:
+        c = np.ascontiguousarray(c0)
+        nan = np.full(c0.shape, np.nan)
+        assert hip.hipMemcpy(cd, c.ctypes.data, nb, 1) == 0
+        assert hip.hipMemcpy(gd, nan.ctypes.data, nb, 1) == 0
+        check(lib().msm_set_spline_table(fm.dev.h, cd, fm.ld, 1), "msm_set_spline_table",
+              fm.dev.h)
+        check(lib().msm_fm_loss_grad(fm.dev.h, dptr(out), gd, fm.ld, 1), "msm_fm_loss_grad",
+              fm.dev.h)
+        got = np.zeros(c0.shape)
+        assert hip.hipMemcpy(got.ctypes.data, gd, nb, 2) == 0
+    finally:
+        hip.hipFree(cd)
+        hip.hipFree(gd)
+        fm.close()
+    assert out[0] == L
+    assert np.all(np.isfinite(got))
+    assert np.max(np.abs(got - g)) <= 1e-12 * np.max(np.abs(g))
