@@ -129,6 +129,8 @@ struct msm_sim {
            *fm_loss = nullptr, *fm_grad = nullptr;
     size_t fm_cap_coef = 0, fm_cap_ft = 0, fm_cap_et = 0, fm_cap_part = 0, fm_cap_loss = 0,
            fm_cap_grad = 0;
+    double* fm_host = nullptr;  // pinned staging: coefficients in, then loss and gradient out
+    size_t fm_cap_host = 0;
     int fm_n = -1;         // atom count the targets were given for (-1: none)
     double fm_w_e = 0.0;
 
@@ -794,6 +796,7 @@ int msm_destroy(msm_sim* s) {
     dfree(&s->d_stats);
     dfree(&s->d_boxwork);
     if (s->h_err) (void)hipHostFree(s->h_err);
+    if (s->fm_host) (void)hipHostFree(s->fm_host);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return MSM_OK;
@@ -2366,6 +2369,20 @@ static int fm_reserve(msm_sim* s, double** p, size_t m, size_t& cap) {
     return rc;
 }
 
+// Pinned host staging of at least m doubles (PCIe copies without the runtime's pageable bounce).
+static int fm_reserve_host(msm_sim* s, size_t m) {
+    if (s->fm_cap_host >= m && s->fm_host) return MSM_OK;
+    if (s->fm_host) (void)hipHostFree(s->fm_host);
+    s->fm_host = nullptr;
+    s->fm_cap_host = 0;
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->fm_host), m * sizeof(double)) != hipSuccess) {
+        s->fm_host = nullptr;
+        return set_err(s, MSM_ERR_NOMEM, "pinned staging allocation failed");
+    }
+    s->fm_cap_host = m;
+    return MSM_OK;
+}
+
 // The spline model's shape checks: one pair kind (one embedding kind for EAM) and ld covering
 // both grids.
 static int fm_check_model(msm_sim* s, int ld) {
@@ -2386,8 +2403,10 @@ int msm_set_spline_table(msm_sim* s, const double* coef, int ld, int coef_on_dev
     const double* c = coef;
     if (!coef_on_device) {
         if ((rc = fm_reserve(s, &s->fm_coef, m, s->fm_cap_coef)) != MSM_OK) return rc;
-        HIPCHK(s, hipMemcpyAsync(s->fm_coef, coef, m * sizeof(double), hipMemcpyHostToDevice,
-                                 s->stream));
+        if ((rc = fm_reserve_host(s, m + 1)) != MSM_OK) return rc;
+        std::memcpy(s->fm_host, coef, m * sizeof(double));
+        HIPCHK(s, hipMemcpyAsync(s->fm_coef, s->fm_host, m * sizeof(double),
+                                 hipMemcpyHostToDevice, s->stream));
         c = s->fm_coef;
     }
     // same buffers, sizes and grid: the table descriptor and captured segments stay valid
@@ -2430,9 +2449,9 @@ int msm_fm_loss_grad(msm_sim* s, double* loss, double* coef_grad, int ld, int gr
     if ((rc = adj_reserve(s, nb1, nb2, stride)) != MSM_OK) return rc;
     if ((rc = fm_reserve(s, &s->fm_part, msm::fm_parts(n), s->fm_cap_part)) != MSM_OK) return rc;
     if ((rc = fm_reserve(s, &s->fm_loss, 1, s->fm_cap_loss)) != MSM_OK) return rc;
-    if (coef_grad && !grad_on_device &&
-        (rc = fm_reserve(s, &s->fm_grad, 3 * static_cast<size_t>(ld), s->fm_cap_grad)) != MSM_OK)
-        return rc;
+    const size_t m = coef_grad && !grad_on_device ? 3 * static_cast<size_t>(ld) : 0;
+    if (m && (rc = fm_reserve(s, &s->fm_grad, m, s->fm_cap_grad)) != MSM_OK) return rc;
+    if ((rc = fm_reserve_host(s, m + 1)) != MSM_OK) return rc;
     clear_flags(s);
     enqueue_density(s);
     enqueue_force(s, 0, 1, 0.0);
@@ -2449,12 +2468,16 @@ int msm_fm_loss_grad(msm_sim* s, double* loss, double* coef_grad, int ld, int gr
         adj_launch(s, nb1, stride);
         double* g = grad_on_device ? coef_grad : s->fm_grad;
         msm::launch_spline_grad(s->adj_out, s->adj_out + nb1, s->tab, ld, g, s->stream);
-        if (!grad_on_device)
-            HIPCHK(s, hipMemcpyAsync(coef_grad, g, 3 * static_cast<size_t>(ld) * sizeof(double),
+        if (m)
+            HIPCHK(s, hipMemcpyAsync(s->fm_host + 1, g, m * sizeof(double),
                                      hipMemcpyDeviceToHost, s->stream));
     }
-    HIPCHK(s, hipMemcpyAsync(loss, s->fm_loss, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    return read_flags(s);
+    HIPCHK(s, hipMemcpyAsync(s->fm_host, s->fm_loss, sizeof(double), hipMemcpyDeviceToHost,
+                             s->stream));
+    rc = read_flags(s);  // synchronises the stream
+    *loss = s->fm_host[0];
+    if (m) std::memcpy(coef_grad, s->fm_host + 1, m * sizeof(double));
+    return rc;
 }
 
 int msm_list_stats(msm_sim* s, long long* total, int* maxk) {
