@@ -44,9 +44,15 @@ __global__ __launch_bounds__(kNlSmallBlock) void k_nlist_small(
     int* __restrict__ kvois, int* __restrict__ indi, unsigned short* __restrict__ sidx,
     int stride, int* err) {
 #pragma clang fp contract(off)
-    // about 40 KB for one atom type, so that four workgroups share a CU
-    __shared__ double lx[kNlCapPass], ly[kNlCapPass], lz[kNlCapPass];
-    __shared__ int lt[MT ? kNlCapPass : 1];
+    // about 40 KB for one atom type, so that four workgroups share a CU.  The x, y, z rows are
+    // one array with kNlGroup - 1 slots of padding at its end: a candidate group reads up to
+    // kNlGroup - 1 slots past its cell (the next cell's atoms, the next row's first ones or the
+    // padding) and masks them, so the loads need no clamped index.
+    __shared__ double lxyz[3 * kNlCapPass + kNlGroup - 1];
+    __shared__ int lt[MT ? kNlCapPass + kNlGroup - 1 : 1];
+    double* const lx = lxyz;
+    double* const ly = lxyz + kNlCapPass;
+    double* const lz = lxyz + 2 * kNlCapPass;
     __shared__ int hbase[kNlMaxHalo];  // sorted index of halo slot q in cell h: hbase[h] + q
     __shared__ signed char hsh[kNlMaxHalo * 3];
     __shared__ double srnl2[MT ? kMaxGroup * kMaxGroup : 1];
@@ -72,6 +78,9 @@ __global__ __launch_bounds__(kNlSmallBlock) void k_nlist_small(
             lz[s] = pq.z + shift_of(hsh[3 * h + 2], B.len[2]);
             if constexpr (MT) lt[s] = ityp[q];
         });
+        // the types read past the last staged atom index srnl2: keep them valid
+        if constexpr (MT)
+            if (tid < kNlGroup - 1) lt[total + tid] = 0;
     }
     __syncthreads();
     const int nhome = L.homeoff[g.nhc];
@@ -89,41 +98,48 @@ __global__ __launch_bounds__(kNlSmallBlock) void k_nlist_small(
         const int p = hbase[h] + si;
         const double xi = lx[si], yi = ly[si], zi = lz[si];
         const int ti = MT ? lt[si] : 0;
-        int* __restrict__ li = indi + list_base(p, mx);
-        unsigned short* __restrict__ ls = SIDX ? sidx + list_base(p, mx) : nullptr;
-        int c = 0;
-        for (int s = 0; s < 27; ++s) {
-            const int hn = h + c_nix[s] + g.H[0] * (c_niy[s] + g.H[1] * c_niz[s]);
+        // slot c of this atom is li[c * kListRow]: the store pointer advances by kListRow per hit
+        // and stops storing at the row's end (the count goes on, for the overflow flag)
+        int* __restrict__ const li = indi + list_base(p, mx);
+        int* const lend = li + static_cast<size_t>(mx) * kListRow;
+        int* __restrict__ w = li;
+        // one stencil cell, candidates in slot order; OWN: the home cell (stencil entry 0), the
+        // only one that holds atom si itself
+        auto scan = [&](auto own, int hn) {
             const int qe = L.hoff[hn + 1];
             const int gb = hbase[hn];
             for (int q0 = L.hoff[hn]; q0 < qe; q0 += kNlGroup) {
                 bool hit[kNlGroup];
 #pragma unroll
                 for (int u = 0; u < kNlGroup; ++u) {
-                    const bool valid = q0 + u < qe && q0 + u != si;
-                    const int q = valid ? q0 + u : si;
+                    const int q = q0 + u;
                     // XPT(I) - (XP(J) + CXYZ), squared and summed left to right, no contraction
                     const double dx = xi - lx[q];
                     const double dy = yi - ly[q];
                     const double dz = zi - lz[q];
                     const double r2 = dx * dx + dy * dy + dz * dz;
                     const double lim = MT ? srnl2[ti * ngroup + lt[q]] : rsingle;
-                    hit[u] = valid && r2 < lim;
+                    bool ok = u == 0 || q < qe;
+                    if constexpr (decltype(own)::value) ok = ok && q != si;
+                    hit[u] = ok && r2 < lim;
                 }
 #pragma unroll
                 for (int u = 0; u < kNlGroup; ++u) {
                     if (hit[u]) {
-                        if (c < mx) {
-                            // the slot stride is kListRow (a shift, not a multiply)
-                            const unsigned at = static_cast<unsigned>(c) * kListRow;
-                            li[at] = gb + q0 + u;
-                            if constexpr (SIDX) ls[at] = static_cast<unsigned short>(q0 + u);
+                        if (w < lend) {
+                            *w = gb + q0 + u;
+                            if constexpr (SIDX)
+                                sidx[w - indi] = static_cast<unsigned short>(q0 + u);
                         }
-                        ++c;
+                        w += kListRow;
                     }
                 }
             }
-        }
+        };
+        scan(std::true_type{}, h);
+        for (int s = 1; s < 27; ++s)
+            scan(std::false_type{}, h + c_nix[s] + g.H[0] * (c_niy[s] + g.H[1] * c_niz[s]));
+        int c = static_cast<int>((w - li) / kListRow);
         if (c > mx) {
             atomicOr(&err[kErrOverflow], 1);
             c = mx;
