@@ -158,6 +158,9 @@ __global__ __launch_bounds__(kNlSmallBlock) void k_nlist_small(
 // (stencil cell, then slot) whatever T is.  A brick whose halo exceeds `cap` falls back to the
 // global-memory scan.
 constexpr int kNlUnroll = 4;
+// Slots of padding after the staged rows: a round of kNlUnroll * T candidates (T <= 8) reads up
+// to kNlUnroll * T - 1 slots past its cell and masks them.
+constexpr int kNlPad = kNlUnroll * 8 - 1;
 
 template <int MT, int T, int SIDX>
 __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
@@ -167,11 +170,12 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
     int* __restrict__ kvois, int* __restrict__ indi, unsigned short* __restrict__ sidx,
     int stride, int* err, int cap) {
 #pragma clang fp contract(off)
+    static_assert(kNlUnroll * T - 1 <= kNlPad, "padding too small for T");
     extern __shared__ double s_nl[];
     double* lx = s_nl;
     double* ly = lx + cap;
     double* lz = ly + cap;
-    int* lt = reinterpret_cast<int*>(lz + cap);  // MT only
+    int* lt = reinterpret_cast<int*>(lz + cap + kNlPad);  // MT only
     __shared__ int hbase[kNlMaxHalo];  // sorted index of halo slot q in cell h: hbase[h] + q
     __shared__ signed char hsh[kNlMaxHalo * 3];
     __shared__ double srnl2[MT ? kMaxGroup * kMaxGroup : 1];
@@ -196,6 +200,9 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
             lz[s] = pq.z + shift_of(hsh[3 * h + 2], B.len[2]);
             if constexpr (MT) lt[s] = ityp[q];
         });
+        // the types read past the last staged atom index srnl2: keep them valid
+        if constexpr (MT)
+            if (tid < kNlPad) lt[total + tid] = 0;
     }
     __syncthreads();
     const int nhome = L.homeoff[g.nhc];
@@ -222,8 +229,10 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
         const int ti = MT ? lt[si] : 0;
         const size_t lb = list_base(p, mx);
         int c = 0;
-        for (int s = 0; s < 27; ++s) {
-            const int hn = h + c_nix[s] + g.H[0] * (c_niy[s] + g.H[1] * c_niz[s]);
+        // one stencil cell; OWN: the home cell (stencil entry 0), the only one holding atom si.
+        // Candidates past the cell are read (the next cell, the next row or the padding) and
+        // masked.
+        auto scan = [&](auto own, int hn) {
             const int qe = L.hoff[hn + 1];
             const int gb = hbase[hn];
             for (int q0 = L.hoff[hn]; q0 < qe; q0 += kNlUnroll * T) {
@@ -231,15 +240,15 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
 #pragma unroll
                 for (int u = 0; u < kNlUnroll; ++u) {
                     const int q = q0 + u * T + sub;
-                    const bool valid = q < qe && q != si;
-                    const int qq = valid ? q : si;
                     // XPT(I) - (XP(J) + CXYZ), squared and summed left to right, no contraction
-                    const double dx = xi - lx[qq];
-                    const double dy = yi - ly[qq];
-                    const double dz = zi - lz[qq];
+                    const double dx = xi - lx[q];
+                    const double dy = yi - ly[q];
+                    const double dz = zi - lz[q];
                     const double r2 = dx * dx + dy * dy + dz * dz;
-                    const double lim = MT ? srnl2[ti * ngroup + lt[qq]] : rsingle;
-                    hit[u] = valid && r2 < lim;
+                    const double lim = MT ? srnl2[ti * ngroup + lt[q]] : rsingle;
+                    bool ok = q < qe;
+                    if constexpr (decltype(own)::value) ok = ok && q != si;
+                    hit[u] = ok && r2 < lim;
                 }
 #pragma unroll
                 for (int u = 0; u < kNlUnroll; ++u) {
@@ -257,7 +266,10 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
                     c += __popc(grp);
                 }
             }
-        }
+        };
+        scan(std::true_type{}, h);
+        for (int s = 1; s < 27; ++s)
+            scan(std::false_type{}, h + c_nix[s] + g.H[0] * (c_niy[s] + g.H[1] * c_niz[s]));
         if (sub == 0) {
             if (c > mx) {
                 atomicOr(&err[kErrOverflow], 1);
@@ -269,7 +281,9 @@ __global__ __launch_bounds__(kNlBlock, MT ? 4 : 8) void k_nlist_bricks(
 }
 
 size_t nlist_lds_bytes(int cap, int ngroup) {
-    return static_cast<size_t>(cap) * (3 * sizeof(double) + (ngroup > 1 ? sizeof(int) : 0));
+    const size_t slots = static_cast<size_t>(cap) + kNlPad;
+    return (static_cast<size_t>(cap) * 2 + slots) * sizeof(double) +
+           (ngroup > 1 ? slots * sizeof(int) : 0);
 }
 
 __global__ void k_zero_range(int* __restrict__ p, const int* __restrict__ first, int n) {

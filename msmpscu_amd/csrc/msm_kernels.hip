@@ -68,23 +68,27 @@ __global__ void k_cell_assign(const double4* __restrict__ pos, const int* __rest
     }
     key[i] = c;
     // one atomic per distinct cell of the wave (after the first build the atoms are cell-sorted,
-    // so a wave spans a few cells): the leader of each cell adds the cell's lane count and every
-    // lane takes base + its rank among those lanes.  Slot order inside a cell is made
-    // deterministic afterwards by k_cell_order.
+    // so a wave spans a few cells): the lanes of each cell are grouped by ballots first, then
+    // the leader of every cell adds the cell's lane count, all leaders in one atomic
+    // instruction (one round trip per wave), and every lane takes base + its rank among its
+    // cell's lanes.  Slot order inside a cell is made deterministic afterwards by k_cell_order.
     const int lane = threadIdx.x & 63;
     unsigned long long todo = __ballot(1);
-    int my = 0;
+    int lead = lane, rank = 0, cnt = 0;
     while (todo) {
         const int leader = __ffsll(static_cast<long long>(todo)) - 1;
         const int lc = __shfl(c, leader, 64);
         const unsigned long long same = __ballot(c == lc) & todo;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&count[lc], __popcll(same));
-        base = __shfl(base, leader, 64);
-        if (c == lc) my = base + __popcll(same & ((1ull << lane) - 1ull));
+        if (c == lc) {
+            lead = leader;
+            rank = __popcll(same & ((1ull << lane) - 1ull));
+            if (lane == leader) cnt = __popcll(same);
+        }
         todo &= ~same;
     }
-    slot[i] = my;
+    int base = 0;
+    if (cnt > 0) base = atomicAdd(&count[c], cnt);
+    slot[i] = __shfl(base, lead, 64) + rank;
 }
 
 void launch_cell_assign(const double4* pos, const int* statu, const int* boxid,
