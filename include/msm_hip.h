@@ -378,7 +378,9 @@ int msm_adjoint(msm_sim* sim, const double* dldf, const double* dlde, double* g_
  * by msm_set_force_table (whose NTAB, CSI, NEMBD, RHOD, KPAIR and KEMBD it keeps; it must hold
  * one pair kind and, for EAM, one embedding kind; ld >= max(NTAB, NEMBD)).  The result is
  * bit-identical to msm_set_force_table on the same tables evaluated on the host with these
- * formulas.  coef is a device pointer when coef_on_device is nonzero.
+ * formulas.  coef is a device pointer when coef_on_device is nonzero (read before the call
+ * returns); host coefficients are staged, and the build is queued on the context's stream
+ * ahead of every later call (msm_set_force_table waits for it).
  *
  * msm_fm_set_targets keeps the target forces (3*N doubles, caller order, FP(N,3) layout) and box
  * energies (one per box) on the device with the energy weight w_e >= 0.

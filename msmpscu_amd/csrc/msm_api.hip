@@ -130,6 +130,8 @@ struct msm_sim {
     size_t fm_cap_coef = 0, fm_cap_ft = 0, fm_cap_et = 0, fm_cap_part = 0, fm_cap_loss = 0,
            fm_cap_grad = 0;
     double* fm_host = nullptr;  // pinned staging: coefficients in, then loss and gradient out
+    hipEvent_t fm_ev = nullptr; // end of the last msm_set_spline_table on the stream
+    bool fm_pending = false;    // host-synchronous table writers wait for fm_ev first
     size_t fm_cap_host = 0;
     int fm_n = -1;         // atom count the targets were given for (-1: none)
     double fm_w_e = 0.0;
@@ -772,6 +774,7 @@ int msm_destroy(msm_sim* s) {
         (void)hipEventDestroy(std::get<2>(p));
     }
     for (hipEvent_t e : s->evpool) (void)hipEventDestroy(e);
+    if (s->fm_ev) (void)hipEventDestroy(s->fm_ev);
     dfree(&s->d_b2);
     dfree(&s->d_f4);
     dfree(&s->d_e4);
@@ -811,6 +814,11 @@ int msm_set_force_table(msm_sim* s, int pot_type, int ngroup, int nkind, int nta
                         const double* dfembd) {
     if (!s) return MSM_ERR_ARG;
     (void)hipSetDevice(s->device);
+    // the copies below are not ordered after table kernels still queued on the stream
+    if (s->fm_pending) {
+        HIPCHK(s, hipEventSynchronize(s->fm_ev));
+        s->fm_pending = false;
+    }
     if (pot_type != MSM_FS_TYPE && pot_type != MSM_EAM_TYPE)
         return set_err(s, MSM_ERR_ARG, "pot_type must be MSM_FS_TYPE or MSM_EAM_TYPE");
     if (ngroup < 1 || ngroup > msm::kMaxGroup || nkind < 1 || ntab < 2 || !(csi > 0.0) ||
@@ -2399,6 +2407,10 @@ int msm_set_spline_table(msm_sim* s, const double* coef, int ld, int coef_on_dev
     int rc = fm_check_model(s, ld);
     if (rc != MSM_OK) return rc;
     (void)hipSetDevice(s->device);
+    if (!s->fm_ev) HIPCHK(s, hipEventCreateWithFlags(&s->fm_ev, hipEventDisableTiming));
+    // the staging buffer and the table buffers are free once the previous build has run
+    if (s->fm_pending) HIPCHK(s, hipEventSynchronize(s->fm_ev));
+    s->fm_pending = false;
     const size_t m = 3 * static_cast<size_t>(ld);
     const double* c = coef;
     if (!coef_on_device) {
@@ -2412,7 +2424,15 @@ int msm_set_spline_table(msm_sim* s, const double* coef, int ld, int coef_on_dev
     // same buffers, sizes and grid: the table descriptor and captured segments stay valid
     msm::launch_spline_tables(c, ld, s->tab, s->d_b2, s->d_f4, s->d_e4, s->d_m4, s->stream);
     HIPCHK(s, hipGetLastError());
-    HIPCHK(s, hipStreamSynchronize(s->stream));
+    // stream-ordered before every later call on this context; msm_set_force_table, whose
+    // copies are not, and the next build wait for the event.  A caller's device coefficients
+    // are read before the call returns.
+    if (coef_on_device) {
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        return MSM_OK;
+    }
+    HIPCHK(s, hipEventRecord(s->fm_ev, s->stream));
+    s->fm_pending = true;
     return MSM_OK;
 }
 
