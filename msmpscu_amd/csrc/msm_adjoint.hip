@@ -131,9 +131,8 @@ __device__ __forceinline__ void fprime(const TabDev& t, int ke, double rho, doub
         return;
     }
     if constexpr (!EAM) {
-        const double sr = sqrt(rho);
-        fp = -0.5 / sr;
-        dfp = 0.25 / (rho * sr);
+        fp = fs_fprime(rho);
+        dfp = 0.25 / (rho * sqrt(rho));
     } else {
         const double sk = rho / t.rhod + 1.0;
         int kk = static_cast<int>(sk + 0.000001);
@@ -352,7 +351,8 @@ void adjoint_bins(const TabDev& t, int& nb1, int& nb2) {
 
 void launch_adjoint(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                     const BoxDev* boxes, int multibox, const double* dldf, const int* gid,
-                    AdjDev d, double* out_pairs, double* out_dens, hipStream_t s) {
+                    AdjDev d, double* out_pairs, double* out_dens, hipStream_t s,
+                    bool rho_ready) {
     if (a.n <= 0) return;
     const bool eam = t.pot_type != 0, mt = t.ngroup > 1;
     const unsigned g1 = static_cast<unsigned>((a.n + 255) / 256);
@@ -360,8 +360,10 @@ void launch_adjoint(const AtomsDev& a, const ListDev& l, const TabDev& t, const 
 #define MSM_AR(MB, E, M) hipLaunchKernelGGL((k_adj_rho<MB, E, M>), dim3(g1), dim3(256), 0, s, a, \
                                             l, t, box0, boxes, d)
 #define MSM_ARM(MB, E) do { if (mt) MSM_AR(MB, E, 1); else MSM_AR(MB, E, 0); } while (0)
-    if (multibox) { if (eam) MSM_ARM(1, 1); else MSM_ARM(1, 0); }
-    else          { if (eam) MSM_ARM(0, 1); else MSM_ARM(0, 0); }
+    if (!rho_ready) {
+        if (multibox) { if (eam) MSM_ARM(1, 1); else MSM_ARM(1, 0); }
+        else          { if (eam) MSM_ARM(0, 1); else MSM_ARM(0, 0); }
+    }
 #undef MSM_ARM
 #undef MSM_AR
     const int nchunks = static_cast<int>((2LL * a.n + kAdjBlock - 1) / kAdjBlock);

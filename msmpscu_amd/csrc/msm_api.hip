@@ -2283,7 +2283,7 @@ static int adj_reserve(msm_sim* s, size_t& nb1, size_t& nb2, size_t& stride) {
 
 // The adjoint passes on dL/dF (s->adj_dldf, caller order) and dL/dE (s->adj_gbox) into the bins
 // s->adj_out (pairs) and s->adj_out + nb1 (density).
-static void adj_launch(msm_sim* s, size_t nb1, size_t stride) {
+static void adj_launch(msm_sim* s, size_t nb1, size_t stride, bool rho_ready = false) {
     msm::AdjDev d{};
     d.g = s->adj_g;
     d.gbox = s->adj_gbox;
@@ -2295,7 +2295,7 @@ static void adj_launch(msm_sim* s, size_t nb1, size_t stride) {
     TimedScope ts(s, MSM_KCLASS_ADJOINT);
     msm::launch_adjoint(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s),
                         s->adj_dldf, s->set[s->scur].gid, d, s->adj_out, s->adj_out + nb1,
-                        s->stream);
+                        s->stream, rho_ready);
 }
 
 int msm_adjoint(msm_sim* s, const double* dldf, const double* dlde, double* g_potr,
@@ -2476,8 +2476,10 @@ int msm_fm_loss_grad(msm_sim* s, double* loss, double* coef_grad, int ld, int gr
     enqueue_density(s);
     enqueue_force(s, 0, 1, 0.0);
     AtomsDev a = atoms_view(s, s->scur, s->pcur);
+    // with a gradient the energy pass also keeps rho for the adjoint, whose F' is the density
+    // pass's (in pos.w), so the adjoint's own rho pass is skipped
     msm::launch_epot(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), s->d_err,
-                     s->stream);
+                     s->stream, coef_grad ? s->adj_rho : nullptr);
     msm::launch_box_sum(s->epot, s->sb.start, s->d_boxes, nbox, s->d_ebox, s->stream);
     const AtomSet& z = s->set[s->scur];
     const size_t na = s->n_alloc;
@@ -2485,7 +2487,7 @@ int msm_fm_loss_grad(msm_sim* s, double* loss, double* coef_grad, int ld, int gr
                         s->d_boxes, nbox, s->fm_w_e, s->adj_dldf, s->adj_gbox, s->fm_part,
                         s->fm_loss, s->stream);
     if (coef_grad) {
-        adj_launch(s, nb1, stride);
+        adj_launch(s, nb1, stride, true);
         double* g = grad_on_device ? coef_grad : s->fm_grad;
         msm::launch_spline_grad(s->adj_out, s->adj_out + nb1, s->tab, ld, g, s->stream);
         if (m)

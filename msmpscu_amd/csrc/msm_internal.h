@@ -217,9 +217,12 @@ int adjoint_workgroups(int n);
 void adjoint_bins(const TabDev& t, int& nb1, int& nb2);
 // out_pairs: 3 x nkind x ntab2 knot gradients [FPOTR | FPOTB | POTR], each kind-major; out_dens:
 // nkind x ntab2 + 2 x nkind1 x nembd2 [POTB | FEMBD | DFEMBD] (embedding parts for EAM only).
+// rho_ready: d.rho and pos.w (F') already hold the current values (the energy pass's rho_out and
+// the density pass), so the rho pass is skipped.
 void launch_adjoint(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                     const BoxDev* boxes, int multibox, const double* dldf, const int* gid,
-                    AdjDev d, double* out_pairs, double* out_dens, hipStream_t s);
+                    AdjDev d, double* out_pairs, double* out_dens, hipStream_t s,
+                    bool rho_ready = false);
 // Device-resident force matching (msm_adjoint.hip).  launch_spline_tables: the knot tables b2,
 // f4, e4 (and m4 for EAM) of one kind from the cubic B-spline coefficients coef (rows pair,
 // density, embedding of ld doubles) on t's grid.  launch_fm_loss: from the sorted forces and the
@@ -245,8 +248,10 @@ void launch_density(const AtomsDev& a, const ListDev& l, const TabDev& t, const 
                     const BoxDev* boxes, int multibox, int* err, hipStream_t s);
 void launch_force(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                   const BoxDev* boxes, int multibox, const StepParams& sp, hipStream_t s);
+// rho_out (may be null): each atom's density sum rho (0 for inactive atoms) as well.
 void launch_epot(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
-                 const BoxDev* boxes, int multibox, int* err, hipStream_t s);
+                 const BoxDev* boxes, int multibox, int* err, hipStream_t s,
+                 double* rho_out = nullptr);
 void launch_predictor(const AtomsDev& a, const TabDev& t, const BoxDev& box0,
                       const BoxDev* boxes, int multibox, double h, hipStream_t s);
 void launch_correction(const AtomsDev& a, const TabDev& t, double h, hipStream_t s);

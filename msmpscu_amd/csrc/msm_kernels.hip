@@ -445,7 +445,8 @@ void launch_force(const AtomsDev& a, const ListDev& l, const TabDev& t, const Bo
 
 template <int MB, int EAM>
 __global__ __launch_bounds__(kBlock) void k_epot(AtomsDev a, ListDev l, TabDev t, BoxDev box0,
-                                                 const BoxDev* __restrict__ boxes, int* err) {
+                                                 const BoxDev* __restrict__ boxes, int* err,
+                                                 double* __restrict__ rho_out) {
     const int i = atom_index();
     if (i >= a.n) return;
     double er0 = 0.0, den0 = 0.0;
@@ -472,6 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_epot(AtomsDev a, ListDev l, TabDev t
                 den0 = den0 + (e.z + dk * e.w);
             }
         }
+        if (rho_out) rho_out[i] = den0;
         if constexpr (!EAM) {
             den0 = -sqrt(den0);
         } else {
@@ -485,21 +487,23 @@ __global__ __launch_bounds__(kBlock) void k_epot(AtomsDev a, ListDev l, TabDev t
             const double4 m = t.m4[t.kembd[ti] * t.nembd2 + kk];
             den0 = m.x + (sk - kk) * m.y;
         }
+    } else if (rho_out) {
+        rho_out[i] = 0.0;
     }
     a.epot[i] = er0 + den0;
 }
 
 void launch_epot(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
-                 const BoxDev* boxes, int multibox, int* err, hipStream_t s) {
+                 const BoxDev* boxes, int multibox, int* err, hipStream_t s, double* rho_out) {
     if (a.n <= 0) return;
     const dim3 g(grid_for(a.n)), b(kBlock);
     const bool eam = t.pot_type != 0;
     if (multibox) {
-        if (eam) hipLaunchKernelGGL((k_epot<1, 1>), g, b, 0, s, a, l, t, box0, boxes, err);
-        else     hipLaunchKernelGGL((k_epot<1, 0>), g, b, 0, s, a, l, t, box0, boxes, err);
+        if (eam) hipLaunchKernelGGL((k_epot<1, 1>), g, b, 0, s, a, l, t, box0, boxes, err, rho_out);
+        else     hipLaunchKernelGGL((k_epot<1, 0>), g, b, 0, s, a, l, t, box0, boxes, err, rho_out);
     } else {
-        if (eam) hipLaunchKernelGGL((k_epot<0, 1>), g, b, 0, s, a, l, t, box0, boxes, err);
-        else     hipLaunchKernelGGL((k_epot<0, 0>), g, b, 0, s, a, l, t, box0, boxes, err);
+        if (eam) hipLaunchKernelGGL((k_epot<0, 1>), g, b, 0, s, a, l, t, box0, boxes, err, rho_out);
+        else     hipLaunchKernelGGL((k_epot<0, 0>), g, b, 0, s, a, l, t, box0, boxes, err, rho_out);
     }
 }
 
