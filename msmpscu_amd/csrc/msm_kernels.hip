@@ -443,19 +443,23 @@ void launch_force(const AtomsDev& a, const ListDev& l, const TabDev& t, const Bo
 // energy pass
 // --------------------------------------------------------------------------------------------
 
+// Two lanes per atom (lanes 2i and 2i+1 take the even and odd list slots; the lane pair's sums
+// are added once at the end), the first lane finishes the atom.
 template <int MB, int EAM>
 __global__ __launch_bounds__(kBlock) void k_epot(AtomsDev a, ListDev l, TabDev t, BoxDev box0,
                                                  const BoxDev* __restrict__ boxes, int* err,
                                                  double* __restrict__ rho_out) {
-    const int i = atom_index();
-    if (i >= a.n) return;
+    const int gt = atom_index();
+    const int i = gt >> 1, sub = gt & 1;
+    const bool have = i < a.n;  // both lanes of a pair agree, so the shuffle below is paired
     double er0 = 0.0, den0 = 0.0;
-    if ((a.statu[i] & STATU_ACTIVE) == STATU_ACTIVE) {
+    const bool active = have && (a.statu[i] & STATU_ACTIVE) == STATU_ACTIVE;
+    if (active) {
         const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, i);
         const double4 pi = a.pos[i];
         const int ti = a.ityp[i];
         const int kv = l.kvois[i];
-        for (int iw = 0; iw < kv; ++iw) {
+        for (int iw = sub; iw < kv; iw += 2) {
             const int j = l.indi[list_base(i, l.mx) + static_cast<size_t>(iw) * l.stride];
             const double4 pj = a.pos[j];
             const double sx = min_image(pi.x - pj.x, b.len[0], b.ilen[0]);
@@ -473,7 +477,12 @@ __global__ __launch_bounds__(kBlock) void k_epot(AtomsDev a, ListDev l, TabDev t
                 den0 = den0 + (e.z + dk * e.w);
             }
         }
-        if (rho_out) rho_out[i] = den0;
+    }
+    er0 = er0 + __shfl_xor(er0, 1, 64);
+    den0 = den0 + __shfl_xor(den0, 1, 64);
+    if (!have || sub != 0) return;
+    if (rho_out) rho_out[i] = den0;
+    if (active) {
         if constexpr (!EAM) {
             den0 = -sqrt(den0);
         } else {
@@ -484,11 +493,9 @@ __global__ __launch_bounds__(kBlock) void k_epot(AtomsDev a, ListDev l, TabDev t
                 atomicOr(&err[kErrEmbd], 1);
                 kk = t.nembd;
             }
-            const double4 m = t.m4[t.kembd[ti] * t.nembd2 + kk];
+            const double4 m = t.m4[t.kembd[a.ityp[i]] * t.nembd2 + kk];
             den0 = m.x + (sk - kk) * m.y;
         }
-    } else if (rho_out) {
-        rho_out[i] = 0.0;
     }
     a.epot[i] = er0 + den0;
 }
@@ -496,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_epot(AtomsDev a, ListDev l, TabDev t
 void launch_epot(const AtomsDev& a, const ListDev& l, const TabDev& t, const BoxDev& box0,
                  const BoxDev* boxes, int multibox, int* err, hipStream_t s, double* rho_out) {
     if (a.n <= 0) return;
-    const dim3 g(grid_for(a.n)), b(kBlock);
+    const dim3 g(grid_for(2 * a.n)), b(kBlock);
     const bool eam = t.pot_type != 0;
     if (multibox) {
         if (eam) hipLaunchKernelGGL((k_epot<1, 1>), g, b, 0, s, a, l, t, box0, boxes, err, rho_out);
