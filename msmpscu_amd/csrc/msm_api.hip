@@ -2380,6 +2380,11 @@ static int fm_reserve(msm_sim* s, double** p, size_t m, size_t& cap) {
 // Pinned host staging of at least m doubles (PCIe copies without the runtime's pageable bounce).
 static int fm_reserve_host(msm_sim* s, size_t m) {
     if (s->fm_cap_host >= m && s->fm_host) return MSM_OK;
+    // a queued table build may still be copying from the old buffer
+    if (s->fm_pending) {
+        HIPCHK(s, hipEventSynchronize(s->fm_ev));
+        s->fm_pending = false;
+    }
     if (s->fm_host) (void)hipHostFree(s->fm_host);
     s->fm_host = nullptr;
     s->fm_cap_host = 0;
