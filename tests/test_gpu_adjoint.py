@@ -306,3 +306,30 @@ def test_device_force_match_with_device_pointers():
     assert out[0] == L
     assert np.all(np.isfinite(got))
     assert np.max(np.abs(got - g)) <= 1e-12 * np.max(np.abs(g))
+
+
+def test_device_force_match_row_lengths():
+    """Coefficient rows longer than the tables (ld beyond max(NTAB, NEMBD), which also grows the
+    pinned staging buffer while a table build may be queued): the same loss bit for bit, the
+    same dL/dc on the common columns to 1e-12 of its largest entry, zeros beyond."""
+    from msmpscu_amd._lib import check, dptr, lib
+    say("device force matching with longer coefficient rows")
+    boxes, ctrl, ft = adj.make_d_config(nbox=4)
+    fm = adj.DeviceForceMatch(boxes, ctrl, ft)
+    c0 = fm.model.fit(ft)
+    fm.set_targets(adj.perturbed(c0, 0.01, seed=7))
+    L, g = fm.loss_and_grad(c0)
+    ld2 = fm.ld + 777
+    c2 = np.zeros((3, ld2))
+    c2[:, : fm.ld] = c0
+    c2[:, fm.ld:] = 1.0e3  # never read: beyond NTAB and NEMBD
+    out = np.zeros(1)
+    g2 = np.full((3, ld2), np.nan)
+    check(lib().msm_set_spline_table(fm.dev.h, dptr(c2), ld2, 0), "msm_set_spline_table",
+          fm.dev.h)
+    check(lib().msm_fm_loss_grad(fm.dev.h, dptr(out), dptr(g2), ld2, 0), "msm_fm_loss_grad",
+          fm.dev.h)
+    fm.close()
+    assert out[0] == L
+    assert np.all(g2[:, fm.ld:] == 0.0)
+    assert np.max(np.abs(g2[:, : fm.ld] - g)) <= 1e-12 * np.max(np.abs(g))
