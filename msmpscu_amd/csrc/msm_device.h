@@ -63,15 +63,14 @@ static __device__ __forceinline__ size_t list_base(int p, int mx) {
     return static_cast<size_t>(p / kListRow) * static_cast<size_t>(mx) * kListRow + (p % kListRow);
 }
 
-// Minimum image: the reference writes "if periodic and |s| > L/2 then s - sign(L, s)" -- one
-// box length at most.  s - L*k with k = rint(s/L) clamped to [-1, 1] is that shift without a
-// branch: for |s| < 3L/2 the clamp is idle, beyond it the pair stays as far apart as the
-// reference leaves it (quench trial points can lie many box lengths out).  ilen = 0 switches
-// it off for free directions.  The subtraction is one rounding either way (fma with an exact
-// product L*k, k in {-1,0,1}).
+// Minimum image as the reference writes it: "if periodic and |s| > L/2 then s - sign(L, s)" --
+// one box length at most, whatever |s| (quench trial points can lie many box lengths out).
+// Branch-free: the shifted value is formed and selected (one rounding, as in the reference).
+// ilen = 0 marks a free direction (no shift).
 static __device__ __forceinline__ double min_image(double s, double len, double ilen) {
-    const double k = fmin(fmax(rint(s * ilen), -1.0), 1.0);
-    return fma(-len, k, s);
+    const double hm = ilen > 0.0 ? 0.5 * len : __builtin_huge_val();
+    const double t = s - __builtin_copysign(len, s);
+    return __builtin_fabs(s) > hm ? t : s;
 }
 
 // 1/sqrt(x), x > 0: the hardware estimate y (relative error below 6e-8 on gfx950) refined by one
