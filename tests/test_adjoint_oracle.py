@@ -193,3 +193,24 @@ def test_spline_fit_reproduces_value_tables():
         sl = slice(20, -20)
         err = np.max(np.abs(tabs[name][sl] - ref[sl]))
         assert err <= 1e-3 * np.max(np.abs(ref[sl])), (name, err)
+
+
+def test_spline_model_of_fs_tables_has_no_embedding_row():
+    """For FS tables (F = -sqrt(rho) built in) the model has no embedding knots: the third
+    coefficient row is zero after the fit, its gradient row is zero, and applying coefficients
+    leaves the table's (absent) embedding arrays alone while the value tables are reproduced."""
+    from msmpscu_amd import adjoint as adj
+    box = cfg.make_bcc_box((4, 4, 4), cfg.W_A, 0.0, 0.0, 1)
+    ctrl = cfg.w_ctrl(mx=64)
+    ft = cfg.w_forcetable(box, ctrl, pot_type="FS_TYPE")
+    m = adj.SplineModel.from_table(ft)
+    assert m.nembd == 0 and m.shape == (3, int(ft.NTAB))
+    c = m.fit(ft)
+    assert np.all(c[2] == 0.0)
+    potr = np.array(ft.POTR[0], dtype=np.float64)
+    m.apply(c, ft)
+    assert ft.FEMBD is None and ft.DFEMBD is None
+    assert np.max(np.abs(ft.POTR[0] - potr)) <= 1e-9 * np.max(np.abs(potr))
+    rng = np.random.default_rng(3)
+    g = {k: rng.normal(size=v.shape) for k, v in m.tables(c).items()}
+    assert np.all(m.coeff_grad(g)[2] == 0.0)
