@@ -277,15 +277,15 @@ class DeviceForceMatch(ForceMatch):
 
     def loss(self, c: np.ndarray) -> float:
         self.set_tables(c)
-        out = np.zeros(1)
+        out = np.empty(1)
         check(lib().msm_fm_loss_grad(self.dev.h, dptr(out), None, self.ld, 0),
               "msm_fm_loss_grad", self.dev.h)
         return float(out[0])
 
     def loss_and_grad(self, c: np.ndarray):
         self.set_tables(c)
-        out = np.zeros(1)
-        g = np.zeros((3, self.ld))
+        out = np.empty(1)
+        g = np.empty((3, self.ld))  # every entry is written
         check(lib().msm_fm_loss_grad(self.dev.h, dptr(out), dptr(g), self.ld, 0),
               "msm_fm_loss_grad", self.dev.h)
         return float(out[0]), g
