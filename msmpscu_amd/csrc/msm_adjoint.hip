@@ -71,14 +71,17 @@ __device__ __forceinline__ double pair_sum2(double v) { return v + __shfl_xor(v,
 
 }  // namespace
 
-// dL/dF in the sorted order, zero for inactive atoms.
+// dL/dF in the sorted order as one 32-byte record per atom (x, y, z, 0), so that the pair pass
+// gathers a neighbour's three components in one load; zero for inactive atoms.
 __global__ void k_adj_gather(const double* __restrict__ dldf, const int* __restrict__ gid,
-                             const int* __restrict__ statu, int n, double* __restrict__ g) {
+                             const int* __restrict__ statu, int n, double4* __restrict__ g) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int q = gid[p];
     const bool act = (statu[p] & STATU_ACTIVE) == STATU_ACTIVE;
-    for (int c = 0; c < 3; ++c) g[c * static_cast<size_t>(n) + p] = act ? dldf[c * static_cast<size_t>(n) + q] : 0.0;
+    const size_t m = static_cast<size_t>(n);
+    g[p] = act ? make_double4(dldf[q], dldf[m + q], dldf[2 * m + q], 0.0)
+               : make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
 template <int EAM>
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_pairs(AtomsDev a, ListDev l, 
     double* b_pr = bins + 2 * nkt;
     int c0, c1;
     adj_range(nchunks, c0, c1);
-    const size_t n = static_cast<size_t>(a.n);
+    const double4* __restrict__ g4 = reinterpret_cast<const double4*>(d.g);
     for (int c = c0; c < c1; ++c) {
         const int gt = c * blockDim.x + threadIdx.x;
         const int k = gt >> 1, sub = gt & 1;
@@ -168,7 +171,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_pairs(AtomsDev a, ListDev l, 
         if (active) {
             const BoxDev& b = box_of<MB>(box0, boxes, a.boxid, k);
             const double4 pk = a.pos[k];
-            const double gkx = d.g[k], gky = d.g[n + k], gkz = d.g[2 * n + k];
+            const double4 gk = g4[k];
+            const double gkx = gk.x, gky = gk.y, gkz = gk.z;
             const double gb = d.gbox[MB ? a.boxid[k] : 0];
             const int kv = l.kvois[k];
             const int tk = MT ? a.ityp[k] : 0;
@@ -193,7 +197,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_pairs(AtomsDev a, ListDev l, 
                 const double4 f = t.f4[kind * nt + kk];
                 const double bp = f.z + dk * f.w;
                 const double ak = gkx * sx + gky * sy + gkz * sz;
-                const double aj = d.g[j] * sx + d.g[n + j] * sy + d.g[2 * n + j] * sz;
+                const double4 gj = g4[j];
+                const double aj = gj.x * sx + gj.y * sy + gj.z * sz;
                 const int row = MT ? kind * nt : 0;
                 bin_add(b_fr + row, kk, dk, ak * inv_r * inv_r);
                 if constexpr (MT) {
@@ -356,7 +361,8 @@ void launch_adjoint(const AtomsDev& a, const ListDev& l, const TabDev& t, const 
     if (a.n <= 0) return;
     const bool eam = t.pot_type != 0, mt = t.ngroup > 1;
     const unsigned g1 = static_cast<unsigned>((a.n + 255) / 256);
-    hipLaunchKernelGGL(k_adj_gather, dim3(g1), dim3(256), 0, s, dldf, gid, a.statu, a.n, d.g);
+    hipLaunchKernelGGL(k_adj_gather, dim3(g1), dim3(256), 0, s, dldf, gid, a.statu, a.n,
+                       reinterpret_cast<double4*>(d.g));
 #define MSM_AR(MB, E, M) hipLaunchKernelGGL((k_adj_rho<MB, E, M>), dim3(g1), dim3(256), 0, s, a, \
                                             l, t, box0, boxes, d)
 #define MSM_ARM(MB, E) do { if (mt) MSM_AR(MB, E, 1); else MSM_AR(MB, E, 0); } while (0)

@@ -2254,7 +2254,7 @@ static int adj_reserve(msm_sim* s, size_t& nb1, size_t& nb2, size_t& stride) {
         dfree(&s->adj_lam);
         s->adj_cap_n = 0;
         rc = dalloc(s, &s->adj_dldf, 3 * static_cast<size_t>(s->n_alloc));
-        if (rc == MSM_OK) rc = dalloc(s, &s->adj_g, 3 * static_cast<size_t>(s->n_alloc));
+        if (rc == MSM_OK) rc = dalloc(s, &s->adj_g, 4 * static_cast<size_t>(s->n_alloc));
         if (rc == MSM_OK) rc = dalloc(s, &s->adj_rho, s->n_alloc);
         if (rc == MSM_OK) rc = dalloc(s, &s->adj_lam, s->n_alloc);
         if (rc != MSM_OK) return rc;

@@ -205,7 +205,8 @@ void launch_force_sb(const LtShape& sh, int nbricks, const int* brick_box, const
                      int ncell_total, const StepParams& sp, hipStream_t s);
 // Reverse-mode table gradients (msm_adjoint.hip): scratch of the adjoint passes.
 struct AdjDev {
-    double* g;           // dL/dF in the sorted order, 3 x n (zero for inactive atoms)
+    double* g;           // dL/dF in the sorted order, (x, y, z, 0) per atom, 32-byte aligned
+                         // (zero for inactive atoms)
     const double* gbox;  // dL/dE per box
     double* rho;         // rho_k
     double* lam;         // dL/dF'_k
