@@ -131,6 +131,9 @@ struct msm_sim {
            fm_cap_grad = 0;
     double* fm_host = nullptr;  // pinned staging: coefficients in, then loss and gradient out
     hipEvent_t fm_ev = nullptr; // end of the last msm_set_spline_table on the stream
+    // msm_fm_loss_grad's kernels and copies as graphs, by what they bake in (a few at most)
+    std::map<std::vector<uint64_t>, hipGraphExec_t> fm_graphs;
+    uint64_t graph_gen = 0;     // bumped by clear_graphs: every cached graph is stale
     bool fm_pending = false;    // host-synchronous table writers wait for fm_ev first
     size_t fm_cap_host = 0;
     int fm_n = -1;         // atom count the targets were given for (-1: none)
@@ -204,6 +207,9 @@ int dreserve(msm_sim* s, T** p, size_t count, size_t& cap) {
 void clear_graphs(msm_sim* s) {
     for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
     s->graphs.clear();
+    for (auto& kv : s->fm_graphs) (void)hipGraphExecDestroy(kv.second);
+    s->fm_graphs.clear();
+    ++s->graph_gen;
 }
 
 void free_atoms(msm_sim* s) {
@@ -2477,30 +2483,72 @@ int msm_fm_loss_grad(msm_sim* s, double* loss, double* coef_grad, int ld, int gr
     const size_t m = coef_grad && !grad_on_device ? 3 * static_cast<size_t>(ld) : 0;
     if (m && (rc = fm_reserve(s, &s->fm_grad, m, s->fm_cap_grad)) != MSM_OK) return rc;
     if ((rc = fm_reserve_host(s, m + 1)) != MSM_OK) return rc;
-    clear_flags(s);
-    enqueue_density(s);
-    enqueue_force(s, 0, 1, 0.0);
-    AtomsDev a = atoms_view(s, s->scur, s->pcur);
-    // with a gradient the energy pass also keeps rho for the adjoint, whose F' is the density
-    // pass's (in pos.w), so the adjoint's own rho pass is skipped
-    msm::launch_epot(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), s->d_err,
-                     s->stream, coef_grad ? s->adj_rho : nullptr);
-    msm::launch_box_sum(s->epot, s->sb.start, s->d_boxes, nbox, s->d_ebox, s->stream);
-    const AtomSet& z = s->set[s->scur];
-    const size_t na = s->n_alloc;
-    msm::launch_fm_loss(z.f, z.f + na, z.f + 2 * na, z.gid, n, s->fm_ft, s->d_ebox, s->fm_et,
-                        s->d_boxes, nbox, s->fm_w_e, s->adj_dldf, s->adj_gbox, s->fm_part,
-                        s->fm_loss, s->stream);
-    if (coef_grad) {
-        adj_launch(s, nb1, stride, true);
-        double* g = grad_on_device ? coef_grad : s->fm_grad;
-        msm::launch_spline_grad(s->adj_out, s->adj_out + nb1, s->tab, ld, g, s->stream);
-        if (m)
-            HIPCHK(s, hipMemcpyAsync(s->fm_host + 1, g, m * sizeof(double),
-                                     hipMemcpyDeviceToHost, s->stream));
+    // everything up to the flags as one graph on repeated calls (one launch instead of about
+    // twenty); the key holds what the captured kernels and copies bake in
+    hipError_t ce = hipSuccess;
+    auto enqueue = [&]() {
+        clear_flags(s);
+        enqueue_density(s);
+        enqueue_force(s, 0, 1, 0.0);
+        AtomsDev a = atoms_view(s, s->scur, s->pcur);
+        // with a gradient the energy pass also keeps rho for the adjoint, whose F' is the
+        // density pass's (in pos.w), so the adjoint's own rho pass is skipped
+        msm::launch_epot(a, list_view(s), s->tab, s->boxes[0], s->d_boxes, multibox(s), s->d_err,
+                         s->stream, coef_grad ? s->adj_rho : nullptr);
+        msm::launch_box_sum(s->epot, s->sb.start, s->d_boxes, nbox, s->d_ebox, s->stream);
+        const AtomSet& z = s->set[s->scur];
+        const size_t na = s->n_alloc;
+        msm::launch_fm_loss(z.f, z.f + na, z.f + 2 * na, z.gid, n, s->fm_ft, s->d_ebox, s->fm_et,
+                            s->d_boxes, nbox, s->fm_w_e, s->adj_dldf, s->adj_gbox, s->fm_part,
+                            s->fm_loss, s->stream);
+        if (coef_grad) {
+            adj_launch(s, nb1, stride, true);
+            double* g = grad_on_device ? coef_grad : s->fm_grad;
+            msm::launch_spline_grad(s->adj_out, s->adj_out + nb1, s->tab, ld, g, s->stream);
+            if (m && ce == hipSuccess)
+                ce = hipMemcpyAsync(s->fm_host + 1, g, m * sizeof(double), hipMemcpyDeviceToHost,
+                                    s->stream);
+        }
+        if (ce == hipSuccess)
+            ce = hipMemcpyAsync(s->fm_host, s->fm_loss, sizeof(double), hipMemcpyDeviceToHost,
+                                s->stream);
+    };
+    if (s->timing) {  // direct launches, each bracketed by events
+        enqueue();
+        HIPCHK(s, ce);
+    } else {
+        auto P = [](const void* q) { return static_cast<uint64_t>(reinterpret_cast<uintptr_t>(q)); };
+        uint64_t webits = 0;
+        std::memcpy(&webits, &s->fm_w_e, sizeof(webits));
+        const std::vector<uint64_t> key = {
+            s->graph_gen, static_cast<uint64_t>(s->scur), static_cast<uint64_t>(s->pcur),
+            static_cast<uint64_t>(s->track_dis), static_cast<uint64_t>(n),
+            static_cast<uint64_t>(ld), coef_grad ? 1u : 0u, static_cast<uint64_t>(m),
+            grad_on_device ? P(coef_grad) : 0u, webits, P(s->adj_dldf), P(s->adj_gbox),
+            P(s->adj_g), P(s->adj_rho), P(s->adj_lam), P(s->adj_part), P(s->adj_out),
+            P(s->fm_part), P(s->fm_loss), P(s->fm_grad), P(s->fm_host), P(s->fm_ft),
+            P(s->fm_et), P(s->epot), P(s->d_ebox)};
+        auto it = s->fm_graphs.find(key);
+        if (it == s->fm_graphs.end()) {
+            if (s->fm_graphs.size() >= 4) {  // buffers or sizes moved on: drop the old ones
+                for (auto& kv : s->fm_graphs) (void)hipGraphExecDestroy(kv.second);
+                s->fm_graphs.clear();
+            }
+            HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            enqueue();
+            hipGraph_t gr = nullptr;
+            const hipError_t ee = hipStreamEndCapture(s->stream, &gr);
+            if (ce == hipSuccess) ce = ee;
+            hipGraphExec_t ge = nullptr;
+            if (ce == hipSuccess) ce = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+            if (gr) (void)hipGraphDestroy(gr);
+            if (ce != hipSuccess)
+                return set_err(s, MSM_ERR_HIP, std::string("fitting-loop graph: ") +
+                                                   hipGetErrorString(ce));
+            it = s->fm_graphs.emplace(key, ge).first;
+        }
+        HIPCHK(s, hipGraphLaunch(it->second, s->stream));
     }
-    HIPCHK(s, hipMemcpyAsync(s->fm_host, s->fm_loss, sizeof(double), hipMemcpyDeviceToHost,
-                             s->stream));
     rc = read_flags(s);  // synchronises the stream
     *loss = s->fm_host[0];
     if (m) std::memcpy(coef_grad, s->fm_host + 1, m * sizeof(double));

@@ -333,3 +333,33 @@ def test_device_force_match_row_lengths():
     assert out[0] == L
     assert np.all(g2[:, fm.ld:] == 0.0)
     assert np.max(np.abs(g2[:, : fm.ld] - g)) <= 1e-12 * np.max(np.abs(g))
+
+
+def test_fitting_loop_graph_replays_match_direct_launches():
+    """msm_fm_loss_grad replays one captured graph on repeated calls; with kernel timing on it
+    launches the same work directly.  Over four coefficient sets, alternating the two modes with
+    eager work (a host-path table upload and force evaluation) and synchronisations in between,
+    every replay gives the direct launches' loss bit for bit and their dL/dc to 1e-12 of its
+    largest entry (the adjoint's LDS atomics may round differently between calls)."""
+    from msmpscu_amd._lib import MSM_OPT_KERNEL_TIMING
+    say("fitting-loop graph replays against direct launches")
+    boxes, ctrl, ft = adj.make_d_config(nbox=4)
+    fm = adj.DeviceForceMatch(boxes, ctrl, ft)
+    c0 = fm.model.fit(ft)
+    fm.set_targets(adj.perturbed(c0, 0.01, seed=11))
+    try:
+        for k in range(4):
+            c = adj.perturbed(c0, 0.004 * (k + 1), seed=20 + k)
+            fm.dev.set_option(MSM_OPT_KERNEL_TIMING, 1)
+            le, ge = fm.loss_and_grad(c)
+            fm.dev.set_option(MSM_OPT_KERNEL_TIMING, 0)
+            lg, gg = fm.loss_and_grad(c)
+            lg2, gg2 = fm.loss_and_grad(c)
+            assert lg == le and lg2 == le, (k, lg, lg2, le)
+            scale = np.max(np.abs(ge))
+            assert np.max(np.abs(gg - ge)) <= 1e-12 * scale
+            assert np.max(np.abs(gg2 - ge)) <= 1e-12 * scale
+            assert fm.loss(c) == le
+            adj.ForceMatch.forward(fm, c)  # eager work on the same context in between
+    finally:
+        fm.close()
