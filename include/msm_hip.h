@@ -391,7 +391,10 @@ int msm_adjoint(msm_sim* sim, const double* dldf, const double* dlde, double* g_
  * through the adjoint passes of msm_adjoint and the transpose of the table map above
  * (coef_grad is a device pointer when grad_on_device is nonzero).  Only the loss (and the
  * gradient, when on the host) cross PCIe.  Sums run in a fixed order except the adjoint's LDS
- * atomics, so the gradient of repeated calls agrees to rounding. */
+ * atomics, so the gradient of repeated calls agrees to rounding.  Repeated calls replay one
+ * captured HIP graph of the whole evaluation (recaptured when the atoms, list, tables' layout,
+ * options, ld, w_e or the gradient pointer change; with MSM_OPT_KERNEL_TIMING the kernels are
+ * launched directly and timed). */
 int msm_set_spline_table(msm_sim* sim, const double* coef, int ld, int coef_on_device);
 int msm_fm_set_targets(msm_sim* sim, const double* f_target, const double* e_target, double w_e);
 int msm_fm_loss_grad(msm_sim* sim, double* loss, double* coef_grad, int ld, int grad_on_device);
