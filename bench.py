@@ -220,7 +220,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
+    # 500 steps (about 0.1 s of GPU work): kernel traces show the first tens of milliseconds
+    # after the host-side setup running 7-20% slower (clock ramp) on some boxes
+    ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--ncell", type=int, default=80)
     ap.add_argument("--pot", default="FS_TYPE", choices=["FS_TYPE", "EAM_TYPE"])
     ap.add_argument("--no-dis", action="store_true", help="do not accumulate DIS in the step")
