@@ -224,6 +224,7 @@ def main():
     # after the host-side setup running 7-20% slower (clock ramp) on some boxes
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--ncell", type=int, default=80)
+    ap.add_argument("--mx", type=int, default=32, help="list capacity MXKVOIS of the P1 leg")
     ap.add_argument("--pot", default="FS_TYPE", choices=["FS_TYPE", "EAM_TYPE"])
     ap.add_argument("--no-dis", action="store_true", help="do not accumulate DIS in the step")
     ap.add_argument("--cpu-steps", type=int, default=100)
@@ -292,7 +293,11 @@ def main():
     nc = args.ncell
     log(f"building workload: bcc {nc}^3, 0.5% vacancies")
     box = cfg.make_bcc_box((nc, nc, nc), cfg.W_A, 0.05, 600.0, 1000, vacancy_frac=0.005)
-    ctrl = cfg.w_ctrl(mx=64)
+    # list capacity MXKVOIS: the longest list of this lattice is 26.  The list's 64-atom blocks
+    # are mx rows deep and only the rows below the block's longest list are read, so a deep
+    # block leaves gaps whose placement in physical memory decides the pass speed (contexts with
+    # mx = 64 ran 0.212-0.244 ms/step, with mx = 32 0.209-0.212); 32 keeps the rows dense.
+    ctrl = cfg.w_ctrl(mx=args.mx)
     ft = cfg.w_forcetable(box, ctrl, pot_type=args.pot)
     fc = md.MDForceClassGPU(ForceTable=ft)
     md.Register_ForceClass(args.pot, fc)
@@ -465,7 +470,8 @@ def main():
         "config": {"workload": f"plain MD, bcc W {nc}^3 cells - 0.5% vacancies "
                                f"({box.NPRT} atoms), {args.pot} 5000-pt tables, dt 1 fs, "
                                "rebuild every 10 steps, skin 0.3 A",
-                   "atoms": box.NPRT, "list_mean": round(kmean, 3), "list_max": kmax,
+                   "atoms": box.NPRT, "mxkvois": args.mx, "list_mean": round(kmean, 3),
+                   "list_max": kmax,
                    "dis_tracking": track, "pass_kernels": pass_kernels,
                    "parallelism": "single GPU"},
         "pcie_inclusive_value": round(pcie_value, 1),
